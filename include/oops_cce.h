@@ -1,0 +1,97 @@
+/*
+ * oops_cce.h — C ABI of the CCE host pipeline around the expv hot path.
+ *
+ * These entry points build what the reference's CCE driver builds on the
+ * host and feed the batched device expv of oops_expv.h.  Reference
+ * interfaces replaced (paths relative to the reference's src/):
+ *   oops_nv_states           <- NVCenter::make_espin_hamiltonian + eig_sym
+ *                               (source/app/DefectCenter.cpp:32-45)
+ *   oops_cce_cluster_terms   <- EnsembleCCE::create_spin_hamiltonian /
+ *                               create_spin_liouvillian / create_incoherent_operator /
+ *                               create_spin_density_state
+ *                               (source/app/ensemble_cce.cpp:57-102), flattened as
+ *                               MatExpVector does (source/math/MatExp.cpp:219-262)
+ *   oops_clusters_enumerate  <- cSpinCollection::make + getConnectionMatrix +
+ *                               cDepthFirstPathTracing::generate
+ *                               (source/spin/SpinCollection/SpinCollection.cpp:31-57,
+ *                                source/spin/SpinCluster/SpinClusterAlgorithm.cpp:98-148)
+ *   oops_clusters_parents    <- cClusterIndex sub-cluster positions
+ *                               (SpinClusterAlgorithm.cpp:50-63, SpinClusterIndex.cpp:29-42)
+ *   oops_cce_reduce          <- CCE::cce_coherence_reduction + compuate_final_coherence
+ *                               (source/app/cce.cpp:151-189)
+ *   oops_cce_hahn_batch_create <- CCE::run_each_clusters (source/app/cce.cpp:54-70)
+ *                               with EnsembleCCE::cluster_evolution for a Hahn echo
+ *                               (source/app/ensemble_cce.cpp:33-55) on the device expv
+ *   oops_bath_diamond        <- no reference counterpart: the seeded synthetic
+ *                               13C bath of the benchmark workloads.
+ */
+#ifndef OOPS_CCE_H
+#define OOPS_CCE_H
+
+#include <stdint.h>
+
+#include "oops_expv.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oops_terms oops_terms;
+typedef struct oops_clusters oops_clusters;
+
+/* Parameters of EnsembleCCE (EnsNVnBath's option set, SI units / Angstrom). */
+typedef struct {
+    double B[3];                /* magnetic field, T */
+    double nv_pos[3];           /* nitrogen position, Angstrom; electron at + 3.57/4 (1,1,1) */
+    int32_t state0, state1;     /* NV eigenstate indices (ascending energy) */
+    double dephasing_rate;      /* Hz (times 2 pi internally); 0 drops the terms */
+    double dephasing_axis[3];
+    double pol[3];              /* bath polarisation of the initial state */
+} oops_cce_params;
+
+/* Seeded diamond lattice (8-site cubic cell of side a, Angstrom), sites of
+ * the NV (origin and a/4 (1,1,1)) excluded, each other site within the
+ * region occupied with probability p (splitmix64 stream of `seed`, one draw
+ * per candidate site in cell-major order).  shape 0: sphere |r| <= radius,
+ * 1: cube max|x_i| <= radius.  Writes up to max_n coordinates (x,y,z). */
+int oops_bath_diamond(double a, double radius, double p, uint64_t seed, int32_t shape, double* coords,
+                      int32_t max_n, int32_t* n_out);
+
+/* NV eigenvalues (rad/s, ascending) and eigenvectors (3x3 complex,
+ * column-major, interleaved). */
+int oops_nv_states(const double* pos, const double* B, double* evals, double* evecs);
+
+/* Term table of one cluster of k 13C spins: which = 0 lvA, 1 lvB,
+ * 2 -lvB^dagger, 3 H0, 4 H1.  rho (n complex) and D may be NULL. */
+int oops_cce_cluster_terms(const double* coords, int32_t k, const oops_cce_params* P, int32_t which,
+                           oops_terms** out, double* rho, double* D);
+int oops_terms_view(const oops_terms* t, oops_term_table* view);
+int oops_terms_sizes(const oops_terms* t, int64_t* nterm, int64_t* npos, int64_t* nmat);
+int oops_terms_destroy(oops_terms* t);
+
+/* Connected clusters of 1..max_order spins for the pair graph d <= cutoff. */
+int oops_clusters_enumerate(const double* coords, int32_t nspin, double cutoff, int32_t max_order,
+                            oops_clusters** out);
+int oops_clusters_count(const oops_clusters* C, int32_t order, int64_t* count);
+/* Spin indices, count x (order+1), clusters in the reference's set order. */
+int oops_clusters_get(const oops_clusters* C, int32_t order, int32_t* idx);
+/* Parent positions (in order-1) of each cluster: CSR offsets [count+1] + list. */
+int oops_clusters_parents(const oops_clusters* C, int32_t order, int64_t* offsets, int32_t* list);
+int oops_clusters_destroy(oops_clusters* C);
+
+/* CCE reduction of L (all clusters' rows, orders concatenated, nt columns):
+ * tilde (same shape), final_result / each_order (nt x max_order, row-major),
+ * and the sign / log-magnitude of final_result.  Outputs may be NULL. */
+int oops_cce_reduce(const oops_clusters* C, int32_t nt, const double* L, double* tilde, double* final_result,
+                    double* each_order, double* log_abs_final, int32_t* sign_final);
+
+/* Build every cluster's Hahn-echo pair of sweeps and create the device batch
+ * (items ordered as the clusters: order 0 first, set order within an order). */
+int oops_cce_hahn_batch_create(oops_ctx* ctx, const double* coords, const oops_clusters* C, const oops_cce_params* P,
+                               int32_t m, int32_t klim, double tol, oops_hahn_batch** out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OOPS_CCE_H */

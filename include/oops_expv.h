@@ -1,0 +1,124 @@
+/*
+ * oops_expv.h — C ABI of the MI355X-native CCE Krylov-expv hot path.
+ *
+ * Plain C types only (no torch, no HIP types).  Every call returns an int
+ * status (OOPS_OK == 0, negative on error; oops_last_error() gives the text)
+ * instead of stopping the process the way the reference's Fortran does.
+ * Complex numbers are passed as interleaved (re, im) doubles, i.e. the
+ * memory layout of std::complex<double> / double _Complex.
+ *
+ * Reference interfaces replaced (paths relative to the reference's src/):
+ *   oops_term_table        <- the flattened term arrays MatExpVector builds
+ *                             (source/math/MatExp.cpp:219-262) and hands to
+ *                             main_mkl_/main_cache_ (expv/include/main_cache.h:6)
+ *   oops_hamvec            <- hamvec_cuda3_ / hamvec_func3_
+ *                             (expv/source/hamvec_cuda3.cu:661-665,
+ *                              expv/source/hamvec_func3.cpp:399)
+ *   oops_expv / main_cache_ <- MAIN_CACHE / MAIN_MKL (expv/source/main_cache.f:1-136,
+ *                             expv/source/main_mkl.f:1-129) as called by
+ *                             MatExpVector::runInexplicitGPU / runInexplicit
+ *                             (source/math/MatExp.cpp:216-521)
+ *   oops_zgpadm            <- zgpadm_ (expv/include/expokit.h:6,
+ *                             expv/source/expokit/expokit.f:762-934)
+ *   oops_hahn_batch_*      <- no reference counterpart: the reference calls one
+ *                             expv per cluster from CCE::run_each_clusters
+ *                             (source/app/cce.cpp:54-70); the batch runs every
+ *                             cluster's two Hahn-echo sweeps on the device and
+ *                             returns L_c(t) instead of the n x nt state matrix.
+ *   oops_cce_*             <- the CCE host pipeline: NV eigenstates
+ *                             (source/app/DefectCenter.cpp:32-45), per-cluster
+ *                             Liouvillians (source/app/ensemble_cce.cpp:33-113),
+ *                             cluster enumeration (source/spin/SpinCluster/
+ *                             SpinClusterAlgorithm.cpp:98-148) and the CCE
+ *                             reduction (source/app/cce.cpp:151-189).
+ */
+#ifndef OOPS_EXPV_H
+#define OOPS_EXPV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OOPS_OK 0
+#define OOPS_ERR_ARG (-1)
+#define OOPS_ERR_HIP (-2)
+#define OOPS_ERR_NOMEM (-3)
+#define OOPS_ERR_UNSUPPORTED (-4)
+#define OOPS_ERR_NUMERIC (-5)
+#define OOPS_ERR_CAPACITY (-6)
+
+typedef struct oops_ctx oops_ctx;
+typedef struct oops_hahn_batch oops_hahn_batch;
+
+/* One operator H = sum_t coeff[t] * kron_b mats[t,b] in the reference's
+ * flattened layout (MatExp.cpp:228-262): term t has nbody[t] bodies at
+ * pos[pos_offset[t] .. pos_offset[t+1]), body b's local matrix is the
+ * dims[b] x dims[b] column-major block mats[mat_offset[b] ..], spin 0 is the
+ * slowest Kronecker factor, spin_dim[s] is the local dimension of spin s. */
+typedef struct {
+    int32_t nspin;
+    int32_t nterm;
+    const double* coeff;        /* [nterm] real multipliers */
+    const int64_t* nbody;       /* [nterm] 0, 1 or 2 */
+    const int64_t* pos_offset;  /* [nterm + 1] */
+    const int64_t* pos;         /* [pos_offset[nterm]] */
+    const int64_t* dims;        /* [pos_offset[nterm]] */
+    const int64_t* mat_offset;  /* [pos_offset[nterm] + 1] (complex entries) */
+    const double* mats;         /* interleaved complex, [mat_offset[last]] */
+    const int64_t* spin_dim;    /* [nspin] */
+} oops_term_table;
+
+/* --- context ------------------------------------------------------------ */
+int oops_ctx_create(int32_t device, oops_ctx** out);
+int oops_ctx_destroy(oops_ctx* ctx);
+const char* oops_last_error(const oops_ctx* ctx);
+/* Device properties: number of compute units, LDS bytes per workgroup. */
+int oops_ctx_info(const oops_ctx* ctx, int32_t* num_cu, int32_t* lds_bytes, char* name, int32_t name_len);
+
+/* --- single operator ---------------------------------------------------- */
+/* w = H v for host vectors of length prod(spin_dim) (hamvec_cuda3_). */
+int oops_hamvec(oops_ctx* ctx, const oops_term_table* H, const double* v, double* w);
+
+/* --- batched Hahn echo ---------------------------------------------------- */
+/* One cluster: opA generates the ket sweep exp(-i opA tau) u0, opB the bra
+ * sweep exp(-i opB tau) y0 (pass -lvB^dagger for lvB), and
+ *   L(t_k) = Re( scale * < bra(tau_k), ket(tau_k) > ). */
+typedef struct {
+    const oops_term_table* opA;
+    const oops_term_table* opB;
+    const double* u0;  /* n complex */
+    const double* y0;  /* n complex */
+    double scale;
+} oops_hahn_item;
+
+/* Compile every item's operators, bucket the items by state dimension and
+ * upload everything to device memory.  m: Krylov dimension (20 or 30);
+ * clusters with n <= m use the dense exponential.  klim: Lanczos steps of the
+ * norm estimate (the reference default is 10), tol: Expokit tolerance. */
+int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* items, int32_t m, int32_t klim,
+                           double tol, oops_hahn_batch** out);
+/* Run every bucket for the ascending sweep times tau[0..nt) (device-resident
+ * results).  Blocks until the device is done. */
+int oops_hahn_batch_run(oops_hahn_batch* b, int32_t nt, const double* tau);
+/* Copy results: L [nitems x nt] (row per item, item order of create), stats
+ * [nitems x 16] int32 per-cluster counters (steps, products, rejections,
+ * exponentials of both sweeps, failure flags, norm estimates).  Either may be NULL. */
+int oops_hahn_batch_results(oops_hahn_batch* b, double* L, int32_t* stats);
+/* Per-bucket timing of the last run: for bucket k, its state dimension,
+ * item count and kernel time in ms (HIP events on the launch stream). */
+int oops_hahn_batch_timing(const oops_hahn_batch* b, int32_t max_buckets, int32_t* n, int32_t* count, double* ms,
+                           int32_t* nbuckets);
+int oops_hahn_batch_destroy(oops_hahn_batch* b);
+
+/* DLARAN (LAPACK 48-bit multiplicative congruential generator, the Lanczos
+ * start vector source of expv/source/expokit/DLARAN.f): draws count values
+ * starting from iseed[4] (12-bit limbs, most significant first), updating it. */
+int oops_dlaran_fill(int32_t* iseed, int64_t count, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OOPS_EXPV_H */

@@ -1,0 +1,371 @@
+"""oops_amd — MI355X-native CCE Krylov-expv hot path (Python binding of the C ABI).
+
+The numeric work runs in ``liboops_amd.so`` (hand-written HIP kernels for
+gfx950 behind ``include/oops_expv.h`` / ``include/oops_cce.h``).  This module
+only marshals arguments through ctypes; there is no CPU fallback: if the
+library or a GPU is missing the calls raise.
+
+Names mirror the reference's host classes where the reference has them
+(``MatExpVector``, ``SumKronProd``, ``cDepthFirstPathTracing``, ``CCE``) so
+the tests read like the reference's own call sites.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "liboops_amd.so")
+
+OOPS_OK = 0
+
+
+class OopsError(RuntimeError):
+    pass
+
+
+class TermTableC(ctypes.Structure):
+    _fields_ = [("nspin", ctypes.c_int32), ("nterm", ctypes.c_int32),
+                ("coeff", ctypes.c_void_p), ("nbody", ctypes.c_void_p),
+                ("pos_offset", ctypes.c_void_p), ("pos", ctypes.c_void_p),
+                ("dims", ctypes.c_void_p), ("mat_offset", ctypes.c_void_p),
+                ("mats", ctypes.c_void_p), ("spin_dim", ctypes.c_void_p)]
+
+
+class HahnItemC(ctypes.Structure):
+    _fields_ = [("opA", ctypes.POINTER(TermTableC)), ("opB", ctypes.POINTER(TermTableC)),
+                ("u0", ctypes.c_void_p), ("y0", ctypes.c_void_p), ("scale", ctypes.c_double)]
+
+
+class CCEParamsC(ctypes.Structure):
+    _fields_ = [("B", ctypes.c_double * 3), ("nv_pos", ctypes.c_double * 3),
+                ("state0", ctypes.c_int32), ("state1", ctypes.c_int32),
+                ("dephasing_rate", ctypes.c_double), ("dephasing_axis", ctypes.c_double * 3),
+                ("pol", ctypes.c_double * 3)]
+
+
+_lib = None
+
+EXPORTS = [
+    "oops_ctx_create", "oops_ctx_destroy", "oops_last_error", "oops_ctx_info",
+    "oops_hahn_batch_create", "oops_hahn_batch_run", "oops_hahn_batch_results",
+    "oops_hahn_batch_timing", "oops_hahn_batch_destroy", "oops_dlaran_fill",
+    "oops_bath_diamond", "oops_nv_states", "oops_cce_cluster_terms", "oops_terms_view",
+    "oops_terms_sizes", "oops_terms_destroy", "oops_clusters_enumerate", "oops_clusters_count",
+    "oops_clusters_get", "oops_clusters_parents", "oops_clusters_destroy", "oops_cce_reduce",
+    "oops_cce_hahn_batch_create",
+]
+
+
+def lib():
+    """Load liboops_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise OopsError(f"{LIBPATH} missing: run oops_amd/build.py (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIBPATH)
+        for name in EXPORTS:
+            getattr(L, name).restype = ctypes.c_int
+        L.oops_last_error.restype = ctypes.c_char_p
+        L.oops_last_error.argtypes = [ctypes.c_void_p]
+        L.oops_ctx_create.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
+        L.oops_ctx_destroy.argtypes = [ctypes.c_void_p]
+        L.oops_hahn_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
+        L.oops_hahn_batch_results.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.oops_hahn_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.oops_bath_diamond.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                                        ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32,
+                                        ctypes.POINTER(ctypes.c_int32)]
+        L.oops_clusters_enumerate.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_double,
+                                              ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
+        L.oops_clusters_destroy.argtypes = [ctypes.c_void_p]
+        L.oops_terms_destroy.argtypes = [ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)
+
+
+def _check(rc, ctx=None, what=""):
+    if rc != OOPS_OK:
+        msg = ""
+        if ctx is not None:
+            m = lib().oops_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise OopsError(f"{what} failed with status {rc}: {msg}")
+
+
+# ---------------------------------------------------------------------------
+class TermTable:
+    """Owning host copy of an operator in the reference's flattened format
+    (MatExp.cpp:228-262) plus its C view."""
+
+    def __init__(self, coeff, nbody, pos_offset, pos, dims, mat_offset, mats, spin_dim):
+        self.coeff = np.ascontiguousarray(coeff, np.float64)
+        self.nbody = np.ascontiguousarray(nbody, np.int64)
+        self.pos_offset = np.ascontiguousarray(pos_offset, np.int64)
+        self.pos = np.ascontiguousarray(pos if len(pos) else [0], np.int64)
+        self.dims = np.ascontiguousarray(dims if len(dims) else [1], np.int64)
+        self.mat_offset = np.ascontiguousarray(mat_offset, np.int64)
+        m = np.ascontiguousarray(mats, np.complex128)
+        self.mats = m if m.size else np.zeros(1, np.complex128)
+        self.spin_dim = np.ascontiguousarray(spin_dim, np.int64)
+        self.n = int(np.prod(self.spin_dim))
+        self.c = TermTableC(len(self.spin_dim), len(self.coeff), self.coeff.ctypes.data,
+                            self.nbody.ctypes.data, self.pos_offset.ctypes.data, self.pos.ctypes.data,
+                            self.dims.ctypes.data, self.mat_offset.ctypes.data, self.mats.ctypes.data,
+                            self.spin_dim.ctypes.data)
+
+    @classmethod
+    def from_fields(cls, t):
+        """From any object with the reference's field names (e.g. an oracle table)."""
+        return cls(t.coeff, t.nbody, getattr(t, "pos_offset", getattr(t, "pos_off", None)), t.pos, t.dims,
+                   getattr(t, "mat_offset", getattr(t, "mat_off", None)), t.mats, t.spin_dim)
+
+
+class Context:
+    """A device context (one HIP stream on one GPU)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        rc = lib().oops_ctx_create(device, ctypes.byref(h))
+        if rc != OOPS_OK:
+            raise OopsError(f"oops_ctx_create({device}) failed with status {rc} (no HIP device?)")
+        self.h = h
+
+    def info(self):
+        cu = ctypes.c_int32()
+        lds = ctypes.c_int32()
+        name = ctypes.create_string_buffer(64)
+        _check(lib().oops_ctx_info(self.h, ctypes.byref(cu), ctypes.byref(lds), name, 64), self.h, "ctx_info")
+        return {"num_cu": cu.value, "lds_bytes": lds.value, "arch": name.value.decode()}
+
+    def close(self):
+        if self.h:
+            lib().oops_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HahnBatch:
+    """Device-resident batch of Hahn-echo clusters (oops_hahn_batch_*)."""
+
+    def __init__(self, ctx: Context, handle, nitems: int):
+        self.ctx = ctx
+        self.h = handle
+        self.nitems = nitems
+        self.nt = 0
+
+    @classmethod
+    def from_items(cls, ctx: Context, items, m=30, klim=10, tol=1e-12):
+        """items: list of (TermTable A, TermTable B', u0, y0, scale)."""
+        keep = []
+        arr = (HahnItemC * len(items))()
+        for i, (A, B, u0, y0, sc) in enumerate(items):
+            u0 = np.ascontiguousarray(u0, np.complex128)
+            y0 = np.ascontiguousarray(y0, np.complex128)
+            keep += [A, B, u0, y0]
+            arr[i].opA = ctypes.pointer(A.c)
+            arr[i].opB = ctypes.pointer(B.c)
+            arr[i].u0 = u0.ctypes.data
+            arr[i].y0 = y0.ctypes.data
+            arr[i].scale = float(sc)
+        h = ctypes.c_void_p()
+        rc = lib().oops_hahn_batch_create(ctx.h, ctypes.c_int32(len(items)), arr, ctypes.c_int32(m),
+                                          ctypes.c_int32(klim), ctypes.c_double(tol), ctypes.byref(h))
+        _check(rc, ctx.h, "oops_hahn_batch_create")
+        return cls(ctx, h, len(items))
+
+    def run(self, tau):
+        tau = np.ascontiguousarray(tau, np.float64)
+        _check(lib().oops_hahn_batch_run(self.h, ctypes.c_int32(len(tau)), _ptr(tau)), self.ctx.h,
+               "oops_hahn_batch_run")
+        self.nt = len(tau)
+
+    def results(self):
+        L = np.zeros((self.nitems, self.nt))
+        st = np.zeros((self.nitems, 16), np.int32)
+        _check(lib().oops_hahn_batch_results(self.h, _ptr(L), _ptr(st)), self.ctx.h, "oops_hahn_batch_results")
+        return L, st
+
+    def timing(self):
+        nb = ctypes.c_int32()
+        n = np.zeros(32, np.int32)
+        cnt = np.zeros(32, np.int32)
+        ms = np.zeros(32)
+        _check(lib().oops_hahn_batch_timing(self.h, ctypes.c_int32(32), _ptr(n), _ptr(cnt), _ptr(ms),
+                                            ctypes.byref(nb)), self.ctx.h, "oops_hahn_batch_timing")
+        k = nb.value
+        return [{"n": int(n[i]), "count": int(cnt[i]), "ms": float(ms[i])} for i in range(k)]
+
+    def close(self):
+        if self.h:
+            lib().oops_hahn_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# CCE host pipeline
+def cce_params(B=(0.0, 0.0, 0.0), nv_pos=(0.0, 0.0, 0.0), state0=0, state1=1, dephasing_rate=0.0,
+               dephasing_axis=(1.0, 1.0, 1.0), pol=(0.0, 0.0, 0.0)) -> CCEParamsC:
+    p = CCEParamsC()
+    p.B[:] = list(B)
+    p.nv_pos[:] = list(nv_pos)
+    p.state0 = state0
+    p.state1 = state1
+    p.dephasing_rate = dephasing_rate
+    p.dephasing_axis[:] = list(dephasing_axis)
+    p.pol[:] = list(pol)
+    return p
+
+
+def bath_diamond(a=3.567, radius=30.0, p=0.011, seed=0, shape="sphere"):
+    """Seeded diamond-lattice 13C bath (Angstrom), NV sites excluded."""
+    sh = 0 if shape == "sphere" else 1
+    n = ctypes.c_int32()
+    _check(lib().oops_bath_diamond(a, radius, p, seed, sh, ctypes.c_void_p(0), 0, ctypes.byref(n)), None,
+           "oops_bath_diamond")
+    out = np.zeros((max(n.value, 1), 3))
+    _check(lib().oops_bath_diamond(a, radius, p, seed, sh, _ptr(out), n.value, ctypes.byref(n)), None,
+           "oops_bath_diamond")
+    return out[:n.value].copy()
+
+
+def nv_states(pos, B):
+    pos = np.ascontiguousarray(pos, float)
+    B = np.ascontiguousarray(B, float)
+    ev = np.zeros(3)
+    vec = np.zeros(9, np.complex128)
+    _check(lib().oops_nv_states(_ptr(pos), _ptr(B), _ptr(ev), _ptr(vec)), None, "oops_nv_states")
+    return ev, vec.reshape(3, 3, order="F")
+
+
+def cluster_terms(coords, params: CCEParamsC, which=0):
+    """Term table of one cluster: which 0 lvA, 1 lvB, 2 -lvB^dagger, 3 H0, 4 H1.
+    Returns (TermTable copy, rho0, D)."""
+    X = np.ascontiguousarray(coords, float).reshape(-1, 3)
+    k = X.shape[0]
+    n = 4 ** k
+    rho = np.zeros(n, np.complex128)
+    D = ctypes.c_double()
+    h = ctypes.c_void_p()
+    _check(lib().oops_cce_cluster_terms(_ptr(X), ctypes.c_int32(k), ctypes.byref(params), ctypes.c_int32(which),
+                                        ctypes.byref(h), _ptr(rho), ctypes.byref(D)), None, "oops_cce_cluster_terms")
+    try:
+        v = TermTableC()
+        _check(lib().oops_terms_view(h, ctypes.byref(v)), None, "oops_terms_view")
+        nterm = v.nterm
+        npos = ctypes.c_int64()
+        nmat = ctypes.c_int64()
+        lib().oops_terms_sizes(h, None, ctypes.byref(npos), ctypes.byref(nmat))
+
+        def arr(ptr, count, dt):
+            if count == 0:
+                return np.zeros(0, dt)
+            return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(dt)), shape=(count,)).copy()
+
+        coeff = arr(v.coeff, nterm, ctypes.c_double)
+        nbody = arr(v.nbody, nterm, ctypes.c_int64)
+        pos_off = arr(v.pos_offset, nterm + 1, ctypes.c_int64)
+        pos = arr(v.pos, npos.value, ctypes.c_int64)
+        dims = arr(v.dims, npos.value, ctypes.c_int64)
+        mat_off = arr(v.mat_offset, npos.value + 1, ctypes.c_int64)
+        mats = arr(v.mats, 2 * nmat.value, ctypes.c_double).view(np.complex128)
+        spin_dim = arr(v.spin_dim, v.nspin, ctypes.c_int64)
+        tt = TermTable(coeff, nbody, pos_off, pos, dims, mat_off, mats, spin_dim)
+    finally:
+        lib().oops_terms_destroy(h)
+    if which >= 3:
+        rho = rho[: 2 ** k]
+    return tt, rho, D.value
+
+
+class Clusters:
+    """cDepthFirstPathTracing result: clusters per order in the reference's set order."""
+
+    def __init__(self, coords, cutoff, max_order):
+        X = np.ascontiguousarray(coords, float).reshape(-1, 3)
+        self.coords = X
+        self.max_order = max_order
+        h = ctypes.c_void_p()
+        _check(lib().oops_clusters_enumerate(_ptr(X), X.shape[0], float(cutoff), max_order, ctypes.byref(h)),
+               None, "oops_clusters_enumerate")
+        self.h = h
+
+    def count(self, order):
+        c = ctypes.c_int64()
+        _check(lib().oops_clusters_count(self.h, ctypes.c_int32(order), ctypes.byref(c)), None, "count")
+        return c.value
+
+    def counts(self):
+        return [self.count(o) for o in range(self.max_order)]
+
+    def get(self, order):
+        c = self.count(order)
+        out = np.zeros((c, order + 1), np.int32)
+        if c:
+            _check(lib().oops_clusters_get(self.h, ctypes.c_int32(order), _ptr(out)), None, "get")
+        return out
+
+    def parents(self, order):
+        c = self.count(order)
+        off = np.zeros(c + 1, np.int64)
+        lib().oops_clusters_parents(self.h, ctypes.c_int32(order), _ptr(off), ctypes.c_void_p(0))
+        lst = np.zeros(max(int(off[-1]), 1), np.int32)
+        lib().oops_clusters_parents(self.h, ctypes.c_int32(order), _ptr(off), _ptr(lst))
+        return [lst[off[i]:off[i + 1]].tolist() for i in range(c)]
+
+    def reduce(self, L, nt):
+        """CCE reduction: returns dict(tilde, final, each, log_abs, sign)."""
+        L = np.ascontiguousarray(L, float)
+        K = self.max_order
+        til = np.zeros_like(L)
+        fin = np.zeros((nt, K))
+        each = np.zeros((nt, K))
+        la = np.zeros((nt, K))
+        sg = np.zeros((nt, K), np.int32)
+        _check(lib().oops_cce_reduce(self.h, ctypes.c_int32(nt), _ptr(L), _ptr(til), _ptr(fin), _ptr(each),
+                                     _ptr(la), _ptr(sg)), None, "oops_cce_reduce")
+        return {"tilde": til, "final": fin, "each": each, "log_abs": la, "sign": sg}
+
+    def hahn_batch(self, ctx: Context, params: CCEParamsC, m=30, klim=10, tol=1e-12):
+        h = ctypes.c_void_p()
+        rc = lib().oops_cce_hahn_batch_create(ctx.h, _ptr(self.coords), self.h, ctypes.byref(params),
+                                              ctypes.c_int32(m), ctypes.c_int32(klim), ctypes.c_double(tol),
+                                              ctypes.byref(h))
+        _check(rc, ctx.h, "oops_cce_hahn_batch_create")
+        return HahnBatch(ctx, h, sum(self.counts()))
+
+    def close(self):
+        if self.h:
+            lib().oops_clusters_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dlaran_fill(iseed, count):
+    seed = (ctypes.c_int32 * 4)(*iseed)
+    out = np.zeros(count)
+    _check(lib().oops_dlaran_fill(seed, ctypes.c_int64(count), _ptr(out)), None, "oops_dlaran_fill")
+    iseed[:] = list(seed)
+    return out

@@ -1,0 +1,456 @@
+// Host runtime behind include/oops_expv.h: operator compilation, device
+// buffers, bucketed launches and timing.  The product path has no CPU
+// fallback: every numeric entry point runs on the GPU and returns an error
+// status when no HIP device is available.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/oops_expv.h"
+#include "oops_kernels.h"
+#include "host_common.h"
+
+using oops::DevOpHeader;
+using oops::DevOpLocal;
+
+struct oops_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+    int lds_bytes = 0;
+    std::string name;
+    std::string err;
+};
+
+namespace oops {
+
+int set_err(oops_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int hip_check(oops_ctx* ctx, hipError_t e, const char* what) {
+    if (e == hipSuccess) return OOPS_OK;
+    return set_err(ctx, OOPS_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------------------
+// Operator compiler: merge the reference's term list into one dense local
+// matrix per spin and per spin pair, then store each as an ELL row table.
+int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err) {
+    if (!T || T->nspin <= 0 || T->nterm < 0) { err = "empty term table"; return OOPS_ERR_ARG; }
+    const int ns = T->nspin;
+    const int64_t d = T->spin_dim[0];
+    for (int s = 0; s < ns; ++s)
+        if (T->spin_dim[s] != d) { err = "all spins of one operator must share the local dimension"; return OOPS_ERR_UNSUPPORTED; }
+    int ld = 0;
+    while ((1LL << ld) < d) ++ld;
+    if ((1LL << ld) != d || d < 2) { err = "local dimension must be a power of two"; return OOPS_ERR_UNSUPPORTED; }
+    int64_t n = 1;
+    for (int s = 0; s < ns; ++s) n *= d;
+    if (n > (1LL << 30)) { err = "state too large"; return OOPS_ERR_UNSUPPORTED; }
+    typedef std::complex<double> cplx;
+    const cplx* mats = reinterpret_cast<const cplx*>(T->mats);
+    cplx c0(0.0, 0.0);
+    std::vector<std::vector<cplx>> single(ns);
+    std::map<std::pair<int, int>, std::vector<cplx>> pair;
+    for (int t = 0; t < T->nterm; ++t) {
+        const double c = T->coeff[t];
+        const int64_t nb = T->nbody[t];
+        const int64_t p0 = T->pos_offset[t];
+        if (nb == 0) {
+            c0 += c;
+        } else if (nb == 1) {
+            const int s = (int)T->pos[p0];
+            if (s < 0 || s >= ns || T->dims[p0] != d) { err = "bad 1-body term"; return OOPS_ERR_ARG; }
+            auto& M = single[s];
+            if (M.empty()) M.assign(d * d, cplx(0.0));
+            const cplx* A = mats + T->mat_offset[p0];
+            for (int64_t k = 0; k < d * d; ++k) M[k] += c * A[k];
+        } else if (nb == 2) {
+            int s = (int)T->pos[p0], u = (int)T->pos[p0 + 1];
+            if (s < 0 || s >= ns || u < 0 || u >= ns || s == u || T->dims[p0] != d || T->dims[p0 + 1] != d) {
+                err = "bad 2-body term";
+                return OOPS_ERR_ARG;
+            }
+            const cplx* A = mats + T->mat_offset[p0];
+            const cplx* B = mats + T->mat_offset[p0 + 1];
+            if (s > u) { std::swap(s, u); std::swap(A, B); }
+            auto& M = pair[{s, u}];
+            const int64_t dd = d * d;
+            if (M.empty()) M.assign(dd * dd, cplx(0.0));
+            // kron(A, B) column-major: row (ra*d + rb), col (ca*d + cb)
+            for (int64_t ca = 0; ca < d; ++ca)
+                for (int64_t cb = 0; cb < d; ++cb)
+                    for (int64_t ra = 0; ra < d; ++ra)
+                        for (int64_t rb = 0; rb < d; ++rb)
+                            M[(ra * d + rb) + dd * (ca * d + cb)] += c * A[ra + d * ca] * B[rb + d * cb];
+        } else {
+            err = "terms with more than two bodies are not supported on the GPU path";
+            return OOPS_ERR_UNSUPPORTED;
+        }
+    }
+    out = CompiledOp();
+    out.n = n;
+    out.nspin = ns;
+    out.ld_bits = ld;
+    out.c0 = c0;
+    auto stride = [&](int s) { return (int64_t)1 << (ld * (ns - 1 - s)); };
+    auto add_local = [&](int nb, int s, int u, const std::vector<cplx>& M, int64_t rows) {
+        int R = 0;
+        for (int64_t r = 0; r < rows; ++r) {
+            int cnt = 0;
+            for (int64_t c = 0; c < rows; ++c) if (M[r + rows * c] != cplx(0.0)) ++cnt;
+            R = std::max(R, cnt);
+        }
+        if (R == 0) return;
+        DevOpLocal lo;
+        lo.nb = (int16_t)nb; lo.s = (int16_t)s; lo.t = (int16_t)u; lo.R = (int16_t)R;
+        lo.ent = (int32_t)out.val.size();
+        lo.rows = (int32_t)rows;
+        for (int64_t r = 0; r < rows; ++r) {
+            int cnt = 0;
+            for (int64_t c = 0; c < rows; ++c) {
+                const cplx v = M[r + rows * c];
+                if (v == cplx(0.0)) continue;
+                int64_t off;
+                if (nb == 1) off = c * stride(s);
+                else off = (c / d) * stride(s) + (c % d) * stride(u);
+                out.val.push_back(v);
+                out.off.push_back((int32_t)off);
+                ++cnt;
+            }
+            for (; cnt < R; ++cnt) { out.val.push_back(cplx(0.0)); out.off.push_back(0); }
+        }
+        out.locs.push_back(lo);
+        out.max_row = std::max(out.max_row, R);
+    };
+    for (int s = 0; s < ns; ++s)
+        if (!single[s].empty()) add_local(1, s, s, single[s], d);
+    for (auto& kv : pair) add_local(2, kv.first.first, kv.first.second, kv.second, d * d);
+    if ((int)out.locs.size() > kMaxLocals) { err = "too many local operators"; return OOPS_ERR_CAPACITY; }
+    return OOPS_OK;
+}
+
+// LAPACK DLARAN: x <- x * 33952834046453 mod 2^48, value x / 2^48.
+void dlaran_fill(int32_t* iseed, int64_t count, double* out) {
+    const uint64_t mult = 33952834046453ull;
+    const uint64_t mask48 = (1ull << 48) - 1;
+    uint64_t x = ((uint64_t)iseed[0] << 36) | ((uint64_t)iseed[1] << 24) | ((uint64_t)iseed[2] << 12) | (uint64_t)iseed[3];
+    for (int64_t i = 0; i < count; ++i) {
+        // (x * mult) mod 2^48 with 64-bit wrap-around arithmetic
+        x = (x * mult) & mask48;
+        out[i] = std::ldexp((double)x, -48);
+    }
+    iseed[0] = (int32_t)((x >> 36) & 4095);
+    iseed[1] = (int32_t)((x >> 24) & 4095);
+    iseed[2] = (int32_t)((x >> 12) & 4095);
+    iseed[3] = (int32_t)(x & 4095);
+}
+
+template <typename T>
+int dev_upload(oops_ctx* ctx, DevBuf& b, const std::vector<T>& v) {
+    b.release();
+    if (v.empty()) return OOPS_OK;
+    int rc = hip_check(ctx, hipMalloc(&b.p, v.size() * sizeof(T)), "hipMalloc");
+    if (rc) return rc;
+    b.bytes = v.size() * sizeof(T);
+    return hip_check(ctx, hipMemcpy(b.p, v.data(), b.bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+}  // namespace oops
+
+using namespace oops;
+
+struct oops_hahn_batch {
+    oops_ctx* ctx = nullptr;
+    int32_t nitems = 0, m = 30, klim = 10;
+    double tol = 1e-12;
+    struct Bucket {
+        int n = 0;
+        std::vector<int32_t> items;  // global item indices
+        DevBuf hdr, loc, val, off, opA, opB, voff, u0, y0, scale, rnd, L, stats, queue;
+        int grid = 0;
+        double ms = 0.0;
+    };
+    std::vector<Bucket> buckets;
+    DevBuf tau;
+    int32_t nt = 0;
+};
+
+extern "C" {
+
+int oops_ctx_create(int32_t device, oops_ctx** out) {
+    if (!out) return OOPS_ERR_ARG;
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) return OOPS_ERR_HIP;
+    if (device < 0 || device >= count) return OOPS_ERR_ARG;
+    std::unique_ptr<oops_ctx> c(new oops_ctx());
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) return OOPS_ERR_HIP;
+    if (hipStreamCreate(&c->stream) != hipSuccess) return OOPS_ERR_HIP;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return OOPS_ERR_HIP;
+    c->num_cu = prop.multiProcessorCount;
+    c->lds_bytes = (int)prop.sharedMemPerBlock;
+    c->name = prop.gcnArchName;
+    *out = c.release();
+    return OOPS_OK;
+}
+
+int oops_ctx_destroy(oops_ctx* ctx) {
+    if (!ctx) return OOPS_OK;
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return OOPS_OK;
+}
+
+const char* oops_last_error(const oops_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int oops_ctx_info(const oops_ctx* ctx, int32_t* num_cu, int32_t* lds_bytes, char* name, int32_t name_len) {
+    if (!ctx) return OOPS_ERR_ARG;
+    if (num_cu) *num_cu = ctx->num_cu;
+    if (lds_bytes) *lds_bytes = ctx->lds_bytes;
+    if (name && name_len > 0) {
+        std::snprintf(name, name_len, "%s", ctx->name.c_str());
+    }
+    return OOPS_OK;
+}
+
+int oops_dlaran_fill(int32_t* iseed, int64_t count, double* out) {
+    if (!iseed || count < 0 || (count > 0 && !out)) return OOPS_ERR_ARG;
+    dlaran_fill(iseed, count, out);
+    return OOPS_OK;
+}
+
+int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* items, int32_t m, int32_t klim,
+                           double tol, oops_hahn_batch** out) {
+    if (!ctx || !out || nitems < 0 || (nitems > 0 && !items)) return set_err(ctx, OOPS_ERR_ARG, "bad arguments");
+    *out = nullptr;
+    if (m != 20 && m != 30) return set_err(ctx, OOPS_ERR_UNSUPPORTED, "Krylov dimension must be 20 or 30");
+    if (klim < 1 || klim > 16) return set_err(ctx, OOPS_ERR_ARG, "klim must be in [1, 16]");
+    std::unique_ptr<oops_hahn_batch> b(new oops_hahn_batch());
+    b->ctx = ctx;
+    b->nitems = nitems;
+    b->m = m;
+    b->klim = klim;
+    b->tol = tol;
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
+    // compile all operators, bucket by n
+    std::map<int64_t, int> bucket_of;
+    struct Host {
+        std::vector<DevOpHeader> hdr;
+        std::vector<DevOpLocal> loc;
+        std::vector<double> val;
+        std::vector<int32_t> off;
+        std::vector<int32_t> opA, opB;
+        std::vector<int64_t> voff;
+        std::vector<double> u0, y0, scale;
+    };
+    std::vector<Host> hosts;
+    const int emax = krylov_entry_capacity(m);
+    for (int32_t it = 0; it < nitems; ++it) {
+        CompiledOp A, B;
+        std::string err;
+        int rc = compile_operator(items[it].opA, A, err);
+        if (!rc) rc = compile_operator(items[it].opB, B, err);
+        if (rc) return set_err(ctx, rc, "item " + std::to_string(it) + ": " + err);
+        if (A.n != B.n) return set_err(ctx, OOPS_ERR_ARG, "item " + std::to_string(it) + ": sweep dimensions differ");
+        const int64_t n = A.n;
+        const bool dense = n <= m;
+        if (dense && n > kDenseNMAX)
+            return set_err(ctx, OOPS_ERR_UNSUPPORTED, "item " + std::to_string(it) + ": n <= m needs n <= 16");
+        if (!dense && n > 256)
+            return set_err(ctx, OOPS_ERR_UNSUPPORTED, "item " + std::to_string(it) + ": batched Krylov path holds n <= 256");
+        const int cap = dense ? kDenseMaxEnt : emax;
+        if ((int)A.val.size() > cap || (int)B.val.size() > cap)
+            return set_err(ctx, OOPS_ERR_CAPACITY, "item " + std::to_string(it) + ": operator table exceeds LDS capacity");
+        auto f = bucket_of.find(n);
+        int bk;
+        if (f == bucket_of.end()) {
+            bk = (int)b->buckets.size();
+            bucket_of[n] = bk;
+            b->buckets.emplace_back();
+            b->buckets.back().n = (int)n;
+            hosts.emplace_back();
+        } else {
+            bk = f->second;
+        }
+        Host& H = hosts[bk];
+        b->buckets[bk].items.push_back(it);
+        for (int s = 0; s < 2; ++s) {
+            const CompiledOp& C = s == 0 ? A : B;
+            DevOpHeader h;
+            std::memset(&h, 0, sizeof(h));
+            h.n = (int32_t)C.n;
+            h.nspin = C.nspin;
+            h.ld_bits = C.ld_bits;
+            h.nloc = (int32_t)C.locs.size();
+            h.nent = (int32_t)C.val.size();
+            h.loc_off = (int32_t)H.loc.size();
+            h.ent_off = (int32_t)(H.val.size() / 2);
+            h.max_row = C.max_row;
+            h.c0re = C.c0.real();
+            h.c0im = C.c0.imag();
+            (s == 0 ? H.opA : H.opB).push_back((int32_t)H.hdr.size());
+            H.hdr.push_back(h);
+            H.loc.insert(H.loc.end(), C.locs.begin(), C.locs.end());
+            for (auto& v : C.val) { H.val.push_back(v.real()); H.val.push_back(v.imag()); }
+            H.off.insert(H.off.end(), C.off.begin(), C.off.end());
+        }
+        H.voff.push_back((int64_t)(H.u0.size() / 2));
+        for (int64_t i = 0; i < 2 * n; ++i) {
+            H.u0.push_back(items[it].u0[i]);
+            H.y0.push_back(items[it].y0[i]);
+        }
+        H.scale.push_back(items[it].scale);
+    }
+    for (size_t bk = 0; bk < b->buckets.size(); ++bk) {
+        auto& B = b->buckets[bk];
+        Host& H = hosts[bk];
+        std::vector<double> rnd(2 * B.n + 16);
+        int32_t seed[4] = {0, 0, 0, 5};
+        dlaran_fill(seed, (int64_t)rnd.size(), rnd.data());
+        int rc = 0;
+        rc = rc ? rc : dev_upload(ctx, B.hdr, H.hdr);
+        rc = rc ? rc : dev_upload(ctx, B.loc, H.loc);
+        rc = rc ? rc : dev_upload(ctx, B.val, H.val);
+        rc = rc ? rc : dev_upload(ctx, B.off, H.off);
+        rc = rc ? rc : dev_upload(ctx, B.opA, H.opA);
+        rc = rc ? rc : dev_upload(ctx, B.opB, H.opB);
+        rc = rc ? rc : dev_upload(ctx, B.voff, H.voff);
+        rc = rc ? rc : dev_upload(ctx, B.u0, H.u0);
+        rc = rc ? rc : dev_upload(ctx, B.y0, H.y0);
+        rc = rc ? rc : dev_upload(ctx, B.scale, H.scale);
+        rc = rc ? rc : dev_upload(ctx, B.rnd, rnd);
+        rc = rc ? rc : dev_upload(ctx, B.queue, std::vector<int32_t>(4, 0));
+        rc = rc ? rc : dev_upload(ctx, B.stats, std::vector<int32_t>(B.items.size() * kStatsPerItem, 0));
+        if (rc) return rc;
+        const bool dense = B.n <= m;
+        const int per_cu = dense ? 4 : 2;
+        B.grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu * per_cu));
+    }
+    *out = b.release();
+    return OOPS_OK;
+}
+
+int oops_hahn_batch_run(oops_hahn_batch* b, int32_t nt, const double* tau) {
+    if (!b || nt <= 0 || !tau) return OOPS_ERR_ARG;
+    oops_ctx* ctx = b->ctx;
+    for (int k = 1; k < nt; ++k)
+        if (!(tau[k] >= tau[k - 1])) return set_err(ctx, OOPS_ERR_ARG, "sweep times must be ascending");
+    if (tau[0] < 0.0) return set_err(ctx, OOPS_ERR_ARG, "sweep times must be >= 0");
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
+    int rc = dev_upload(ctx, b->tau, std::vector<double>(tau, tau + nt));
+    if (rc) return rc;
+    b->nt = nt;
+    std::vector<hipEvent_t> ev(2 * b->buckets.size());
+    for (auto& e : ev) hipEventCreate(&e);
+    // largest buckets first so the long Krylov work starts early
+    for (size_t bk = 0; bk < b->buckets.size(); ++bk) {
+        auto& B = b->buckets[bk];
+        B.L.release();
+        rc = hip_check(ctx, hipMalloc(&B.L.p, sizeof(double) * B.items.size() * nt), "hipMalloc L");
+        if (rc) break;
+        B.L.bytes = sizeof(double) * B.items.size() * nt;
+        KrylovArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.hdr = (const DevOpHeader*)B.hdr.p;
+        a.loc = (const DevOpLocal*)B.loc.p;
+        a.val = (const double2*)B.val.p;
+        a.off = (const int32_t*)B.off.p;
+        a.item_opA = (const int32_t*)B.opA.p;
+        a.item_opB = (const int32_t*)B.opB.p;
+        a.vec_off = (const int64_t*)B.voff.p;
+        a.u0 = (const double2*)B.u0.p;
+        a.y0 = (const double2*)B.y0.p;
+        a.scale = (const double*)B.scale.p;
+        a.tau = (const double*)b->tau.p;
+        a.rnd = (const double*)B.rnd.p;
+        a.L = (double*)B.L.p;
+        a.stats = (int32_t*)B.stats.p;
+        a.queue = (int32_t*)B.queue.p;
+        a.nitems = (int32_t)B.items.size();
+        a.n = B.n;
+        a.nt = nt;
+        a.klim = b->klim;
+        a.max_steps = 50000000;
+        a.tol = b->tol;
+        hipEventRecord(ev[2 * bk], ctx->stream);
+        const int lrc = launch_hahn_bucket(a, b->m, B.grid, ctx->stream);
+        hipEventRecord(ev[2 * bk + 1], ctx->stream);
+        if (lrc != 0) {
+            rc = set_err(ctx, lrc < 0 ? OOPS_ERR_UNSUPPORTED : OOPS_ERR_HIP,
+                         "launch failed for n=" + std::to_string(B.n) + " code " + std::to_string(lrc));
+            break;
+        }
+    }
+    hipError_t se = hipStreamSynchronize(ctx->stream);
+    if (!rc) rc = hip_check(ctx, se, "hipStreamSynchronize");
+    if (!rc) {
+        for (size_t bk = 0; bk < b->buckets.size(); ++bk) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, ev[2 * bk], ev[2 * bk + 1]);
+            b->buckets[bk].ms = ms;
+        }
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    return rc;
+}
+
+int oops_hahn_batch_results(oops_hahn_batch* b, double* L, int32_t* stats) {
+    if (!b) return OOPS_ERR_ARG;
+    oops_ctx* ctx = b->ctx;
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
+    for (auto& B : b->buckets) {
+        const size_t ni = B.items.size();
+        if (L) {
+            if (!B.L.p) return set_err(ctx, OOPS_ERR_ARG, "batch has not been run");
+            std::vector<double> h(ni * b->nt);
+            int rc = hip_check(ctx, hipMemcpy(h.data(), B.L.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost), "hipMemcpy L");
+            if (rc) return rc;
+            for (size_t i = 0; i < ni; ++i)
+                std::memcpy(L + (size_t)B.items[i] * b->nt, h.data() + i * b->nt, sizeof(double) * b->nt);
+        }
+        if (stats) {
+            std::vector<int32_t> h(ni * kStatsPerItem);
+            int rc = hip_check(ctx, hipMemcpy(h.data(), B.stats.p, sizeof(int32_t) * h.size(), hipMemcpyDeviceToHost), "hipMemcpy stats");
+            if (rc) return rc;
+            for (size_t i = 0; i < ni; ++i)
+                std::memcpy(stats + (size_t)B.items[i] * kStatsPerItem, h.data() + i * kStatsPerItem,
+                            sizeof(int32_t) * kStatsPerItem);
+        }
+    }
+    return OOPS_OK;
+}
+
+int oops_hahn_batch_timing(const oops_hahn_batch* b, int32_t max_buckets, int32_t* n, int32_t* count, double* ms,
+                           int32_t* nbuckets) {
+    if (!b) return OOPS_ERR_ARG;
+    if (nbuckets) *nbuckets = (int32_t)b->buckets.size();
+    for (int k = 0; k < (int)b->buckets.size() && k < max_buckets; ++k) {
+        if (n) n[k] = b->buckets[k].n;
+        if (count) count[k] = (int32_t)b->buckets[k].items.size();
+        if (ms) ms[k] = b->buckets[k].ms;
+    }
+    return OOPS_OK;
+}
+
+int oops_hahn_batch_destroy(oops_hahn_batch* b) {
+    if (!b) return OOPS_OK;
+    if (b->ctx) hipSetDevice(b->ctx->device);
+    delete b;
+    return OOPS_OK;
+}
+
+}  // extern "C"

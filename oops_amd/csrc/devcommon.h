@@ -1,0 +1,335 @@
+// Device-side building blocks shared by the CCE expv kernels (gfx950).
+//
+//  * complex FP64 arithmetic on a plain {re, im} pair;
+//  * block reductions built on wavefront butterflies (64-lane waves);
+//  * the compiled-operator matvec: every merged 1-body / 2-body local
+//    operator stored as a padded row table (ELL) whose entries carry the
+//    gather offset of the source element, so one thread computes one output
+//    element of  y = c0*x + sum_ops (I x A_op x I) x  from an LDS copy of x;
+//  * Pade(6) + scaling and squaring of a small dense matrix held in LDS.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "oops_device_types.h"
+
+namespace oops {
+
+struct cd {
+    double re, im;
+};
+
+__device__ __forceinline__ cd mkcd(double a, double b) { cd r; r.re = a; r.im = b; return r; }
+__device__ __forceinline__ cd cadd(cd a, cd b) { return mkcd(a.re + b.re, a.im + b.im); }
+__device__ __forceinline__ cd csub(cd a, cd b) { return mkcd(a.re - b.re, a.im - b.im); }
+__device__ __forceinline__ cd cscale(cd a, double s) { return mkcd(a.re * s, a.im * s); }
+__device__ __forceinline__ cd cmul(cd a, cd b) {
+    return mkcd(fma(a.re, b.re, -a.im * b.im), fma(a.re, b.im, a.im * b.re));
+}
+// a*b + c
+__device__ __forceinline__ cd cfma(cd a, cd b, cd c) {
+    return mkcd(fma(a.re, b.re, fma(-a.im, b.im, c.re)), fma(a.re, b.im, fma(a.im, b.re, c.im)));
+}
+// conj(a)*b + c
+__device__ __forceinline__ cd cjfma(cd a, cd b, cd c) {
+    return mkcd(fma(a.re, b.re, fma(a.im, b.im, c.re)), fma(a.re, b.im, fma(-a.im, b.re, c.im)));
+}
+// c - a*b
+__device__ __forceinline__ cd cfms(cd a, cd b, cd c) {
+    return mkcd(fma(-a.re, b.re, fma(a.im, b.im, c.re)), fma(-a.re, b.im, fma(-a.im, b.re, c.im)));
+}
+__device__ __forceinline__ double cabs2(cd a) { return fma(a.re, a.re, a.im * a.im); }
+// -i * a
+__device__ __forceinline__ cd cmulmi(cd a) { return mkcd(a.im, -a.re); }
+__device__ __forceinline__ cd cdiv(cd a, cd b) {
+    // Smith's algorithm
+    if (fabs(b.re) >= fabs(b.im)) {
+        const double r = b.im / b.re, d = b.re + b.im * r;
+        return mkcd((a.re + a.im * r) / d, (a.im - a.re * r) / d);
+    }
+    const double r = b.re / b.im, d = b.im + b.re * r;
+    return mkcd((a.re * r + a.im) / d, (a.im * r - a.re) / d);
+}
+__device__ __forceinline__ double cabs(cd a) { return hypot(a.re, a.im); }
+
+constexpr int kWave = 64;
+
+constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
+constexpr int pow2ceil(int v) { int p = 1; while (p < v) p *= 2; return p; }
+
+// ---------------------------------------------------------------------------
+// Block sum of P doubles per thread (P a power of two, P <= 64); value k of a
+// thread is produced on demand by gen(k), so at most P/2 partial sums are live
+// in registers at any time.
+// Stage s of a halving butterfly exchanges half of the remaining values with
+// the lane at distance 32>>s, so after log2(P) stages lane L holds the partial
+// sum of value (L >> (6-log2 P)); a plain butterfly over the remaining low
+// lane bits completes the wave sum.  Wave partials are combined through LDS.
+// On return out[0..P) holds the block totals (visible to every thread).
+template <int P, int NT, typename Gen>
+__device__ __forceinline__ void block_sum_gen(Gen gen, double* ws, double* out) {
+    static_assert((P & (P - 1)) == 0 && P <= 64, "P must be a power of two <= 64");
+    constexpr int LP = ilog2c(P);
+    constexpr int NW = NT / kWave;
+    constexpr int H0 = P > 1 ? P / 2 : 1;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    double v[H0];
+    if constexpr (P == 1) {
+        v[0] = gen(0);
+    } else {
+        const bool up = (lane & 32) != 0;
+#pragma unroll
+        for (int k = 0; k < H0; ++k) {
+            const double a = gen(k), b = gen(k + H0);
+            const double send = up ? a : b;
+            const double keep = up ? b : a;
+            v[k] = keep + __shfl_xor(send, 32, kWave);
+        }
+#pragma unroll
+        for (int st = 1; st < LP; ++st) {
+            const int mask = 32 >> st;
+            const int half = P >> (st + 1);
+            const bool upm = (lane & mask) != 0;
+#pragma unroll
+            for (int k = 0; k < half; ++k) {
+                const double a = v[k], b = v[k + half];
+                const double send = upm ? a : b;
+                const double keep = upm ? b : a;
+                v[k] = keep + __shfl_xor(send, mask, kWave);
+            }
+        }
+    }
+#pragma unroll
+    for (int mask = (32 >> LP); mask >= 1; mask >>= 1) v[0] += __shfl_xor(v[0], mask, kWave);
+    constexpr int GROUP = kWave / P;  // lanes sharing one value index
+    const int idx = lane >> (6 - LP);
+    if (NW == 1) {
+        if ((lane & (GROUP - 1)) == 0) out[idx] = v[0];
+        __syncthreads();
+    } else {
+        if ((lane & (GROUP - 1)) == 0) ws[wave * P + idx] = v[0];
+        __syncthreads();
+        if ((int)threadIdx.x < P) {
+            double s = 0.0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) s += ws[w * P + threadIdx.x];
+            out[threadIdx.x] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// Value k (compile-time after unrolling) of a wave-distributed vector whose
+// lane L holds entry L: a VGPR -> SGPR lane read, no memory traffic.
+__device__ __forceinline__ double lane_val(double x, int k) {
+    const unsigned long long bits = __double_as_longlong(x);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)(bits & 0xffffffffull), k);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(bits >> 32), k);
+    return __longlong_as_double(((unsigned long long)hi << 32) | lo);
+}
+
+template <int P, int NT>
+__device__ __forceinline__ void block_sum(double (&v)[P], double* ws, double* out) {
+    block_sum_gen<P, NT>([&](int k) { return v[k]; }, ws, out);
+}
+
+// Block max of one double (values >= 0).
+template <int NT>
+__device__ __forceinline__ double block_max1(double v, double* ws) {
+    constexpr int NW = NT / kWave;
+#pragma unroll
+    for (int mask = 32; mask >= 1; mask >>= 1) v = fmax(v, __shfl_xor(v, mask, kWave));
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    if (lane == 0) ws[wave] = v;
+    __syncthreads();
+    double r = ws[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) r = fmax(r, ws[w]);
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Compiled-operator matvec for output element i (i < n) from xs (LDS).
+// Returns  c0*x_i + sum_ops sum_e val[row(i), e] * x[base(i) + off[row(i), e]].
+__device__ __forceinline__ cd op_apply_elem(int i, const cd* __restrict__ xs, const DevOpHeader& h,
+                                            const DevOpLocal* __restrict__ locs,
+                                            const cd* __restrict__ vals, const int32_t* __restrict__ offs) {
+    cd acc = cmul(mkcd(h.c0re, h.c0im), xs[i]);
+    const int ld = h.ld_bits;
+    const int lmask = (1 << ld) - 1;
+    const int top = h.nspin - 1;
+    for (int L = 0; L < h.nloc; ++L) {
+        const DevOpLocal lo = locs[L];
+        const int shs = ld * (top - lo.s);
+        const int ls = (i >> shs) & lmask;
+        int base = i - (ls << shs);
+        int row = ls;
+        if (lo.nb == 2) {
+            const int sht = ld * (top - lo.t);
+            const int lt = (i >> sht) & lmask;
+            base -= lt << sht;
+            row = (ls << ld) | lt;
+        }
+        const int e0 = lo.ent + row * lo.R;
+        for (int e = 0; e < lo.R; ++e) acc = cfma(vals[e0 + e], xs[base + offs[e0 + e]], acc);
+    }
+    return acc;
+}
+
+// Cooperative copy of one compiled operator into LDS.
+template <int NT>
+__device__ __forceinline__ void op_load(const DevOpHeader* __restrict__ hdr_g, const DevOpLocal* __restrict__ loc_g,
+                                        const double2* __restrict__ val_g, const int32_t* __restrict__ off_g,
+                                        int op, DevOpHeader& h, DevOpLocal* locs, cd* vals, int32_t* offs) {
+    h = hdr_g[op];
+    const int tid = threadIdx.x;
+    for (int k = tid; k < h.nloc; k += NT) locs[k] = loc_g[h.loc_off + k];
+    for (int k = tid; k < h.nent; k += NT) {
+        const double2 v = val_g[h.ent_off + k];
+        vals[k] = mkcd(v.x, v.y);
+        offs[k] = off_g[h.ent_off + k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Small dense complex matrices in LDS, column-major with leading dim ld.
+template <int NT>
+__device__ __forceinline__ void lds_matmul(cd* C, int ldc, const cd* A, int lda, const cd* B, int ldb, int m,
+                                           cd alpha) {
+    for (int idx = threadIdx.x; idx < m * m; idx += NT) {
+        const int i = idx % m, j = idx / m;
+        cd s = mkcd(0.0, 0.0);
+        for (int l = 0; l < m; ++l) s = cfma(A[i + lda * l], B[l + ldb * j], s);
+        C[i + ldc * j] = cmul(alpha, s);
+    }
+    __syncthreads();
+}
+
+// exp(t*H) for the leading m x m block of H (leading dim ldh) by the
+// irreducible diagonal Pade(6) approximant with scaling and squaring.
+// Workspace: four m x m matrices w0..w3 (leading dim m).  Returns the
+// buffer index (0..3) holding the result, or -1 for a null H / -2 for a
+// singular denominator.  ns receives the number of squarings.
+template <int NT>
+__device__ int lds_pade6(const cd* H, int ldh, int m, double t, cd* w0, cd* w1, cd* w2, cd* w3, double* ws,
+                         int* sh_int, int& ns) {
+    const int tid = threadIdx.x;
+    // infinity norm (max row sum of |H|)
+    double rs = 0.0;
+    for (int i = tid; i < m; i += NT) {
+        double s = 0.0;
+        for (int j = 0; j < m; ++j) s += cabs(H[i + ldh * j]);
+        rs = fmax(rs, s);
+    }
+    const double hnorm = fabs(t * block_max1<NT>(rs, ws));
+    if (hnorm == 0.0) return -1;
+    if (!(hnorm < 1e300)) return -3;  // non-finite input
+    ns = (int)(log(hnorm) / log(2.0)) + 2;
+    if (ns < 0) ns = 0;
+    const double scale = t / ldexp(1.0, ns);
+    // Pade(6) coefficients by the recurrence c_k = c_{k-1} (7-k) / (k (13-k))
+    double c[7];
+    c[0] = 1.0;
+#pragma unroll
+    for (int k = 1; k <= 6; ++k) c[k] = c[k - 1] * (double)(7 - k) / (double)(k * (13 - k));
+    cd* buf[4] = {w0, w1, w2, w3};
+    // X2 = scale^2 H H  -> buf 0
+    lds_matmul<NT>(w0, m, H, ldh, H, ldh, m, mkcd(scale * scale, 0.0));
+    // even part starts at c6 I (buf 1), odd part at c5 I (buf 2)
+    for (int idx = tid; idx < m * m; idx += NT) {
+        const bool dg = (idx % m) == (idx / m);
+        w1[idx] = mkcd(dg ? c[6] : 0.0, 0.0);
+        w2[idx] = mkcd(dg ? c[5] : 0.0, 0.0);
+    }
+    __syncthreads();
+    int ie = 1, io = 2, it = 3;
+    bool even_turn = true;
+    for (int k = 5; k > 0; --k) {
+        const int src = even_turn ? ie : io;
+        lds_matmul<NT>(buf[it], m, buf[src], m, w0, m, m, mkcd(1.0, 0.0));
+        for (int i = tid; i < m; i += NT) buf[it][i + m * i].re += c[k - 1];
+        __syncthreads();
+        if (even_turn) { const int tmp = ie; ie = it; it = tmp; }
+        else { const int tmp = io; io = it; it = tmp; }
+        even_turn = !even_turn;
+    }
+    // odd <- scale * odd * H  (into the free buffer, then X2 slot is free)
+    lds_matmul<NT>(buf[it], m, buf[io], m, H, ldh, m, mkcd(scale, 0.0));
+    {
+        const int tmp = io; io = it; it = tmp;
+    }
+    // D = even - odd  (in the even buffer), N = odd  (RHS)
+    cd* D = buf[ie];
+    cd* N = buf[io];
+    for (int idx = tid; idx < m * m; idx += NT) D[idx] = csub(D[idx], N[idx]);
+    __syncthreads();
+    // LU with partial pivoting, applied to N
+    for (int k = 0; k < m; ++k) {
+        if (tid == 0) {
+            int p = k;
+            double best = cabs(D[k + m * k]);
+            for (int i = k + 1; i < m; ++i) {
+                const double a = cabs(D[i + m * k]);
+                if (a > best) { best = a; p = i; }
+            }
+            sh_int[0] = best == 0.0 ? -1 : p;
+        }
+        __syncthreads();
+        const int p = sh_int[0];
+        if (p < 0) return -2;
+        if (p != k) {
+            for (int j = tid; j < 2 * m; j += NT) {
+                cd* M_ = j < m ? D : N;
+                const int jj = j < m ? j : j - m;
+                const cd a = M_[k + m * jj];
+                M_[k + m * jj] = M_[p + m * jj];
+                M_[p + m * jj] = a;
+            }
+            __syncthreads();
+        }
+        const cd piv = D[k + m * k];
+        for (int i = k + 1 + tid; i < m; i += NT) D[i + m * k] = cdiv(D[i + m * k], piv);
+        __syncthreads();
+        const int rows = m - k - 1;
+        const int cols = (m - k - 1) + m;
+        for (int idx = tid; idx < rows * cols; idx += NT) {
+            const int i = k + 1 + idx % rows;
+            const int jj = idx / rows;
+            const cd f = D[i + m * k];
+            if (jj < m - k - 1) {
+                const int j = k + 1 + jj;
+                D[i + m * j] = cfms(f, D[k + m * j], D[i + m * j]);
+            } else {
+                const int j = jj - (m - k - 1);
+                N[i + m * j] = cfms(f, N[k + m * j], N[i + m * j]);
+            }
+        }
+        __syncthreads();
+    }
+    for (int j = tid; j < m; j += NT) {
+        for (int i = m - 1; i >= 0; --i) {
+            cd s = N[i + m * j];
+            for (int l = i + 1; l < m; ++l) s = cfms(D[i + m * l], N[l + m * j], s);
+            N[i + m * j] = cdiv(s, D[i + m * i]);
+        }
+    }
+    __syncthreads();
+    // R = I + 2 N
+    for (int idx = tid; idx < m * m; idx += NT) {
+        const bool dg = (idx % m) == (idx / m);
+        N[idx] = mkcd(2.0 * N[idx].re + (dg ? 1.0 : 0.0), 2.0 * N[idx].im);
+    }
+    __syncthreads();
+    int ir = io;
+    int free_a = ie;  // D is no longer needed
+    for (int k = 0; k < ns; ++k) {
+        lds_matmul<NT>(buf[free_a], m, buf[ir], m, buf[ir], m, m, mkcd(1.0, 0.0));
+        const int tmp = ir; ir = free_a; free_a = tmp;
+    }
+    return ir;
+}
+
+}  // namespace oops

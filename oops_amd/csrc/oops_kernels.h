@@ -1,0 +1,44 @@
+// Launch interface between the host runtime (capi.cpp) and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "oops_device_types.h"
+
+namespace oops {
+
+constexpr int kMaxLocals = 64;      // merged local operators per compiled operator
+constexpr int kDenseMaxEnt = 1024;  // ELL entries per operator on the dense path
+constexpr int kDenseNMAX = 16;      // largest state handled by the dense path
+constexpr int kStatsPerItem = 16;   // int32 slots of per-cluster counters
+
+// One bucket of Hahn-echo clusters sharing the state dimension n.
+struct KrylovArgs {
+    const DevOpHeader* hdr;
+    const DevOpLocal* loc;
+    const double2* val;
+    const int32_t* off;
+    const int32_t* item_opA;   // operator index of sweep A (ket side)
+    const int32_t* item_opB;   // operator index of sweep B (bra side, already -lvB^dagger)
+    const int64_t* vec_off;    // offset of the item's u0 / y0 in the vector pool
+    const double2* u0;
+    const double2* y0;
+    const double* scale;       // observable prefactor (the Hilbert dimension D)
+    const double* tau;         // ascending sweep times
+    const double* rnd;         // DLARAN draws: 2n start-vector values + klim power-iteration values
+    double* L;                 // nitems x nt
+    int32_t* stats;            // nitems x kStatsPerItem
+    int32_t* queue;            // work counter (zeroed by the launcher)
+    int32_t nitems;
+    int32_t n;
+    int32_t nt;
+    int32_t klim;
+    int32_t max_steps;
+    int32_t pad;
+    double tol;
+};
+
+int launch_hahn_bucket(const KrylovArgs& a, int m, int grid, hipStream_t stream);
+int krylov_entry_capacity(int m);
+
+}  // namespace oops
