@@ -88,7 +88,7 @@ int oops_cce_reduce(const oops_clusters* C, int32_t nt, const double* L, double*
 /* Build every cluster's Hahn-echo pair of sweeps and create the device batch
  * (items ordered as the clusters: order 0 first, set order within an order). */
 int oops_cce_hahn_batch_create(oops_ctx* ctx, const double* coords, const oops_clusters* C, const oops_cce_params* P,
-                               int32_t m, int32_t klim, double tol, oops_hahn_batch** out);
+                               int32_t m, int32_t klim, double tol, int32_t precision, oops_hahn_batch** out);
 
 #ifdef __cplusplus
 }

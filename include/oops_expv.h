@@ -96,15 +96,19 @@ typedef struct {
 /* Compile every item's operators, bucket the items by state dimension and
  * upload everything to device memory.  m: Krylov dimension (20 or 30);
  * clusters with n <= m use the dense exponential.  klim: Lanczos steps of the
- * norm estimate (the reference default is 10), tol: Expokit tolerance. */
+ * norm estimate (the reference default is 10), tol: Expokit tolerance.
+ * precision 0: complex FP64 throughout; 1: mixed (2-body couplings applied in
+ * complex FP32, 1-body terms, basis, orthogonalisation and the small
+ * exponential in FP64). */
 int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* items, int32_t m, int32_t klim,
-                           double tol, oops_hahn_batch** out);
+                           double tol, int32_t precision, oops_hahn_batch** out);
 /* Run every bucket for the ascending sweep times tau[0..nt) (device-resident
  * results).  Blocks until the device is done. */
 int oops_hahn_batch_run(oops_hahn_batch* b, int32_t nt, const double* tau);
 /* Copy results: L [nitems x nt] (row per item, item order of create), stats
- * [nitems x 16] int32 per-cluster counters (steps, products, rejections,
- * exponentials of both sweeps, failure flags, norm estimates).  Either may be NULL. */
+ * [nitems x 24] int32 per-cluster counters: [0..3] steps, products, rejections,
+ * small exponentials of sweep A, [4..7] the same for sweep B, [8] failure flags,
+ * [9] exponential substeps, [10..13] the two norm estimates (doubles).  Either may be NULL. */
 int oops_hahn_batch_results(oops_hahn_batch* b, double* L, int32_t* stats);
 /* Per-bucket timing of the last run: for bucket k, its state dimension,
  * item count and kernel time in ms (HIP events on the launch stream). */

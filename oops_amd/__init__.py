@@ -20,6 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "liboops_amd.so")
 
 OOPS_OK = 0
+STATS_PER_ITEM = 24  # int32 counters per cluster (oops_hahn_batch_results)
 
 
 class OopsError(RuntimeError):
@@ -88,6 +89,13 @@ def lib():
 
 def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)
+
+
+def _precision_code(p):
+    codes = {"fp64": 0, "mixed": 1}
+    if p not in codes:
+        raise ValueError(f"precision must be one of {sorted(codes)}")
+    return codes[p]
 
 
 def _check(rc, ctx=None, what=""):
@@ -166,7 +174,7 @@ class HahnBatch:
         self.nt = 0
 
     @classmethod
-    def from_items(cls, ctx: Context, items, m=30, klim=10, tol=1e-12):
+    def from_items(cls, ctx: Context, items, m=30, klim=10, tol=1e-12, precision="fp64"):
         """items: list of (TermTable A, TermTable B', u0, y0, scale)."""
         keep = []
         arr = (HahnItemC * len(items))()
@@ -181,7 +189,8 @@ class HahnBatch:
             arr[i].scale = float(sc)
         h = ctypes.c_void_p()
         rc = lib().oops_hahn_batch_create(ctx.h, ctypes.c_int32(len(items)), arr, ctypes.c_int32(m),
-                                          ctypes.c_int32(klim), ctypes.c_double(tol), ctypes.byref(h))
+                                          ctypes.c_int32(klim), ctypes.c_double(tol),
+                                          ctypes.c_int32(_precision_code(precision)), ctypes.byref(h))
         _check(rc, ctx.h, "oops_hahn_batch_create")
         return cls(ctx, h, len(items))
 
@@ -193,7 +202,7 @@ class HahnBatch:
 
     def results(self):
         L = np.zeros((self.nitems, self.nt))
-        st = np.zeros((self.nitems, 16), np.int32)
+        st = np.zeros((self.nitems, STATS_PER_ITEM), np.int32)
         _check(lib().oops_hahn_batch_results(self.h, _ptr(L), _ptr(st)), self.ctx.h, "oops_hahn_batch_results")
         return L, st
 
@@ -343,11 +352,11 @@ class Clusters:
                                      _ptr(la), _ptr(sg)), None, "oops_cce_reduce")
         return {"tilde": til, "final": fin, "each": each, "log_abs": la, "sign": sg}
 
-    def hahn_batch(self, ctx: Context, params: CCEParamsC, m=30, klim=10, tol=1e-12):
+    def hahn_batch(self, ctx: Context, params: CCEParamsC, m=30, klim=10, tol=1e-12, precision="fp64"):
         h = ctypes.c_void_p()
         rc = lib().oops_cce_hahn_batch_create(ctx.h, _ptr(self.coords), self.h, ctypes.byref(params),
                                               ctypes.c_int32(m), ctypes.c_int32(klim), ctypes.c_double(tol),
-                                              ctypes.byref(h))
+                                              ctypes.c_int32(_precision_code(precision)), ctypes.byref(h))
         _check(rc, ctx.h, "oops_cce_hahn_batch_create")
         return HahnBatch(ctx, h, sum(self.counts()))
 

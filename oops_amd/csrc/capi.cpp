@@ -112,6 +112,7 @@ int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err
             R = std::max(R, cnt);
         }
         if (R == 0) return;
+        R = R <= 4 ? 4 : (R <= 8 ? 8 : 16);  // kernel row widths
         DevOpLocal lo;
         lo.nb = (int16_t)nb; lo.s = (int16_t)s; lo.t = (int16_t)u; lo.R = (int16_t)R;
         lo.ent = (int32_t)out.val.size();
@@ -135,8 +136,10 @@ int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err
     };
     for (int s = 0; s < ns; ++s)
         if (!single[s].empty()) add_local(1, s, s, single[s], d);
+    out.ent2_first = (int32_t)out.val.size();
     for (auto& kv : pair) add_local(2, kv.first.first, kv.first.second, kv.second, d * d);
     if ((int)out.locs.size() > kMaxLocals) { err = "too many local operators"; return OOPS_ERR_CAPACITY; }
+    if (out.max_row > 16) { err = "local operator rows wider than 16 entries"; return OOPS_ERR_UNSUPPORTED; }
     return OOPS_OK;
 }
 
@@ -172,7 +175,7 @@ using namespace oops;
 
 struct oops_hahn_batch {
     oops_ctx* ctx = nullptr;
-    int32_t nitems = 0, m = 30, klim = 10;
+    int32_t nitems = 0, m = 30, klim = 10, precision = 0;
     double tol = 1e-12;
     struct Bucket {
         int n = 0;
@@ -234,9 +237,10 @@ int oops_dlaran_fill(int32_t* iseed, int64_t count, double* out) {
 }
 
 int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* items, int32_t m, int32_t klim,
-                           double tol, oops_hahn_batch** out) {
+                           double tol, int32_t precision, oops_hahn_batch** out) {
     if (!ctx || !out || nitems < 0 || (nitems > 0 && !items)) return set_err(ctx, OOPS_ERR_ARG, "bad arguments");
     *out = nullptr;
+    if (precision != 0 && precision != 1) return set_err(ctx, OOPS_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed)");
     if (m != 20 && m != 30) return set_err(ctx, OOPS_ERR_UNSUPPORTED, "Krylov dimension must be 20 or 30");
     if (klim < 1 || klim > 16) return set_err(ctx, OOPS_ERR_ARG, "klim must be in [1, 16]");
     std::unique_ptr<oops_hahn_batch> b(new oops_hahn_batch());
@@ -245,6 +249,7 @@ int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* 
     b->m = m;
     b->klim = klim;
     b->tol = tol;
+    b->precision = precision;
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
     // compile all operators, bucket by n
     std::map<int64_t, int> bucket_of;
@@ -302,6 +307,7 @@ int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* 
             h.max_row = C.max_row;
             h.c0re = C.c0.real();
             h.c0im = C.c0.imag();
+            h.ent2_first = C.ent2_first;
             (s == 0 ? H.opA : H.opB).push_back((int32_t)H.hdr.size());
             H.hdr.push_back(h);
             H.loc.insert(H.loc.end(), C.locs.begin(), C.locs.end());
@@ -387,7 +393,7 @@ int oops_hahn_batch_run(oops_hahn_batch* b, int32_t nt, const double* tau) {
         a.max_steps = 50000000;
         a.tol = b->tol;
         hipEventRecord(ev[2 * bk], ctx->stream);
-        const int lrc = launch_hahn_bucket(a, b->m, B.grid, ctx->stream);
+        const int lrc = launch_hahn_bucket(a, b->m, b->precision, B.grid, ctx->stream);
         hipEventRecord(ev[2 * bk + 1], ctx->stream);
         if (lrc != 0) {
             rc = set_err(ctx, lrc < 0 ? OOPS_ERR_UNSUPPORTED : OOPS_ERR_HIP,

@@ -698,7 +698,7 @@ int oops_cce_reduce(const oops_clusters* C, int32_t nt, const double* L, double*
 // Build every cluster's Hahn-echo problem (orders in sequence, clusters in
 // set order) and hand the batch to the device runtime.
 int oops_cce_hahn_batch_create(oops_ctx* ctx, const double* coords, const oops_clusters* C, const oops_cce_params* P,
-                               int32_t m, int32_t klim, double tol, oops_hahn_batch** out) {
+                               int32_t m, int32_t klim, double tol, int32_t precision, oops_hahn_batch** out) {
     if (!ctx || !coords || !C || !P || !out) return OOPS_ERR_ARG;
     NV nv = make_nv(P->nv_pos, P->B);
     std::vector<ClusterProblem> probs;
@@ -722,7 +722,7 @@ int oops_cce_hahn_batch_create(oops_ctx* ctx, const double* coords, const oops_c
         items[i].y0 = reinterpret_cast<const double*>(probs[i].rho.data());
         items[i].scale = probs[i].D;
     }
-    return oops_hahn_batch_create(ctx, (int32_t)items.size(), items.data(), m, klim, tol, out);
+    return oops_hahn_batch_create(ctx, (int32_t)items.size(), items.data(), m, klim, tol, precision, out);
 }
 
 }  // extern "C"

@@ -129,6 +129,85 @@ __device__ __forceinline__ double lane_val(double x, int k) {
     return __longlong_as_double(((unsigned long long)hi << 32) | lo);
 }
 
+// Same reduction with a single barrier: every wave leaves its P partial sums
+// in one of two alternating LDS slots, each lane adds the NW partials of the
+// value it owns, and BlockTotals::get broadcasts a total from its owner lane
+// with a lane read.  A slot is rewritten two reductions later, after every
+// wave has passed the barrier of the reduction in between.
+template <int NT>
+struct BlockRed {
+    double* ws;  // 3 * (NT/64) * 64 doubles: two partial slots + per-wave totals
+    int slot;
+    __device__ explicit BlockRed(double* w) : ws(w), slot(0) {}
+};
+
+template <int NT, int P>
+struct BlockTotals {
+    const double* vals;  // this wave's copy of the P block totals (LDS)
+    __device__ __forceinline__ double get(int k) const { return vals[k]; }
+};
+
+template <int P, int NT, typename Gen>
+__device__ __forceinline__ BlockTotals<NT, P> block_sum1(BlockRed<NT>& br, Gen gen) {
+    static_assert((P & (P - 1)) == 0 && P <= 64, "P must be a power of two <= 64");
+    constexpr int LP = ilog2c(P);
+    constexpr int H0 = P > 1 ? P / 2 : 1;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    double v[H0];
+    if constexpr (P == 1) {
+        v[0] = gen(0);
+    } else {
+        const bool up = (lane & 32) != 0;
+#pragma unroll
+        for (int k = 0; k < H0; ++k) {
+            const double a = gen(k), b = gen(k + H0);
+            const double send = up ? a : b;
+            const double keep = up ? b : a;
+            v[k] = keep + __shfl_xor(send, 32, kWave);
+            // generate operands pair by pair instead of all P at once
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int st = 1; st < LP; ++st) {
+            const int mask = 32 >> st;
+            const int half = P >> (st + 1);
+            const bool upm = (lane & mask) != 0;
+#pragma unroll
+            for (int k = 0; k < half; ++k) {
+                const double a = v[k], b = v[k + half];
+                const double send = upm ? a : b;
+                const double keep = upm ? b : a;
+                v[k] = keep + __shfl_xor(send, mask, kWave);
+            }
+        }
+    }
+#pragma unroll
+    for (int mask = (32 >> LP); mask >= 1; mask >>= 1) v[0] += __shfl_xor(v[0], mask, kWave);
+    constexpr int GROUP = kWave / P;
+    const int idx = lane >> (6 - LP);
+    constexpr int NW = NT / kWave;
+    BlockTotals<NT, P> t;
+    double* mine = br.ws + 2 * NW * 64 + wave * 64;  // per-wave copy of the totals
+    double tot = v[0];
+    if constexpr (NW > 1) {
+        double* part = br.ws + br.slot * NW * 64;
+        if ((lane & (GROUP - 1)) == 0) part[wave * P + idx] = v[0];
+        br.slot ^= 1;
+        __syncthreads();
+        tot = part[idx];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) tot += part[w * P + idx];
+    }
+    // lanes of one wave exchange through their own LDS copy: no block barrier
+    if ((lane & (GROUP - 1)) == 0) mine[idx] = tot;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    t.vals = mine;
+    return t;
+}
+
 template <int P, int NT>
 __device__ __forceinline__ void block_sum(double (&v)[P], double* ws, double* out) {
     block_sum_gen<P, NT>([&](int k) { return v[k]; }, ws, out);
@@ -154,6 +233,24 @@ __device__ __forceinline__ double block_max1(double v, double* ws) {
 // ---------------------------------------------------------------------------
 // Compiled-operator matvec for output element i (i < n) from xs (LDS).
 // Returns  c0*x_i + sum_ops sum_e val[row(i), e] * x[base(i) + off[row(i), e]].
+// Row widths are padded on the host to multiples of 4 (4, 8 or 16 entries).
+// One ELL row of R entries (R a multiple of 4), processed four entries at a
+// time: the offsets arrive as one 16-byte load and the four gathers are
+// independent, while the live set stays small.
+__device__ __forceinline__ cd ell_row(const cd* __restrict__ xs, int base, const cd* __restrict__ vals,
+                                      const int32_t* __restrict__ offs, int e0, int R, cd acc) {
+#pragma unroll 1
+    for (int e = 0; e < R; e += 4) {
+        const int4 q = *reinterpret_cast<const int4*>(offs + e0 + e);
+        const cd x0 = xs[base + q.x], x1 = xs[base + q.y], x2 = xs[base + q.z], x3 = xs[base + q.w];
+        acc = cfma(vals[e0 + e], x0, acc);
+        acc = cfma(vals[e0 + e + 1], x1, acc);
+        acc = cfma(vals[e0 + e + 2], x2, acc);
+        acc = cfma(vals[e0 + e + 3], x3, acc);
+    }
+    return acc;
+}
+
 __device__ __forceinline__ cd op_apply_elem(int i, const cd* __restrict__ xs, const DevOpHeader& h,
                                             const DevOpLocal* __restrict__ locs,
                                             const cd* __restrict__ vals, const int32_t* __restrict__ offs) {
@@ -173,10 +270,59 @@ __device__ __forceinline__ cd op_apply_elem(int i, const cd* __restrict__ xs, co
             base -= lt << sht;
             row = (ls << ld) | lt;
         }
-        const int e0 = lo.ent + row * lo.R;
-        for (int e = 0; e < lo.R; ++e) acc = cfma(vals[e0 + e], xs[base + offs[e0 + e]], acc);
+        acc = ell_row(xs, base, vals, offs, lo.ent + row * lo.R, lo.R, acc);
     }
     return acc;
+}
+
+// Mixed-precision variant: 1-body operators (Zeeman, hyperfine field,
+// dephasing; the large part of the operator) in FP64 from xs, 2-body
+// couplings in complex FP32 from the FP32 copy xs32 with FP32 values vals32
+// (entry e of a 2-body operator at vals32[e - ent2_first]).
+__device__ __forceinline__ float2 ell_row32(const float2* __restrict__ xs32, int base,
+                                            const float2* __restrict__ v32, const int32_t* __restrict__ offs,
+                                            int e0, int e0v, int R, float2 acc) {
+#pragma unroll 1
+    for (int e = 0; e < R; e += 4) {
+        const int4 q = *reinterpret_cast<const int4*>(offs + e0 + e);
+        const float2 xv[4] = {xs32[base + q.x], xs32[base + q.y], xs32[base + q.z], xs32[base + q.w]};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const float2 vv = v32[e0v + e + u];
+            acc.x = fmaf(vv.x, xv[u].x, fmaf(-vv.y, xv[u].y, acc.x));
+            acc.y = fmaf(vv.x, xv[u].y, fmaf(vv.y, xv[u].x, acc.y));
+        }
+    }
+    return acc;
+}
+
+__device__ __forceinline__ cd op_apply_elem_mixed(int i, const cd* __restrict__ xs, const float2* __restrict__ xs32,
+                                                  const DevOpHeader& h, const DevOpLocal* __restrict__ locs,
+                                                  const cd* __restrict__ vals, const int32_t* __restrict__ offs) {
+    cd acc = cmul(mkcd(h.c0re, h.c0im), xs[i]);
+    float2 acc32 = make_float2(0.f, 0.f);
+    const float2* v32 = reinterpret_cast<const float2*>(vals + h.ent2_first);
+    const int ld = h.ld_bits;
+    const int lmask = (1 << ld) - 1;
+    const int top = h.nspin - 1;
+    for (int L = 0; L < h.nloc; ++L) {
+        const DevOpLocal lo = locs[L];
+        const int shs = ld * (top - lo.s);
+        const int ls = (i >> shs) & lmask;
+        int base = i - (ls << shs);
+        int row = ls;
+        if (lo.nb == 2) {
+            const int sht = ld * (top - lo.t);
+            const int lt = (i >> sht) & lmask;
+            base -= lt << sht;
+            row = (ls << ld) | lt;
+            const int e0 = lo.ent + row * lo.R;
+            acc32 = ell_row32(xs32, base, v32, offs, e0, e0 - h.ent2_first, lo.R, acc32);
+        } else {
+            acc = ell_row(xs, base, vals, offs, lo.ent + row * lo.R, lo.R, acc);
+        }
+    }
+    return mkcd(acc.re + (double)acc32.x, acc.im + (double)acc32.y);
 }
 
 // Cooperative copy of one compiled operator into LDS.
@@ -330,6 +476,291 @@ __device__ int lds_pade6(const cd* H, int ldh, int m, double t, cd* w0, cd* w1, 
         const int tmp = ir; ir = free_a; free_a = tmp;
     }
     return ir;
+}
+
+// ---------------------------------------------------------------------------
+// Faster Pade(6) + scaling and squaring for the Krylov kernel (same algorithm
+// as lds_pade6, ZGPADM's): matrix products use 2x2 register tiles per thread,
+// the denominator solve is Gauss-Jordan with implicit partial pivoting (the
+// pivot of column k is the largest |D(i,k)| over unused rows, no row swaps, no
+// back substitution), and the last squaring only forms the first column.
+// H: m x m block of an LDS matrix with leading dim ld; W0..W3: ld x ld LDS
+// workspaces; F receives exp(t H) e1 (m entries).  NT threads cooperate.
+// Returns ns (>= 0) or a negative code (-1 null H, -2 singular, -3 non-finite).
+template <int NT>
+__device__ __forceinline__ void tile_matmul(cd* C, const cd* A, const cd* B, int m, int ld, double alpha) {
+    const int tiles = (m + 1) >> 1;
+    for (int t = threadIdx.x; t < tiles * tiles; t += NT) {
+        const int i0 = 2 * (t % tiles), j0 = 2 * (t / tiles);
+        cd c00 = mkcd(0.0, 0.0), c10 = c00, c01 = c00, c11 = c00;
+        for (int l = 0; l < m; ++l) {
+            const cd a0 = A[i0 + ld * l], a1 = A[i0 + 1 + ld * l];
+            const cd b0 = B[l + ld * j0], b1 = B[l + ld * (j0 + 1)];
+            c00 = cfma(a0, b0, c00);
+            c10 = cfma(a1, b0, c10);
+            c01 = cfma(a0, b1, c01);
+            c11 = cfma(a1, b1, c11);
+        }
+        C[i0 + ld * j0] = cscale(c00, alpha);
+        if (i0 + 1 < m) C[i0 + 1 + ld * j0] = cscale(c10, alpha);
+        if (j0 + 1 < m) {
+            C[i0 + ld * (j0 + 1)] = cscale(c01, alpha);
+            if (i0 + 1 < m) C[i0 + 1 + ld * (j0 + 1)] = cscale(c11, alpha);
+        }
+    }
+    __syncthreads();
+}
+
+template <int NT>
+__device__ int pade6_first_column(const cd* H, int ld, int m, double t, cd* W0, cd* W1, cd* W2, cd* W3, cd* F,
+                                  int* piv_lds, double* scal) {
+    const int tid = threadIdx.x;
+    // ||H||_inf with complex moduli (the reference's scaling rule)
+    if (tid < kWave) {
+        double rs = 0.0;
+        if (tid < m)
+            for (int j = 0; j < m; ++j) rs += cabs(H[tid + ld * j]);
+#pragma unroll
+        for (int mask = 32; mask >= 1; mask >>= 1) rs = fmax(rs, __shfl_xor(rs, mask, kWave));
+        if (tid == 0) scal[0] = rs;
+    }
+    __syncthreads();
+    const double hnorm = fabs(t * scal[0]);
+    if (hnorm == 0.0) return -1;
+    if (!(hnorm < 1e300)) return -3;
+    int ns = (int)(log(hnorm) / log(2.0)) + 2;
+    if (ns < 0) ns = 0;
+    const double scale = t / ldexp(1.0, ns);
+    double c[7];
+    c[0] = 1.0;
+#pragma unroll
+    for (int k = 1; k <= 6; ++k) c[k] = c[k - 1] * (double)(7 - k) / (double)(k * (13 - k));
+    // W0 = X2 = scale^2 H H
+    tile_matmul<NT>(W0, H, H, m, ld, scale * scale);
+    // even part E (c6, c4, c2, c0) in W1, odd part O (c5, c3, c1) in W2
+    for (int idx = tid; idx < m * m; idx += NT) {
+        const int i = idx % m, j = idx / m;
+        W1[i + ld * j] = cscale(W0[i + ld * j], c[6]);
+        W2[i + ld * j] = cscale(W0[i + ld * j], c[5]);
+        if (i == j) { W1[i + ld * j].re += c[4]; W2[i + ld * j].re += c[3]; }
+    }
+    __syncthreads();
+    // E = (E X2 + c2) X2 + c0 ; O = O X2 + c1   (W3 scratch)
+    tile_matmul<NT>(W3, W1, W0, m, ld, 1.0);
+    for (int i = tid; i < m; i += NT) W3[i + ld * i].re += c[2];
+    __syncthreads();
+    tile_matmul<NT>(W1, W3, W0, m, ld, 1.0);
+    for (int i = tid; i < m; i += NT) W1[i + ld * i].re += c[0];
+    tile_matmul<NT>(W3, W2, W0, m, ld, 1.0);
+    for (int i = tid; i < m; i += NT) W3[i + ld * i].re += c[1];
+    __syncthreads();
+    // zO = scale * O H -> W2 ; D = E - zO -> W1 ; N = zO (W2)
+    tile_matmul<NT>(W2, W3, H, m, ld, scale);
+    for (int idx = tid; idx < m * m; idx += NT) {
+        const int i = idx % m, j = idx / m;
+        W1[i + ld * j] = csub(W1[i + ld * j], W2[i + ld * j]);
+    }
+    if (tid < kWave) piv_lds[tid] = 0;  // used-row flags (m <= 64)
+    __syncthreads();
+    // Gauss-Jordan on [D | N] with implicit partial pivoting
+    for (int k = 0; k < m; ++k) {
+        if (tid < kWave) {
+            double a = -1.0;
+            if (tid < m && piv_lds[tid] == 0) a = cabs(W1[tid + ld * k]);
+            int p = tid;
+#pragma unroll
+            for (int mask = 32; mask >= 1; mask >>= 1) {
+                const double ao = __shfl_xor(a, mask, kWave);
+                const int po = __shfl_xor(p, mask, kWave);
+                if (ao > a || (ao == a && po < p)) { a = ao; p = po; }
+            }
+            if (tid == 0) piv_lds[64 + k] = (a > 0.0) ? p : -1;
+        }
+        __syncthreads();
+        const int p = piv_lds[64 + k];
+        if (p < 0) return -2;
+        if (tid == 0) piv_lds[p] = 1;
+        const cd inv = cdiv(mkcd(1.0, 0.0), W1[p + ld * k]);
+        const int ncol = (m - k - 1) + m;
+        for (int idx = tid; idx < m * ncol; idx += NT) {
+            const int i = idx % m;
+            const int jj = idx / m;
+            if (i == p) continue;
+            const cd f = cmul(W1[i + ld * k], inv);
+            if (jj < m - k - 1) {
+                const int j = k + 1 + jj;
+                W1[i + ld * j] = cfms(f, W1[p + ld * j], W1[i + ld * j]);
+            } else {
+                const int j = jj - (m - k - 1);
+                W2[i + ld * j] = cfms(f, W2[p + ld * j], W2[i + ld * j]);
+            }
+        }
+        __syncthreads();
+    }
+    // R = I + 2 Y, Y(k,:) = N(perm k,:) / D(perm k, k)   -> W0
+    for (int idx = tid; idx < m * m; idx += NT) {
+        const int k = idx % m, j = idx / m;
+        const int pr = piv_lds[64 + k];
+        const cd y = cdiv(W2[pr + ld * j], W1[pr + ld * k]);
+        W0[k + ld * j] = mkcd(2.0 * y.re + (k == j ? 1.0 : 0.0), 2.0 * y.im);
+    }
+    __syncthreads();
+    // squarings; the last one only forms the first column
+    cd* R = W0;
+    cd* T = W3;
+    for (int q = 0; q + 1 < ns; ++q) {
+        tile_matmul<NT>(T, R, R, m, ld, 1.0);
+        cd* tmp = R; R = T; T = tmp;
+    }
+    if (ns == 0) {
+        for (int i = tid; i < m; i += NT) F[i] = R[i];
+    } else {
+        for (int i = tid; i < m; i += NT) {
+            cd acc = mkcd(0.0, 0.0);
+            for (int l = 0; l < m; ++l) acc = cfma(R[i + ld * l], R[l], acc);
+            F[i] = acc;
+        }
+    }
+    __syncthreads();
+    return ns;
+}
+
+// In-place variant of tile_matmul (C may alias A or B): every thread keeps
+// its 2x2 tile in registers until all threads have finished reading.
+// Needs ((m+1)/2)^2 <= NT, i.e. m <= 32 for 256 threads.
+template <int NT>
+__device__ __forceinline__ void tile_matmul_ip(cd* C, const cd* A, const cd* B, int m, int ld, double alpha) {
+    const int tiles = (m + 1) >> 1;
+    const int t = threadIdx.x;
+    const bool active = t < tiles * tiles;
+    const int i0 = 2 * (t % tiles), j0 = 2 * (t / tiles);
+    cd c00 = mkcd(0.0, 0.0), c10 = c00, c01 = c00, c11 = c00;
+    if (active) {
+        for (int l = 0; l < m; ++l) {
+            const cd a0 = A[i0 + ld * l], a1 = A[i0 + 1 + ld * l];
+            const cd b0 = B[l + ld * j0], b1 = B[l + ld * (j0 + 1)];
+            c00 = cfma(a0, b0, c00);
+            c10 = cfma(a1, b0, c10);
+            c01 = cfma(a0, b1, c01);
+            c11 = cfma(a1, b1, c11);
+        }
+    }
+    __syncthreads();
+    if (active) {
+        C[i0 + ld * j0] = cscale(c00, alpha);
+        if (i0 + 1 < m) C[i0 + 1 + ld * j0] = cscale(c10, alpha);
+        if (j0 + 1 < m) {
+            C[i0 + ld * (j0 + 1)] = cscale(c01, alpha);
+            if (i0 + 1 < m) C[i0 + 1 + ld * (j0 + 1)] = cscale(c11, alpha);
+        }
+    }
+    __syncthreads();
+}
+
+// pade6_first_column with three workspaces (W0 = X2 then R, W1 = even part
+// then the denominator, W2 = odd part then the numerator), using in-place
+// register-tiled products.  m <= 32.
+template <int NT>
+__device__ int pade6_first_column3(const cd* H, int ld, int m, double t, cd* W0, cd* W1, cd* W2, cd* F,
+                                   int* piv_lds, double* scal) {
+    const int tid = threadIdx.x;
+    if (tid < kWave) {
+        double rs = 0.0;
+        if (tid < m)
+            for (int j = 0; j < m; ++j) rs += cabs(H[tid + ld * j]);
+#pragma unroll
+        for (int mask = 32; mask >= 1; mask >>= 1) rs = fmax(rs, __shfl_xor(rs, mask, kWave));
+        if (tid == 0) scal[0] = rs;
+    }
+    __syncthreads();
+    const double hnorm = fabs(t * scal[0]);
+    if (hnorm == 0.0) return -1;
+    if (!(hnorm < 1e300)) return -3;
+    int ns = (int)(log(hnorm) / log(2.0)) + 2;
+    if (ns < 0) ns = 0;
+    const double scale = t / ldexp(1.0, ns);
+    double c[7];
+    c[0] = 1.0;
+#pragma unroll
+    for (int k = 1; k <= 6; ++k) c[k] = c[k - 1] * (double)(7 - k) / (double)(k * (13 - k));
+    tile_matmul<NT>(W0, H, H, m, ld, scale * scale);  // X2
+    for (int idx = tid; idx < m * m; idx += NT) {
+        const int i = idx % m, j = idx / m;
+        W1[i + ld * j] = cscale(W0[i + ld * j], c[6]);
+        W2[i + ld * j] = cscale(W0[i + ld * j], c[5]);
+        if (i == j) { W1[i + ld * j].re += c[4]; W2[i + ld * j].re += c[3]; }
+    }
+    __syncthreads();
+    tile_matmul_ip<NT>(W1, W0, W1, m, ld, 1.0);  // E = X2 (c6 X2 + c4)
+    for (int i = tid; i < m; i += NT) W1[i + ld * i].re += c[2];
+    __syncthreads();
+    tile_matmul_ip<NT>(W1, W0, W1, m, ld, 1.0);  // E = X2 E + c0
+    tile_matmul_ip<NT>(W2, W0, W2, m, ld, 1.0);  // O = X2 (c5 X2 + c3) + c1
+    for (int i = tid; i < m; i += NT) {
+        W1[i + ld * i].re += c[0];
+        W2[i + ld * i].re += c[1];
+    }
+    __syncthreads();
+    tile_matmul_ip<NT>(W2, W2, H, m, ld, scale);  // zO = scale O H
+    for (int idx = tid; idx < m * m; idx += NT) {
+        const int i = idx % m, j = idx / m;
+        W1[i + ld * j] = csub(W1[i + ld * j], W2[i + ld * j]);  // D = E - zO
+    }
+    if (tid < kWave) piv_lds[tid] = 0;
+    __syncthreads();
+    for (int k = 0; k < m; ++k) {
+        if (tid < kWave) {
+            double a = -1.0;
+            if (tid < m && piv_lds[tid] == 0) a = cabs(W1[tid + ld * k]);
+            int p = tid;
+#pragma unroll
+            for (int mask = 32; mask >= 1; mask >>= 1) {
+                const double ao = __shfl_xor(a, mask, kWave);
+                const int po = __shfl_xor(p, mask, kWave);
+                if (ao > a || (ao == a && po < p)) { a = ao; p = po; }
+            }
+            if (tid == 0) piv_lds[64 + k] = (a > 0.0) ? p : -1;
+        }
+        __syncthreads();
+        const int p = piv_lds[64 + k];
+        if (p < 0) return -2;
+        if (tid == 0) piv_lds[p] = 1;
+        const cd inv = cdiv(mkcd(1.0, 0.0), W1[p + ld * k]);
+        const int ncol = (m - k - 1) + m;
+        for (int idx = tid; idx < m * ncol; idx += NT) {
+            const int i = idx % m;
+            const int jj = idx / m;
+            if (i == p) continue;
+            const cd f = cmul(W1[i + ld * k], inv);
+            if (jj < m - k - 1) {
+                const int j = k + 1 + jj;
+                W1[i + ld * j] = cfms(f, W1[p + ld * j], W1[i + ld * j]);
+            } else {
+                const int j = jj - (m - k - 1);
+                W2[i + ld * j] = cfms(f, W2[p + ld * j], W2[i + ld * j]);
+            }
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < m * m; idx += NT) {
+        const int k = idx % m, j = idx / m;
+        const int pr = piv_lds[64 + k];
+        const cd y = cdiv(W2[pr + ld * j], W1[pr + ld * k]);
+        W0[k + ld * j] = mkcd(2.0 * y.re + (k == j ? 1.0 : 0.0), 2.0 * y.im);
+    }
+    __syncthreads();
+    for (int q = 0; q + 1 < ns; ++q) tile_matmul_ip<NT>(W0, W0, W0, m, ld, 1.0);
+    if (ns == 0) {
+        for (int i = tid; i < m; i += NT) F[i] = W0[i];
+    } else {
+        for (int i = tid; i < m; i += NT) {
+            cd acc = mkcd(0.0, 0.0);
+            for (int l = 0; l < m; ++l) acc = cfma(W0[i + ld * l], W0[l], acc);
+            F[i] = acc;
+        }
+    }
+    __syncthreads();
+    return ns;
 }
 
 }  // namespace oops

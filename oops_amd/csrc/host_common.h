@@ -17,6 +17,7 @@ struct CompiledOp {
     int nspin = 0;
     int ld_bits = 0;
     int max_row = 0;
+    int32_t ent2_first = 0;
     std::complex<double> c0{0.0, 0.0};
     std::vector<DevOpLocal> locs;
     std::vector<std::complex<double>> val;

@@ -16,7 +16,9 @@ struct DevOpHeader {      // 64 bytes
     int32_t ent_off;      // first entry of this operator
     int32_t max_row;      // largest R over the local operators
     double c0re, c0im;    // 0-body (identity) coefficient
-    double pad[2];
+    int32_t ent2_first;   // first entry of the 2-body operators (1-body entries come first)
+    int32_t pad0;
+    double pad1;
 };
 
 struct DevOpLocal {       // 16 bytes

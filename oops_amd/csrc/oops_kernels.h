@@ -9,8 +9,9 @@ namespace oops {
 
 constexpr int kMaxLocals = 64;      // merged local operators per compiled operator
 constexpr int kDenseMaxEnt = 1024;  // ELL entries per operator on the dense path
+constexpr int kKrylovMaxEnt = 1024; // ELL entries per operator on the batched Krylov path
 constexpr int kDenseNMAX = 16;      // largest state handled by the dense path
-constexpr int kStatsPerItem = 16;   // int32 slots of per-cluster counters
+constexpr int kStatsPerItem = 24;   // int32 slots of per-cluster counters
 
 // One bucket of Hahn-echo clusters sharing the state dimension n.
 struct KrylovArgs {
@@ -38,7 +39,7 @@ struct KrylovArgs {
     double tol;
 };
 
-int launch_hahn_bucket(const KrylovArgs& a, int m, int grid, hipStream_t stream);
+int launch_hahn_bucket(const KrylovArgs& a, int m, int precision, int grid, hipStream_t stream);
 int krylov_entry_capacity(int m);
 
 }  // namespace oops

@@ -478,9 +478,46 @@ struct OrcTermTable {
     int64_t n;
 };
 
+// w(t_k) = exp(-i H (t_k - t_{k-1})) w(t_{k-1}) with a dense Pade(6)
+// exponential, recomputed only when the output spacing changes.
+int orc_dense_sweep(const OrcTermTable& T, const cplx* v, int32_t nt, const double* ta, cplx* out) {
+    const int64_t n = T.n;
+    std::vector<cplx> H((size_t)n * n), e(n, cplx(0.0)), col(n), wm(n), cz(T.nterm), U((size_t)n * n), X((size_t)n * n);
+    for (int t = 0; t < T.nterm; ++t) cz[t] = cplx(T.coeff[t], 0.0);
+    for (int64_t j = 0; j < n; ++j) {
+        e[j] = 1.0;
+        orc_hamvec(T.nspin, T.nterm, cz.data(), T.nbody, T.pos_off, T.pos, T.dims, T.mat_off, T.mats, n,
+                   T.spin_dim, e.data(), col.data(), wm.data());
+        e[j] = 0.0;
+        for (int64_t i = 0; i < n; ++i) H[i + n * j] = cplx(0.0, -1.0) * col[i];
+    }
+    std::vector<cplx> w(v, v + n), w2(n);
+    double tprev = 0.0, dprev = -1.0;
+    for (int k = 0; k < nt; ++k) {
+        const double d = ta[k] - tprev;
+        tprev = ta[k];
+        if (d > 0.0) {
+            if (d != dprev) {
+                int ns = 0;
+                if (orc_zgpadm(6, (int)n, d, H.data(), (int)n, U.data(), &ns) != 0) return -4;
+                dprev = d;
+            }
+            for (int64_t i = 0; i < n; ++i) {
+                cplx s(0.0);
+                for (int64_t j = 0; j < n; ++j) s += U[i + n * j] * w[j];
+                w2[i] = s;
+            }
+            w.swap(w2);
+        }
+        std::memcpy(out + (size_t)n * k, w.data(), sizeof(cplx) * n);
+    }
+    return 0;
+}
+
 // Hahn-echo coherence of a batch of clusters by two Krylov sweeps each:
 //   L[c, k] = Re( scale[c] * < expv(opB[c], y0[c], ta[k]), expv(opA[c], u0[c], ta[k]) > )
-// opB must already be the bra-side generator (-lvB^dagger).  One cluster per
+// opB must already be the bra-side generator (-lvB^dagger).  Clusters with
+// n <= m use orc_dense_sweep (the reference's driver stops there).  One cluster per
 // thread, nthreads threads (the reference runs one cluster per MPI rank,
 // src/source/app/cce.cpp:54-70).  Each expv starts from iseed = (0,0,0,5).
 int orc_hahn_batch(int32_t nclu, const OrcTermTable* opA, const OrcTermTable* opB,
@@ -495,17 +532,25 @@ int orc_hahn_batch(int32_t nclu, const OrcTermTable* opA, const OrcTermTable* op
             const OrcTermTable& A = opA[c];
             const OrcTermTable& B = opB[c];
             const int64_t n = A.n;
-            const int mm = (int)std::min<int64_t>(m, n - 1);
             std::vector<cplx> ua((size_t)n * nt), yb((size_t)n * nt);
             int32_t seed[4] = {0, 0, 0, 5};
             OrcExpvStats sa, sb;
-            int fa = orc_expv(A.nspin, A.nterm, A.coeff, A.nbody, A.pos_off, A.pos, A.dims,
-                              A.mat_off, A.mats, n, A.spin_dim, u0[c], klim, nt, ta, mm, tol,
+            std::memset(&sa, 0, sizeof(sa));
+            std::memset(&sb, 0, sizeof(sb));
+            int fa, fb;
+            if (n <= m) {
+                // the reference stops for m >= n; use the exact dense exponential
+                fa = orc_dense_sweep(A, u0[c], nt, ta, ua.data());
+                fb = orc_dense_sweep(B, y0[c], nt, ta, yb.data());
+            } else {
+                fa = orc_expv(A.nspin, A.nterm, A.coeff, A.nbody, A.pos_off, A.pos, A.dims,
+                              A.mat_off, A.mats, n, A.spin_dim, u0[c], klim, nt, ta, m, tol,
                               ua.data(), seed, &sa);
-            seed[0] = 0; seed[1] = 0; seed[2] = 0; seed[3] = 5;
-            int fb = orc_expv(B.nspin, B.nterm, B.coeff, B.nbody, B.pos_off, B.pos, B.dims,
-                              B.mat_off, B.mats, n, B.spin_dim, y0[c], klim, nt, ta, mm, tol,
+                seed[0] = 0; seed[1] = 0; seed[2] = 0; seed[3] = 5;
+                fb = orc_expv(B.nspin, B.nterm, B.coeff, B.nbody, B.pos_off, B.pos, B.dims,
+                              B.mat_off, B.mats, n, B.spin_dim, y0[c], klim, nt, ta, m, tol,
                               yb.data(), seed, &sb);
+            }
             if (fa != 0 || fb != 0) err.store(fa != 0 ? fa : fb);
             for (int k = 0; k < nt; ++k) {
                 const cplx d = zdotc(n, &yb[(size_t)n * k], &ua[(size_t)n * k]);
