@@ -18,8 +18,6 @@
  *                             expv/source/main_mkl.f:1-129) as called by
  *                             MatExpVector::runInexplicitGPU / runInexplicit
  *                             (source/math/MatExp.cpp:216-521)
- *   oops_zgpadm            <- zgpadm_ (expv/include/expokit.h:6,
- *                             expv/source/expokit/expokit.f:762-934)
  *   oops_hahn_batch_*      <- no reference counterpart: the reference calls one
  *                             expv per cluster from CCE::run_each_clusters
  *                             (source/app/cce.cpp:54-70); the batch runs every
@@ -35,6 +33,7 @@
 #ifndef OOPS_EXPV_H
 #define OOPS_EXPV_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -80,6 +79,27 @@ int oops_ctx_info(const oops_ctx* ctx, int32_t* num_cu, int32_t* lds_bytes, char
 /* --- single operator ---------------------------------------------------- */
 /* w = H v for host vectors of length prod(spin_dim) (hamvec_cuda3_). */
 int oops_hamvec(oops_ctx* ctx, const oops_term_table* H, const double* v, double* w);
+
+/* Multi-output Krylov expv of one state: w(t_k) = exp(-i H t_k) v for the
+ * ascending times ta[0..nt) (MAIN_CACHE / zgexpv_cache: Lanczos(klim) norm
+ * estimate from DLARAN seed (0,0,0,5), Krylov dimension m (2..40, < n),
+ * tolerance tol per unit time).  The basis (n x (m+1) complex) stays in
+ * device memory; only scalars cross to the host per Krylov step.
+ * out_mode 0: out = w_seq, n x nt complex column-major (the reference's
+ * w_seq); out_mode 1: out = <v, w(t_k)>, nt complex.
+ * stats (may be NULL): [0] products, [1] small exponentials, [2] squarings,
+ * [3] steps, [4] rejections, [5] breakdown flag, [6] norm estimate,
+ * [7] device time of the Krylov steps in ms. */
+int oops_expv(oops_ctx* ctx, const oops_term_table* H, const double* v, int32_t nt, const double* ta, int32_t klim,
+              int32_t m, double tol, int32_t out_mode, double* out, double* stats);
+
+/* Fortran-ABI drop-in for the reference's main_cache_ (main_cache.h:6),
+ * same argument list; runs oops_expv with out_mode 0 on device 0. */
+void main_cache_(size_t* nspin, size_t* nTerm, double* coeff_lst, size_t* nbody_lst, size_t* pos_i_idx,
+                 size_t* pos_i_lst, size_t* dim_i_lst, size_t* mat_i_idx, double* mat_i_lst, size_t* ham_dim,
+                 size_t* nspin_dim, double* v, size_t* pos_idx, size_t* mat_idx, size_t* k,
+                 size_t* maxThreadsPerBlock, size_t* maxGridSize, size_t* tn, double* ta, size_t* m, double* tol,
+                 size_t* itrace, double* w_seq, size_t* w_seq_len);
 
 /* --- batched Hahn echo ---------------------------------------------------- */
 /* One cluster: opA generates the ket sweep exp(-i opA tau) u0, opB the bra

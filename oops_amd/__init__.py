@@ -56,7 +56,7 @@ EXPORTS = [
     "oops_bath_diamond", "oops_nv_states", "oops_cce_cluster_terms", "oops_terms_view",
     "oops_terms_sizes", "oops_terms_destroy", "oops_clusters_enumerate", "oops_clusters_count",
     "oops_clusters_get", "oops_clusters_parents", "oops_clusters_destroy", "oops_cce_reduce",
-    "oops_cce_hahn_batch_create",
+    "oops_cce_hahn_batch_create", "oops_hamvec", "oops_expv",
 ]
 
 
@@ -83,6 +83,10 @@ def lib():
                                               ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
         L.oops_clusters_destroy.argtypes = [ctypes.c_void_p]
         L.oops_terms_destroy.argtypes = [ctypes.c_void_p]
+        L.oops_hamvec.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.oops_expv.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_void_p,
+                                ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -162,6 +166,35 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+def hamvec(ctx: Context, H: TermTable, v):
+    """w = H v on the device (hamvec_cuda3_, hamvec_cuda3.cu:661-665)."""
+    v = np.ascontiguousarray(v, np.complex128)
+    if v.shape != (H.n,):
+        raise ValueError(f"v must have length {H.n}")
+    w = np.empty_like(v)
+    _check(lib().oops_hamvec(ctx.h, ctypes.byref(H.c), _ptr(v), _ptr(w)), ctx.h, "oops_hamvec")
+    return w
+
+
+def expv(ctx: Context, H: TermTable, v, ta, m=30, klim=10, tol=1e-12, overlaps=False):
+    """exp(-i H t) v at every time of ta (MatExpVector::runInexplicitGPU,
+    MatExp.cpp:367-521).  Returns (w_seq [n x nt], stats) or, with
+    overlaps=True, (<v, w(t_k)> [nt], stats)."""
+    v = np.ascontiguousarray(v, np.complex128)
+    ta = np.ascontiguousarray(ta, np.float64)
+    if v.shape != (H.n,):
+        raise ValueError(f"v must have length {H.n}")
+    out = np.empty(len(ta) if overlaps else (len(ta), H.n), np.complex128)
+    st = np.zeros(8)
+    rc = lib().oops_expv(ctx.h, ctypes.byref(H.c), _ptr(v), ctypes.c_int32(len(ta)), _ptr(ta), ctypes.c_int32(klim),
+                         ctypes.c_int32(m), ctypes.c_double(tol), ctypes.c_int32(1 if overlaps else 0), _ptr(out),
+                         _ptr(st))
+    _check(rc, ctx.h, "oops_expv")
+    keys = ["nmult", "nexph", "nscale", "nstep", "nreject", "ibrkflag", "anorm", "kernel_ms"]
+    stats = dict(zip(keys, st.tolist()))
+    return (out if overlaps else out.T), stats
 
 
 class HahnBatch:

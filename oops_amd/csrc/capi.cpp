@@ -21,15 +21,6 @@
 using oops::DevOpHeader;
 using oops::DevOpLocal;
 
-struct oops_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int num_cu = 0;
-    int lds_bytes = 0;
-    std::string name;
-    std::string err;
-};
-
 namespace oops {
 
 int set_err(oops_ctx* ctx, int code, const std::string& msg) {
@@ -45,7 +36,7 @@ int hip_check(oops_ctx* ctx, hipError_t e, const char* what) {
 // ---------------------------------------------------------------------------
 // Operator compiler: merge the reference's term list into one dense local
 // matrix per spin and per spin pair, then store each as an ELL row table.
-int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err) {
+int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err, int max_locals) {
     if (!T || T->nspin <= 0 || T->nterm < 0) { err = "empty term table"; return OOPS_ERR_ARG; }
     const int ns = T->nspin;
     const int64_t d = T->spin_dim[0];
@@ -138,7 +129,7 @@ int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err
         if (!single[s].empty()) add_local(1, s, s, single[s], d);
     out.ent2_first = (int32_t)out.val.size();
     for (auto& kv : pair) add_local(2, kv.first.first, kv.first.second, kv.second, d * d);
-    if ((int)out.locs.size() > kMaxLocals) { err = "too many local operators"; return OOPS_ERR_CAPACITY; }
+    if ((int)out.locs.size() > max_locals) { err = "too many local operators"; return OOPS_ERR_CAPACITY; }
     if (out.max_row > 16) { err = "local operator rows wider than 16 entries"; return OOPS_ERR_UNSUPPORTED; }
     return OOPS_OK;
 }

@@ -1,0 +1,278 @@
+// Krylov expv for one large state vector (any n up to 2^30), basis in HBM.
+//
+// Device side of MatExpVector::runInexplicitGPU / MAIN_CACHE
+// (src/source/math/MatExp.cpp:367-521, src/expv/source/zgexpv_cache.f:4-690)
+// and of hamvec_cuda3 (src/expv/source/hamvec_cuda3.cu:392-645), redesigned
+// so that nothing but a few scalars per Krylov step crosses PCIe: the basis
+// V (n x (m+1)), the iterate w and the Hessenberg matrix stay in device
+// memory, the small exponential runs on the device, and the host keeps only
+// Expokit's scalar step control.
+//
+// Kernels:
+//   large_matvec     y = alpha * H * (x * xscale), optionally storing the
+//                    scaled x as the next basis vector; one thread per
+//                    element, the compiled operator (merged 1-/2-body ELL
+//                    tables) in LDS, gathers from L2/HBM.
+//   large_dots       partial sums of conj(V_l) . p for l <= J and |p|^2
+//                    (grid.y = vector index), 256-thread block reductions.
+//   large_finish     sums the partials, stores h into the Hessenberg column
+//                    and the reciprocal norm for the next product.
+//   large_update     p -= sum_l h_l V_l (one read of each basis vector).
+//   large_pade       Pade(6) first column of exp(t H_aug) on one workgroup.
+//   large_combine    w = beta * sum_j F_j V_j.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "devcommon.h"
+#include "oops_kernels.h"
+#include "oops_large.h"
+
+namespace oops {
+
+constexpr int kLT = 256;
+
+// The local-operator list sits in LDS; the ELL rows (up to a few thousand
+// entries for a 10-spin Liouvillian) are read through the L1/L2 caches.
+struct LargeOpLds {
+    DevOpHeader hdr;
+    DevOpLocal locs[kLargeMaxLocals];
+};
+
+__device__ __forceinline__ void large_op_load(const LargeOpArgs& o, LargeOpLds& L) {
+    if (threadIdx.x == 0) L.hdr = *o.hdr;
+    const int nloc = o.hdr->nloc;
+    for (int k = threadIdx.x; k < nloc; k += kLT) L.locs[k] = o.loc[k];
+    __syncthreads();
+}
+
+// y = alpha * H (xscale * x); if xstore, xstore = xscale * x.
+// xscale is read from device memory (the reciprocal norm of the previous
+// Arnoldi step), alpha: 0 -> +1 (norm estimate), 1 -> -i.
+__global__ __launch_bounds__(kLT) void large_matvec(LargeOpArgs o, const double2* __restrict__ x,
+                                                    const double* __restrict__ xscale_p, double2* __restrict__ y,
+                                                    double2* __restrict__ xstore, int64_t n, int alpha_mode) {
+    __shared__ LargeOpLds L;
+    large_op_load(o, L);
+    const double xs = xscale_p ? *xscale_p : 1.0;
+    const cd* xc = reinterpret_cast<const cd*>(x);
+    const cd* vals = reinterpret_cast<const cd*>(o.val);
+    for (int64_t i = (int64_t)blockIdx.x * kLT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kLT) {
+        cd acc = op_apply_elem((int)i, xc, L.hdr, L.locs, vals, o.off);
+        acc = cscale(acc, xs);
+        if (alpha_mode == 1) acc = cmulmi(acc);
+        y[i] = make_double2(acc.re, acc.im);
+        if (xstore) {
+            const double2 xv = x[i];
+            xstore[i] = make_double2(xv.x * xs, xv.y * xs);
+        }
+    }
+}
+
+// part[(l * gridDim.x + block) * 2 + {re, im}] = partial conj(V_l) . p,
+// l = 0..J; l = J+1 (if with_norm) holds |p|^2 in the re slot.
+__global__ __launch_bounds__(kLT) void large_dots(const double2* __restrict__ V, int64_t ldv,
+                                                  const double2* __restrict__ p, int64_t n, int J, int with_norm,
+                                                  double* __restrict__ part) {
+    __shared__ double red[2 * (kLT / kWave)];
+    const int l = blockIdx.y;
+    double sr = 0.0, si = 0.0;
+    if (l <= J) {
+        const double2* v = V + (int64_t)l * ldv;
+        for (int64_t i = (int64_t)blockIdx.x * kLT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kLT) {
+            const double2 a = v[i], b = p[i];
+            sr = fma(a.x, b.x, fma(a.y, b.y, sr));
+            si = fma(a.x, b.y, fma(-a.y, b.x, si));
+        }
+    } else {
+        for (int64_t i = (int64_t)blockIdx.x * kLT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kLT) {
+            const double2 b = p[i];
+            sr = fma(b.x, b.x, fma(b.y, b.y, sr));
+        }
+    }
+#pragma unroll
+    for (int mask = 32; mask >= 1; mask >>= 1) {
+        sr += __shfl_xor(sr, mask, kWave);
+        si += __shfl_xor(si, mask, kWave);
+    }
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    if (lane == 0) { red[2 * wave] = sr; red[2 * wave + 1] = si; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < kLT / kWave; ++w) { a += red[2 * w]; b += red[2 * w + 1]; }
+        part[((int64_t)l * gridDim.x + blockIdx.x) * 2] = a;
+        part[((int64_t)l * gridDim.x + blockIdx.x) * 2 + 1] = b;
+    }
+}
+
+// Sum the partials of large_dots.
+//   mode 0: first Gram-Schmidt pass of Arnoldi column jcol: h(l, jcol) = sums,
+//           coef = sums (rows 0..nrows-1 of part)
+//   mode 1: re-orthogonalisation pass: h(l, jcol) += sums, coef = sums
+//   mode 2: norm (row 0 of part): h(jcol+1, jcol) = sqrt(|p|^2) unless it is
+//           <= break_tol (then flags = {1, jcol+1} once); scal[0] = 1/h
+//           (0 after a breakdown), scal[1] = h
+//   mode 3: norm (row 0 of part) into scal[2] (the extra product's norm)
+// One block.
+__global__ __launch_bounds__(kLT) void large_finish(const double* __restrict__ part, int nblk, int nrows, int jcol,
+                                                    int mode, double2* __restrict__ hess, int ldh,
+                                                    double2* __restrict__ coef, double* __restrict__ scal,
+                                                    int* __restrict__ flags, double break_tol) {
+    for (int l = threadIdx.x; l < nrows; l += kLT) {
+        double a = 0.0, b = 0.0;
+        for (int k = 0; k < nblk; ++k) {
+            a += part[((int64_t)l * nblk + k) * 2];
+            b += part[((int64_t)l * nblk + k) * 2 + 1];
+        }
+        if (mode == 2) {
+            const double h = sqrt(a);
+            const bool broke = flags[0] != 0 || h <= break_tol;
+            if (!broke) hess[(jcol + 1) + (int64_t)ldh * jcol] = make_double2(h, 0.0);
+            if (flags[0] == 0 && h <= break_tol) {
+                flags[0] = 1;
+                flags[1] = jcol + 1;  // basis size at the breakdown
+            }
+            scal[0] = broke ? 0.0 : 1.0 / h;
+            scal[1] = h;
+        } else if (mode == 3) {
+            scal[2] = sqrt(a);
+        } else {
+            coef[l] = make_double2(a, b);
+            double2& hv = hess[l + (int64_t)ldh * jcol];
+            if (mode == 0) hv = make_double2(a, b);
+            else hv = make_double2(hv.x + a, hv.y + b);
+        }
+    }
+}
+
+// p -= sum_{l<=J} coef_l V_l
+__global__ __launch_bounds__(kLT) void large_update(const double2* __restrict__ V, int64_t ldv, double2* __restrict__ p,
+                                                    int64_t n, int J, const double2* __restrict__ coef) {
+    __shared__ double2 c[kLargeMaxM + 2];
+    for (int l = threadIdx.x; l <= J; l += kLT) c[l] = coef[l];
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * kLT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kLT) {
+        cd acc = mkcd(p[i].x, p[i].y);
+        for (int l = 0; l <= J; ++l) {
+            const double2 v = V[(int64_t)l * ldv + i];
+            acc = cfms(mkcd(c[l].x, c[l].y), mkcd(v.x, v.y), acc);
+        }
+        p[i] = make_double2(acc.re, acc.im);
+    }
+}
+
+// F = exp(t H(0:mx,0:mx)) e1 for the augmented Hessenberg matrix (Pade(6),
+// one workgroup).  status[0] = number of squarings or a negative error.
+struct LargePadeSmem {
+    cd W[4][(kLargeMaxM + 2) * (kLargeMaxM + 3)];  // one spare column per matrix (tiled reads)
+    cd F[kLargeMaxM + 2];
+    int piv[128];
+    double scal[16];
+};
+
+__global__ __launch_bounds__(kLT) void large_pade(const double2* __restrict__ hess, int ldh, int mx, double t,
+                                                  double2* __restrict__ F, int* __restrict__ status) {
+    extern __shared__ unsigned char dyn[];
+    LargePadeSmem& S = *reinterpret_cast<LargePadeSmem*>(dyn);
+    const int ld = kLargeMaxM + 2;
+    // H is read in place from device memory (leading dimension ldh)
+    const cd* H = reinterpret_cast<const cd*>(hess);
+    (void)ld;
+    const int ns = pade6_first_column<kLT>(H, ldh, mx, t, S.W[0], S.W[1], S.W[2], S.W[3], S.F, S.piv, S.scal);
+    for (int i = threadIdx.x; i < mx; i += kLT) F[i] = make_double2(S.F[i].re, S.F[i].im);
+    if (threadIdx.x == 0) status[0] = ns;
+}
+
+// w = beta * sum_{j<mx} F_j V_j ; partial |w|^2 per block into part
+__global__ __launch_bounds__(kLT) void large_combine(const double2* __restrict__ V, int64_t ldv, int mx,
+                                                     const double2* __restrict__ F, double beta,
+                                                     double2* __restrict__ w, int64_t n, double* __restrict__ part) {
+    __shared__ double2 c[kLargeMaxM + 2];
+    __shared__ double red[kLT / kWave];
+    for (int l = threadIdx.x; l < mx; l += kLT) c[l] = make_double2(F[l].x * beta, F[l].y * beta);
+    __syncthreads();
+    double s2 = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kLT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kLT) {
+        cd acc = mkcd(0.0, 0.0);
+        for (int l = 0; l < mx; ++l) {
+            const double2 v = V[(int64_t)l * ldv + i];
+            acc = cfma(mkcd(c[l].x, c[l].y), mkcd(v.x, v.y), acc);
+        }
+        w[i] = make_double2(acc.re, acc.im);
+        s2 = fma(acc.re, acc.re, fma(acc.im, acc.im, s2));
+    }
+#pragma unroll
+    for (int mask = 32; mask >= 1; mask >>= 1) s2 += __shfl_xor(s2, mask, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = s2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int q = 0; q < kLT / kWave; ++q) a += red[q];
+        part[blockIdx.x] = a;
+    }
+}
+
+// y_i = a x_i + b y_i (complex a, b), small BLAS-1 helper (scaled copies).
+// x may alias y; y is not read when b == 0.
+__global__ __launch_bounds__(kLT) void large_axpby(double2 a, const double2* x, double2 b, double2* y, int64_t n) {
+    const bool read_y = b.x != 0.0 || b.y != 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kLT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kLT) {
+        const cd xv = mkcd(x[i].x, x[i].y);
+        cd r = cmul(mkcd(a.x, a.y), xv);
+        if (read_y) r = cadd(r, cmul(mkcd(b.x, b.y), mkcd(y[i].x, y[i].y)));
+        y[i] = make_double2(r.re, r.im);
+    }
+}
+
+// Launch wrappers -------------------------------------------------------------
+int large_grid(int64_t n, int num_cu) {
+    const int64_t want = (n + kLT - 1) / kLT;
+    const int64_t cap = (int64_t)num_cu * 8;
+    return (int)(want < cap ? want : cap);
+}
+
+void launch_large_matvec(const LargeOpArgs& o, const double2* x, const double* xscale, double2* y, double2* xstore,
+                         int64_t n, int alpha_mode, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(large_matvec, dim3(grid), dim3(kLT), 0, s, o, x, xscale, y, xstore, n, alpha_mode);
+}
+
+void launch_large_dots(const double2* V, int64_t ldv, const double2* p, int64_t n, int J, int with_norm, double* part,
+                       int nblk, hipStream_t s) {
+    hipLaunchKernelGGL(large_dots, dim3(nblk, J + 1 + (with_norm ? 1 : 0)), dim3(kLT), 0, s, V, ldv, p, n, J,
+                       with_norm, part);
+}
+
+void launch_large_finish(const double* part, int nblk, int nrows, int jcol, int mode, double2* hess, int ldh,
+                         double2* coef, double* scal, int* flags, double break_tol, hipStream_t s) {
+    hipLaunchKernelGGL(large_finish, dim3(1), dim3(kLT), 0, s, part, nblk, nrows, jcol, mode, hess, ldh, coef, scal,
+                       flags, break_tol);
+}
+
+void launch_large_update(const double2* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef, int grid,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(large_update, dim3(grid), dim3(kLT), 0, s, V, ldv, p, n, J, coef);
+}
+
+int launch_large_pade(const double2* hess, int ldh, int mx, double t, double2* F, int* status, hipStream_t s) {
+    const size_t bytes = sizeof(LargePadeSmem);
+    static bool attr_set = false;
+    if (!attr_set) {
+        if (hipFuncSetAttribute((const void*)large_pade, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) !=
+            hipSuccess)
+            return -1;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(large_pade, dim3(1), dim3(kLT), bytes, s, hess, ldh, mx, t, F, status);
+    return 0;
+}
+
+void launch_large_combine(const double2* V, int64_t ldv, int mx, const double2* F, double beta, double2* w, int64_t n,
+                          double* part, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(large_combine, dim3(grid), dim3(kLT), 0, s, V, ldv, mx, F, beta, w, n, part);
+}
+
+void launch_large_axpby(double2 a, const double2* x, double2 b, double2* y, int64_t n, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(large_axpby, dim3(grid), dim3(kLT), 0, s, a, x, b, y, n);
+}
+
+}  // namespace oops

@@ -9,6 +9,16 @@
 #include "../../include/oops_expv.h"
 #include "oops_device_types.h"
 
+// One device, its stream and the last error text.
+struct oops_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+    int lds_bytes = 0;
+    std::string name;
+    std::string err;
+};
+
 namespace oops {
 
 // A term table compiled into merged local operators with ELL row tables.
@@ -44,7 +54,8 @@ struct DevBuf {
     }
 };
 
-int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err);
+// max_locals: capacity of the kernel's local-operator table (LDS).
+int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err, int max_locals = 64);
 int set_err(oops_ctx* ctx, int code, const std::string& msg);
 int hip_check(oops_ctx* ctx, hipError_t e, const char* what);
 void dlaran_fill(int32_t* iseed, int64_t count, double* out);

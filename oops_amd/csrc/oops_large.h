@@ -1,0 +1,34 @@
+// Launch interface of the large-state Krylov path (expv_large.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "oops_device_types.h"
+
+namespace oops {
+
+constexpr int kLargeMaxM = 40;      // Krylov dimension limit of the large path (LDS of the Pade kernel)
+constexpr int kLargeMaxLocals = 1024;  // merged local operators of one operator (LDS of the product kernel)
+
+struct LargeOpArgs {
+    const DevOpHeader* hdr;  // device copy of the header
+    const DevOpLocal* loc;
+    const double2* val;
+    const int32_t* off;
+};
+
+int large_grid(int64_t n, int num_cu);
+void launch_large_matvec(const LargeOpArgs& o, const double2* x, const double* xscale, double2* y, double2* xstore,
+                         int64_t n, int alpha_mode, int grid, hipStream_t s);
+void launch_large_dots(const double2* V, int64_t ldv, const double2* p, int64_t n, int J, int with_norm, double* part,
+                       int nblk, hipStream_t s);
+void launch_large_finish(const double* part, int nblk, int nrows, int jcol, int mode, double2* hess, int ldh,
+                         double2* coef, double* scal, int* flags, double break_tol, hipStream_t s);
+void launch_large_update(const double2* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef, int grid,
+                         hipStream_t s);
+int launch_large_pade(const double2* hess, int ldh, int mx, double t, double2* F, int* status, hipStream_t s);
+void launch_large_combine(const double2* V, int64_t ldv, int mx, const double2* F, double beta, double2* w, int64_t n,
+                          double* part, int grid, hipStream_t s);
+void launch_large_axpby(double2 a, const double2* x, double2 b, double2* y, int64_t n, int grid, hipStream_t s);
+
+}  // namespace oops

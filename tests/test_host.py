@@ -1,0 +1,111 @@
+"""Product host logic (liboops_amd.so, no GPU needed) against the oracle:
+term tables, NV states, enumeration order and parents, CCE reduction, the
+synthetic bath, and the C-ABI export list."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oops_amd as O
+from oracle import physics as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+B111 = np.ones(3) * 0.05 / np.sqrt(3.0)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = O.lib()
+    names = set()
+    for h in ("oops_expv.h", "oops_cce.h"):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        names |= set(re.findall(r"^\s*(?:int|double|const char\*)\s+(oops_\w+)\s*\(", txt, re.M))
+    assert len(names) > 20
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_nv_states_match_oracle():
+    ev, vec = O.nv_states([0, 0, 0], B111)
+    nv = P.NVCenter((0, 0, 0), B111)
+    assert np.allclose(ev, nv.evals, rtol=1e-14, atol=1e-5)
+    for k in range(3):
+        assert np.allclose(P.spin_vector(3, vec[:, k]), P.spin_vector(3, nv.state(k)), atol=1e-12)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_cluster_terms_match_oracle(k):
+    a = 3.567
+    rng = np.random.default_rng(k)
+    coords = np.array([[a * x for x in rng.integers(-4, 5, 3)] for _ in range(k)]) + \
+        np.array([[0.25 * a * (i % 2)] * 3 for i in range(k)])
+    par = O.cce_params(B=B111, dephasing_rate=2.0 if k == 2 else 0.0)
+    nv = P.NVCenter((0, 0, 0), B111)
+    prob = P.cluster_problem([P.Spin(c) for c in coords], B111, nv, dephasing_rate=2.0 if k == 2 else 0.0)
+    for which, key in [(0, "lvA"), (1, "lvB"), (2, "lvBdag")]:
+        tt, rho, D = O.cluster_terms(coords, par, which)
+        ot = P.flatten(prob[key], prob["ldims"])
+        assert len(tt.coeff) == len(ot.coeff)
+        assert np.array_equal(tt.nbody, ot.nbody)
+        assert np.array_equal(tt.pos[:len(ot.pos)], ot.pos[:len(tt.pos)])
+        assert np.allclose(tt.coeff, ot.coeff, rtol=1e-13, atol=1e-9)
+        assert np.abs(tt.mats - ot.mats).max() < 1e-15
+        assert np.abs(rho - prob["rho"]).max() < 1e-15 and D == prob["D"]
+
+
+def test_enumeration_bit_exact_against_oracle():
+    X = O.bath_diamond(3.567, 22.0, 0.011, seed=9)
+    C = O.Clusters(X, 8.0, 4)
+    cl, par = P.enumerate_clusters(P.connection_matrix(X, 8.0), 4)
+    for o in range(4):
+        assert np.array_equal(C.get(o), np.array(cl[o], np.int32).reshape(-1, o + 1))
+        assert C.parents(o) == par[o]
+
+
+def test_enumeration_counts_roycoord():
+    path = os.path.join(ROOT, "tests", "golden", "RoyCoord.xyz")
+    lines = open(path).read().split("\n")
+    n = int(lines[0])
+    X = np.array([[float(v) for v in l.split()[1:4]] for l in lines[1:n + 1]])
+    assert O.Clusters(X, 6.0, 4).counts() == [548, 504, 672, 1107]
+    assert O.Clusters(X, 8.0, 4).counts() == [548, 1179, 3530, 12662]
+
+
+def test_reduction_bit_exact_against_oracle():
+    X = O.bath_diamond(3.567, 18.0, 0.02, seed=4)
+    C = O.Clusters(X, 7.0, 3)
+    cl, par = P.enumerate_clusters(P.connection_matrix(X, 7.0), 3)
+    rng = np.random.default_rng(2)
+    nt = 5
+    L = [rng.uniform(0.7, 1.0, size=(len(c), nt)) for c in cl]
+    tilde, final, each = P.cce_reduce(L, par)
+    r = C.reduce(np.concatenate(L), nt)
+    assert np.array_equal(r["tilde"], np.concatenate(tilde))
+    assert np.array_equal(r["final"], final)
+    assert np.array_equal(r["each"], each)
+    assert np.allclose(np.exp(r["log_abs"]) * r["sign"], final, rtol=1e-12)
+
+
+def test_bath_diamond_properties():
+    a = 3.567
+    X = O.bath_diamond(a, 30.0, 0.011, seed=0)
+    X2 = O.bath_diamond(a, 30.0, 0.011, seed=0)
+    assert np.array_equal(X, X2)
+    assert not np.array_equal(X, O.bath_diamond(a, 30.0, 0.011, seed=1))
+    assert np.all(np.linalg.norm(X, axis=1) <= 30.0 + 1e-12)
+    # every site is a diamond lattice site: 4x/a integer and parity rule
+    q = X * 4.0 / a
+    assert np.allclose(q, np.round(q), atol=1e-9)
+    qi = np.round(q).astype(int) % 4
+    fcc = np.all(qi % 2 == 0, axis=1) & ((qi // 2).sum(axis=1) % 2 == 0)
+    shifted = np.all(qi % 2 == 1, axis=1) & (((qi - 1) // 2).sum(axis=1) % 2 == 0)
+    assert np.all(fcc | shifted)
+    # NV sites excluded; occupancy ~ p
+    assert not np.any(np.all(np.abs(X) < 1e-9, axis=1))
+    nsites = 8 * (4.0 / 3.0) * np.pi * 30.0 ** 3 / a ** 3
+    assert abs(len(X) - 0.011 * nsites) < 5 * np.sqrt(0.011 * nsites)
+
+
+def test_precision_and_arguments_rejected_without_gpu_calls():
+    with pytest.raises(ValueError):
+        O._precision_code("fp16")
