@@ -1,18 +1,26 @@
-"""Benchmark: cluster expv/s of the batched CCE-4 Hahn echo on one MI355X.
+"""Benchmark: cluster expv/s of the batched CCE-4 Hahn echo on one MI355X,
+mixed precision against FP64.
 
 Workload (the batched CCE-4 configuration): a diamond-lattice 13C bath
 (a = 3.567 A, p = 0.011, sphere r = 5 nm, seed 0, NV sites excluded), every
 connected cluster of 1-4 spins with pair cutoff 8 A, B = 0.05 T along [111],
-Hahn echo sampled at 401 points on 0-5 ms, FP64 complex, Krylov m = 30,
-tol = 1e-12.  One step = the two Krylov sweeps of EVERY cluster of that
-enumeration, reduced on the device to each cluster's L_c(t).  Inputs (term
-tables, initial vectors) are uploaded before the timed region.
+Hahn echo sampled at 401 points on 0-5 ms.  One step = the two Krylov sweeps
+of EVERY cluster of that enumeration in the mixed-precision kernel (complex
+FP32 couplings and basis, FP64 orthogonalisation and Hessenberg exponential;
+Krylov dimension and tolerance from oops_amd/tuning.json, written by
+oops_amd/autotune.py), reduced on the device to each cluster's L_c(t).
+Inputs (term tables, initial vectors) are uploaded before the timed region.
 
-The JSON line carries the dominant kernel's roofline (FP64 compute: the
-algorithmic flops of the executed matvecs and Arnoldi steps over the kernel's
-launch time measured with HIP events on its stream) and a CPU baseline (the
-oracle's restatement of the reference CPU path, timed on a bounded sample
-and extrapolated; see the `sample` field).
+Beside it, the FP64 path (m = 30, tol = 1e-12, the reference's settings) runs
+on a stratified sample of the same clusters (every k-th cluster of every
+order): its throughput gives the mixed/FP64 ratio and its L_c(t) the error
+max |L_mixed - L_fp64| over every sampled cluster and time point.
+
+The JSON line carries the dominant kernel's roofline (the algorithmic flops
+of its executed matvecs and Arnoldi steps over its launch time, measured with
+HIP events on its stream) and a CPU baseline (the oracle's restatement of the
+reference CPU path, FP64, timed on a bounded sample and extrapolated; see the
+`sample` field).
 """
 import argparse
 import json
@@ -96,6 +104,42 @@ def cpu_baseline(X, C, par, tlist, args, counts):
     }
 
 
+def stratified(counts, frac):
+    """Every round(1/frac)-th cluster of every order (global indices)."""
+    step = max(1, int(round(1.0 / frac)))
+    sel, base = [], 0
+    for c in counts:
+        sel += list(range(base, base + c, step))
+        base += c
+    return np.array(sel, np.int64)
+
+
+def tuned_mixed():
+    path = os.path.join(ROOT, "oops_amd", "tuning.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            t = json.load(f)
+        return int(t["mixed"]["m"]), float(t["mixed"]["tol"]), "oops_amd/tuning.json"
+    return 30, 1e-4, "default"
+
+
+def roofline(counts, st, timing, m):
+    dom = max(timing, key=lambda b: b["ms"])
+    n_dom = dom["n"]
+    k_dom = int(round(np.log(n_dom) / np.log(4)))
+    base = 0
+    flops = 0.0
+    for o, cnt in enumerate(counts):
+        if 4 ** (o + 1) == n_dom:
+            sl = st[base:base + cnt]
+            flops = krylov_flops(n_dom, k_dom, sl[:, 1].sum() + sl[:, 5].sum(), sl[:, 0].sum() + sl[:, 4].sum(), m)
+        base += cnt
+    achieved = flops / (dom["ms"] * 1e-3) / 1e12
+    return dom, {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                 "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None, "kernel_ms": dom["ms"],
+                 "algorithmic_flops": flops}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,8 +152,12 @@ def main():
     ap.add_argument("--field", type=float, default=0.05)
     ap.add_argument("--t-end", dest="t_end", type=float, default=5e-3)
     ap.add_argument("--nt", type=int, default=401)
-    ap.add_argument("--m", type=int, default=30)
-    ap.add_argument("--tol", type=float, default=1e-12)
+    ap.add_argument("--m", type=int, default=30, help="Krylov dimension of the FP64 path")
+    ap.add_argument("--tol", type=float, default=1e-12, help="tolerance of the FP64 path")
+    ap.add_argument("--mixed-m", dest="mixed_m", type=int, default=None)
+    ap.add_argument("--mixed-tol", dest="mixed_tol", type=float, default=None)
+    ap.add_argument("--fp64-frac", dest="fp64_frac", type=float, default=0.25,
+                    help="fraction of every order's clusters run on the FP64 path")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-per", dest="cpu_per", type=int, default=None)
     args = ap.parse_args()
@@ -119,18 +167,24 @@ def main():
         sys.exit(2)
 
     import oops_amd as O
+    mm, mtol, msrc = tuned_mixed()
+    mm = args.mixed_m or mm
+    mtol = args.mixed_tol or mtol
     ctx = O.Context(0)
     X, C, par, tlist = build_workload(args)
     counts = C.counts()
     nclu = sum(counts)
     tau = tlist / 2.0
     t0 = time.perf_counter()
-    batch = C.hahn_batch(ctx, par, m=args.m, tol=args.tol)
+    batch = C.hahn_batch(ctx, par, m=mm, tol=mtol, precision="mixed")
+    sel = stratified(counts, args.fp64_frac)
+    batch64 = C.hahn_batch(ctx, par, m=args.m, tol=args.tol, precision="fp64", select=sel)
     setup_s = time.perf_counter() - t0
-    print(f"# bath {len(X)} spins, clusters per order {counts} ({nclu}), setup {setup_s:.2f}s", file=sys.stderr,
-          flush=True)
+    print(f"# bath {len(X)} spins, clusters per order {counts} ({nclu}), fp64 sample {len(sel)}, "
+          f"setup {setup_s:.2f}s", file=sys.stderr, flush=True)
     # prime code objects on a small batch (not a step)
     Cs = O.Clusters(X[:40], args.cutoff, 2)
+    Cs.hahn_batch(ctx, par, m=mm, tol=mtol, precision="mixed").run(tau[:3])
     Cs.hahn_batch(ctx, par, m=args.m, tol=args.tol).run(tau[:3])
     for w in range(args.warmup):
         batch.run(tau)
@@ -142,24 +196,26 @@ def main():
     elapsed = time.perf_counter() - t0
     L, st = batch.results()
     timing = batch.timing()
-    fails = int((st[:, 8] & 0xFF).max()) if len(st) else 0
+    value = nclu * args.steps / elapsed
 
-    # dominant kernel: the longest bucket launch
-    dom = max(timing, key=lambda b: b["ms"])
-    n_dom = dom["n"]
-    k_dom = int(round(np.log(n_dom) / np.log(4)))
-    base = 0
-    flops = 0.0
-    for o, cnt in enumerate(counts):
-        if 4 ** (o + 1) == n_dom:
-            sl = st[base:base + cnt]
-            flops = krylov_flops(n_dom, k_dom, sl[:, 1].sum() + sl[:, 5].sum(), sl[:, 0].sum() + sl[:, 4].sum(),
-                                 args.m)
-        base += cnt
-    achieved = flops / (dom["ms"] * 1e-3) / 1e12
+    # FP64 path on the stratified sample (same clusters, same times)
+    t1 = time.perf_counter()
+    batch64.run(tau)
+    el64 = time.perf_counter() - t1
+    L64, st64 = batch64.results()
+    rate64 = len(sel) / el64
+    print(f"# fp64 sample {len(sel)} clusters in {el64:.2f}s", file=sys.stderr, flush=True)
+    dL = np.abs(L[sel] - L64)
+    order_of = np.repeat(np.arange(len(counts)), counts)[sel]
+    err_per_order = [float(dL[order_of == o].max()) if np.any(order_of == o) else 0.0 for o in range(len(counts))]
+
+    dom, roof = roofline(counts, st, timing, mm)
+    roof["kernel"] = f"cce_hahn_krylov_kernel<{mm}, mixed> n={dom['n']}"
+    dom64, roof64 = roofline([len(np.nonzero(order_of == o)[0]) for o in range(len(counts))], st64,
+                             batch64.timing(), args.m)
     out = {
-        "metric": "cluster expv/sec at CCE-4 (FP64 Hahn echo, m=30, tol=1e-12)",
-        "value": nclu * args.steps / elapsed,
+        "metric": "cluster expv/sec at CCE-4 (mixed vs FP64, |\u0394L|\u22641e-6); p99 request latency",
+        "value": value,
         "unit": "cluster expv/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -168,20 +224,24 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": "batched CCE-4 Hahn echo, diamond 13C bath r=5nm p=0.011 seed 0, cutoff 8A, "
                                "B=0.05T||[111], 401 points on 0-5ms",
-                   "clusters": nclu, "clusters_per_order": counts, "spins": int(len(X)), "m": args.m,
-                   "tol": args.tol, "buckets": timing, "failed_clusters": int((st[:, 8] != 0).sum()),
-                   "fail_flags": fails},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": f"cce_hahn_krylov_kernel<{args.m}> n={n_dom}",
-                     "kernel_ms": dom["ms"], "algorithmic_flops": flops},
+                   "clusters": nclu, "clusters_per_order": counts, "spins": int(len(X)),
+                   "mixed": {"m": mm, "tol": mtol, "source": msrc, "buckets": timing,
+                             "failed_clusters": int((st[:, 8] != 0).sum())},
+                   "fp64": {"m": args.m, "tol": args.tol, "sample_clusters": int(len(sel)),
+                            "value": rate64, "seconds": el64, "buckets": batch64.timing(),
+                            "failed_clusters": int((st64[:, 8] != 0).sum()),
+                            "roofline_frac": roof64["frac"], "roofline_achieved": roof64["achieved"]},
+                   "mixed_over_fp64": value / rate64,
+                   "max_abs_dL": float(dL.max()), "max_abs_dL_per_order": err_per_order},
+        "roofline": roof,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(X, C, par, tlist, args, counts)
+        out["config"]["mixed_over_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 
 

@@ -85,10 +85,13 @@ int oops_clusters_destroy(oops_clusters* C);
 int oops_cce_reduce(const oops_clusters* C, int32_t nt, const double* L, double* tilde, double* final_result,
                     double* each_order, double* log_abs_final, int32_t* sign_final);
 
-/* Build every cluster's Hahn-echo pair of sweeps and create the device batch
- * (items ordered as the clusters: order 0 first, set order within an order). */
+/* Build the Hahn-echo pair of sweeps of every selected cluster and create the
+ * device batch.  Clusters are numbered globally, order 0 first and set order
+ * within an order; sel (nsel ascending global indices) picks a subset, or
+ * NULL for every cluster.  Items keep that order. */
 int oops_cce_hahn_batch_create(oops_ctx* ctx, const double* coords, const oops_clusters* C, const oops_cce_params* P,
-                               int32_t m, int32_t klim, double tol, int32_t precision, oops_hahn_batch** out);
+                               int64_t nsel, const int64_t* sel, int32_t m, int32_t klim, double tol,
+                               int32_t precision, oops_hahn_batch** out);
 
 #ifdef __cplusplus
 }

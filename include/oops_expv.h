@@ -117,9 +117,12 @@ typedef struct {
  * upload everything to device memory.  m: Krylov dimension (20 or 30);
  * clusters with n <= m use the dense exponential.  klim: Lanczos steps of the
  * norm estimate (the reference default is 10), tol: Expokit tolerance.
- * precision 0: complex FP64 throughout; 1: mixed (2-body couplings applied in
- * complex FP32, 1-body terms, basis, orthogonalisation and the small
- * exponential in FP64). */
+ * precision 0: complex FP64 throughout (the reference's Arnoldi + Pade).
+ * 1: mixed — 2-body couplings applied in complex FP32 and the basis stored in
+ * FP32; 1-body terms, the recurrence coefficients, re-orthogonalisation and
+ * the small exponential in FP64.  Buckets whose generators are all Hermitian
+ * run Lanczos with an eigendecomposition of the tridiagonal matrix, the
+ * others Arnoldi + Pade.  2: mixed, always Arnoldi + Pade. */
 int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* items, int32_t m, int32_t klim,
                            double tol, int32_t precision, oops_hahn_batch** out);
 /* Run every bucket for the ascending sweep times tau[0..nt) (device-resident

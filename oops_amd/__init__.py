@@ -96,7 +96,7 @@ def _ptr(a):
 
 
 def _precision_code(p):
-    codes = {"fp64": 0, "mixed": 1}
+    codes = {"fp64": 0, "mixed": 1, "mixed-arnoldi": 2}
     if p not in codes:
         raise ValueError(f"precision must be one of {sorted(codes)}")
     return codes[p]
@@ -385,13 +385,18 @@ class Clusters:
                                      _ptr(la), _ptr(sg)), None, "oops_cce_reduce")
         return {"tilde": til, "final": fin, "each": each, "log_abs": la, "sign": sg}
 
-    def hahn_batch(self, ctx: Context, params: CCEParamsC, m=30, klim=10, tol=1e-12, precision="fp64"):
+    def hahn_batch(self, ctx: Context, params: CCEParamsC, m=30, klim=10, tol=1e-12, precision="fp64", select=None):
+        """Device batch of the Hahn-echo sweeps of every cluster, or of the
+        ascending global cluster indices in `select`."""
         h = ctypes.c_void_p()
+        sel = None if select is None else np.ascontiguousarray(select, np.int64)
+        nsel = 0 if sel is None else len(sel)
         rc = lib().oops_cce_hahn_batch_create(ctx.h, _ptr(self.coords), self.h, ctypes.byref(params),
-                                              ctypes.c_int32(m), ctypes.c_int32(klim), ctypes.c_double(tol),
+                                              ctypes.c_int64(nsel), _ptr(sel), ctypes.c_int32(m),
+                                              ctypes.c_int32(klim), ctypes.c_double(tol),
                                               ctypes.c_int32(_precision_code(precision)), ctypes.byref(h))
         _check(rc, ctx.h, "oops_cce_hahn_batch_create")
-        return HahnBatch(ctx, h, sum(self.counts()))
+        return HahnBatch(ctx, h, sum(self.counts()) if sel is None else nsel)
 
     def close(self):
         if self.h:

@@ -1,0 +1,101 @@
+"""Autotuner of the mixed-precision batched Hahn-echo path.
+
+Picks the Krylov dimension m and the Expokit tolerance of the mixed kernel
+(complex FP32 couplings and basis, FP64 orthogonalisation and Hessenberg
+exponential) that give the highest cluster throughput while every L_c(t) of
+a tuning sample stays within budget/safety of the FP64 path (m = 30,
+tol = 1e-12, the reference's settings).  The tuning sample is a CCE-4
+enumeration of a bath drawn from a different seed than the benchmark's, so
+the benchmark checks the choice on data it was not tuned on.
+
+    python -m oops_amd.autotune [--per-order 128] [--out oops_amd/tuning.json]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def sample(counts, per):
+    sel, base = [], 0
+    for c in counts:
+        k = min(per, c)
+        sel += list(base + np.linspace(0, c - 1, k).astype(np.int64))
+        base += c
+    return np.array(sorted(set(sel)), np.int64)
+
+
+def kernel_seconds(batch):
+    return sum(b["ms"] for b in batch.timing()) * 1e-3
+
+
+def main(argv=None):
+    import oops_amd as O
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--radius", type=float, default=50.0)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cutoff", type=float, default=8.0)
+    ap.add_argument("--order", type=int, default=4)
+    ap.add_argument("--field", type=float, default=0.05)
+    ap.add_argument("--t-end", dest="t_end", type=float, default=5e-3)
+    ap.add_argument("--nt", type=int, default=401)
+    ap.add_argument("--per-order", dest="per", type=int, default=128)
+    ap.add_argument("--budget", type=float, default=1e-6, help="max |L_mixed - L_fp64| allowed")
+    ap.add_argument("--safety", type=float, default=4.0)
+    ap.add_argument("--ms", default="20,30")
+    ap.add_argument("--tols", default="1e-3,3e-4,1e-4,3e-5,1e-5,1e-6")
+    ap.add_argument("--out", default=os.path.join(HERE, "tuning.json"))
+    a = ap.parse_args(argv)
+
+    ctx = O.Context(0)
+    X = O.bath_diamond(3.567, a.radius, 0.011, seed=a.seed)
+    C = O.Clusters(X, a.cutoff, a.order)
+    par = O.cce_params(B=np.ones(3) * a.field / np.sqrt(3.0))
+    tau = np.linspace(0.0, a.t_end, a.nt) / 2.0
+    counts = C.counts()
+    sel = sample(counts, a.per)
+    print(f"# tuning sample: {len(sel)} clusters of {counts} (seed {a.seed})", flush=True)
+    ref = C.hahn_batch(ctx, par, m=30, tol=1e-12, precision="fp64", select=sel)
+    ref.run(tau)
+    L64, st64 = ref.results()
+    t64 = kernel_seconds(ref)
+    if (st64[:, 8] != 0).any():
+        raise RuntimeError("FP64 reference run reported failures")
+    print(f"# fp64: {t64:.3f} s kernel time", flush=True)
+    rows = []
+    limit = a.budget / a.safety
+    for m in [int(x) for x in a.ms.split(",")]:
+        for tol in [float(x) for x in a.tols.split(",")]:
+            b = C.hahn_batch(ctx, par, m=m, tol=tol, precision="mixed", select=sel)
+            b.run(tau)
+            L, st = b.results()
+            t = kernel_seconds(b)
+            err = float(np.abs(L - L64).max())
+            ok = bool(err <= limit and not (st[:, 8] != 0).any())
+            rows.append({"m": m, "tol": tol, "kernel_s": t, "max_abs_dL": err, "speedup_vs_fp64": t64 / t,
+                         "accepted": ok})
+            print(f"# m={m} tol={tol:g}: {t:.3f} s, max|dL| {err:.2e}, x{t64 / t:.2f} {'ok' if ok else 'rejected'}",
+                  flush=True)
+            b.close()
+    good = [r for r in rows if r["accepted"]]
+    if not good:
+        raise RuntimeError("no candidate met the error budget")
+    best = min(good, key=lambda r: r["kernel_s"])
+    out = {"mixed": {"m": best["m"], "tol": best["tol"]},
+           "criterion": f"fastest with max|L_mixed - L_fp64| <= {a.budget:g}/{a.safety:g} on every cluster and time",
+           "sample": {"radius": a.radius, "seed": a.seed, "cutoff": a.cutoff, "order": a.order,
+                      "clusters": int(len(sel)), "nt": a.nt, "t_end": a.t_end},
+           "fp64_kernel_s": t64, "device": ctx.info(), "candidates": rows}
+    with open(a.out, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out["mixed"]), flush=True)
+    return out
+
+
+if __name__ == "__main__":
+    sys.exit(0 if main() else 1)

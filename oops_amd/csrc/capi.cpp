@@ -129,6 +129,20 @@ int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err
         if (!single[s].empty()) add_local(1, s, s, single[s], d);
     out.ent2_first = (int32_t)out.val.size();
     for (auto& kv : pair) add_local(2, kv.first.first, kv.first.second, kv.second, d * d);
+    // Hermitian check of every merged local operator (sufficient for H = H^dagger)
+    auto is_herm = [](const std::vector<cplx>& M, int64_t rows) {
+        double mx = 0.0;
+        for (const auto& z : M) mx = std::max(mx, std::abs(z));
+        for (int64_t r = 0; r < rows; ++r)
+            for (int64_t c = 0; c <= r; ++c)
+                if (std::abs(M[r + rows * c] - std::conj(M[c + rows * r])) > 1e-13 * mx) return false;
+        return true;
+    };
+    out.hermitian = c0.imag() == 0.0;
+    for (int s = 0; s < ns && out.hermitian; ++s)
+        if (!single[s].empty() && !is_herm(single[s], d)) out.hermitian = false;
+    for (auto& kv : pair)
+        if (out.hermitian && !is_herm(kv.second, d * d)) out.hermitian = false;
     if ((int)out.locs.size() > max_locals) { err = "too many local operators"; return OOPS_ERR_CAPACITY; }
     if (out.max_row > 16) { err = "local operator rows wider than 16 entries"; return OOPS_ERR_UNSUPPORTED; }
     return OOPS_OK;
@@ -173,6 +187,7 @@ struct oops_hahn_batch {
         std::vector<int32_t> items;  // global item indices
         DevBuf hdr, loc, val, off, opA, opB, voff, u0, y0, scale, rnd, L, stats, queue;
         int grid = 0;
+        bool hermitian = true;  // every generator Hermitian: mixed precision may use Lanczos
         double ms = 0.0;
     };
     std::vector<Bucket> buckets;
@@ -231,7 +246,8 @@ int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* 
                            double tol, int32_t precision, oops_hahn_batch** out) {
     if (!ctx || !out || nitems < 0 || (nitems > 0 && !items)) return set_err(ctx, OOPS_ERR_ARG, "bad arguments");
     *out = nullptr;
-    if (precision != 0 && precision != 1) return set_err(ctx, OOPS_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed)");
+    if (precision < 0 || precision > 2)
+        return set_err(ctx, OOPS_ERR_ARG, "precision must be 0 (FP64), 1 (mixed) or 2 (mixed, Arnoldi)");
     if (m != 20 && m != 30) return set_err(ctx, OOPS_ERR_UNSUPPORTED, "Krylov dimension must be 20 or 30");
     if (klim < 1 || klim > 16) return set_err(ctx, OOPS_ERR_ARG, "klim must be in [1, 16]");
     std::unique_ptr<oops_hahn_batch> b(new oops_hahn_batch());
@@ -284,6 +300,7 @@ int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* 
         }
         Host& H = hosts[bk];
         b->buckets[bk].items.push_back(it);
+        b->buckets[bk].hermitian = b->buckets[bk].hermitian && A.hermitian && B.hermitian;
         for (int s = 0; s < 2; ++s) {
             const CompiledOp& C = s == 0 ? A : B;
             DevOpHeader h;
@@ -384,7 +401,9 @@ int oops_hahn_batch_run(oops_hahn_batch* b, int32_t nt, const double* tau) {
         a.max_steps = 50000000;
         a.tol = b->tol;
         hipEventRecord(ev[2 * bk], ctx->stream);
-        const int lrc = launch_hahn_bucket(a, b->m, b->precision, B.grid, ctx->stream);
+        const bool lanczos = b->precision == 1 && B.hermitian && B.n > b->m;
+        const int lrc = lanczos ? (launch_zero_counter(a.queue, ctx->stream), launch_hahn_lanczos(a, b->m, B.grid, ctx->stream))
+                                : launch_hahn_bucket(a, b->m, b->precision == 0 ? 0 : 1, B.grid, ctx->stream);
         hipEventRecord(ev[2 * bk + 1], ctx->stream);
         if (lrc != 0) {
             rc = set_err(ctx, lrc < 0 ? OOPS_ERR_UNSUPPORTED : OOPS_ERR_HIP,

@@ -698,12 +698,19 @@ int oops_cce_reduce(const oops_clusters* C, int32_t nt, const double* L, double*
 // Build every cluster's Hahn-echo problem (orders in sequence, clusters in
 // set order) and hand the batch to the device runtime.
 int oops_cce_hahn_batch_create(oops_ctx* ctx, const double* coords, const oops_clusters* C, const oops_cce_params* P,
-                               int32_t m, int32_t klim, double tol, int32_t precision, oops_hahn_batch** out) {
-    if (!ctx || !coords || !C || !P || !out) return OOPS_ERR_ARG;
+                               int64_t nsel, const int64_t* sel, int32_t m, int32_t klim, double tol,
+                               int32_t precision, oops_hahn_batch** out) {
+    if (!ctx || !coords || !C || !P || !out || nsel < 0 || (nsel > 0 && !sel)) return OOPS_ERR_ARG;
     NV nv = make_nv(P->nv_pos, P->B);
     std::vector<ClusterProblem> probs;
+    int64_t g = 0, next = 0;  // global cluster index, next selection
     for (int o = 0; o < C->max_order; ++o)
         for (const auto& cl : C->idx[o]) {
+            const int64_t gi = g++;
+            if (sel) {
+                if (next >= nsel || sel[next] != gi) continue;
+                ++next;
+            }
             std::vector<SpinH> spins(cl.size());
             for (size_t i = 0; i < cl.size(); ++i) {
                 for (int c = 0; c < 3; ++c) spins[i].r[c] = coords[3 * cl[i] + c];
@@ -711,6 +718,7 @@ int oops_cce_hahn_batch_create(oops_ctx* ctx, const double* coords, const oops_c
             }
             probs.push_back(build_problem(spins, *P, nv));
         }
+    if (sel && next != nsel) return oops::set_err(ctx, OOPS_ERR_ARG, "selection must be ascending cluster indices");
     std::vector<oops_term_table> views(2 * probs.size());
     std::vector<oops_hahn_item> items(probs.size());
     for (size_t i = 0; i < probs.size(); ++i) {

@@ -28,12 +28,9 @@
 
 #include <utility>
 
-#include "devcommon.h"
-#include "oops_kernels.h"
+#include "krylov_common.h"
 
 namespace oops {
-
-constexpr int kNT = 256;
 
 template <int M>
 struct KrylovLds {
@@ -43,35 +40,6 @@ struct KrylovLds {
     static constexpr int EMAX = kKrylovMaxEnt;  // ELL entries per operator
 };
 
-struct IntegState {
-    cd w;
-    double beta, t_now, t_new, anorm, rndoff, xm;
-    int nstep, nmult, nrej, nexph, nscale, brk, fail, n;
-#ifdef OOPS_PHASE_TIMERS
-    unsigned long long tmv, tred, texp, tupd;
-#endif
-};
-
-#ifdef OOPS_PHASE_TIMERS
-#define OOPS_TIC(v) const unsigned long long v = clock64()
-#define OOPS_TOC(g, field, v) (g).field += clock64() - (v)
-#else
-#define OOPS_TIC(v)
-#define OOPS_TOC(g, field, v)
-#endif
-
-// One operator in LDS (ELL tables of the merged local operators).
-struct OpLds {
-    DevOpHeader hdr;
-    DevOpLocal locs[kMaxLocals];
-    alignas(16) cd vals[kKrylovMaxEnt];
-    alignas(16) int32_t offs[kKrylovMaxEnt];
-};
-
-// LDS of one workgroup (< 80 KB so two workgroups share a CU): the operator
-// tables and the product input are only needed during the Arnoldi phase and
-// the Pade workspace only during the small exponential, so they share one
-// region; the operator is re-read from L2 at the start of every Krylov step.
 template <int M>
 struct KrylovSmem {
     cd hess[KrylovLds<M>::MAT];
@@ -90,153 +58,6 @@ struct KrylovSmem {
         } a;
         cd pw[3][KrylovLds<M>::MAT];
     } u;
-};
-
-// Global operator tables of a bucket.
-struct OpSrc {
-    const DevOpHeader* hdr;
-    const DevOpLocal* loc;
-    const double2* val;
-    const int32_t* off;
-};
-
-// Expokit's step rounding to two significant digits (+inf stays +inf).
-__device__ __forceinline__ double round_step2(double t) {
-    if (!(t > 0.0) || isinf(t)) return __builtin_huge_val();
-    const double sqr1 = 0.31622776601683794;  // sqrt(0.1)
-    const double x = log10(t) - sqr1;
-    const int k = (int)(x >= 0.0 ? floor(x + 0.5) : -floor(-x + 0.5)) - 1;
-    double p = 1.0;
-    const int a = k < 0 ? -k : k;
-    for (int i = 0; i < a; ++i) p *= 10.0;
-    if (k < 0) p = 1.0 / p;
-    return trunc(t / p + 0.55) * p;
-}
-
-// Stage x_i for the product (FP64, plus an FP32 copy on the mixed path).
-template <bool MIXED>
-__device__ __forceinline__ void stage_x(cd* xs, float2* xs32, int i, int n, cd x) {
-    if (i < n) {
-        xs[i] = x;
-        if (MIXED) xs32[i] = make_float2((float)x.re, (float)x.im);
-    }
-}
-
-// y_i = H x (x staged in LDS by the caller), zero for i >= n.
-template <bool MIXED>
-__device__ __forceinline__ cd matvec_elem(const OpLds& o, const cd* xs, const float2* xs32, int i) {
-    if (i >= o.hdr.n) return mkcd(0.0, 0.0);
-    if (MIXED) return op_apply_elem_mixed(i, xs, xs32, o.hdr, o.locs, o.vals, o.offs);
-    return op_apply_elem(i, xs, o.hdr, o.locs, o.vals, o.offs);
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int mask = 32; mask >= 1; mask >>= 1) v = fmax(v, __shfl_xor(v, mask, kWave));
-    return v;
-}
-
-__device__ __forceinline__ cd readlane_cd(cd x, int c) {
-    return mkcd(lane_val(x.re, c), lane_val(x.im, c));
-}
-
-// ----------------------------------------------------------------------------
-// Norm estimate: k Lanczos steps on +H from the DLARAN start vector, then
-// power iteration on the k x k tridiagonal (reference :210-265).
-template <int M, bool MIXED>
-__device__ double lanczos_anorm(KrylovSmem<M>& S, BlockRed<kNT>& br, const OpLds& o, const double* rnd, int klim,
-                                double tol) {
-    const int tid = threadIdx.x;
-    const int n = o.hdr.n;
-    int k = klim;
-    if (k > n) k = n;
-    double* tdiag = S.tdiag;
-    double* toff = S.toff;
-    for (int q2 = tid; q2 < 16; q2 += kNT) { tdiag[q2] = 0.0; toff[q2] = 0.0; }
-    cd q = tid < n ? mkcd(rnd[2 * tid], rnd[2 * tid + 1]) : mkcd(0.0, 0.0);
-    cd qm = mkcd(0.0, 0.0);
-    {
-        const auto t0 = block_sum1<1, kNT>(br, [&](int) { return cabs2(q); });
-        q = cscale(q, 1.0 / sqrt(t0.get(0)));
-    }
-    double bjm = 0.0, bj = 0.0;
-    for (int j = 1; j <= k; ++j) {
-        stage_x<MIXED>(S.u.a.xs, S.u.a.xs32, tid, n, q);
-        __syncthreads();
-        cd w = matvec_elem<MIXED>(o, S.u.a.xs, S.u.a.xs32, tid);
-        const auto ta = block_sum1<2, kNT>(br, [&](int c) {
-            return c == 0 ? fma(q.re, w.re, q.im * w.im) : fma(q.re, w.im, -q.im * w.re);
-        });
-        const cd aj = mkcd(ta.get(0), ta.get(1));
-        w = cfms(aj, q, w);
-        w = mkcd(w.re - bjm * qm.re, w.im - bjm * qm.im);
-        const auto tb = block_sum1<1, kNT>(br, [&](int) { return cabs2(w); });
-        bj = sqrt(tb.get(0));
-        if (tid == 0) tdiag[j - 1] = aj.re;
-        if (j == k) break;
-        if (tid == 0) toff[j - 1] = bj;
-        if (bj < tol) break;
-        qm = q;
-        q = cscale(w, 1.0 / bj);
-        bjm = bj;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        // power iteration on T (k x k symmetric tridiagonal; entries not set stay 0)
-        double qr[16], wr[16];
-        for (int j = 0; j < k; ++j) qr[j] = rnd[2 * n + j];
-        double cjm = 0.0, cj = 0.0;
-        for (int it = 0; it < 10000; ++it) {
-            for (int i = 0; i < k; ++i) {
-                double s = tdiag[i] * qr[i];
-                if (i > 0) s += toff[i - 1] * qr[i - 1];
-                if (i + 1 < k) s += toff[i] * qr[i + 1];
-                wr[i] = s;
-            }
-            cj = 0.0;
-            for (int i = 0; i < k; ++i) cj += qr[i] * wr[i];
-            if (fabs(1.0 - cjm / cj) < tol) break;
-            cjm = cj;
-            double nr = 0.0;
-            for (int i = 0; i < k; ++i) nr += wr[i] * wr[i];
-            cj = sqrt(nr);
-            for (int i = 0; i < k; ++i) qr[i] = wr[i] / cj;
-        }
-        S.scal[0] = fabs(cj) + fabs(bj * qr[k - 1]);
-    }
-    __syncthreads();
-    const double an = S.scal[0];
-    __syncthreads();
-    return an;
-}
-
-// ----------------------------------------------------------------------------
-// Register-resident Krylov basis of one state element.  FP64 path: all m+1
-// vectors in FP64.  Mixed path: v0 = w/beta in FP64 and v1..vm stored in
-// FP32 (half the registers); every product with them accumulates in FP64.
-template <int M, bool B32>
-struct Basis;
-
-template <int M>
-struct Basis<M, false> {
-    cd v[M + 1];
-    __device__ __forceinline__ cd get(int l) const { return v[l]; }
-    __device__ __forceinline__ void set(int l, cd x) { v[l] = x; }
-};
-
-template <int M>
-struct Basis<M, true> {
-    cd v0;
-    float2 v[M];
-    __device__ __forceinline__ cd get(int l) const {
-        if (l == 0) return v0;
-        const float2 f = v[l > 0 ? l - 1 : 0];
-        return mkcd((double)f.x, (double)f.y);
-    }
-    __device__ __forceinline__ void set(int l, cd x) {
-        if (l == 0) v0 = x;
-        else v[l > 0 ? l - 1 : 0] = make_float2((float)x.re, (float)x.im);
-    }
 };
 
 // ----------------------------------------------------------------------------
@@ -408,45 +229,6 @@ __device__ __forceinline__ void advance(KrylovSmem<M>& S, BlockRed<kNT>& br, Int
 }
 
 template <int M, bool MIXED>
-__device__ void init_integ(KrylovSmem<M>& S, BlockRed<kNT>& br, IntegState& g, cd w0, const double* rnd, int klim,
-                           double tol, const OpSrc& src, int op, int m) {
-    const int tid = threadIdx.x;
-    __syncthreads();
-    op_load_lds<kNT, MIXED>(src, op, S.u.a.op);
-    __syncthreads();
-    const OpLds& o = S.u.a.op;
-    g.n = o.hdr.n;
-    g.w = tid < o.hdr.n ? w0 : mkcd(0.0, 0.0);
-    g.anorm = lanczos_anorm<M, MIXED>(S, br, o, rnd, klim, tol);
-    const auto tb = block_sum1<1, kNT>(br, [&](int) { return cabs2(g.w); });
-    g.beta = sqrt(tb.get(0));
-    g.t_now = 0.0;
-    g.xm = 1.0 / (double)m;
-    g.rndoff = 2.220446049250313e-16 * g.anorm;
-    const double p2 = tol * pow((m + 1) / 2.72, (double)(m + 1)) * sqrt(2.0 * 3.14 * (m + 1));
-    g.t_new = (1.0 / g.anorm) * pow(p2 / (4.0 * g.beta * g.anorm), g.xm);
-    g.t_new = round_step2(g.t_new);
-    g.nstep = g.nmult = g.nrej = g.nexph = g.nscale = g.brk = g.fail = 0;
-#ifdef OOPS_PHASE_TIMERS
-    g.tmv = g.tred = g.texp = g.tupd = 0;
-#endif
-}
-
-template <int NT, bool MIXED>
-__device__ __forceinline__ void op_load_lds(const OpSrc& a, int op, OpLds& o) {
-    o.hdr = a.hdr[op];
-    const DevOpHeader h = a.hdr[op];
-    for (int k = threadIdx.x; k < h.nloc; k += NT) o.locs[k] = a.loc[h.loc_off + k];
-    float2* v32 = reinterpret_cast<float2*>(o.vals + h.ent2_first);
-    for (int k = threadIdx.x; k < h.nent; k += NT) {
-        const double2 v = a.val[h.ent_off + k];
-        if (MIXED && k >= h.ent2_first) v32[k - h.ent2_first] = make_float2((float)v.x, (float)v.y);
-        else o.vals[k] = mkcd(v.x, v.y);
-        o.offs[k] = a.off[h.ent_off + k];
-    }
-}
-
-template <int M, bool MIXED>
 __global__ __launch_bounds__(kNT) void cce_hahn_krylov_kernel(KrylovArgs a) {
     __shared__ KrylovSmem<M> S;
     const int tid = threadIdx.x;
@@ -472,7 +254,11 @@ __global__ __launch_bounds__(kNT) void cce_hahn_krylov_kernel(KrylovArgs a) {
         __syncthreads();
         for (int s = 0; s < 2; ++s) {
             IntegState g;
-            init_integ<M, MIXED>(S, br, g, s ? y0 : u0, rnd, klim, tol, src, s ? ops[1] : ops[0], M);
+            __syncthreads();
+            op_load_lds<kNT, MIXED>(src, s ? ops[1] : ops[0], S.u.a.op);
+            __syncthreads();
+            const NormWork nw{S.u.a.xs, S.u.a.xs32, S.tdiag, S.toff, S.scal};
+            init_integ<MIXED>(nw, S.u.a.op, br, g, s ? y0 : u0, rnd, klim, tol, M);
             if (s) gB = g; else gA = g;
         }
         const double scale = a.scale[item];
@@ -604,6 +390,10 @@ __global__ __launch_bounds__(kDenseNT) void cce_hahn_dense_kernel(KrylovArgs a) 
 
 // ----------------------------------------------------------------------------
 __global__ void zero_counter_kernel(int32_t* p) { p[threadIdx.x] = 0; }
+
+void launch_zero_counter(int32_t* counter, hipStream_t stream) {
+    hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
+}
 
 int launch_hahn_bucket(const KrylovArgs& a, int m, int precision, int grid, hipStream_t stream) {
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, stream, a.queue);

@@ -28,6 +28,7 @@ struct CompiledOp {
     int ld_bits = 0;
     int max_row = 0;
     int32_t ent2_first = 0;
+    bool hermitian = false;  // every merged local operator is Hermitian
     std::complex<double> c0{0.0, 0.0};
     std::vector<DevOpLocal> locs;
     std::vector<std::complex<double>> val;

@@ -40,6 +40,10 @@ struct KrylovArgs {
 };
 
 int launch_hahn_bucket(const KrylovArgs& a, int m, int precision, int grid, hipStream_t stream);
+// Mixed-precision Lanczos kernel for buckets whose generators are all Hermitian
+// (n > m); the caller zeroes the work counter first.
+int launch_hahn_lanczos(const KrylovArgs& a, int m, int grid, hipStream_t stream);
+void launch_zero_counter(int32_t* counter, hipStream_t stream);
 int krylov_entry_capacity(int m);
 
 }  // namespace oops
