@@ -46,6 +46,17 @@ def krylov_flops(n, k, nmult, nstep, m):
     return nmult * mv + nstep * 8.0 * n * m * (m + 1)
 
 
+def lanczos_flops(n, k, nmult, nstep, m):
+    """Algorithmic flops of the mixed Lanczos kernel: merged matvec per
+    product + per Lanczos step one fused <v_j, p>, |p|^2 reduction and the
+    three-term recurrence update (4 complex dot/axpy passes, 32 n) + the
+    (m+1)-term combination of the update (8 n (m+1))."""
+    s = 4
+    P = k * (k - 1) // 2
+    mv = 8.0 * n * (k * s + P * s * s)
+    return nmult * mv + nstep * (32.0 * n * m + 8.0 * n * (m + 1))
+
+
 def build_workload(args):
     import oops_amd as O
     X = O.bath_diamond(3.567, args.radius, 0.011, seed=args.seed, shape="sphere")
@@ -114,6 +125,21 @@ def stratified(counts, frac):
     return np.array(sel, np.int64)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None when no summary covers it."""
+    import re
+    path = os.path.join(ROOT, "profiles", "cce_hahn_lanczos_cce4_r5nm_pmc.md")
+    if not os.path.exists(path):
+        return None
+    txt = open(path).read()
+    if kernel.split(" ")[0] not in txt:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+-]+)", txt)
+    return float(m.group(1)) if m else None
+
+
 def tuned_mixed():
     path = os.path.join(ROOT, "oops_amd", "tuning.json")
     if os.path.exists(path):
@@ -123,7 +149,7 @@ def tuned_mixed():
     return 30, 1e-4, "default"
 
 
-def roofline(counts, st, timing, m):
+def roofline(counts, st, timing, m, flops_fn=krylov_flops):
     dom = max(timing, key=lambda b: b["ms"])
     n_dom = dom["n"]
     k_dom = int(round(np.log(n_dom) / np.log(4)))
@@ -132,7 +158,7 @@ def roofline(counts, st, timing, m):
     for o, cnt in enumerate(counts):
         if 4 ** (o + 1) == n_dom:
             sl = st[base:base + cnt]
-            flops = krylov_flops(n_dom, k_dom, sl[:, 1].sum() + sl[:, 5].sum(), sl[:, 0].sum() + sl[:, 4].sum(), m)
+            flops = flops_fn(n_dom, k_dom, sl[:, 1].sum() + sl[:, 5].sum(), sl[:, 0].sum() + sl[:, 4].sum(), m)
         base += cnt
     achieved = flops / (dom["ms"] * 1e-3) / 1e12
     return dom, {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
@@ -159,6 +185,8 @@ def main():
     ap.add_argument("--fp64-frac", dest="fp64_frac", type=float, default=0.25,
                     help="fraction of every order's clusters run on the FP64 path")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", dest="no_fp64", action="store_true",
+                    help="skip the FP64 comparison (profiling runs)")
     ap.add_argument("--cpu-per", dest="cpu_per", type=int, default=None)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -178,14 +206,16 @@ def main():
     t0 = time.perf_counter()
     batch = C.hahn_batch(ctx, par, m=mm, tol=mtol, precision="mixed")
     sel = stratified(counts, args.fp64_frac)
-    batch64 = C.hahn_batch(ctx, par, m=args.m, tol=args.tol, precision="fp64", select=sel)
+    batch64 = None if args.no_fp64 else C.hahn_batch(ctx, par, m=args.m, tol=args.tol, precision="fp64",
+                                                        select=sel)
     setup_s = time.perf_counter() - t0
     print(f"# bath {len(X)} spins, clusters per order {counts} ({nclu}), fp64 sample {len(sel)}, "
           f"setup {setup_s:.2f}s", file=sys.stderr, flush=True)
     # prime code objects on a small batch (not a step)
     Cs = O.Clusters(X[:40], args.cutoff, 2)
     Cs.hahn_batch(ctx, par, m=mm, tol=mtol, precision="mixed").run(tau[:3])
-    Cs.hahn_batch(ctx, par, m=args.m, tol=args.tol).run(tau[:3])
+    if batch64 is not None:
+        Cs.hahn_batch(ctx, par, m=args.m, tol=args.tol).run(tau[:3])
     for w in range(args.warmup):
         batch.run(tau)
         print(f"# warmup {w + 1}/{args.warmup} done", file=sys.stderr, flush=True)
@@ -198,21 +228,29 @@ def main():
     timing = batch.timing()
     value = nclu * args.steps / elapsed
 
-    # FP64 path on the stratified sample (same clusters, same times)
-    t1 = time.perf_counter()
-    batch64.run(tau)
-    el64 = time.perf_counter() - t1
-    L64, st64 = batch64.results()
-    rate64 = len(sel) / el64
-    print(f"# fp64 sample {len(sel)} clusters in {el64:.2f}s", file=sys.stderr, flush=True)
-    dL = np.abs(L[sel] - L64)
-    order_of = np.repeat(np.arange(len(counts)), counts)[sel]
-    err_per_order = [float(dL[order_of == o].max()) if np.any(order_of == o) else 0.0 for o in range(len(counts))]
-
-    dom, roof = roofline(counts, st, timing, mm)
-    roof["kernel"] = f"cce_hahn_krylov_kernel<{mm}, mixed> n={dom['n']}"
-    dom64, roof64 = roofline([len(np.nonzero(order_of == o)[0]) for o in range(len(counts))], st64,
-                             batch64.timing(), args.m)
+    dom, roof = roofline(counts, st, timing, mm, lanczos_flops)
+    roof["kernel"] = f"cce_hahn_lanczos_kernel<{mm}> n={dom['n']}"
+    roof["traffic"] = measured_traffic(roof["kernel"])
+    fp64 = None
+    if batch64 is not None:
+        # FP64 path on the stratified sample (same clusters, same times)
+        t1 = time.perf_counter()
+        batch64.run(tau)
+        el64 = time.perf_counter() - t1
+        L64, st64 = batch64.results()
+        rate64 = len(sel) / el64
+        print(f"# fp64 sample {len(sel)} clusters in {el64:.2f}s", file=sys.stderr, flush=True)
+        dL = np.abs(L[sel] - L64)
+        order_of = np.repeat(np.arange(len(counts)), counts)[sel]
+        err_per_order = [float(dL[order_of == o].max()) if np.any(order_of == o) else 0.0
+                         for o in range(len(counts))]
+        _, roof64 = roofline([int(np.sum(order_of == o)) for o in range(len(counts))], st64, batch64.timing(),
+                             args.m)
+        fp64 = {"m": args.m, "tol": args.tol, "sample_clusters": int(len(sel)), "value": rate64, "seconds": el64,
+                "buckets": batch64.timing(), "failed_clusters": int(((st64[:, 8] & 0xFF) != 0).sum()),
+                "roofline_frac": roof64["frac"], "roofline_achieved": roof64["achieved"],
+                "mixed_over_fp64": value / rate64, "max_abs_dL": float(dL.max()),
+                "max_abs_dL_per_order": err_per_order}
     out = {
         "metric": "cluster expv/sec at CCE-4 (mixed vs FP64, |\u0394L|\u22641e-6); p99 request latency",
         "value": value,
@@ -230,13 +268,10 @@ def main():
                                "B=0.05T||[111], 401 points on 0-5ms",
                    "clusters": nclu, "clusters_per_order": counts, "spins": int(len(X)),
                    "mixed": {"m": mm, "tol": mtol, "source": msrc, "buckets": timing,
-                             "failed_clusters": int((st[:, 8] != 0).sum())},
-                   "fp64": {"m": args.m, "tol": args.tol, "sample_clusters": int(len(sel)),
-                            "value": rate64, "seconds": el64, "buckets": batch64.timing(),
-                            "failed_clusters": int((st64[:, 8] != 0).sum()),
-                            "roofline_frac": roof64["frac"], "roofline_achieved": roof64["achieved"]},
-                   "mixed_over_fp64": value / rate64,
-                   "max_abs_dL": float(dL.max()), "max_abs_dL_per_order": err_per_order},
+                             "failed_clusters": int(((st[:, 8] & 0xFF) != 0).sum())},
+                   "fp64": fp64,
+                   "mixed_over_fp64": fp64["mixed_over_fp64"] if fp64 else None,
+                   "max_abs_dL": fp64["max_abs_dL"] if fp64 else None},
         "roofline": roof,
     }
     if not args.no_cpu:

@@ -262,8 +262,9 @@ __global__ __launch_bounds__(kNT) void cce_hahn_krylov_kernel(KrylovArgs a) {
             if (s) gB = g; else gA = g;
         }
         const double scale = a.scale[item];
-        for (int k = 0; k < a.nt; ++k) {
-            const double t_out = a.tau[k];
+        const ItemSeq q = item_seq(a, item);
+        for (int k = 0; k < q.nt; ++k) {
+            const double t_out = q.tau[k];
             // one call site, so the unrolled Arnoldi code is inlined once
             for (int s = 0; s < 2; ++s) {
                 IntegState g = s ? gB : gA;
@@ -273,7 +274,7 @@ __global__ __launch_bounds__(kNT) void cce_hahn_krylov_kernel(KrylovArgs a) {
             const auto tl = block_sum1<1, kNT>(br, [&](int) {
                 return fma(gB.w.re, gA.w.re, gB.w.im * gA.w.im);
             });
-            if (tid == 0) a.L[(int64_t)item * a.nt + k] = scale * tl.get(0);
+            if (tid == 0) q.L[k] = scale * tl.get(0);
             if ((gA.fail | gB.fail) != 0) break;
         }
         if (tid == 0) {
@@ -297,10 +298,12 @@ __global__ __launch_bounds__(kNT) void cce_hahn_krylov_kernel(KrylovArgs a) {
 }
 
 // ----------------------------------------------------------------------------
-// Dense path for n <= m (the reference's Krylov driver stops there):
-// w(tau_k) = exp(-i H (tau_k - tau_{k-1})) w(tau_{k-1}) with the exponential
-// from Pade(6), reusing it while the output spacing repeats.  One 64-thread
-// workgroup per cluster, two integrators, same observable as above.
+// Dense path for n <= m (the reference's Krylov driver stops there).
+// Hahn echo: w(tau_k) = exp(-i H (tau_k - tau_{k-1})) w(tau_{k-1}) for both
+// sweeps with the exponential from Pade(6), reused while the output spacing
+// repeats.  CPMG-N: for every output time the whole sequence
+// U_N .. U_0 u0 (QuantumEvolutionAlgorithm.cpp:60-83), L = scale Re<y0, s>.
+// One 64-thread workgroup per cluster.
 constexpr int kDenseNT = 64;
 
 struct DenseSmem {
@@ -328,7 +331,10 @@ __global__ __launch_bounds__(kDenseNT) void cce_hahn_dense_kernel(KrylovArgs a) 
         const int item = S.ints[7];
         __syncthreads();
         if (item >= a.nitems) break;
-        const int ops[2] = {a.item_opA[item], a.item_opB[item]};
+        const ItemSeq q = item_seq(a, item);
+        const bool cpmg = q.npulse > 1;
+        // Hahn echo: sweeps under lvA and -lvB^dagger; CPMG-N: ket-side lvA and lvB
+        const int ops[2] = {a.item_opA[item], cpmg ? a.item_opC[item] : a.item_opB[item]};
         const int64_t voff = a.vec_off[item];
         int fail = 0;
         // dense -i*H for both sweeps: column c = -i H e_c
@@ -349,9 +355,45 @@ __global__ __launch_bounds__(kDenseNT) void cce_hahn_dense_kernel(KrylovArgs a) 
         }
         const double scale = a.scale[item];
         double tprev = 0.0;
-        for (int k = 0; k < a.nt; ++k) {
-            const double d = a.tau[k] - tprev;
-            tprev = a.tau[k];
+        for (int k = 0; k < q.nt; ++k) {
+            if (cpmg) {
+                // s = U_N .. U_0 u0 with U_j = exp(-i L_j f_j t): exponentials of the
+                // two generators over t/(2N), applied once (end segments) or twice
+                const double t = q.tau[k];
+                if (tid < n) S.ws[0][tid] = mkcd(a.u0[voff + tid].x, a.u0[voff + tid].y);
+                __syncthreads();
+                if (t > 0.0) {
+                    for (int g2 = 0; g2 < 2 && !fail; ++g2) {
+                        int ns = 0;
+                        const int which = lds_pade6<kDenseNT>(S.H[g2], n, n, t * (0.5 / q.npulse), S.work[0],
+                                                              S.work[1], S.work[2], S.work[3], S.red, S.ints, ns);
+                        if (which < 0) { fail = 2; break; }
+                        for (int idx = tid; idx < n * n; idx += kDenseNT) S.U[g2][idx] = S.work[which][idx];
+                        __syncthreads();
+                    }
+                    if (fail) break;
+                    for (int j = 0; j <= q.npulse; ++j) {
+                        const int reps = (j == 0 || j == q.npulse) ? 1 : 2;
+                        for (int rep = 0; rep < reps; ++rep) {
+                            cd acc = mkcd(0.0, 0.0);
+                            if (tid < n)
+                                for (int c = 0; c < n; ++c) acc = cfma(S.U[j & 1][tid + n * c], S.ws[0][c], acc);
+                            __syncthreads();
+                            if (tid < n) S.ws[0][tid] = acc;
+                            __syncthreads();
+                        }
+                    }
+                }
+                const cd ya = tid < n ? mkcd(a.y0[voff + tid].x, a.y0[voff + tid].y) : mkcd(0.0, 0.0);
+                const cd ua = tid < n ? S.ws[0][tid] : mkcd(0.0, 0.0);
+                double r[2] = {fma(ya.re, ua.re, ya.im * ua.im), fma(ya.re, ua.im, -ya.im * ua.re)};
+                block_sum<2, kDenseNT>(r, S.red, S.red + 64);
+                if (tid == 0) q.L[k] = scale * S.red[64];
+                __syncthreads();
+                continue;
+            }
+            const double d = q.tau[k] - tprev;
+            tprev = q.tau[k];
             if (d > 0.0) {
                 for (int s = 0; s < 2; ++s) {
                     if (d != S.dprev[s]) {
@@ -376,7 +418,7 @@ __global__ __launch_bounds__(kDenseNT) void cce_hahn_dense_kernel(KrylovArgs a) 
             const cd ua = tid < n ? S.ws[0][tid] : mkcd(0.0, 0.0);
             double r[2] = {fma(ya.re, ua.re, ya.im * ua.im), fma(ya.re, ua.im, -ya.im * ua.re)};
             block_sum<2, kDenseNT>(r, S.red, S.red + 64);
-            if (tid == 0) a.L[(int64_t)item * a.nt + k] = scale * S.red[64];
+            if (tid == 0) q.L[k] = scale * S.red[64];
             __syncthreads();
         }
         if (tid == 0) {

@@ -90,6 +90,33 @@ __device__ __forceinline__ cd readlane_cd(cd x, int c) {
     return mkcd(lane_val(x.re, c), lane_val(x.im, c));
 }
 
+// Output times, result row and pulse count of one item.
+struct ItemSeq {
+    int nt;
+    int npulse;
+    const double* tau;
+    double* L;
+};
+
+__device__ __forceinline__ ItemSeq item_seq(const KrylovArgs& a, int item) {
+    ItemSeq q;
+    q.nt = a.item_nt ? a.item_nt[item] : a.nt;
+    q.npulse = a.item_npulse ? a.item_npulse[item] : 1;
+    q.tau = a.tau + (a.item_tau_off ? a.item_tau_off[item] : 0);
+    q.L = a.L + (a.item_L_off ? a.item_L_off[item] : (int64_t)item * a.nt);
+    return q;
+}
+
+// CPMG-N (Pulse_Interval / riffle, misc.cpp:59-84): segment j of N+1 has the
+// fraction 1/(2N) at both ends and 1/N inside, and generator lvA for even j,
+// lvB for odd j.  The ket side applies segments 0..h-1 to rho0, the bra side
+// the adjoints of segments N..h, h = (N+2)/2, so that
+// <rho0, U_N..U_0 rho0> = <U_h^+..U_N^+ rho0, U_{h-1}..U_0 rho0>.
+__device__ __forceinline__ double cpmg_fraction(int j, int npulse) {
+    return (j == 0 || j == npulse) ? 0.5 / npulse : 1.0 / npulse;
+}
+__device__ __forceinline__ int cpmg_split(int npulse) { return (npulse + 2) / 2; }
+
 // LDS scratch of the norm estimate.
 struct NormWork {
     cd* xs;

@@ -19,13 +19,19 @@ struct KrylovArgs {
     const DevOpLocal* loc;
     const double2* val;
     const int32_t* off;
-    const int32_t* item_opA;   // operator index of sweep A (ket side)
+    const int32_t* item_opA;   // operator index of sweep A (ket side, lvA)
     const int32_t* item_opB;   // operator index of sweep B (bra side, already -lvB^dagger)
+    const int32_t* item_opC;   // CPMG-N only: lvB (ket side), or NULL
+    const int32_t* item_opD;   // CPMG-N only: -lvA^dagger (bra side), or NULL
+    const int32_t* item_npulse;  // pulses N per item (NULL: 1, the Hahn echo)
+    const int32_t* item_nt;      // output times per item (NULL: nt for every item)
+    const int64_t* item_tau_off; // offset of the item's times in tau (NULL: 0)
+    const int64_t* item_L_off;   // offset of the item's L in L (NULL: item * nt)
     const int64_t* vec_off;    // offset of the item's u0 / y0 in the vector pool
     const double2* u0;
     const double2* y0;
     const double* scale;       // observable prefactor (the Hilbert dimension D)
-    const double* tau;         // ascending sweep times
+    const double* tau;         // ascending times: sweep times t/2 (Hahn), full times t (CPMG-N)
     const double* rnd;         // DLARAN draws: 2n start-vector values + klim power-iteration values
     double* L;                 // nitems x nt
     int32_t* stats;            // nitems x kStatsPerItem
