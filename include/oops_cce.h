@@ -62,7 +62,7 @@ int oops_bath_diamond(double a, double radius, double p, uint64_t seed, int32_t 
 int oops_nv_states(const double* pos, const double* B, double* evals, double* evecs);
 
 /* Term table of one cluster of k 13C spins: which = 0 lvA, 1 lvB,
- * 2 -lvB^dagger, 3 H0, 4 H1.  rho (n complex) and D may be NULL. */
+ * 2 -lvB^dagger, 3 H0, 4 H1, 5 -lvA^dagger.  rho (n complex) and D may be NULL. */
 int oops_cce_cluster_terms(const double* coords, int32_t k, const oops_cce_params* P, int32_t which,
                            oops_terms** out, double* rho, double* D);
 int oops_terms_view(const oops_terms* t, oops_term_table* view);

@@ -133,6 +133,43 @@ int oops_hahn_batch_run(oops_hahn_batch* b, int32_t nt, const double* tau);
  * small exponentials of sweep A, [4..7] the same for sweep B, [8] failure flags,
  * [9] exponential substeps, [10..13] the two norm estimates (doubles).  Either may be NULL. */
 int oops_hahn_batch_results(oops_hahn_batch* b, double* L, int32_t* stats);
+/* --- pulse sequences with per-item time grids ------------------------------
+ * One cluster under a Hahn echo (npulse = 1) or CPMG-N (npulse = N >= 2,
+ * Pulse_Interval / riffle of misc.cpp:59-84: segment fractions 1/(2N), 1/N,
+ * ..., 1/N, 1/(2N) with generators lvA, lvB, lvA, ...).  Hahn echo: t holds
+ * the sweep times t/2, L(t) = Re(scale <exp(-i opB t/2) y0, exp(-i opA t/2) u0>)
+ * as for oops_hahn_item.  CPMG-N: t holds the times t and
+ * L(t) = Re(scale <y0, U_N .. U_0 u0>), U_j = exp(-i L_j f_j t); opC = lvB and
+ * opD = -lvA^dagger are required.  CPMG-N runs on the dense path (n <= m) or
+ * on the mixed Lanczos path (precision 1, Hermitian generators). */
+typedef struct {
+    const oops_term_table* opA;  /* lvA */
+    const oops_term_table* opB;  /* -lvB^dagger */
+    const oops_term_table* opC;  /* lvB (CPMG-N) or NULL */
+    const oops_term_table* opD;  /* -lvA^dagger (CPMG-N) or NULL */
+    const double* u0;
+    const double* y0;
+    double scale;
+    int32_t npulse;
+    int32_t nt;
+    const double* t;             /* nt ascending times >= 0 */
+} oops_echo_item;
+
+/* Like oops_hahn_batch_create, with each item's times fixed at creation.  The
+ * batch owns a HIP stream, so several batches run concurrently on one device.
+ * Results: oops_hahn_batch_results with L concatenated in item order (item i
+ * at offset nt_0 + .. + nt_{i-1}). */
+int oops_echo_batch_create(oops_ctx* ctx, int32_t nitems, const oops_echo_item* items, int32_t m, int32_t klim,
+                           double tol, int32_t precision, oops_hahn_batch** out);
+/* Asynchronous launch on the batch's stream; query without blocking; wait. */
+int oops_echo_batch_launch(oops_hahn_batch* b);
+int oops_echo_batch_query(oops_hahn_batch* b, int32_t* done);
+int oops_echo_batch_wait(oops_hahn_batch* b);
+/* launch + wait */
+int oops_echo_batch_run(oops_hahn_batch* b);
+/* Device bytes held by the batch (its admission footprint). */
+int oops_echo_batch_footprint(const oops_hahn_batch* b, int64_t* bytes);
+
 /* Per-bucket timing of the last run: for bucket k, its state dimension,
  * item count and kernel time in ms (HIP events on the launch stream). */
 int oops_hahn_batch_timing(const oops_hahn_batch* b, int32_t max_buckets, int32_t* n, int32_t* count, double* ms,

@@ -56,7 +56,8 @@ EXPORTS = [
     "oops_bath_diamond", "oops_nv_states", "oops_cce_cluster_terms", "oops_terms_view",
     "oops_terms_sizes", "oops_terms_destroy", "oops_clusters_enumerate", "oops_clusters_count",
     "oops_clusters_get", "oops_clusters_parents", "oops_clusters_destroy", "oops_cce_reduce",
-    "oops_cce_hahn_batch_create", "oops_hamvec", "oops_expv",
+    "oops_cce_hahn_batch_create", "oops_hamvec", "oops_expv", "oops_echo_batch_create", "oops_echo_batch_launch",
+    "oops_echo_batch_query", "oops_echo_batch_wait", "oops_echo_batch_run", "oops_echo_batch_footprint",
 ]
 
 
@@ -197,6 +198,13 @@ def expv(ctx: Context, H: TermTable, v, ta, m=30, klim=10, tol=1e-12, overlaps=F
     return (out if overlaps else out.T), stats
 
 
+class EchoItemC(ctypes.Structure):
+    _fields_ = [("opA", ctypes.POINTER(TermTableC)), ("opB", ctypes.POINTER(TermTableC)),
+                ("opC", ctypes.POINTER(TermTableC)), ("opD", ctypes.POINTER(TermTableC)),
+                ("u0", ctypes.c_void_p), ("y0", ctypes.c_void_p), ("scale", ctypes.c_double),
+                ("npulse", ctypes.c_int32), ("nt", ctypes.c_int32), ("t", ctypes.c_void_p)]
+
+
 class HahnBatch:
     """Device-resident batch of Hahn-echo clusters (oops_hahn_batch_*)."""
 
@@ -259,6 +267,75 @@ class HahnBatch:
             self.close()
         except Exception:
             pass
+
+
+class EchoBatch(HahnBatch):
+    """Device batch of pulse-sequence items with their own time grids
+    (oops_echo_batch_*): Hahn echo (npulse 1, times are sweep times t/2) or
+    CPMG-N (times are t).  Runs asynchronously on its own HIP stream."""
+
+    @classmethod
+    def from_items(cls, ctx: Context, items, m=30, klim=10, tol=1e-12, precision="mixed"):
+        """items: dicts with opA, opB, [opC, opD] (TermTable), u0, y0, scale, npulse, t."""
+        keep = []
+        arr = (EchoItemC * len(items))()
+        nts = []
+        for i, it in enumerate(items):
+            u0 = np.ascontiguousarray(it["u0"], np.complex128)
+            y0 = np.ascontiguousarray(it["y0"], np.complex128)
+            t = np.ascontiguousarray(it["t"], np.float64)
+            keep += [u0, y0, t, it["opA"], it["opB"]]
+            arr[i].opA = ctypes.pointer(it["opA"].c)
+            arr[i].opB = ctypes.pointer(it["opB"].c)
+            if it.get("opC") is not None:
+                keep += [it["opC"], it["opD"]]
+                arr[i].opC = ctypes.pointer(it["opC"].c)
+                arr[i].opD = ctypes.pointer(it["opD"].c)
+            arr[i].u0 = u0.ctypes.data
+            arr[i].y0 = y0.ctypes.data
+            arr[i].scale = float(it["scale"])
+            arr[i].npulse = int(it.get("npulse", 1))
+            arr[i].nt = len(t)
+            arr[i].t = t.ctypes.data
+            nts.append(len(t))
+        h = ctypes.c_void_p()
+        rc = lib().oops_echo_batch_create(ctx.h, ctypes.c_int32(len(items)), arr, ctypes.c_int32(m),
+                                          ctypes.c_int32(klim), ctypes.c_double(tol),
+                                          ctypes.c_int32(_precision_code(precision)), ctypes.byref(h))
+        _check(rc, ctx.h, "oops_echo_batch_create")
+        b = cls(ctx, h, len(items))
+        b.nts = nts
+        return b
+
+    def launch(self):
+        _check(lib().oops_echo_batch_launch(self.h), self.ctx.h, "oops_echo_batch_launch")
+
+    def done(self):
+        d = ctypes.c_int32()
+        _check(lib().oops_echo_batch_query(self.h, ctypes.byref(d)), self.ctx.h, "oops_echo_batch_query")
+        return bool(d.value)
+
+    def wait(self):
+        _check(lib().oops_echo_batch_wait(self.h), self.ctx.h, "oops_echo_batch_wait")
+
+    def run(self):
+        _check(lib().oops_echo_batch_run(self.h), self.ctx.h, "oops_echo_batch_run")
+
+    def footprint(self):
+        v = ctypes.c_int64()
+        _check(lib().oops_echo_batch_footprint(self.h, ctypes.byref(v)), self.ctx.h, "oops_echo_batch_footprint")
+        return v.value
+
+    def results(self):
+        """(list of per-item L arrays, stats)"""
+        L = np.zeros(sum(self.nts))
+        st = np.zeros((self.nitems, STATS_PER_ITEM), np.int32)
+        _check(lib().oops_hahn_batch_results(self.h, _ptr(L), _ptr(st)), self.ctx.h, "oops_hahn_batch_results")
+        out, off = [], 0
+        for k in self.nts:
+            out.append(L[off:off + k].copy())
+            off += k
+        return out, st
 
 
 # ---------------------------------------------------------------------------

@@ -185,14 +185,38 @@ struct oops_hahn_batch {
     struct Bucket {
         int n = 0;
         std::vector<int32_t> items;  // global item indices
-        DevBuf hdr, loc, val, off, opA, opB, voff, u0, y0, scale, rnd, L, stats, queue;
+        DevBuf hdr, loc, val, off, opA, opB, opC, opD, voff, u0, y0, scale, rnd, L, stats, queue;
+        DevBuf npulse, nt, tau_off, L_off, tau;  // per-item sequences (echo batches)
+        int64_t L_len = 0;                       // doubles of L (echo batches)
         int grid = 0;
         bool hermitian = true;  // every generator Hermitian: mixed precision may use Lanczos
+        bool cpmg = false;      // some item has N >= 2 pulses
         double ms = 0.0;
     };
     std::vector<Bucket> buckets;
     DevBuf tau;
     int32_t nt = 0;
+    // echo batches: per-item times fixed at creation, own stream, asynchronous launch
+    bool per_item = false;
+    std::vector<int64_t> item_L_off;  // offset of item i's L in the concatenated result
+    std::vector<int32_t> item_nt;
+    int64_t footprint = 0;            // device bytes held by the batch
+    hipStream_t stream = nullptr;
+    std::vector<hipEvent_t> ev;
+    bool launched = false;
+    ~oops_hahn_batch() {
+        for (auto& e : ev) hipEventDestroy(e);
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+// One item of either create call.
+struct EchoIn {
+    const oops_term_table *opA, *opB, *opC, *opD;
+    const double *u0, *y0;
+    double scale;
+    int32_t npulse, nt;
+    const double* t;
 };
 
 extern "C" {
@@ -242,8 +266,8 @@ int oops_dlaran_fill(int32_t* iseed, int64_t count, double* out) {
     return OOPS_OK;
 }
 
-int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* items, int32_t m, int32_t klim,
-                           double tol, int32_t precision, oops_hahn_batch** out) {
+static int batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool per_item, int32_t m, int32_t klim,
+                        double tol, int32_t precision, oops_hahn_batch** out) {
     if (!ctx || !out || nitems < 0 || (nitems > 0 && !items)) return set_err(ctx, OOPS_ERR_ARG, "bad arguments");
     *out = nullptr;
     if (precision < 0 || precision > 2)
@@ -268,16 +292,36 @@ int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* 
         std::vector<int32_t> opA, opB;
         std::vector<int64_t> voff;
         std::vector<double> u0, y0, scale;
+        std::vector<int32_t> opC, opD, npulse, nt;
+        std::vector<int64_t> tau_off, L_off;
+        std::vector<double> tau;
     };
+    b->per_item = per_item;
     std::vector<Host> hosts;
     const int emax = krylov_entry_capacity(m);
+    int64_t L_total = 0;
     for (int32_t it = 0; it < nitems; ++it) {
-        CompiledOp A, B;
+        const EchoIn& I = items[it];
+        const bool cpmg = I.npulse > 1;
+        if (I.npulse < 1 || I.npulse > 64) return set_err(ctx, OOPS_ERR_ARG, "item " + std::to_string(it) + ": npulse must be in [1, 64]");
+        if (per_item) {
+            if (I.nt < 1 || !I.t) return set_err(ctx, OOPS_ERR_ARG, "item " + std::to_string(it) + ": no output times");
+            for (int k = 0; k < I.nt; ++k)
+                if (!(I.t[k] >= (k ? I.t[k - 1] : 0.0))) return set_err(ctx, OOPS_ERR_ARG, "item " + std::to_string(it) + ": times must be ascending and >= 0");
+        }
+        if (cpmg && (!I.opC || !I.opD)) return set_err(ctx, OOPS_ERR_ARG, "item " + std::to_string(it) + ": CPMG-N needs lvB and -lvA^dagger");
+        CompiledOp A, B, Cc, Dd;
         std::string err;
-        int rc = compile_operator(items[it].opA, A, err);
-        if (!rc) rc = compile_operator(items[it].opB, B, err);
+        int rc = compile_operator(I.opA, A, err);
+        if (!rc) rc = compile_operator(I.opB, B, err);
+        if (!rc && cpmg) rc = compile_operator(I.opC, Cc, err);
+        if (!rc && cpmg) rc = compile_operator(I.opD, Dd, err);
         if (rc) return set_err(ctx, rc, "item " + std::to_string(it) + ": " + err);
-        if (A.n != B.n) return set_err(ctx, OOPS_ERR_ARG, "item " + std::to_string(it) + ": sweep dimensions differ");
+        if (A.n != B.n || (cpmg && (Cc.n != A.n || Dd.n != A.n)))
+            return set_err(ctx, OOPS_ERR_ARG, "item " + std::to_string(it) + ": sweep dimensions differ");
+        if (cpmg && !(A.n <= m || (precision == 1 && A.hermitian && B.hermitian && Cc.hermitian && Dd.hermitian)))
+            return set_err(ctx, OOPS_ERR_UNSUPPORTED, "item " + std::to_string(it) +
+                           ": CPMG-N runs on the dense path (n <= m) or the mixed Lanczos path (Hermitian generators)");
         const int64_t n = A.n;
         const bool dense = n <= m;
         if (dense && n > kDenseNMAX)
@@ -285,7 +329,7 @@ int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* 
         if (!dense && n > 256)
             return set_err(ctx, OOPS_ERR_UNSUPPORTED, "item " + std::to_string(it) + ": batched Krylov path holds n <= 256");
         const int cap = dense ? kDenseMaxEnt : emax;
-        if ((int)A.val.size() > cap || (int)B.val.size() > cap)
+        if ((int)A.val.size() > cap || (int)B.val.size() > cap || (cpmg && ((int)Cc.val.size() > cap || (int)Dd.val.size() > cap)))
             return set_err(ctx, OOPS_ERR_CAPACITY, "item " + std::to_string(it) + ": operator table exceeds LDS capacity");
         auto f = bucket_of.find(n);
         int bk;
@@ -300,9 +344,15 @@ int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* 
         }
         Host& H = hosts[bk];
         b->buckets[bk].items.push_back(it);
-        b->buckets[bk].hermitian = b->buckets[bk].hermitian && A.hermitian && B.hermitian;
-        for (int s = 0; s < 2; ++s) {
-            const CompiledOp& C = s == 0 ? A : B;
+        b->buckets[bk].hermitian = b->buckets[bk].hermitian && A.hermitian && B.hermitian &&
+                                   (!cpmg || (Cc.hermitian && Dd.hermitian));
+        b->buckets[bk].cpmg = b->buckets[bk].cpmg || cpmg;
+        for (int s = 0; s < 4; ++s) {
+            if (s >= 2 && !cpmg) {
+                (s == 2 ? H.opC : H.opD).push_back(-1);
+                continue;
+            }
+            const CompiledOp& C = s == 0 ? A : (s == 1 ? B : (s == 2 ? Cc : Dd));
             DevOpHeader h;
             std::memset(&h, 0, sizeof(h));
             h.n = (int32_t)C.n;
@@ -316,7 +366,7 @@ int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* 
             h.c0re = C.c0.real();
             h.c0im = C.c0.imag();
             h.ent2_first = C.ent2_first;
-            (s == 0 ? H.opA : H.opB).push_back((int32_t)H.hdr.size());
+            (s == 0 ? H.opA : (s == 1 ? H.opB : (s == 2 ? H.opC : H.opD))).push_back((int32_t)H.hdr.size());
             H.hdr.push_back(h);
             H.loc.insert(H.loc.end(), C.locs.begin(), C.locs.end());
             for (auto& v : C.val) { H.val.push_back(v.real()); H.val.push_back(v.imag()); }
@@ -324,10 +374,21 @@ int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* 
         }
         H.voff.push_back((int64_t)(H.u0.size() / 2));
         for (int64_t i = 0; i < 2 * n; ++i) {
-            H.u0.push_back(items[it].u0[i]);
-            H.y0.push_back(items[it].y0[i]);
+            H.u0.push_back(I.u0[i]);
+            H.y0.push_back(I.y0[i]);
         }
-        H.scale.push_back(items[it].scale);
+        H.scale.push_back(I.scale);
+        H.npulse.push_back(I.npulse);
+        if (per_item) {
+            H.nt.push_back(I.nt);
+            H.tau_off.push_back((int64_t)H.tau.size());
+            H.tau.insert(H.tau.end(), I.t, I.t + I.nt);
+            H.L_off.push_back(b->buckets[bk].L_len);
+            b->buckets[bk].L_len += I.nt;
+            b->item_L_off.push_back(L_total);
+            b->item_nt.push_back(I.nt);
+            L_total += I.nt;
+        }
     }
     for (size_t bk = 0; bk < b->buckets.size(); ++bk) {
         auto& B = b->buckets[bk];
@@ -349,18 +410,164 @@ int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* 
         rc = rc ? rc : dev_upload(ctx, B.rnd, rnd);
         rc = rc ? rc : dev_upload(ctx, B.queue, std::vector<int32_t>(4, 0));
         rc = rc ? rc : dev_upload(ctx, B.stats, std::vector<int32_t>(B.items.size() * kStatsPerItem, 0));
+        if (B.cpmg) {
+            rc = rc ? rc : dev_upload(ctx, B.opC, H.opC);
+            rc = rc ? rc : dev_upload(ctx, B.opD, H.opD);
+        }
+        rc = rc ? rc : dev_upload(ctx, B.npulse, H.npulse);
+        if (per_item) {
+            rc = rc ? rc : dev_upload(ctx, B.nt, H.nt);
+            rc = rc ? rc : dev_upload(ctx, B.tau_off, H.tau_off);
+            rc = rc ? rc : dev_upload(ctx, B.L_off, H.L_off);
+            rc = rc ? rc : dev_upload(ctx, B.tau, H.tau);
+            if (!rc) {
+                rc = hip_check(ctx, hipMalloc(&B.L.p, sizeof(double) * std::max<int64_t>(B.L_len, 1)), "hipMalloc L");
+                if (!rc) B.L.bytes = sizeof(double) * std::max<int64_t>(B.L_len, 1);
+            }
+        }
         if (rc) return rc;
         const bool dense = B.n <= m;
         const int per_cu = dense ? 4 : 2;
         B.grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu * per_cu));
     }
+    for (auto& B : b->buckets)
+        for (DevBuf* d : {&B.hdr, &B.loc, &B.val, &B.off, &B.opA, &B.opB, &B.opC, &B.opD, &B.voff, &B.u0, &B.y0,
+                          &B.scale, &B.rnd, &B.L, &B.stats, &B.queue, &B.npulse, &B.nt, &B.tau_off, &B.L_off, &B.tau})
+            b->footprint += (int64_t)d->bytes;
+    if (per_item) {
+        if (hipStreamCreate(&b->stream) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipStreamCreate");
+        b->ev.resize(2 * b->buckets.size());
+        for (auto& e : b->ev)
+            if (hipEventCreate(&e) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipEventCreate");
+    }
     *out = b.release();
+    return OOPS_OK;
+}
+
+// Kernel arguments of bucket B; per-item sequences when the batch has them.
+static KrylovArgs bucket_args(const oops_hahn_batch* b, const oops_hahn_batch::Bucket& B, int32_t nt, const double* tau) {
+    KrylovArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.hdr = (const DevOpHeader*)B.hdr.p;
+    a.loc = (const DevOpLocal*)B.loc.p;
+    a.val = (const double2*)B.val.p;
+    a.off = (const int32_t*)B.off.p;
+    a.item_opA = (const int32_t*)B.opA.p;
+    a.item_opB = (const int32_t*)B.opB.p;
+    a.item_opC = (const int32_t*)B.opC.p;
+    a.item_opD = (const int32_t*)B.opD.p;
+    a.item_npulse = (const int32_t*)B.npulse.p;
+    a.vec_off = (const int64_t*)B.voff.p;
+    a.u0 = (const double2*)B.u0.p;
+    a.y0 = (const double2*)B.y0.p;
+    a.scale = (const double*)B.scale.p;
+    a.rnd = (const double*)B.rnd.p;
+    a.L = (double*)B.L.p;
+    a.stats = (int32_t*)B.stats.p;
+    a.queue = (int32_t*)B.queue.p;
+    a.nitems = (int32_t)B.items.size();
+    a.n = B.n;
+    a.klim = b->klim;
+    a.max_steps = 50000000;
+    a.tol = b->tol;
+    if (b->per_item) {
+        a.item_nt = (const int32_t*)B.nt.p;
+        a.item_tau_off = (const int64_t*)B.tau_off.p;
+        a.item_L_off = (const int64_t*)B.L_off.p;
+        a.tau = (const double*)B.tau.p;
+        a.nt = 0;
+    } else {
+        a.tau = tau;
+        a.nt = nt;
+    }
+    return a;
+}
+
+static int bucket_launch(oops_hahn_batch* b, oops_hahn_batch::Bucket& B, const KrylovArgs& a, hipStream_t st) {
+    const bool lanczos = b->precision == 1 && B.hermitian && B.n > b->m;
+    int lrc;
+    if (lanczos) {
+        launch_zero_counter(a.queue, st);
+        lrc = launch_hahn_lanczos(a, b->m, B.grid, st);
+    } else {
+        lrc = launch_hahn_bucket(a, b->m, b->precision == 0 ? 0 : 1, B.grid, st);
+    }
+    if (lrc != 0)
+        return set_err(b->ctx, lrc < 0 ? OOPS_ERR_UNSUPPORTED : OOPS_ERR_HIP,
+                       "launch failed for n=" + std::to_string(B.n) + " code " + std::to_string(lrc));
+    return OOPS_OK;
+}
+
+int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* items, int32_t m, int32_t klim,
+                           double tol, int32_t precision, oops_hahn_batch** out) {
+    std::vector<EchoIn> in(std::max(nitems, 0));
+    for (int32_t i = 0; i < nitems; ++i)
+        in[i] = EchoIn{items[i].opA, items[i].opB, nullptr, nullptr, items[i].u0, items[i].y0, items[i].scale, 1, 0,
+                       nullptr};
+    return batch_create(ctx, nitems, nitems > 0 && items ? in.data() : nullptr, false, m, klim, tol, precision, out);
+}
+
+int oops_echo_batch_create(oops_ctx* ctx, int32_t nitems, const oops_echo_item* items, int32_t m, int32_t klim,
+                           double tol, int32_t precision, oops_hahn_batch** out) {
+    std::vector<EchoIn> in(std::max(nitems, 0));
+    for (int32_t i = 0; i < nitems; ++i)
+        in[i] = EchoIn{items[i].opA, items[i].opB, items[i].opC, items[i].opD, items[i].u0, items[i].y0,
+                       items[i].scale, items[i].npulse, items[i].nt, items[i].t};
+    return batch_create(ctx, nitems, nitems > 0 && items ? in.data() : nullptr, true, m, klim, tol, precision, out);
+}
+
+int oops_echo_batch_launch(oops_hahn_batch* b) {
+    if (!b || !b->per_item) return OOPS_ERR_ARG;
+    oops_ctx* ctx = b->ctx;
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
+    for (size_t bk = 0; bk < b->buckets.size(); ++bk) {
+        auto& B = b->buckets[bk];
+        const KrylovArgs a = bucket_args(b, B, 0, nullptr);
+        hipEventRecord(b->ev[2 * bk], b->stream);
+        int rc = bucket_launch(b, B, a, b->stream);
+        hipEventRecord(b->ev[2 * bk + 1], b->stream);
+        if (rc) return rc;
+    }
+    b->launched = true;
+    return OOPS_OK;
+}
+
+int oops_echo_batch_query(oops_hahn_batch* b, int32_t* done) {
+    if (!b || !done || !b->per_item) return OOPS_ERR_ARG;
+    if (!b->launched) { *done = 0; return OOPS_OK; }
+    const hipError_t e = hipStreamQuery(b->stream);
+    if (e == hipSuccess) { *done = 1; return OOPS_OK; }
+    if (e == hipErrorNotReady) { *done = 0; return OOPS_OK; }
+    return hip_check(b->ctx, e, "hipStreamQuery");
+}
+
+int oops_echo_batch_wait(oops_hahn_batch* b) {
+    if (!b || !b->per_item || !b->launched) return OOPS_ERR_ARG;
+    int rc = hip_check(b->ctx, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    if (rc) return rc;
+    for (size_t bk = 0; bk < b->buckets.size(); ++bk) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, b->ev[2 * bk], b->ev[2 * bk + 1]);
+        b->buckets[bk].ms = ms;
+    }
+    return OOPS_OK;
+}
+
+int oops_echo_batch_run(oops_hahn_batch* b) {
+    int rc = oops_echo_batch_launch(b);
+    return rc ? rc : oops_echo_batch_wait(b);
+}
+
+int oops_echo_batch_footprint(const oops_hahn_batch* b, int64_t* bytes) {
+    if (!b || !bytes) return OOPS_ERR_ARG;
+    *bytes = b->footprint;
     return OOPS_OK;
 }
 
 int oops_hahn_batch_run(oops_hahn_batch* b, int32_t nt, const double* tau) {
     if (!b || nt <= 0 || !tau) return OOPS_ERR_ARG;
     oops_ctx* ctx = b->ctx;
+    if (b->per_item) return set_err(ctx, OOPS_ERR_ARG, "echo batches carry their times: use oops_echo_batch_run");
     for (int k = 1; k < nt; ++k)
         if (!(tau[k] >= tau[k - 1])) return set_err(ctx, OOPS_ERR_ARG, "sweep times must be ascending");
     if (tau[0] < 0.0) return set_err(ctx, OOPS_ERR_ARG, "sweep times must be >= 0");
@@ -377,39 +584,11 @@ int oops_hahn_batch_run(oops_hahn_batch* b, int32_t nt, const double* tau) {
         rc = hip_check(ctx, hipMalloc(&B.L.p, sizeof(double) * B.items.size() * nt), "hipMalloc L");
         if (rc) break;
         B.L.bytes = sizeof(double) * B.items.size() * nt;
-        KrylovArgs a;
-        std::memset(&a, 0, sizeof(a));
-        a.hdr = (const DevOpHeader*)B.hdr.p;
-        a.loc = (const DevOpLocal*)B.loc.p;
-        a.val = (const double2*)B.val.p;
-        a.off = (const int32_t*)B.off.p;
-        a.item_opA = (const int32_t*)B.opA.p;
-        a.item_opB = (const int32_t*)B.opB.p;
-        a.vec_off = (const int64_t*)B.voff.p;
-        a.u0 = (const double2*)B.u0.p;
-        a.y0 = (const double2*)B.y0.p;
-        a.scale = (const double*)B.scale.p;
-        a.tau = (const double*)b->tau.p;
-        a.rnd = (const double*)B.rnd.p;
-        a.L = (double*)B.L.p;
-        a.stats = (int32_t*)B.stats.p;
-        a.queue = (int32_t*)B.queue.p;
-        a.nitems = (int32_t)B.items.size();
-        a.n = B.n;
-        a.nt = nt;
-        a.klim = b->klim;
-        a.max_steps = 50000000;
-        a.tol = b->tol;
+        const KrylovArgs a = bucket_args(b, B, nt, (const double*)b->tau.p);
         hipEventRecord(ev[2 * bk], ctx->stream);
-        const bool lanczos = b->precision == 1 && B.hermitian && B.n > b->m;
-        const int lrc = lanczos ? (launch_zero_counter(a.queue, ctx->stream), launch_hahn_lanczos(a, b->m, B.grid, ctx->stream))
-                                : launch_hahn_bucket(a, b->m, b->precision == 0 ? 0 : 1, B.grid, ctx->stream);
+        rc = bucket_launch(b, B, a, ctx->stream);
         hipEventRecord(ev[2 * bk + 1], ctx->stream);
-        if (lrc != 0) {
-            rc = set_err(ctx, lrc < 0 ? OOPS_ERR_UNSUPPORTED : OOPS_ERR_HIP,
-                         "launch failed for n=" + std::to_string(B.n) + " code " + std::to_string(lrc));
-            break;
-        }
+        if (rc) break;
     }
     hipError_t se = hipStreamSynchronize(ctx->stream);
     if (!rc) rc = hip_check(ctx, se, "hipStreamSynchronize");
@@ -430,7 +609,17 @@ int oops_hahn_batch_results(oops_hahn_batch* b, double* L, int32_t* stats) {
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
     for (auto& B : b->buckets) {
         const size_t ni = B.items.size();
-        if (L) {
+        if (L && b->per_item) {
+            std::vector<double> h(B.L_len);
+            int rc = hip_check(ctx, hipMemcpy(h.data(), B.L.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost), "hipMemcpy L");
+            if (rc) return rc;
+            int64_t off = 0;
+            for (size_t i = 0; i < ni; ++i) {
+                const int32_t it = B.items[i];
+                std::memcpy(L + b->item_L_off[it], h.data() + off, sizeof(double) * b->item_nt[it]);
+                off += b->item_nt[it];
+            }
+        } else if (L) {
             if (!B.L.p) return set_err(ctx, OOPS_ERR_ARG, "batch has not been run");
             std::vector<double> h(ni * b->nt);
             int rc = hip_check(ctx, hipMemcpy(h.data(), B.L.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost), "hipMemcpy L");

@@ -401,13 +401,17 @@ NV make_nv(const double* pos, const double* B) {
 }
 
 struct ClusterProblem {
-    Owned A, Bd;          // lvA and -lvB^dagger
-    Owned H0, H1, lvB;    // for inspection
+    Owned A, Bd;          // lvA and -lvB^dagger (Hahn-echo sweeps)
+    Owned lvB, Ad;        // lvB and -lvA^dagger (CPMG-N segments)
+    Owned H0, H1;         // for inspection
     std::vector<cplx> rho;
     double D = 1.0;
 };
 
-ClusterProblem build_problem(const std::vector<SpinH>& spins, const oops_cce_params& P, const NV& nv) {
+enum : int { kNeedEcho = 1, kNeedCpmg = 2, kNeedHam = 4, kNeedAll = 7 };
+
+ClusterProblem build_problem(const std::vector<SpinH>& spins, const oops_cce_params& P, const NV& nv,
+                             int need = kNeedAll) {
     ClusterProblem cp;
     const cplx* st0 = &nv.evecs[3 * P.state0];
     const cplx* st1 = &nv.evecs[3 * P.state1];
@@ -424,11 +428,16 @@ ClusterProblem build_problem(const std::vector<SpinH>& spins, const oops_cce_par
     lvB.insert(lvB.end(), deph.begin(), deph.end());
     std::vector<int64_t> hd, ldim;
     for (const auto& s : spins) { hd.push_back(s.iso.mult); ldim.push_back((int64_t)s.iso.mult * s.iso.mult); }
-    cp.H0 = flatten(h0, hd);
-    cp.H1 = flatten(h1, hd);
+    if (need & kNeedHam) {
+        cp.H0 = flatten(h0, hd);
+        cp.H1 = flatten(h1, hd);
+    }
     cp.A = flatten(lvA, ldim);
-    cp.lvB = flatten(lvB, ldim);
     cp.Bd = flatten(negated_adjoint(lvB), ldim);
+    if (need & (kNeedCpmg | kNeedHam)) {
+        cp.lvB = flatten(lvB, ldim);
+        cp.Ad = flatten(negated_adjoint(lvA), ldim);
+    }
     cp.rho = density_vector(spins, P.pol, cp.D);
     return cp;
 }
@@ -515,6 +524,7 @@ int oops_cce_cluster_terms(const double* coords, int32_t k, const oops_cce_param
         case 2: t->o = cp.Bd; break;
         case 3: t->o = cp.H0; break;
         case 4: t->o = cp.H1; break;
+        case 5: t->o = cp.Ad; break;
         default: return OOPS_ERR_ARG;
     }
     if (rho) for (size_t i = 0; i < cp.rho.size(); ++i) { rho[2 * i] = cp.rho[i].real(); rho[2 * i + 1] = cp.rho[i].imag(); }
@@ -716,7 +726,7 @@ int oops_cce_hahn_batch_create(oops_ctx* ctx, const double* coords, const oops_c
                 for (int c = 0; c < 3; ++c) spins[i].r[c] = coords[3 * cl[i] + c];
                 isotope_of("13C", spins[i].iso);
             }
-            probs.push_back(build_problem(spins, *P, nv));
+            probs.push_back(build_problem(spins, *P, nv, kNeedEcho));
         }
     if (sel && next != nsel) return oops::set_err(ctx, OOPS_ERR_ARG, "selection must be ascending cluster indices");
     std::vector<oops_term_table> views(2 * probs.size());

@@ -40,7 +40,7 @@ constexpr int kBesMax = 128;  // Bessel coefficients per substep
 
 template <int M>
 struct LanczosSmem {
-    OpLds op[2];                        // both sweeps' generators stay resident
+    OpLds op[4];                        // lvA, -lvB^dagger, lvB, -lvA^dagger stay resident
     cd xs[256];
     float2 xs32[256];
     double red[3 * (kNT / kWave) * 64];
@@ -290,7 +290,9 @@ __device__ __forceinline__ void advance_lz(LanczosSmem<M>& S, BlockRed<kNT>& br,
         }
         g.t_new = gamma * t_step * pow(t_step * tol / err_loc, g.xm);
         g.t_new = round_step2(g.t_new);
-        g.t_now += t_step;
+        // land exactly on t_out when the step was clamped to it (segment ends)
+        if (t_step >= t_out - g.t_now) g.t_now = t_out;
+        else g.t_now += t_step;
         OOPS_TOC(g, tupd, tu0);
     }
 }
@@ -309,9 +311,15 @@ __global__ __launch_bounds__(kNT) void cce_hahn_lanczos_kernel(KrylovArgs a) {
         __syncthreads();
         if (item >= a.nitems) break;
         const int64_t voff = a.vec_off[item];
+        const ItemSeq q = item_seq(a, item);
+        const bool cpmg = q.npulse > 1;
         const OpSrc src{a.hdr, a.loc, a.val, a.off};
         op_load_lds<kNT, true>(src, a.item_opA[item], S.op[0]);
         op_load_lds<kNT, true>(src, a.item_opB[item], S.op[1]);
+        if (cpmg) {
+            op_load_lds<kNT, true>(src, a.item_opC[item], S.op[2]);
+            op_load_lds<kNT, true>(src, a.item_opD[item], S.op[3]);
+        }
         __syncthreads();
         Basis<M, true> V;
         IntegState gA, gB;
@@ -319,23 +327,43 @@ __global__ __launch_bounds__(kNT) void cce_hahn_lanczos_kernel(KrylovArgs a) {
         const cd u0 = tid < n ? mkcd(a.u0[voff + tid].x, a.u0[voff + tid].y) : mkcd(0.0, 0.0);
         const cd y0 = tid < n ? mkcd(a.y0[voff + tid].x, a.y0[voff + tid].y) : mkcd(0.0, 0.0);
         const NormWork nw{S.xs, S.xs32, S.tdiag, S.toff, S.scal};
+        double beta0[2];
         for (int s = 0; s < 2; ++s) {
             IntegState g;
             init_integ<true>(nw, S.op[s], br, g, s ? y0 : u0, a.rnd, a.klim, tol, M);
+            beta0[s] = g.beta;
             if (s) gB = g; else gA = g;
         }
         const double scale = a.scale[item];
-        for (int k = 0; k < a.nt; ++k) {
-            const double t_out = a.tau[k];
+        const int h = cpmg_split(q.npulse);
+        for (int k = 0; k < q.nt; ++k) {
             for (int s = 0; s < 2; ++s) {
                 IntegState g = s ? gB : gA;
-                advance_lz<M>(S, br, g, V, t_out, max_steps, S.op[s], tol);
+                // Hahn echo: one segment, the sweep continues to tau_k.  CPMG-N:
+                // restart from rho0 and run this side's segments over t_k.
+                int nseg = 1;
+                if (cpmg) {
+                    g.w = s ? y0 : u0;
+                    g.beta = beta0[s];
+                    g.t_now = 0.0;
+                    nseg = s == 0 ? h : q.npulse + 1 - h;
+                }
+                for (int sg = 0; sg < nseg; ++sg) {
+                    int opi = s;
+                    double target = q.tau[k];
+                    if (cpmg) {
+                        const int j = s == 0 ? sg : q.npulse - sg;
+                        opi = s == 0 ? ((j & 1) ? 2 : 0) : ((j & 1) ? 1 : 3);
+                        target = g.t_now + cpmg_fraction(j, q.npulse) * q.tau[k];
+                    }
+                    advance_lz<M>(S, br, g, V, target, max_steps, S.op[opi], tol);
+                }
                 if (s) gB = g; else gA = g;
             }
             const auto tl = block_sum1<1, kNT>(br, [&](int) {
                 return fma(gB.w.re, gA.w.re, gB.w.im * gA.w.im);
             });
-            if (tid == 0) a.L[(int64_t)item * a.nt + k] = scale * tl.get(0);
+            if (tid == 0) q.L[k] = scale * tl.get(0);
             if ((gA.fail | gB.fail) != 0) break;
         }
         if (tid == 0) {

@@ -1,0 +1,109 @@
+"""GPU: pulse sequences with per-item time grids (oops_echo_batch_*).
+
+CPMG-N (Pulse_Interval / riffle, misc.cpp:59-84) against the oracle's
+restatement of the reference's production dense propagator
+(PiecewiseFullMatrixVectorEvolution, QuantumEvolutionAlgorithm.cpp:60-83):
+the dense path (n <= m, FP64) to 1e-10, the mixed Lanczos path to 1e-6.
+Items of different sizes, pulse counts and time grids give the same L(t)
+alone and mixed in one batch (the kernels compute each item independently),
+and an echo batch with npulse = 1 equals the Hahn batch on the same times."""
+import numpy as np
+import pytest
+
+import oops_amd as O
+from oracle import physics as P
+
+B111 = np.ones(3) * 0.05 / np.sqrt(3.0)
+
+
+def _clusters(k, count, seed=3):
+    X = O.bath_diamond(3.567, 25.0, 0.02, seed=seed)
+    C = O.Clusters(X, 7.0, k)
+    cl = C.get(k - 1)
+    return [X[list(r)] for r in cl[np.linspace(0, len(cl) - 1, count).astype(int)]]
+
+
+def _item(coords, npulse, t):
+    par = O.cce_params(B=B111)
+    A, rho, D = O.cluster_terms(coords, par, 0)
+    Bd, _, _ = O.cluster_terms(coords, par, 2)
+    it = {"opA": A, "opB": Bd, "u0": rho, "y0": rho, "scale": D, "npulse": npulse, "t": t}
+    if npulse > 1:
+        it["opC"], _, _ = O.cluster_terms(coords, par, 1)
+        it["opD"], _, _ = O.cluster_terms(coords, par, 5)
+    return it
+
+
+def _oracle_cpmg(coords, npulse, t):
+    nv = P.NVCenter((0, 0, 0), B111)
+    prob = P.cluster_problem([P.Spin(c) for c in coords], B111, nv)
+    n = int(np.prod(prob["ldims"]))
+    A = P.full(prob["lvA"], n)
+    Bm = P.full(prob["lvB"], n)
+    ops = P.riffle(A, Bm, npulse)
+    seg = P.pulse_interval(npulse)
+    st = P.piecewise_dense(ops, seg, t[1] - t[0], prob["rho"], len(t))
+    return np.real(prob["D"] * (prob["rho"].conj() @ st))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k,npulse", [(1, 2), (2, 4), (2, 16), (1, 8)])
+def test_cpmg_dense_path_matches_reference_propagator(ctx, k, npulse):
+    t = np.linspace(0.0, 1e-3, 41)
+    for coords in _clusters(k, 3):
+        print(f"dense cpmg k={k} N={npulse}", flush=True)
+        b = O.EchoBatch.from_items(ctx, [_item(coords, npulse, t)], m=30, precision="fp64")
+        b.run()
+        L, st = b.results()
+        assert (st[:, 8] & 0xFF).max() == 0
+        assert np.abs(L[0] - _oracle_cpmg(coords, npulse, t)).max() < 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k,npulse", [(3, 2), (3, 4), (4, 2)])
+def test_cpmg_lanczos_path_matches_reference_propagator(ctx, k, npulse):
+    t = np.linspace(0.0, 4e-4, 17)
+    for coords in _clusters(k, 2):
+        print(f"lanczos cpmg k={k} N={npulse}", flush=True)
+        b = O.EchoBatch.from_items(ctx, [_item(coords, npulse, t)], m=30, tol=1e-6, precision="mixed")
+        b.run()
+        L, st = b.results()
+        assert (st[:, 8] & 0xFF).max() == 0
+        assert np.abs(L[0] - _oracle_cpmg(coords, npulse, t)).max() < 1e-6
+
+
+@pytest.mark.gpu
+def test_items_alone_and_in_the_mix_agree(ctx):
+    rng = np.random.default_rng(0)
+    items = []
+    for k in (1, 2, 3, 4):
+        for coords in _clusters(k, 2):
+            npulse = int(rng.choice([1, 2, 4]))
+            nt = int(rng.integers(5, 30))
+            t = np.linspace(0.0, float(rng.uniform(2e-4, 8e-4)), nt)
+            items.append(_item(coords, npulse, t / 2 if npulse == 1 else t))
+    mix = O.EchoBatch.from_items(ctx, items, m=30, tol=1e-4, precision="mixed")
+    mix.run()
+    Lmix, _ = mix.results()
+    for i, it in enumerate(items):
+        print(f"item {i} npulse={it['npulse']} nt={len(it['t'])}", flush=True)
+        alone = O.EchoBatch.from_items(ctx, [it], m=30, tol=1e-4, precision="mixed")
+        alone.run()
+        La, _ = alone.results()
+        assert np.abs(La[0] - Lmix[i]).max() <= 1e-12
+
+
+@pytest.mark.gpu
+def test_echo_npulse1_equals_hahn_batch(ctx):
+    tau = np.linspace(0.0, 5e-4, 21)
+    coords = _clusters(3, 2) + _clusters(4, 2)
+    items = [_item(c, 1, tau) for c in coords]
+    e = O.EchoBatch.from_items(ctx, items, m=30, tol=1e-4, precision="mixed")
+    e.run()
+    Le, _ = e.results()
+    h = O.HahnBatch.from_items(ctx, [(it["opA"], it["opB"], it["u0"], it["y0"], it["scale"]) for it in items],
+                               m=30, tol=1e-4, precision="mixed")
+    h.run(tau)
+    Lh, _ = h.results()
+    for i in range(len(items)):
+        assert np.abs(Le[i] - Lh[i]).max() <= 1e-13
