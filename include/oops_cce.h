@@ -93,6 +93,14 @@ int oops_cce_hahn_batch_create(oops_ctx* ctx, const double* coords, const oops_c
                                int64_t nsel, const int64_t* sel, int32_t m, int32_t klim, double tol,
                                int32_t precision, oops_hahn_batch** out);
 
+/* Pulse sequence on every cluster (oops_echo_item semantics, same time grid
+ * t[0..nt) for all clusters: sweep times t/2 for npulse = 1, times t for
+ * CPMG-N); operators built on the host's cores.  Items are ordered as the
+ * clusters; the batch runs with oops_echo_batch_run / _launch. */
+int oops_cce_echo_batch_create(oops_ctx* ctx, const double* coords, const oops_clusters* C, const oops_cce_params* P,
+                               int32_t npulse, int32_t nt, const double* t, int32_t m, int32_t klim, double tol,
+                               int32_t precision, oops_hahn_batch** out);
+
 #ifdef __cplusplus
 }
 #endif

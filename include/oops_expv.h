@@ -101,6 +101,37 @@ void main_cache_(size_t* nspin, size_t* nTerm, double* coeff_lst, size_t* nbody_
                  size_t* maxThreadsPerBlock, size_t* maxGridSize, size_t* tn, double* ta, size_t* m, double* tol,
                  size_t* itrace, double* w_seq, size_t* w_seq_len);
 
+/* --- link-compatible symbols of the reference's expv library ------------------
+ * Same names and argument order as expv/include/{main_mkl,krylov_expv,
+ * expokit}.h and hamvec_func3.cpp:399 / hamvec_cuda3.cu:661-665; complex
+ * arrays are interleaved doubles; they run on device 0. */
+void main_mkl_(size_t* nspin, size_t* nTerm, double* coeff_lst, size_t* nbody_lst, size_t* pos_i_idx,
+               size_t* pos_i_lst, size_t* dim_i_lst, size_t* mat_i_idx, double* mat_i_lst, size_t* ham_dim,
+               size_t* nspin_dim, double* v, size_t* pos_idx, size_t* mat_idx, size_t* k, size_t* tn, double* ta,
+               size_t* m, double* tol, size_t* itrace, double* w_seq, size_t* w_seq_len);
+/* w_seq(:, k) = exp(ta[k] A) v, A dense n x n column-major (krylov_zgexpv) or
+ * 1-based COO (krylov_zcooexpv).  Returns 0 or a negative OOPS_ERR_*. */
+int krylov_zgexpv(const int n, const double* a, const double* v, const int tn, const double* ta, double* w_seq,
+                  const int klim, const int m, const double tol, const int itrace);
+int krylov_zcooexpv(const int n, const int nz, const int* ia, const int* ja, const double* a, const double* v,
+                    const int tn, const double* ta, double* w_seq, const int klim, const int m, const double tol,
+                    const int itrace);
+/* w = sum_t coeff_zplx[t] (x)_b A_tb v (host vectors; strides and w_med unused). */
+void hamvec_func3_(int* nspin, int* nTerm, double* coeff_lst_zplx, size_t* nbody_lst, size_t* pos_i_idx,
+                   size_t* pos_i_lst, size_t* dim_i_lst, size_t* mat_i_idx, double* mat_i_lst, size_t* ham_dim,
+                   size_t* nspin_dim, size_t* nspin_m_lst, size_t* nspin_n_lst, double* v, double* w, double* w_med);
+/* Handle = an oops_ctx on device 0; device pointers passed as size_t. */
+void hamvec_cuda3_init_(size_t* handle);
+void hamvec_cuda3_term_(size_t* handle);
+void hamvec_cuda3_(size_t* handle, int* nspin, int* nTerm, double* coeff_lst_zplx, size_t* nbody_lst,
+                   size_t* pos_i_idx, size_t* pos_i_lst, size_t* dim_i_lst, size_t* mat_i_idx,
+                   size_t* dev_mat_i_lst, size_t* ham_dim, size_t* nspin_dim, size_t* nspin_m_lst,
+                   size_t* nspin_n_lst, size_t* dev_v, size_t* dev_w, size_t* dev_w_med, size_t* dev_coeff,
+                   size_t* maxThreadsPerBlock, size_t* maxGridSize);
+/* exp(t H) (m x m, m <= 42, ideg = 6) into wsp(1..m*m), *iexph = 1. */
+void zgpadm_(int* ideg, int* m, double* t, double* H, int* ldh, double* wsp, int* lwsp, int* ipiv, int* iexph,
+             int* ns, int* iflag);
+
 /* --- batched Hahn echo ---------------------------------------------------- */
 /* One cluster: opA generates the ket sweep exp(-i opA tau) u0, opB the bra
  * sweep exp(-i opB tau) y0 (pass -lvB^dagger for lvB), and

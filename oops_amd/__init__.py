@@ -58,6 +58,8 @@ EXPORTS = [
     "oops_clusters_get", "oops_clusters_parents", "oops_clusters_destroy", "oops_cce_reduce",
     "oops_cce_hahn_batch_create", "oops_hamvec", "oops_expv", "oops_echo_batch_create", "oops_echo_batch_launch",
     "oops_echo_batch_query", "oops_echo_batch_wait", "oops_echo_batch_run", "oops_echo_batch_footprint",
+    "oops_cce_echo_batch_create", "oops_server_create", "oops_server_submit", "oops_server_poll", "oops_server_state",
+    "oops_server_result", "oops_server_pending", "oops_server_destroy",
 ]
 
 
@@ -84,6 +86,16 @@ def lib():
                                               ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
         L.oops_clusters_destroy.argtypes = [ctypes.c_void_p]
         L.oops_terms_destroy.argtypes = [ctypes.c_void_p]
+        L.oops_server_create.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.POINTER(ctypes.c_void_p)]
+        L.oops_server_submit.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]
+        L.oops_server_poll.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
+        L.oops_server_state.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32)]
+        L.oops_server_result.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p]
+        L.oops_server_pending.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
+        L.oops_server_destroy.argtypes = [ctypes.c_void_p]
         L.oops_hamvec.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.oops_expv.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
                                 ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_void_p,
@@ -196,6 +208,12 @@ def expv(ctx: Context, H: TermTable, v, ta, m=30, klim=10, tol=1e-12, overlaps=F
     keys = ["nmult", "nexph", "nscale", "nstep", "nreject", "ibrkflag", "anorm", "kernel_ms"]
     stats = dict(zip(keys, st.tolist()))
     return (out if overlaps else out.T), stats
+
+
+class RequestC(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("radius", ctypes.c_double), ("p", ctypes.c_double),
+                ("B", ctypes.c_double * 3), ("cutoff", ctypes.c_double), ("order", ctypes.c_int32),
+                ("npulse", ctypes.c_int32), ("nt", ctypes.c_int32), ("t_end", ctypes.c_double)]
 
 
 class EchoItemC(ctypes.Structure):
@@ -336,6 +354,80 @@ class EchoBatch(HahnBatch):
             out.append(L[off:off + k].copy())
             off += k
         return out, st
+
+
+class Server:
+    """Serving loop (oops_server_*): continuous batching of CCE requests on
+    one device.  A request is a dict with seed, radius, p, B (3), cutoff,
+    order, npulse, nt, t_end; its result is the CCE product L(t)."""
+
+    def __init__(self, ctx: Context, hbm_budget=8 << 30, max_inflight=8, m=30, tol=1e-3, precision="mixed",
+                 threads=16):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(lib().oops_server_create(ctx.h, int(hbm_budget), int(max_inflight), int(m), float(tol),
+                                        _precision_code(precision), int(threads), ctypes.byref(h)),
+               ctx.h, "oops_server_create")
+        self.h = h
+        self.nt = {}
+
+    def submit(self, req):
+        r = RequestC()
+        r.seed = int(req["seed"])
+        r.radius = float(req["radius"])
+        r.p = float(req.get("p", 0.011))
+        for c in range(3):
+            r.B[c] = float(req["B"][c])
+        r.cutoff = float(req.get("cutoff", 8.0))
+        r.order = int(req["order"])
+        r.npulse = int(req.get("npulse", 1))
+        r.nt = int(req["nt"])
+        r.t_end = float(req["t_end"])
+        i = ctypes.c_int64()
+        _check(lib().oops_server_submit(self.h, ctypes.byref(r), ctypes.byref(i)), self.ctx.h, "oops_server_submit")
+        self.nt[i.value] = r.nt
+        return i.value
+
+    def poll(self):
+        c = ctypes.c_int32()
+        _check(lib().oops_server_poll(self.h, ctypes.byref(c)), self.ctx.h, "oops_server_poll")
+        return c.value
+
+    def state(self, rid):
+        s = ctypes.c_int32()
+        _check(lib().oops_server_state(self.h, ctypes.c_int64(rid), ctypes.byref(s)), self.ctx.h, "oops_server_state")
+        return s.value
+
+    def pending(self):
+        p = ctypes.c_int32()
+        _check(lib().oops_server_pending(self.h, ctypes.byref(p)), self.ctx.h, "oops_server_pending")
+        return p.value
+
+    def result(self, rid):
+        L = np.zeros(self.nt[rid])
+        lat = ctypes.c_double()
+        ncl = ctypes.c_int64()
+        fp = ctypes.c_int64()
+        _check(lib().oops_server_result(self.h, ctypes.c_int64(rid), _ptr(L), ctypes.byref(lat), ctypes.byref(ncl),
+                                        ctypes.byref(fp)), self.ctx.h, "oops_server_result")
+        return {"L": L, "latency_s": lat.value, "clusters": ncl.value, "footprint": fp.value}
+
+    def drain(self, sleep=0.002):
+        import time
+        while self.pending():
+            self.poll()
+            time.sleep(sleep)
+
+    def close(self):
+        if self.h:
+            lib().oops_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------

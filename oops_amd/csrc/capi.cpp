@@ -36,7 +36,9 @@ int hip_check(oops_ctx* ctx, hipError_t e, const char* what) {
 // ---------------------------------------------------------------------------
 // Operator compiler: merge the reference's term list into one dense local
 // matrix per spin and per spin pair, then store each as an ELL row table.
-int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err, int max_locals) {
+// max_locals: capacity of the kernel's local-operator table (LDS).
+int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err, int max_locals,
+                     const std::complex<double>* ccoeff) {
     if (!T || T->nspin <= 0 || T->nterm < 0) { err = "empty term table"; return OOPS_ERR_ARG; }
     const int ns = T->nspin;
     const int64_t d = T->spin_dim[0];
@@ -54,7 +56,7 @@ int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err
     std::vector<std::vector<cplx>> single(ns);
     std::map<std::pair<int, int>, std::vector<cplx>> pair;
     for (int t = 0; t < T->nterm; ++t) {
-        const double c = T->coeff[t];
+        const cplx c = ccoeff ? ccoeff[t] : cplx(T->coeff[t], 0.0);
         const int64_t nb = T->nbody[t];
         const int64_t p0 = T->pos_offset[t];
         if (nb == 0) {
@@ -191,6 +193,16 @@ struct oops_hahn_batch {
         int grid = 0;
         bool hermitian = true;  // every generator Hermitian: mixed precision may use Lanczos
         bool cpmg = false;      // some item has N >= 2 pulses
+        bool uniform = true;    // every item's times are k * dt from 0 (dense propagator powers apply)
+        DevBuf work;            // dense64 workspace
+        int dense_grid = 0;
+        // states above 256 elements: every item on the single-state path, in turn
+        bool large = false;
+        std::vector<CompiledOp> hA, hB;
+        std::vector<std::vector<double>> hu0, hy0;
+        std::vector<double> hscale, hL, htau;
+        std::vector<int32_t> hstats, hnt;
+        std::vector<int64_t> htau_off;
         double ms = 0.0;
     };
     std::vector<Bucket> buckets;
@@ -210,14 +222,7 @@ struct oops_hahn_batch {
     }
 };
 
-// One item of either create call.
-struct EchoIn {
-    const oops_term_table *opA, *opB, *opC, *opD;
-    const double *u0, *y0;
-    double scale;
-    int32_t npulse, nt;
-    const double* t;
-};
+
 
 extern "C" {
 
@@ -266,8 +271,10 @@ int oops_dlaran_fill(int32_t* iseed, int64_t count, double* out) {
     return OOPS_OK;
 }
 
-static int batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool per_item, int32_t m, int32_t klim,
-                        double tol, int32_t precision, oops_hahn_batch** out) {
+}  // extern "C"
+
+int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool per_item, int32_t m, int32_t klim,
+                       double tol, int32_t precision, oops_hahn_batch** out) {
     if (!ctx || !out || nitems < 0 || (nitems > 0 && !items)) return set_err(ctx, OOPS_ERR_ARG, "bad arguments");
     *out = nullptr;
     if (precision < 0 || precision > 2)
@@ -319,17 +326,30 @@ static int batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool
         if (rc) return set_err(ctx, rc, "item " + std::to_string(it) + ": " + err);
         if (A.n != B.n || (cpmg && (Cc.n != A.n || Dd.n != A.n)))
             return set_err(ctx, OOPS_ERR_ARG, "item " + std::to_string(it) + ": sweep dimensions differ");
-        if (cpmg && !(A.n <= m || (precision == 1 && A.hermitian && B.hermitian && Cc.hermitian && Dd.hermitian)))
+        bool uniform_item = false;
+        if (per_item && I.nt >= 2) {
+            const double dt = I.t[1] - I.t[0];
+            uniform_item = I.t[0] == 0.0 && dt > 0.0;
+            for (int k = 2; k < I.nt && uniform_item; ++k)
+                uniform_item = std::fabs(I.t[k] - k * dt) <= 1e-12 * I.t[I.nt - 1];
+        }
+        if (cpmg && !(A.n <= m || (A.n == 64 && uniform_item) ||
+                      (precision == 1 && A.hermitian && B.hermitian && Cc.hermitian && Dd.hermitian)))
             return set_err(ctx, OOPS_ERR_UNSUPPORTED, "item " + std::to_string(it) +
-                           ": CPMG-N runs on the dense path (n <= m) or the mixed Lanczos path (Hermitian generators)");
+                           ": CPMG-N runs on the dense paths (n <= m, or n = 64 on a uniform grid) or the mixed "
+                           "Lanczos path (Hermitian generators)");
         const int64_t n = A.n;
         const bool dense = n <= m;
         if (dense && n > kDenseNMAX)
             return set_err(ctx, OOPS_ERR_UNSUPPORTED, "item " + std::to_string(it) + ": n <= m needs n <= 16");
-        if (!dense && n > 256)
-            return set_err(ctx, OOPS_ERR_UNSUPPORTED, "item " + std::to_string(it) + ": batched Krylov path holds n <= 256");
+        const bool large = !dense && n > 256;
+        if (large && cpmg)
+            return set_err(ctx, OOPS_ERR_UNSUPPORTED, "item " + std::to_string(it) + ": CPMG-N needs n <= 256");
+        if (large && (A.locs.size() > 1024 || B.locs.size() > 1024))
+            return set_err(ctx, OOPS_ERR_CAPACITY, "item " + std::to_string(it) + ": too many local operators");
         const int cap = dense ? kDenseMaxEnt : emax;
-        if ((int)A.val.size() > cap || (int)B.val.size() > cap || (cpmg && ((int)Cc.val.size() > cap || (int)Dd.val.size() > cap)))
+        if (!large && ((int)A.val.size() > cap || (int)B.val.size() > cap ||
+                       (cpmg && ((int)Cc.val.size() > cap || (int)Dd.val.size() > cap))))
             return set_err(ctx, OOPS_ERR_CAPACITY, "item " + std::to_string(it) + ": operator table exceeds LDS capacity");
         auto f = bucket_of.find(n);
         int bk;
@@ -344,6 +364,24 @@ static int batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool
         }
         Host& H = hosts[bk];
         b->buckets[bk].items.push_back(it);
+        if (large) {
+            auto& BB = b->buckets[bk];
+            BB.large = true;
+            BB.hA.push_back(std::move(A));
+            BB.hB.push_back(std::move(B));
+            BB.hu0.emplace_back(I.u0, I.u0 + 2 * n);
+            BB.hy0.emplace_back(I.y0, I.y0 + 2 * n);
+            BB.hscale.push_back(I.scale);
+            if (per_item) {
+                BB.hnt.push_back(I.nt);
+                BB.htau_off.push_back((int64_t)BB.htau.size());
+                BB.htau.insert(BB.htau.end(), I.t, I.t + I.nt);
+                b->item_L_off.push_back(L_total);
+                b->item_nt.push_back(I.nt);
+                L_total += I.nt;
+            }
+            continue;
+        }
         b->buckets[bk].hermitian = b->buckets[bk].hermitian && A.hermitian && B.hermitian &&
                                    (!cpmg || (Cc.hermitian && Dd.hermitian));
         b->buckets[bk].cpmg = b->buckets[bk].cpmg || cpmg;
@@ -380,6 +418,7 @@ static int batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool
         H.scale.push_back(I.scale);
         H.npulse.push_back(I.npulse);
         if (per_item) {
+            if (I.nt >= 2) b->buckets[bk].uniform = b->buckets[bk].uniform && uniform_item;
             H.nt.push_back(I.nt);
             H.tau_off.push_back((int64_t)H.tau.size());
             H.tau.insert(H.tau.end(), I.t, I.t + I.nt);
@@ -390,9 +429,15 @@ static int batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool
             L_total += I.nt;
         }
     }
+    for (auto& B : b->buckets)
+        if (B.cpmg && !(B.n <= m || (B.n == 64 && B.uniform) || (precision == 1 && B.hermitian)))
+            return set_err(ctx, OOPS_ERR_UNSUPPORTED,
+                           "CPMG-N items of dimension " + std::to_string(B.n) +
+                           " need one uniform time grid from 0 for the whole bucket, or mixed precision");
     for (size_t bk = 0; bk < b->buckets.size(); ++bk) {
         auto& B = b->buckets[bk];
         Host& H = hosts[bk];
+        if (B.large) continue;
         std::vector<double> rnd(2 * B.n + 16);
         int32_t seed[4] = {0, 0, 0, 5};
         dlaran_fill(seed, (int64_t)rnd.size(), rnd.data());
@@ -429,10 +474,19 @@ static int batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool
         const bool dense = B.n <= m;
         const int per_cu = dense ? 4 : 2;
         B.grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu * per_cu));
+        if (B.n == 64) {
+            // one workgroup per CU (the kernel holds 152 KB of LDS)
+            B.dense_grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu));
+            const size_t bytes = sizeof(double) * (size_t)dense64_workspace_doubles(B.dense_grid);
+            rc = hip_check(ctx, hipMalloc(&B.work.p, bytes), "hipMalloc dense workspace");
+            if (rc) return rc;
+            B.work.bytes = bytes;
+        }
     }
     for (auto& B : b->buckets)
         for (DevBuf* d : {&B.hdr, &B.loc, &B.val, &B.off, &B.opA, &B.opB, &B.opC, &B.opD, &B.voff, &B.u0, &B.y0,
-                          &B.scale, &B.rnd, &B.L, &B.stats, &B.queue, &B.npulse, &B.nt, &B.tau_off, &B.L_off, &B.tau})
+                          &B.scale, &B.rnd, &B.L, &B.stats, &B.queue, &B.npulse, &B.nt, &B.tau_off, &B.L_off, &B.tau,
+                          &B.work})
             b->footprint += (int64_t)d->bytes;
     if (per_item) {
         if (hipStreamCreate(&b->stream) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipStreamCreate");
@@ -483,10 +537,35 @@ static KrylovArgs bucket_args(const oops_hahn_batch* b, const oops_hahn_batch::B
     return a;
 }
 
-static int bucket_launch(oops_hahn_batch* b, oops_hahn_batch::Bucket& B, const KrylovArgs& a, hipStream_t st) {
+// Buckets of large states: each item's two sweeps on the single-state path
+// (synchronous); L and stats kept on the host.
+static int bucket_run_large(oops_hahn_batch* b, oops_hahn_batch::Bucket& B, int32_t nt, const double* tau) {
+    const size_t ni = B.items.size();
+    int64_t total = 0;
+    for (size_t i = 0; i < ni; ++i) total += b->per_item ? B.hnt[i] : nt;
+    B.hL.assign((size_t)total, 0.0);
+    B.hstats.assign(ni * kStatsPerItem, 0);
+    int64_t off = 0;
+    for (size_t i = 0; i < ni; ++i) {
+        const int nti = b->per_item ? B.hnt[i] : nt;
+        const double* ti = b->per_item ? B.htau.data() + B.htau_off[i] : tau;
+        int rc = large_hahn_item(b->ctx, B.hA[i], B.hB[i], B.hu0[i].data(), B.hy0[i].data(), B.hscale[i], nti, ti,
+                                 b->m, b->klim, b->tol, B.hL.data() + off, B.hstats.data() + i * kStatsPerItem);
+        if (rc) return rc;
+        off += nti;
+    }
+    return OOPS_OK;
+}
+
+static int bucket_launch(oops_hahn_batch* b, oops_hahn_batch::Bucket& B, const KrylovArgs& a, hipStream_t st,
+                         bool uniform) {
+    const bool dense64 = B.n == 64 && uniform && B.work.p;
     const bool lanczos = b->precision == 1 && B.hermitian && B.n > b->m;
     int lrc;
-    if (lanczos) {
+    if (dense64) {
+        launch_zero_counter(a.queue, st);
+        lrc = launch_dense64(a, (double*)B.work.p, B.dense_grid, st);
+    } else if (lanczos) {
         launch_zero_counter(a.queue, st);
         lrc = launch_hahn_lanczos(a, b->m, B.grid, st);
     } else {
@@ -497,6 +576,8 @@ static int bucket_launch(oops_hahn_batch* b, oops_hahn_batch::Bucket& B, const K
                        "launch failed for n=" + std::to_string(B.n) + " code " + std::to_string(lrc));
     return OOPS_OK;
 }
+
+extern "C" {
 
 int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* items, int32_t m, int32_t klim,
                            double tol, int32_t precision, oops_hahn_batch** out) {
@@ -522,9 +603,16 @@ int oops_echo_batch_launch(oops_hahn_batch* b) {
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
     for (size_t bk = 0; bk < b->buckets.size(); ++bk) {
         auto& B = b->buckets[bk];
+        if (B.large) {
+            hipEventRecord(b->ev[2 * bk], b->stream);
+            int rc = bucket_run_large(b, B, 0, nullptr);
+            hipEventRecord(b->ev[2 * bk + 1], b->stream);
+            if (rc) return rc;
+            continue;
+        }
         const KrylovArgs a = bucket_args(b, B, 0, nullptr);
         hipEventRecord(b->ev[2 * bk], b->stream);
-        int rc = bucket_launch(b, B, a, b->stream);
+        int rc = bucket_launch(b, B, a, b->stream, B.uniform);
         hipEventRecord(b->ev[2 * bk + 1], b->stream);
         if (rc) return rc;
     }
@@ -575,18 +663,28 @@ int oops_hahn_batch_run(oops_hahn_batch* b, int32_t nt, const double* tau) {
     int rc = dev_upload(ctx, b->tau, std::vector<double>(tau, tau + nt));
     if (rc) return rc;
     b->nt = nt;
+    // uniform grid from 0: the dense propagator powers apply to n = 64 buckets
+    bool uniform = nt >= 2 && tau[0] == 0.0 && tau[1] > 0.0;
+    for (int k = 2; k < nt && uniform; ++k) uniform = std::fabs(tau[k] - k * tau[1]) <= 1e-12 * tau[nt - 1];
     std::vector<hipEvent_t> ev(2 * b->buckets.size());
     for (auto& e : ev) hipEventCreate(&e);
     // largest buckets first so the long Krylov work starts early
     for (size_t bk = 0; bk < b->buckets.size(); ++bk) {
         auto& B = b->buckets[bk];
+        if (B.large) {
+            hipEventRecord(ev[2 * bk], ctx->stream);
+            rc = bucket_run_large(b, B, nt, tau);
+            hipEventRecord(ev[2 * bk + 1], ctx->stream);
+            if (rc) break;
+            continue;
+        }
         B.L.release();
         rc = hip_check(ctx, hipMalloc(&B.L.p, sizeof(double) * B.items.size() * nt), "hipMalloc L");
         if (rc) break;
         B.L.bytes = sizeof(double) * B.items.size() * nt;
         const KrylovArgs a = bucket_args(b, B, nt, (const double*)b->tau.p);
         hipEventRecord(ev[2 * bk], ctx->stream);
-        rc = bucket_launch(b, B, a, ctx->stream);
+        rc = bucket_launch(b, B, a, ctx->stream, uniform);
         hipEventRecord(ev[2 * bk + 1], ctx->stream);
         if (rc) break;
     }
@@ -609,6 +707,19 @@ int oops_hahn_batch_results(oops_hahn_batch* b, double* L, int32_t* stats) {
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
     for (auto& B : b->buckets) {
         const size_t ni = B.items.size();
+        if (B.large) {
+            int64_t off = 0;
+            for (size_t i = 0; i < ni; ++i) {
+                const int32_t it = B.items[i];
+                const int nti = b->per_item ? B.hnt[i] : b->nt;
+                if (L) std::memcpy(L + (b->per_item ? b->item_L_off[it] : (int64_t)it * b->nt), B.hL.data() + off,
+                                   sizeof(double) * nti);
+                if (stats) std::memcpy(stats + (size_t)it * kStatsPerItem, B.hstats.data() + i * kStatsPerItem,
+                                       sizeof(int32_t) * kStatsPerItem);
+                off += nti;
+            }
+            continue;
+        }
         if (L && b->per_item) {
             std::vector<double> h(B.L_len);
             int rc = hip_check(ctx, hipMemcpy(h.data(), B.L.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost), "hipMemcpy L");

@@ -26,6 +26,7 @@
 #include <memory>
 #include <set>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/oops_cce.h"
@@ -743,4 +744,84 @@ int oops_cce_hahn_batch_create(oops_ctx* ctx, const double* coords, const oops_c
     return oops_hahn_batch_create(ctx, (int32_t)items.size(), items.data(), m, klim, tol, precision, out);
 }
 
+int oops_cce_echo_batch_create(oops_ctx* ctx, const double* coords, const oops_clusters* C, const oops_cce_params* P,
+                               int32_t npulse, int32_t nt, const double* t, int32_t m, int32_t klim, double tol,
+                               int32_t precision, oops_hahn_batch** out) {
+    if (!ctx || !coords || !C || !P || !out || nt < 1 || !t || npulse < 1) return OOPS_ERR_ARG;
+    oops::CceItems it;
+    const int threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    int rc = oops::cce_build_items(coords, C, P, npulse, nt, t, threads, it);
+    if (rc) return oops::set_err(ctx, rc, "building cluster operators failed");
+    return oops::batch_create(ctx, (int32_t)it.items.size(), it.items.data(), true, m, klim, tol, precision, out);
+}
+
 }  // extern "C"
+
+namespace oops {
+
+// Every cluster's sweep operators (lvA, -lvB^dagger and, for CPMG-N, lvB and
+// -lvA^dagger) and rho0, built in parallel over clusters.
+int cce_build_items(const double* coords, const oops_clusters* C, const oops_cce_params* P, int32_t npulse,
+                    int32_t nt, const double* t, int threads, CceItems& out) {
+    if (!coords || !C || !P || nt < 1 || !t || npulse < 1) return OOPS_ERR_ARG;
+    struct Hold {
+        std::vector<ClusterProblem> probs;
+        std::vector<oops_term_table> views;
+    };
+    auto hold = std::make_shared<Hold>();
+    const NV nv = make_nv(P->nv_pos, P->B);
+    std::vector<const std::vector<int32_t>*> cls;
+    for (int o = 0; o < C->max_order; ++o)
+        for (const auto& cl : C->idx[o]) cls.push_back(&cl);
+    const size_t nc = cls.size();
+    hold->probs.resize(nc);
+    const int need = npulse > 1 ? (kNeedEcho | kNeedCpmg) : kNeedEcho;
+    auto work = [&](size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; ++i) {
+            const auto& cl = *cls[i];
+            std::vector<SpinH> spins(cl.size());
+            for (size_t q = 0; q < cl.size(); ++q) {
+                for (int c = 0; c < 3; ++c) spins[q].r[c] = coords[3 * cl[q] + c];
+                isotope_of("13C", spins[q].iso);
+            }
+            hold->probs[i] = build_problem(spins, *P, nv, need);
+        }
+    };
+    const int T = std::max(1, std::min<int>(threads, (int)(nc / 64) + 1));
+    if (T == 1) {
+        work(0, nc);
+    } else {
+        std::vector<std::thread> pool;
+        for (int k = 0; k < T; ++k) pool.emplace_back(work, nc * k / T, nc * (k + 1) / T);
+        for (auto& th : pool) th.join();
+    }
+    hold->views.resize(4 * nc);
+    out.t.assign(t, t + nt);
+    out.items.resize(nc);
+    for (size_t i = 0; i < nc; ++i) {
+        ClusterProblem& p = hold->probs[i];
+        oops_term_table* v = &hold->views[4 * i];
+        v[0] = p.A.view();
+        v[1] = p.Bd.view();
+        EchoIn& e = out.items[i];
+        e.opA = &v[0];
+        e.opB = &v[1];
+        e.opC = e.opD = nullptr;
+        if (npulse > 1) {
+            v[2] = p.lvB.view();
+            v[3] = p.Ad.view();
+            e.opC = &v[2];
+            e.opD = &v[3];
+        }
+        e.u0 = e.y0 = reinterpret_cast<const double*>(p.rho.data());
+        e.scale = p.D;
+        e.npulse = npulse;
+        e.nt = nt;
+        e.t = nullptr;  // set below, after out.t is final
+    }
+    for (auto& e : out.items) e.t = out.t.data();
+    out.hold = hold;
+    return OOPS_OK;
+}
+
+}  // namespace oops

@@ -327,24 +327,22 @@ __global__ __launch_bounds__(kNT) void cce_hahn_lanczos_kernel(KrylovArgs a) {
         const cd u0 = tid < n ? mkcd(a.u0[voff + tid].x, a.u0[voff + tid].y) : mkcd(0.0, 0.0);
         const cd y0 = tid < n ? mkcd(a.y0[voff + tid].x, a.y0[voff + tid].y) : mkcd(0.0, 0.0);
         const NormWork nw{S.xs, S.xs32, S.tdiag, S.toff, S.scal};
-        double beta0[2];
-        for (int s = 0; s < 2; ++s) {
-            IntegState g;
-            init_integ<true>(nw, S.op[s], br, g, s ? y0 : u0, a.rnd, a.klim, tol, M);
-            beta0[s] = g.beta;
-            if (s) gB = g; else gA = g;
-        }
+        init_integ<true>(nw, S.op[0], br, gA, u0, a.rnd, a.klim, tol, M);
+        init_integ<true>(nw, S.op[1], br, gB, y0, a.rnd, a.klim, tol, M);
+        const double beta0A = gA.beta, beta0B = gB.beta;
         const double scale = a.scale[item];
         const int h = cpmg_split(q.npulse);
         for (int k = 0; k < q.nt; ++k) {
+            // the active sweep's state is always gA: the two states swap after
+            // each sweep (register moves, no dynamically indexed storage)
             for (int s = 0; s < 2; ++s) {
-                IntegState g = s ? gB : gA;
+                IntegState& g = gA;
                 // Hahn echo: one segment, the sweep continues to tau_k.  CPMG-N:
                 // restart from rho0 and run this side's segments over t_k.
                 int nseg = 1;
                 if (cpmg) {
                     g.w = s ? y0 : u0;
-                    g.beta = beta0[s];
+                    g.beta = s ? beta0B : beta0A;
                     g.t_now = 0.0;
                     nseg = s == 0 ? h : q.npulse + 1 - h;
                 }
@@ -358,7 +356,9 @@ __global__ __launch_bounds__(kNT) void cce_hahn_lanczos_kernel(KrylovArgs a) {
                     }
                     advance_lz<M>(S, br, g, V, target, max_steps, S.op[opi], tol);
                 }
-                if (s) gB = g; else gA = g;
+                const IntegState tmp = gA;
+                gA = gB;
+                gB = tmp;
             }
             const auto tl = block_sum1<1, kNT>(br, [&](int) {
                 return fma(gB.w.re, gA.w.re, gB.w.im * gA.w.im);

@@ -511,9 +511,11 @@ __device__ __forceinline__ void tile_matmul(cd* C, const cd* A, const cd* B, int
     __syncthreads();
 }
 
+// Pade(6) rational part: R = r(H t / 2^ns) into W0 (W1..W3 are workspace);
+// returns ns (the squarings still to do) or a negative code.
 template <int NT>
-__device__ int pade6_first_column(const cd* H, int ld, int m, double t, cd* W0, cd* W1, cd* W2, cd* W3, cd* F,
-                                  int* piv_lds, double* scal) {
+__device__ int pade6_rational(const cd* H, int ld, int m, double t, cd* W0, cd* W1, cd* W2, cd* W3, int* piv_lds,
+                              double* scal) {
     const int tid = threadIdx.x;
     // ||H||_inf with complex moduli (the reference's scaling rule)
     if (tid < kWave) {
@@ -605,6 +607,15 @@ __device__ int pade6_first_column(const cd* H, int ld, int m, double t, cd* W0, 
         W0[k + ld * j] = mkcd(2.0 * y.re + (k == j ? 1.0 : 0.0), 2.0 * y.im);
     }
     __syncthreads();
+    return ns;
+}
+
+template <int NT>
+__device__ int pade6_first_column(const cd* H, int ld, int m, double t, cd* W0, cd* W1, cd* W2, cd* W3, cd* F,
+                                  int* piv_lds, double* scal) {
+    const int tid = threadIdx.x;
+    const int ns = pade6_rational<NT>(H, ld, m, t, W0, W1, W2, W3, piv_lds, scal);
+    if (ns < 0) return ns;
     // squarings; the last one only forms the first column
     cd* R = W0;
     cd* T = W3;
@@ -622,6 +633,23 @@ __device__ int pade6_first_column(const cd* H, int ld, int m, double t, cd* W0, 
         }
     }
     __syncthreads();
+    return ns;
+}
+
+// Full exp(t H): Pade(6) and all ns squarings; *out points at the result
+// (W0 or W3, leading dimension ld).
+template <int NT>
+__device__ int pade6_full(const cd* H, int ld, int m, double t, cd* W0, cd* W1, cd* W2, cd* W3, int* piv_lds,
+                          double* scal, cd** out) {
+    const int ns = pade6_rational<NT>(H, ld, m, t, W0, W1, W2, W3, piv_lds, scal);
+    if (ns < 0) return ns;
+    cd* R = W0;
+    cd* T = W3;
+    for (int q = 0; q < ns; ++q) {
+        tile_matmul<NT>(T, R, R, m, ld, 1.0);
+        cd* tmp = R; R = T; T = tmp;
+    }
+    *out = R;
     return ns;
 }
 

@@ -68,6 +68,30 @@ __global__ __launch_bounds__(kLT) void large_matvec(LargeOpArgs o, const double2
     }
 }
 
+// Same product for a general sparse matrix in CSR form (row_ptr[n+1], col,
+// val): y = alpha * A (xscale * x), alpha_mode 0: +1, 1: -i.
+__global__ __launch_bounds__(kLT) void large_csr_matvec(const int64_t* __restrict__ row_ptr,
+                                                        const int32_t* __restrict__ col,
+                                                        const double2* __restrict__ val, const double2* __restrict__ x,
+                                                        const double* __restrict__ xscale_p, double2* __restrict__ y,
+                                                        double2* __restrict__ xstore, int64_t n, int alpha_mode) {
+    const double xs = xscale_p ? *xscale_p : 1.0;
+    for (int64_t i = (int64_t)blockIdx.x * kLT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kLT) {
+        cd acc = mkcd(0.0, 0.0);
+        for (int64_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+            const double2 a = val[e], xv = x[col[e]];
+            acc = cfma(mkcd(a.x, a.y), mkcd(xv.x, xv.y), acc);
+        }
+        acc = cscale(acc, xs);
+        if (alpha_mode == 1) acc = cmulmi(acc);
+        y[i] = make_double2(acc.re, acc.im);
+        if (xstore) {
+            const double2 xv = x[i];
+            xstore[i] = make_double2(xv.x * xs, xv.y * xs);
+        }
+    }
+}
+
 // part[(l * gridDim.x + block) * 2 + {re, im}] = partial conj(V_l) . p,
 // l = 0..J; l = J+1 (if with_norm) holds |p|^2 in the re slot.
 __global__ __launch_bounds__(kLT) void large_dots(const double2* __restrict__ V, int64_t ldv,
@@ -145,6 +169,99 @@ __global__ __launch_bounds__(kLT) void large_finish(const double* __restrict__ p
     }
 }
 
+// Fused classical Gram-Schmidt passes (one read of each basis vector per pass):
+//   large_gs_dots:    part_l = V_l^H p                      (l <= J)
+//   large_gs_update:  p -= sum_l coef_l V_l, then part_l = V_l^H p (next pass)
+//                     or, with norm_only, part_0 = |p|^2
+// Each thread keeps the J+1 basis values of its element in registers between
+// the update and the dot products; partial sums are reduced per wave with
+// shuffles, then across the block's waves in LDS.  MAXJ bounds J + 1.
+template <int MAXJ>
+__device__ __forceinline__ void gs_block_reduce(const double (&sr)[MAXJ], const double (&si)[MAXJ], int nrows,
+                                                double* red, double* __restrict__ part, int64_t row0) {
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int l = 0; l < MAXJ; ++l) {
+        if (l < nrows) {
+            double a = sr[l], b = si[l];
+#pragma unroll
+            for (int mask = 32; mask >= 1; mask >>= 1) {
+                a += __shfl_xor(a, mask, kWave);
+                b += __shfl_xor(b, mask, kWave);
+            }
+            if (lane == 0) {
+                red[(wave * MAXJ + l) * 2] = a;
+                red[(wave * MAXJ + l) * 2 + 1] = b;
+            }
+        }
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < nrows; l += kLT) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < kLT / kWave; ++w) { a += red[(w * MAXJ + l) * 2]; b += red[(w * MAXJ + l) * 2 + 1]; }
+        part[((int64_t)l * gridDim.x + blockIdx.x) * 2] = a;
+        part[((int64_t)l * gridDim.x + blockIdx.x) * 2 + 1] = b;
+    }
+    (void)row0;
+}
+
+template <int MAXJ>
+__global__ __launch_bounds__(kLT) void large_gs_dots(const double2* __restrict__ V, int64_t ldv,
+                                                     const double2* __restrict__ p, int64_t n, int J,
+                                                     double* __restrict__ part) {
+    __shared__ double red[2 * MAXJ * (kLT / kWave)];
+    double sr[MAXJ], si[MAXJ];
+#pragma unroll
+    for (int l = 0; l < MAXJ; ++l) { sr[l] = 0.0; si[l] = 0.0; }
+    for (int64_t i = (int64_t)blockIdx.x * kLT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kLT) {
+        const double2 b = p[i];
+#pragma unroll
+        for (int l = 0; l < MAXJ; ++l)
+            if (l <= J) {
+                const double2 a = V[(int64_t)l * ldv + i];
+                sr[l] = fma(a.x, b.x, fma(a.y, b.y, sr[l]));
+                si[l] = fma(a.x, b.y, fma(-a.y, b.x, si[l]));
+            }
+    }
+    gs_block_reduce<MAXJ>(sr, si, J + 1, red, part, 0);
+}
+
+template <int MAXJ>
+__global__ __launch_bounds__(kLT) void large_gs_update(const double2* __restrict__ V, int64_t ldv,
+                                                       double2* __restrict__ p, int64_t n, int J,
+                                                       const double2* __restrict__ coef, int norm_only,
+                                                       double* __restrict__ part) {
+    __shared__ double red[2 * MAXJ * (kLT / kWave)];
+    __shared__ double2 c[MAXJ];
+    for (int l = threadIdx.x; l <= J; l += kLT) c[l] = coef[l];
+    __syncthreads();
+    double sr[MAXJ], si[MAXJ];
+#pragma unroll
+    for (int l = 0; l < MAXJ; ++l) { sr[l] = 0.0; si[l] = 0.0; }
+    for (int64_t i = (int64_t)blockIdx.x * kLT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kLT) {
+        double2 v[MAXJ];
+        cd acc = mkcd(p[i].x, p[i].y);
+#pragma unroll
+        for (int l = 0; l < MAXJ; ++l)
+            if (l <= J) {
+                v[l] = V[(int64_t)l * ldv + i];
+                acc = cfms(mkcd(c[l].x, c[l].y), mkcd(v[l].x, v[l].y), acc);
+            }
+        p[i] = make_double2(acc.re, acc.im);
+        if (norm_only) {
+            sr[0] = fma(acc.re, acc.re, fma(acc.im, acc.im, sr[0]));
+        } else {
+#pragma unroll
+            for (int l = 0; l < MAXJ; ++l)
+                if (l <= J) {
+                    sr[l] = fma(v[l].x, acc.re, fma(v[l].y, acc.im, sr[l]));
+                    si[l] = fma(v[l].x, acc.im, fma(-v[l].y, acc.re, si[l]));
+                }
+        }
+    }
+    gs_block_reduce<MAXJ>(sr, si, norm_only ? 1 : J + 1, red, part, 0);
+}
+
 // p -= sum_{l<=J} coef_l V_l
 __global__ __launch_bounds__(kLT) void large_update(const double2* __restrict__ V, int64_t ldv, double2* __restrict__ p,
                                                     int64_t n, int J, const double2* __restrict__ coef) {
@@ -180,6 +297,24 @@ __global__ __launch_bounds__(kLT) void large_pade(const double2* __restrict__ he
     (void)ld;
     const int ns = pade6_first_column<kLT>(H, ldh, mx, t, S.W[0], S.W[1], S.W[2], S.W[3], S.F, S.piv, S.scal);
     for (int i = threadIdx.x; i < mx; i += kLT) F[i] = make_double2(S.F[i].re, S.F[i].im);
+    if (threadIdx.x == 0) status[0] = ns;
+}
+
+// Full exp(t H) (m x m, leading dimension ldh, column-major) by Pade(6) with
+// scaling and squaring (ZGPADM, expokit.f:762-934), one workgroup; out has
+// leading dimension m.  status[0] = squarings or a negative error.
+__global__ __launch_bounds__(kLT) void large_pade_full(const double2* __restrict__ hess, int ldh, int mx, double t,
+                                                       double2* __restrict__ out, int* __restrict__ status) {
+    extern __shared__ unsigned char dyn[];
+    LargePadeSmem& S = *reinterpret_cast<LargePadeSmem*>(dyn);
+    const cd* H = reinterpret_cast<const cd*>(hess);
+    cd* R = nullptr;
+    const int ns = pade6_full<kLT>(H, ldh, mx, t, S.W[0], S.W[1], S.W[2], S.W[3], S.piv, S.scal, &R);
+    if (ns >= 0)
+        for (int idx = threadIdx.x; idx < mx * mx; idx += kLT) {
+            const int i = idx % mx, j = idx / mx;
+            out[idx] = make_double2(R[i + ldh * j].re, R[i + ldh * j].im);
+        }
     if (threadIdx.x == 0) status[0] = ns;
 }
 
@@ -248,6 +383,17 @@ void launch_large_finish(const double* part, int nblk, int nrows, int jcol, int 
                        flags, break_tol);
 }
 
+void launch_large_gs_dots(const double2* V, int64_t ldv, const double2* p, int64_t n, int J, double* part, int nblk,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(large_gs_dots<kLargeMaxM>, dim3(nblk), dim3(kLT), 0, s, V, ldv, p, n, J, part);
+}
+
+void launch_large_gs_update(const double2* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef,
+                            int norm_only, double* part, int nblk, hipStream_t s) {
+    hipLaunchKernelGGL(large_gs_update<kLargeMaxM>, dim3(nblk), dim3(kLT), 0, s, V, ldv, p, n, J, coef, norm_only,
+                       part);
+}
+
 void launch_large_update(const double2* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef, int grid,
                          hipStream_t s) {
     hipLaunchKernelGGL(large_update, dim3(grid), dim3(kLT), 0, s, V, ldv, p, n, J, coef);
@@ -263,6 +409,26 @@ int launch_large_pade(const double2* hess, int ldh, int mx, double t, double2* F
         attr_set = true;
     }
     hipLaunchKernelGGL(large_pade, dim3(1), dim3(kLT), bytes, s, hess, ldh, mx, t, F, status);
+    return 0;
+}
+
+void launch_large_csr_matvec(const int64_t* row_ptr, const int32_t* col, const double2* val, const double2* x,
+                             const double* xscale, double2* y, double2* xstore, int64_t n, int alpha_mode, int grid,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(large_csr_matvec, dim3(grid), dim3(kLT), 0, s, row_ptr, col, val, x, xscale, y, xstore, n,
+                       alpha_mode);
+}
+
+int launch_large_pade_full(const double2* hess, int ldh, int mx, double t, double2* out, int* status, hipStream_t s) {
+    const size_t bytes = sizeof(LargePadeSmem);
+    static bool attr_set = false;
+    if (!attr_set) {
+        if (hipFuncSetAttribute((const void*)large_pade_full, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)bytes) != hipSuccess)
+            return -1;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(large_pade_full, dim3(1), dim3(kLT), bytes, s, hess, ldh, mx, t, out, status);
     return 0;
 }
 

@@ -3,9 +3,11 @@
 #include <hip/hip_runtime.h>
 
 #include <complex>
+#include <memory>
 #include <string>
 #include <vector>
 
+#include "../../include/oops_cce.h"
 #include "../../include/oops_expv.h"
 #include "oops_device_types.h"
 
@@ -55,8 +57,38 @@ struct DevBuf {
     }
 };
 
-// max_locals: capacity of the kernel's local-operator table (LDS).
-int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err, int max_locals = 64);
+// One item of a batch (Hahn items have npulse = 1, nt = 0, t = NULL).
+struct EchoIn {
+    const oops_term_table *opA, *opB, *opC, *opD;
+    const double *u0, *y0;
+    double scale;
+    int32_t npulse, nt;
+    const double* t;
+};
+
+// Compile, bucket and upload items (capi.cpp); per_item: times carried by the items.
+int batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool per_item, int32_t m, int32_t klim,
+                 double tol, int32_t precision, oops_hahn_batch** out);
+
+// Items of every cluster of an enumeration for one pulse sequence and time
+// grid, built on `threads` host threads (cce_host.cpp).  `hold` owns the term
+// tables and vectors the items point to.
+struct CceItems {
+    std::shared_ptr<void> hold;
+    std::vector<EchoIn> items;
+    std::vector<double> t;
+};
+int cce_build_items(const double* coords, const oops_clusters* C, const oops_cce_params* P, int32_t npulse,
+                    int32_t nt, const double* t, int threads, CceItems& out);
+
+// ccoeff (optional, nterm complex): complex multipliers replacing T->coeff.
+// One Hahn-echo item on the single-state path (large_host.cpp), FP64;
+// L[0..nt) and 24 stats slots as the batched kernels write them.
+int large_hahn_item(oops_ctx* ctx, const CompiledOp& A, const CompiledOp& B, const double* u0, const double* y0,
+                    double scale, int nt, const double* tau, int m, int klim, double tol, double* L, int32_t* stats);
+
+int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err, int max_locals = 64,
+                     const std::complex<double>* ccoeff = nullptr);
 int set_err(oops_ctx* ctx, int code, const std::string& msg);
 int hip_check(oops_ctx* ctx, hipError_t e, const char* what);
 void dlaran_fill(int32_t* iseed, int64_t count, double* out);

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -47,11 +48,26 @@ struct LargeRun {
     CompiledOp op;
     DevBuf hdr, loc, val, off;
     LargeOpArgs args{};
+    // general sparse operator (CSR) instead of the Kronecker tables
+    bool csr = false;
+    DevBuf rp, ci, cv;
     int64_t n = 0;
     int m = 0, ldh = 0, grid = 0, nblk = 0;
     DevBuf V, p0, p1, w, v0, part, hess, coef, scal, flags, F, status, wpart;
     int rc = 0;
 };
+
+// y = alpha Op (xscale x), alpha_mode 0: +1, 1: -i.  A Kronecker term table
+// carries the Hamiltonian H (the driver works on -iH); a CSR matrix carries
+// the generator A itself (the driver works on A, as krylov_zgexpv does).
+void matvec(LargeRun& R, const double2* x, const double* xs, double2* y, double2* xstore, int alpha_mode) {
+    if (R.csr)
+        launch_large_csr_matvec((const int64_t*)R.rp.p, (const int32_t*)R.ci.p, (const double2*)R.cv.p, x, xs, y,
+                                xstore, R.n, alpha_mode, R.grid, R.s);
+    else
+        launch_large_matvec(R.args, x, xs, y, xstore, R.n, alpha_mode, R.grid, R.s);
+}
+int arnoldi_mode(const LargeRun& R) { return R.csr ? 0 : 1; }
 
 template <typename T>
 int upload(oops_ctx* ctx, DevBuf& b, const std::vector<T>& v) {
@@ -70,17 +86,23 @@ int alloc(oops_ctx* ctx, DevBuf& b, size_t bytes) {
     return rc;
 }
 
-int setup(oops_ctx* ctx, const oops_term_table* T, int m, LargeRun& R) {
+int setup_buffers(oops_ctx* ctx, int m, LargeRun& R);
+
+int setup_compiled(oops_ctx* ctx, int m, LargeRun& R);
+
+int setup(oops_ctx* ctx, const oops_term_table* T, int m, LargeRun& R, const std::complex<double>* ccoeff = nullptr) {
     std::string err;
-    int rc = compile_operator(T, R.op, err, kLargeMaxLocals);
+    int rc = compile_operator(T, R.op, err, kLargeMaxLocals, ccoeff);
     if (rc) return set_err(ctx, rc, err);
+    return setup_compiled(ctx, m, R);
+}
+
+// Upload R.op (already compiled) and allocate the Krylov buffers.
+int setup_compiled(oops_ctx* ctx, int m, LargeRun& R) {
+    int rc = OOPS_OK;
     R.ctx = ctx;
     R.n = R.op.n;
-    R.m = m;
-    R.ldh = m + 2;
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
-    R.grid = large_grid(R.n, ctx->num_cu);
-    R.nblk = std::min(R.grid, 512);
     DevOpHeader h;
     std::memset(&h, 0, sizeof(h));
     h.n = (int32_t)R.op.n;
@@ -108,6 +130,29 @@ int setup(oops_ctx* ctx, const oops_term_table* T, int m, LargeRun& R) {
     R.args.loc = (const DevOpLocal*)R.loc.p;
     R.args.val = (const double2*)R.val.p;
     R.args.off = (const int32_t*)R.off.p;
+    return setup_buffers(ctx, m, R);
+}
+
+// CSR operator from row-sorted (row, col, value) triples, 0-based.
+int setup_csr(oops_ctx* ctx, int64_t n, const std::vector<int64_t>& rp, const std::vector<int32_t>& ci,
+              const std::vector<double>& cv, int m, LargeRun& R) {
+    R.ctx = ctx;
+    R.n = n;
+    R.csr = true;
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
+    int rc = upload(ctx, R.rp, rp);
+    rc = rc ? rc : upload(ctx, R.ci, ci.empty() ? std::vector<int32_t>{0} : ci);
+    rc = rc ? rc : upload(ctx, R.cv, cv.empty() ? std::vector<double>{0.0, 0.0} : cv);
+    if (rc) return rc;
+    return setup_buffers(ctx, m, R);
+}
+
+int setup_buffers(oops_ctx* ctx, int m, LargeRun& R) {
+    int rc = OOPS_OK;
+    R.m = m;
+    R.ldh = m + 2;
+    R.grid = large_grid(R.n, ctx->num_cu);
+    R.nblk = std::min(R.grid, 512);
     const size_t vb = sizeof(double2) * (size_t)R.n;
     rc = rc ? rc : alloc(ctx, R.V, vb * (m + 1));
     rc = rc ? rc : alloc(ctx, R.p0, vb);
@@ -173,7 +218,7 @@ int norm_estimate(LargeRun& R, int klim, double tol, double& anorm) {
     std::vector<double> tk((size_t)k * k, 0.0);
     double bjm = 0.0;
     for (int j = 1; j <= k; ++j) {
-        launch_large_matvec(R.args, q, nullptr, wv, nullptr, n, 0, R.grid, R.s);
+        matvec(R, q, nullptr, wv, nullptr, 0);
         cplx aj;
         rc = dev_dot(R, q, wv, aj);
         if (rc) return rc;
@@ -218,9 +263,10 @@ struct Counters {
 };
 
 // The Expokit integration.  out_mode 0: w_seq (n x nt) to the host;
-// 1: overlaps <v0, w(t_k)> to the host (2 doubles per time).
+// 1: overlaps <v0, w(t_k)> to the host (2 doubles per time); 2: w(t_k) into
+// the device array dev (n x nt); 3: overlaps <w(t_k), dev(:, k)> to the host.
 int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int klim, double tol, int out_mode,
-             double* out, Counters& C) {
+             double* out, Counters& C, double2* dev = nullptr) {
     const int m = R.m;
     const int64_t n = R.n;
     const double delta = 1.2, gamma = 0.9, break_tol = 1.0e-7;
@@ -267,23 +313,22 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
             const double2* x = (const double2*)R.w.p;
             for (int j = 0; j < m; ++j) {
                 double2* p = bufs[j & 1];
-                launch_large_matvec(R.args, x, scal, p, V + (size_t)j * n, n, 1, R.grid, R.s);
-                launch_large_dots(V, n, p, n, j, 0, (double*)R.part.p, R.nblk, R.s);
+                matvec(R, x, scal, p, V + (size_t)j * n, arnoldi_mode(R));
+                // CGS2: dots, update + dots, update + norm (one read of V per pass)
+                launch_large_gs_dots(V, n, p, n, j, (double*)R.part.p, R.nblk, R.s);
                 launch_large_finish((double*)R.part.p, R.nblk, j + 1, j, 0, hess, R.ldh, (double2*)R.coef.p, scal,
                                     (int*)R.flags.p, break_tol, R.s);
-                launch_large_update(V, n, p, n, j, (double2*)R.coef.p, R.grid, R.s);
-                launch_large_dots(V, n, p, n, j, 0, (double*)R.part.p, R.nblk, R.s);
+                launch_large_gs_update(V, n, p, n, j, (double2*)R.coef.p, 0, (double*)R.part.p, R.nblk, R.s);
                 launch_large_finish((double*)R.part.p, R.nblk, j + 1, j, 1, hess, R.ldh, (double2*)R.coef.p, scal,
                                     (int*)R.flags.p, break_tol, R.s);
-                launch_large_update(V, n, p, n, j, (double2*)R.coef.p, R.grid, R.s);
-                launch_large_dots(p, 0, p, n, -1, 1, (double*)R.part.p, R.nblk, R.s);
+                launch_large_gs_update(V, n, p, n, j, (double2*)R.coef.p, 1, (double*)R.part.p, R.nblk, R.s);
                 launch_large_finish((double*)R.part.p, R.nblk, 1, j, 2, hess, R.ldh, nullptr, scal, (int*)R.flags.p,
                                     break_tol, R.s);
                 x = p;
             }
             // extra product for the error estimate: V[m] = p / h(m, m-1), p' = -iH V[m]
             double2* pe = bufs[m & 1];
-            launch_large_matvec(R.args, x, scal, pe, V + (size_t)m * n, n, 1, R.grid, R.s);
+            matvec(R, x, scal, pe, V + (size_t)m * n, arnoldi_mode(R));
             launch_large_dots(pe, 0, pe, n, -1, 1, (double*)R.part.p, R.nblk, R.s);
             launch_large_finish((double*)R.part.p, R.nblk, 1, 0, 3, hess, R.ldh, nullptr, scal, (int*)R.flags.p,
                                 break_tol, R.s);
@@ -356,7 +401,17 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
             t_new = round_step(gamma * t_step * std::pow(t_step * tol / err_loc, xm));
             t_now += t_step;
         }
-        if (out) {
+        if (out_mode == 2) {
+            rc = hip_check(R.ctx, hipMemcpyAsync(dev + (size_t)n * tt, R.w.p, sizeof(double2) * n,
+                                                 hipMemcpyDeviceToDevice, R.s), "D2D w");
+            if (rc) return rc;
+        } else if (out_mode == 3) {
+            cplx ov;
+            rc = dev_dot(R, (const double2*)R.w.p, dev + (size_t)n * tt, ov);
+            if (rc) return rc;
+            out[2 * tt] = ov.real();
+            out[2 * tt + 1] = ov.imag();
+        } else if (out) {
             if (out_mode == 0) {
                 rc = hip_check(R.ctx, hipMemcpyAsync(out + 2 * (size_t)n * tt, R.w.p, sizeof(double2) * n,
                                                      hipMemcpyDeviceToHost, R.s), "D2H w");
@@ -377,6 +432,47 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
 }
 
 }  // namespace
+
+namespace oops {
+
+// One Hahn-echo cluster on the single-state path (states too large for the
+// batched kernels): the ket sweep's states are kept on the device, the bra
+// sweep forms L(tau_k) = Re(scale <bra(tau_k), ket(tau_k)>) at each output.
+int large_hahn_item(oops_ctx* ctx, const CompiledOp& A, const CompiledOp& B, const double* u0, const double* y0,
+                    double scale, int nt, const double* tau, int m, int klim, double tol, double* L, int32_t* stats) {
+    LargeRun RA, RB;
+    RA.op = A;
+    RB.op = B;
+    int rc = setup_compiled(ctx, m, RA);
+    rc = rc ? rc : setup_compiled(ctx, m, RB);
+    if (rc) return rc;
+    if (m >= RA.n) return set_err(ctx, OOPS_ERR_ARG, "the Krylov dimension must be smaller than the state dimension");
+    DevBuf ket;
+    rc = alloc(ctx, ket, sizeof(double2) * (size_t)RA.n * nt);
+    if (rc) return rc;
+    std::vector<double> ov(2 * (size_t)nt);
+    Counters CA, CB;
+    hipStreamCreate(&RA.s);
+    RB.s = RA.s;
+    rc = run_expv(RA, u0, nt, tau, klim, tol, 2, nullptr, CA, (double2*)ket.p);
+    rc = rc ? rc : run_expv(RB, y0, nt, tau, klim, tol, 3, ov.data(), CB, (double2*)ket.p);
+    hipStreamDestroy(RA.s);
+    if (rc) return rc;
+    for (int k = 0; k < nt; ++k) L[k] = scale * ov[2 * k];
+    if (stats) {
+        stats[0] = (int32_t)CA.nstep; stats[1] = (int32_t)CA.nmult; stats[2] = (int32_t)CA.nreject;
+        stats[3] = (int32_t)CA.nexph;
+        stats[4] = (int32_t)CB.nstep; stats[5] = (int32_t)CB.nmult; stats[6] = (int32_t)CB.nreject;
+        stats[7] = (int32_t)CB.nexph;
+        stats[8] = (int32_t)((CA.ibrk << 8) | (CB.ibrk << 9));
+        stats[9] = (int32_t)(CA.nscale + CB.nscale);
+        std::memcpy(stats + 10, &CA.anorm, sizeof(double));
+        std::memcpy(stats + 12, &CB.anorm, sizeof(double));
+    }
+    return OOPS_OK;
+}
+
+}  // namespace oops
 
 extern "C" {
 
@@ -448,6 +544,236 @@ void main_cache_(size_t* nspin, size_t* nTerm, double* coeff_lst, size_t* nbody_
     if (oops_ctx_create(0, &ctx) != OOPS_OK) return;
     oops_expv(ctx, &T, v, (int32_t)*tn, ta, (int32_t)*k, (int32_t)*m, *tol, 0, w_seq, nullptr);
     oops_ctx_destroy(ctx);
+}
+
+// ---------------------------------------------------------------------------
+// Link-compatible symbols of the reference's expv library (expv/include/*.h).
+// Integers are by reference where the reference's Fortran ABI has them so;
+// complex arrays are interleaved (re, im) doubles.  They run on device 0
+// through one process-wide context.
+
+static oops_ctx* abi_ctx() {
+    static oops_ctx* ctx = nullptr;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        if (oops_ctx_create(0, &ctx) != OOPS_OK) ctx = nullptr;
+    });
+    return ctx;
+}
+
+static oops_term_table term_table(size_t nspin, size_t nterm, const double* coeff, const size_t* nbody,
+                                  const size_t* pos_i_idx, const size_t* pos_i_lst, const size_t* dim_i_lst,
+                                  const size_t* mat_i_idx, const double* mats, const size_t* nspin_dim) {
+    oops_term_table T;
+    T.nspin = (int32_t)nspin;
+    T.nterm = (int32_t)nterm;
+    T.coeff = coeff;
+    T.nbody = reinterpret_cast<const int64_t*>(nbody);
+    T.pos_offset = reinterpret_cast<const int64_t*>(pos_i_idx);
+    T.pos = reinterpret_cast<const int64_t*>(pos_i_lst);
+    T.dims = reinterpret_cast<const int64_t*>(dim_i_lst);
+    T.mat_offset = reinterpret_cast<const int64_t*>(mat_i_idx);
+    T.mats = mats;
+    T.spin_dim = reinterpret_cast<const int64_t*>(nspin_dim);
+    return T;
+}
+
+// MAIN_MKL (main_mkl.h:10): main_cache_ without the launch-shape arguments.
+void main_mkl_(size_t* nspin, size_t* nTerm, double* coeff_lst, size_t* nbody_lst, size_t* pos_i_idx,
+               size_t* pos_i_lst, size_t* dim_i_lst, size_t* mat_i_idx, double* mat_i_lst, size_t* ham_dim,
+               size_t* nspin_dim, double* v, size_t* pos_idx, size_t* mat_idx, size_t* k, size_t* tn, double* ta,
+               size_t* m, double* tol, size_t* itrace, double* w_seq, size_t* w_seq_len) {
+    (void)pos_idx; (void)mat_idx; (void)itrace; (void)ham_dim; (void)w_seq_len;
+    oops_ctx* ctx = abi_ctx();
+    if (!ctx) return;
+    oops_term_table T = term_table(*nspin, *nTerm, coeff_lst, nbody_lst, pos_i_idx, pos_i_lst, dim_i_lst, mat_i_idx,
+                                   mat_i_lst, nspin_dim);
+    oops_expv(ctx, &T, v, (int32_t)*tn, ta, (int32_t)*k, (int32_t)*m, *tol, 0, w_seq, nullptr);
+}
+
+static int csr_expv(int64_t n, const std::vector<int64_t>& rp, const std::vector<int32_t>& ci,
+                    const std::vector<double>& cv, const double* v, int tn, const double* ta, double* w_seq, int klim,
+                    int m, double tol) {
+    oops_ctx* ctx = abi_ctx();
+    if (!ctx) return OOPS_ERR_HIP;
+    if (m < 2 || m > kLargeMaxM || m >= n) return set_err(ctx, OOPS_ERR_ARG, "m must be in [2, 40] and below n");
+    if (klim < 1 || klim > 64) klim = 10;
+    for (int k = 1; k < tn; ++k)
+        if (!(ta[k] >= ta[k - 1])) return set_err(ctx, OOPS_ERR_ARG, "output times must be ascending");
+    LargeRun R;
+    int rc = setup_csr(ctx, n, rp, ci, cv, m, R);
+    if (rc) return rc;
+    hipStreamCreate(&R.s);
+    Counters C;
+    rc = run_expv(R, v, tn, ta, klim, tol, 0, w_seq, C);
+    hipStreamDestroy(R.s);
+    return rc;
+}
+
+// krylov_zgexpv (krylov_expv.h:6): w(t_k) = exp(t_k A) v for a dense n x n
+// column-major A (the caller folds the -i prefactor into A, MatExp.cpp:108).
+// Returns 0 or a negative status (the reference returns Expokit's iflag).
+int krylov_zgexpv(const int n, const double* a, const double* v, const int tn, const double* ta, double* w_seq,
+                  const int klim, const int m, const double tol, const int itrace) {
+    (void)itrace;
+    if (n <= 0 || !a || !v || tn <= 0 || !ta || !w_seq) return OOPS_ERR_ARG;
+    std::vector<int64_t> rp(n + 1, 0);
+    std::vector<int32_t> ci;
+    std::vector<double> cv;
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < n; ++j) {
+            const double re = a[2 * ((size_t)i + (size_t)n * j)], im = a[2 * ((size_t)i + (size_t)n * j) + 1];
+            if (re == 0.0 && im == 0.0) continue;
+            ci.push_back(j);
+            cv.push_back(re);
+            cv.push_back(im);
+        }
+        rp[i + 1] = (int64_t)ci.size();
+    }
+    return csr_expv(n, rp, ci, cv, v, tn, ta, w_seq, klim, m, tol);
+}
+
+// krylov_zcooexpv (krylov_expv.h:9): the same for a sparse A in 1-based COO
+// (ia rows, ja columns, MatExp.cpp:188-196); duplicates are summed.
+int krylov_zcooexpv(const int n, const int nz, const int* ia, const int* ja, const double* a, const double* v,
+                    const int tn, const double* ta, double* w_seq, const int klim, const int m, const double tol,
+                    const int itrace) {
+    (void)itrace;
+    if (n <= 0 || nz < 0 || (nz > 0 && (!ia || !ja || !a)) || !v || tn <= 0 || !ta || !w_seq) return OOPS_ERR_ARG;
+    std::vector<std::vector<std::pair<int32_t, std::complex<double>>>> rows(n);
+    for (int e = 0; e < nz; ++e) {
+        const int i = ia[e] - 1, j = ja[e] - 1;
+        if (i < 0 || i >= n || j < 0 || j >= n) return OOPS_ERR_ARG;
+        rows[i].push_back({j, std::complex<double>(a[2 * e], a[2 * e + 1])});
+    }
+    std::vector<int64_t> rp(n + 1, 0);
+    std::vector<int32_t> ci;
+    std::vector<double> cv;
+    for (int i = 0; i < n; ++i) {
+        auto& r = rows[i];
+        std::sort(r.begin(), r.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+        for (size_t q = 0; q < r.size(); ++q) {
+            if (!ci.empty() && (int64_t)ci.size() > rp[i] && ci.back() == r[q].first) {
+                cv[cv.size() - 2] += r[q].second.real();
+                cv[cv.size() - 1] += r[q].second.imag();
+                continue;
+            }
+            ci.push_back(r[q].first);
+            cv.push_back(r[q].second.real());
+            cv.push_back(r[q].second.imag());
+        }
+        rp[i + 1] = (int64_t)ci.size();
+    }
+    return csr_expv(n, rp, ci, cv, v, tn, ta, w_seq, klim, m, tol);
+}
+
+static int hamvec_complex(oops_ctx* ctx, const oops_term_table& T, const std::complex<double>* cc,
+                          const double2* dv, double2* dw, hipStream_t st) {
+    LargeRun R;
+    int rc = setup(ctx, &T, 1, R, cc);
+    if (rc) return rc;
+    R.s = st;
+    matvec(R, dv, nullptr, dw, nullptr, 0);
+    rc = hip_check(ctx, hipGetLastError(), "matvec launch");
+    return rc ? rc : hip_check(ctx, hipStreamSynchronize(st), "sync matvec");
+}
+
+// hamvec_func3_ (hamvec_func3.cpp:399): w = sum_t coeff_zplx[t] (x)_b A_tb v,
+// host vectors; w_med is not needed.
+void hamvec_func3_(int* nspin, int* nTerm, double* coeff_lst_zplx, size_t* nbody_lst, size_t* pos_i_idx,
+                   size_t* pos_i_lst, size_t* dim_i_lst, size_t* mat_i_idx, double* mat_i_lst, size_t* ham_dim,
+                   size_t* nspin_dim, size_t* nspin_m_lst, size_t* nspin_n_lst, double* v, double* w, double* w_med) {
+    (void)nspin_m_lst; (void)nspin_n_lst; (void)w_med;
+    oops_ctx* ctx = abi_ctx();
+    if (!ctx) return;
+    std::vector<double> ones(*nTerm, 1.0);
+    oops_term_table T = term_table(*nspin, *nTerm, ones.data(), nbody_lst, pos_i_idx, pos_i_lst, dim_i_lst, mat_i_idx,
+                                   mat_i_lst, nspin_dim);
+    const size_t n = *ham_dim;
+    DevBuf dv, dw;
+    if (alloc(ctx, dv, sizeof(double2) * n) || alloc(ctx, dw, sizeof(double2) * n)) return;
+    if (hipMemcpy(dv.p, v, sizeof(double2) * n, hipMemcpyHostToDevice) != hipSuccess) return;
+    if (hamvec_complex(ctx, T, reinterpret_cast<const std::complex<double>*>(coeff_lst_zplx), (const double2*)dv.p,
+                       (double2*)dw.p, ctx->stream))
+        return;
+    (void)hipMemcpy(w, dw.p, sizeof(double2) * n, hipMemcpyDeviceToHost);
+}
+
+// hamvec_cuda3_init_ / _term_ (hamvec_cuda3.cu:661-665): the handle is an
+// oops_ctx on device 0 (the reference stores a cuBLAS handle there).
+void hamvec_cuda3_init_(size_t* handle) {
+    oops_ctx* c = nullptr;
+    if (handle) *handle = oops_ctx_create(0, &c) == OOPS_OK ? (size_t)c : 0;
+}
+
+void hamvec_cuda3_term_(size_t* handle) {
+    if (handle && *handle) {
+        oops_ctx_destroy((oops_ctx*)*handle);
+        *handle = 0;
+    }
+}
+
+// hamvec_cuda3_ (hamvec_cuda3.cu:661): device pointers passed as size_t; the
+// local matrices are device memory (dev_mat_i_lst), coefficients on the host.
+void hamvec_cuda3_(size_t* handle, int* nspin, int* nTerm, double* coeff_lst_zplx, size_t* nbody_lst,
+                   size_t* pos_i_idx, size_t* pos_i_lst, size_t* dim_i_lst, size_t* mat_i_idx,
+                   size_t* dev_mat_i_lst, size_t* ham_dim, size_t* nspin_dim, size_t* nspin_m_lst,
+                   size_t* nspin_n_lst, size_t* dev_v, size_t* dev_w, size_t* dev_w_med, size_t* dev_coeff,
+                   size_t* maxThreadsPerBlock, size_t* maxGridSize) {
+    (void)ham_dim; (void)nspin_m_lst; (void)nspin_n_lst; (void)dev_w_med; (void)dev_coeff;
+    (void)maxThreadsPerBlock; (void)maxGridSize;
+    oops_ctx* ctx = handle ? (oops_ctx*)*handle : nullptr;
+    if (!ctx) return;
+    size_t nbodies = 0;
+    for (int t = 0; t < *nTerm; ++t) nbodies += nbody_lst[t];
+    const size_t nmat = mat_i_idx[nbodies];
+    std::vector<double> mats(2 * std::max<size_t>(nmat, 1));
+    if (nmat && hipMemcpy(mats.data(), (const void*)*dev_mat_i_lst, sizeof(double2) * nmat, hipMemcpyDeviceToHost) !=
+                    hipSuccess)
+        return;
+    std::vector<double> ones(*nTerm, 1.0);
+    oops_term_table T = term_table(*nspin, *nTerm, ones.data(), nbody_lst, pos_i_idx, pos_i_lst, dim_i_lst, mat_i_idx,
+                                   mats.data(), nspin_dim);
+    hamvec_complex(ctx, T, reinterpret_cast<const std::complex<double>*>(coeff_lst_zplx), (const double2*)*dev_v,
+                   (double2*)*dev_w, ctx->stream);
+}
+
+// zgpadm_ (expokit.h:6): exp(t H) of an m x m matrix by Pade(ideg = 6) with
+// scaling and squaring on the device; the result is written to wsp(1..m*m)
+// (*iexph = 1), *ns squarings; *iflag < 0 on error.  m <= 42.
+void zgpadm_(int* ideg, int* m, double* t, double* H, int* ldh, double* wsp, int* lwsp, int* ipiv, int* iexph,
+             int* ns, int* iflag) {
+    (void)ipiv;
+    *iflag = 0;
+    const int mm = *m;
+    if (*ideg != 6 || mm < 1 || mm > kLargeMaxM + 2 || *ldh < mm || *lwsp < mm * mm) { *iflag = -1; return; }
+    oops_ctx* ctx = abi_ctx();
+    if (!ctx) { *iflag = -2; return; }
+    std::vector<double> h((size_t)2 * mm * (mm + 1), 0.0);  // one spare column (tiled reads)
+    for (int j = 0; j < mm; ++j)
+        for (int i = 0; i < mm; ++i) {
+            h[2 * (i + (size_t)mm * j)] = H[2 * (i + (size_t)(*ldh) * j)];
+            h[2 * (i + (size_t)mm * j) + 1] = H[2 * (i + (size_t)(*ldh) * j) + 1];
+        }
+    DevBuf dh, dout, dst;
+    if (upload(ctx, dh, h) || alloc(ctx, dout, sizeof(double2) * mm * mm) || alloc(ctx, dst, sizeof(int) * 4)) {
+        *iflag = -2;
+        return;
+    }
+    if (launch_large_pade_full((const double2*)dh.p, mm, mm, *t, (double2*)dout.p, (int*)dst.p, ctx->stream) != 0) {
+        *iflag = -2;
+        return;
+    }
+    int st = 0;
+    if (hipMemcpyAsync(wsp, dout.p, sizeof(double2) * mm * mm, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(&st, dst.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        *iflag = -2;
+        return;
+    }
+    if (st < 0) { *iflag = st == -2 ? -8 : -9; return; }
+    *ns = st;
+    *iexph = 1;
 }
 
 }  // extern "C"

@@ -50,6 +50,11 @@ int launch_hahn_bucket(const KrylovArgs& a, int m, int precision, int grid, hipS
 // (n > m); the caller zeroes the work counter first.
 int launch_hahn_lanczos(const KrylovArgs& a, int m, int grid, hipStream_t stream);
 void launch_zero_counter(int32_t* counter, hipStream_t stream);
+// Dense FP64-MFMA propagator kernel for n = 64 buckets whose items all have
+// uniform time grids starting at 0 (Hahn echo or CPMG-N); `work` holds
+// dense64_workspace_doubles(grid) doubles.  The caller zeroes the counter.
+int launch_dense64(const KrylovArgs& a, double* work, int grid, hipStream_t stream);
+int64_t dense64_workspace_doubles(int grid);
 int krylov_entry_capacity(int m);
 
 }  // namespace oops

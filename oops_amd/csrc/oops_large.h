@@ -24,11 +24,23 @@ void launch_large_dots(const double2* V, int64_t ldv, const double2* p, int64_t 
                        int nblk, hipStream_t s);
 void launch_large_finish(const double* part, int nblk, int nrows, int jcol, int mode, double2* hess, int ldh,
                          double2* coef, double* scal, int* flags, double break_tol, hipStream_t s);
+// Fused Gram-Schmidt passes: part_l = V_l^H p (l <= J < kLargeMaxM); and
+// p -= sum_l coef_l V_l followed by part_l = V_l^H p (norm_only = 0) or
+// part_0 = |p|^2 (norm_only = 1).  part rows are laid out as for large_dots.
+void launch_large_gs_dots(const double2* V, int64_t ldv, const double2* p, int64_t n, int J, double* part, int nblk,
+                          hipStream_t s);
+void launch_large_gs_update(const double2* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef,
+                            int norm_only, double* part, int nblk, hipStream_t s);
 void launch_large_update(const double2* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef, int grid,
                          hipStream_t s);
 int launch_large_pade(const double2* hess, int ldh, int mx, double t, double2* F, int* status, hipStream_t s);
 void launch_large_combine(const double2* V, int64_t ldv, int mx, const double2* F, double beta, double2* w, int64_t n,
                           double* part, int grid, hipStream_t s);
+void launch_large_csr_matvec(const int64_t* row_ptr, const int32_t* col, const double2* val, const double2* x,
+                             const double* xscale, double2* y, double2* xstore, int64_t n, int alpha_mode, int grid,
+                             hipStream_t s);
+// exp(t H) for H m x m (m <= kLargeMaxM + 2), full matrix, leading dimension m in out
+int launch_large_pade_full(const double2* hess, int ldh, int mx, double t, double2* out, int* status, hipStream_t s);
 void launch_large_axpby(double2 a, const double2* x, double2 b, double2* y, int64_t n, int grid, hipStream_t s);
 
 }  // namespace oops

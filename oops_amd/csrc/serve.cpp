@@ -1,0 +1,315 @@
+// Serving loop: continuous batching of CCE decoherence requests on one GPU.
+//
+// Pipeline per request: queued -> built on a host worker (bath, cluster
+// enumeration, every cluster's operators) -> admitted when an in-flight slot
+// and enough of the device-memory budget are free (upload + asynchronous
+// launch of its size-bucketed kernels on its own stream) -> completed on a
+// later poll (results, CCE product, memory back to the budget).  HIP calls
+// happen only on the thread that calls oops_server_poll.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/oops_cce.h"
+#include "../../include/oops_expv.h"
+#include "../../include/oops_serve.h"
+#include "host_common.h"
+
+using namespace oops;
+
+namespace {
+
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+enum State : int32_t { kQueued = 0, kBuilding = 1, kBuilt = 2, kRunning = 3, kDone = 4 };
+
+struct Req {
+    int64_t id = 0;
+    oops_request r{};
+    double t_submit = 0.0;
+    int32_t state = kQueued;
+    int32_t err = 0;
+    std::vector<double> coords;
+    oops_clusters* C = nullptr;
+    CceItems items;
+    int64_t est_bytes = 0;
+    oops_hahn_batch* batch = nullptr;
+    int64_t footprint = 0;
+    int64_t nclusters = 0;
+    std::vector<double> L;
+    double latency = 0.0;
+    ~Req() {
+        if (batch) oops_hahn_batch_destroy(batch);
+        if (C) oops_clusters_destroy(C);
+    }
+};
+
+// Device bytes a request will hold, from its operators' term counts (an upper
+// bound of the merged ELL tables: one d x d block per spin, one d^2 x d^2 per pair).
+int64_t estimate_bytes(const CceItems& it) {
+    int64_t b = 0;
+    for (const EchoIn& e : it.items) {
+        const oops_term_table* ops[4] = {e.opA, e.opB, e.opC, e.opD};
+        int64_t n = 1;
+        for (int s = 0; s < e.opA->nspin; ++s) n *= e.opA->spin_dim[s];
+        for (const oops_term_table* o : ops) {
+            if (!o) continue;
+            const int64_t k = o->nspin, d = o->spin_dim[0];
+            const int64_t ent = k * d * 4 + k * (k - 1) / 2 * d * d * 16;
+            b += 64 + (k + k * (k - 1) / 2) * 16 + ent * 20;
+        }
+        b += 32 * n + 8 * (int64_t)e.nt + 4 * 24 + 64;
+    }
+    return b + 4096;
+}
+
+}  // namespace
+
+struct oops_server {
+    oops_ctx* ctx = nullptr;
+    int64_t budget = 0, used = 0;
+    int32_t max_inflight = 1, m = 30, precision = 1, inner_threads = 1;
+    double tol = 1e-3;
+    std::mutex mu;
+    std::condition_variable cv;
+    bool stop = false;
+    std::vector<std::thread> workers;
+    std::map<int64_t, std::unique_ptr<Req>> reqs;
+    std::deque<int64_t> to_build, built;
+    std::vector<int64_t> running;
+    int64_t next_id = 1;
+
+    void worker() {
+        for (;;) {
+            Req* q = nullptr;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return stop || !to_build.empty(); });
+                if (stop) return;
+                q = reqs[to_build.front()].get();
+                to_build.pop_front();
+                q->state = kBuilding;
+            }
+            int rc = build(*q);
+            std::lock_guard<std::mutex> lk(mu);
+            if (rc) {
+                q->err = rc;
+                q->state = rc;
+            } else {
+                q->state = kBuilt;
+                built.push_back(q->id);
+            }
+        }
+    }
+
+    int build(Req& q) {
+        const oops_request& r = q.r;
+        if (r.order < 1 || r.order > 6 || r.npulse < 1 || r.nt < 2 || !(r.t_end > 0.0)) return OOPS_ERR_ARG;
+        const double a = 3.567;
+        int32_t n = 0;
+        int rc = oops_bath_diamond(a, r.radius, r.p, r.seed, 0, nullptr, 0, &n);
+        if (rc && rc != OOPS_ERR_CAPACITY) return rc;
+        q.coords.resize(3 * (size_t)std::max(n, 1));
+        rc = oops_bath_diamond(a, r.radius, r.p, r.seed, 0, q.coords.data(), n, &n);
+        if (rc) return rc;
+        rc = oops_clusters_enumerate(q.coords.data(), n, r.cutoff, r.order, &q.C);
+        if (rc) return rc;
+        oops_cce_params P;
+        std::memset(&P, 0, sizeof(P));
+        for (int c = 0; c < 3; ++c) P.B[c] = r.B[c];
+        P.state0 = 0;
+        P.state1 = 1;
+        P.dephasing_axis[2] = 1.0;
+        std::vector<double> t(r.nt);
+        for (int k = 0; k < r.nt; ++k) t[k] = r.t_end * k / (r.nt - 1) * (r.npulse == 1 ? 0.5 : 1.0);
+        rc = cce_build_items(q.coords.data(), q.C, &P, r.npulse, r.nt, t.data(), inner_threads, q.items);
+        if (rc) return rc;
+        q.nclusters = (int64_t)q.items.items.size();
+        q.est_bytes = estimate_bytes(q.items);
+        return OOPS_OK;
+    }
+
+    // Admit built requests in arrival order while a slot and the budget allow.
+    int admit(int32_t& launched) {
+        launched = 0;
+        for (;;) {
+            Req* q = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                if (built.empty() || (int32_t)running.size() >= max_inflight) return OOPS_OK;
+                q = reqs[built.front()].get();
+                if (q->est_bytes > budget) {
+                    built.pop_front();
+                    q->err = q->state = OOPS_ERR_CAPACITY;
+                    continue;
+                }
+                if (!running.empty() && used + q->est_bytes > budget) return OOPS_OK;
+                built.pop_front();
+            }
+            int rc = batch_create(ctx, (int32_t)q->items.items.size(), q->items.items.data(), true, m, 10, tol,
+                                  precision, &q->batch);
+            if (!rc) rc = oops_echo_batch_footprint(q->batch, &q->footprint);
+            if (!rc) rc = oops_echo_batch_launch(q->batch);
+            q->items.items.clear();
+            q->items.hold.reset();  // host copies of the operators are no longer needed
+            std::lock_guard<std::mutex> lk(mu);
+            if (rc) {
+                q->err = q->state = rc;
+                if (q->batch) { oops_hahn_batch_destroy(q->batch); q->batch = nullptr; }
+                continue;
+            }
+            q->state = kRunning;
+            used += q->footprint;
+            running.push_back(q->id);
+            ++launched;
+        }
+    }
+
+    int complete(int32_t& done) {
+        done = 0;
+        std::vector<int64_t> ids;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            ids = running;
+        }
+        for (int64_t id : ids) {
+            Req* q = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                q = reqs[id].get();
+            }
+            int32_t fin = 0;
+            int rc = oops_echo_batch_query(q->batch, &fin);
+            if (!rc && !fin) continue;
+            if (!rc) rc = oops_echo_batch_wait(q->batch);
+            const int nt = q->r.nt, K = q->r.order;
+            std::vector<double> Lall((size_t)q->nclusters * nt), fin_prod((size_t)nt * K);
+            if (!rc) rc = oops_hahn_batch_results(q->batch, Lall.data(), nullptr);
+            if (!rc) rc = oops_cce_reduce(q->C, nt, Lall.data(), nullptr, fin_prod.data(), nullptr, nullptr, nullptr);
+            if (!rc) {
+                q->L.resize(nt);
+                for (int k = 0; k < nt; ++k) q->L[k] = fin_prod[(size_t)k * K + (K - 1)];
+            }
+            oops_hahn_batch_destroy(q->batch);
+            q->batch = nullptr;
+            std::lock_guard<std::mutex> lk(mu);
+            used -= q->footprint;
+            running.erase(std::find(running.begin(), running.end(), id));
+            q->latency = now_s() - q->t_submit;
+            q->state = rc ? rc : kDone;
+            q->err = rc;
+            ++done;
+        }
+        return OOPS_OK;
+    }
+};
+
+extern "C" {
+
+int oops_server_create(oops_ctx* ctx, int64_t hbm_budget, int32_t max_inflight, int32_t m, double tol,
+                       int32_t precision, int32_t threads, oops_server** out) {
+    if (!ctx || !out || hbm_budget <= 0 || max_inflight < 1 || threads < 1) return OOPS_ERR_ARG;
+    std::unique_ptr<oops_server> s(new oops_server());
+    s->ctx = ctx;
+    s->budget = hbm_budget;
+    s->max_inflight = max_inflight;
+    s->m = m;
+    s->tol = tol;
+    s->precision = precision;
+    const int nworkers = std::max(1, std::min(4, threads / 4));
+    s->inner_threads = std::max(1, threads / nworkers);
+    oops_server* raw = s.get();
+    for (int w = 0; w < nworkers; ++w) s->workers.emplace_back([raw] { raw->worker(); });
+    *out = s.release();
+    return OOPS_OK;
+}
+
+int oops_server_submit(oops_server* s, const oops_request* r, int64_t* id) {
+    if (!s || !r) return OOPS_ERR_ARG;
+    std::unique_ptr<Req> q(new Req());
+    q->r = *r;
+    q->t_submit = now_s();
+    std::lock_guard<std::mutex> lk(s->mu);
+    q->id = s->next_id++;
+    if (id) *id = q->id;
+    s->to_build.push_back(q->id);
+    s->reqs[q->id] = std::move(q);
+    s->cv.notify_one();
+    return OOPS_OK;
+}
+
+int oops_server_poll(oops_server* s, int32_t* completed) {
+    if (!s) return OOPS_ERR_ARG;
+    if (hipSetDevice(s->ctx->device) != hipSuccess) return set_err(s->ctx, OOPS_ERR_HIP, "hipSetDevice");
+    int32_t done = 0, launched = 0;
+    int rc = s->complete(done);
+    if (!rc) rc = s->admit(launched);
+    if (completed) *completed = done;
+    return rc;
+}
+
+int oops_server_state(oops_server* s, int64_t id, int32_t* state) {
+    if (!s || !state) return OOPS_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto f = s->reqs.find(id);
+    if (f == s->reqs.end()) return OOPS_ERR_ARG;
+    *state = f->second->state;
+    return OOPS_OK;
+}
+
+int oops_server_result(oops_server* s, int64_t id, double* L, double* latency_s, int64_t* nclusters,
+                       int64_t* footprint) {
+    if (!s) return OOPS_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto f = s->reqs.find(id);
+    if (f == s->reqs.end()) return OOPS_ERR_ARG;
+    const Req& q = *f->second;
+    if (q.state != kDone) return q.state < 0 ? q.state : OOPS_ERR_ARG;
+    if (L) std::memcpy(L, q.L.data(), sizeof(double) * q.L.size());
+    if (latency_s) *latency_s = q.latency;
+    if (nclusters) *nclusters = q.nclusters;
+    if (footprint) *footprint = q.footprint;
+    return OOPS_OK;
+}
+
+int oops_server_pending(oops_server* s, int32_t* pending) {
+    if (!s || !pending) return OOPS_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    int32_t p = 0;
+    for (auto& kv : s->reqs)
+        if (kv.second->state >= 0 && kv.second->state != kDone) ++p;
+    *pending = p;
+    return OOPS_OK;
+}
+
+int oops_server_destroy(oops_server* s) {
+    if (!s) return OOPS_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv.notify_all();
+    for (auto& w : s->workers) w.join();
+    if (s->ctx) hipSetDevice(s->ctx->device);
+    for (int64_t id : s->running) {
+        Req* q = s->reqs[id].get();
+        if (q->batch) oops_echo_batch_wait(q->batch);
+    }
+    delete s;
+    return OOPS_OK;
+}
+
+}  // extern "C"

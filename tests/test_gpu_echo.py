@@ -3,7 +3,9 @@
 CPMG-N (Pulse_Interval / riffle, misc.cpp:59-84) against the oracle's
 restatement of the reference's production dense propagator
 (PiecewiseFullMatrixVectorEvolution, QuantumEvolutionAlgorithm.cpp:60-83):
-the dense path (n <= m, FP64) to 1e-10, the mixed Lanczos path to 1e-6.
+the dense paths (n <= m VALU kernel, n = 64 FP64-MFMA kernel on uniform
+grids) to 1e-10, the mixed Lanczos path (n = 256, or non-uniform grids) to
+1e-6.
 Items of different sizes, pulse counts and time grids give the same L(t)
 alone and mixed in one batch (the kernels compute each item independently),
 and an echo batch with npulse = 1 equals the Hahn batch on the same times."""
@@ -60,7 +62,34 @@ def test_cpmg_dense_path_matches_reference_propagator(ctx, k, npulse):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("k,npulse", [(3, 2), (3, 4), (4, 2)])
+@pytest.mark.parametrize("npulse", [1, 2, 4, 16])
+def test_dense64_mfma_path_matches_reference_propagator(ctx, npulse):
+    t = np.linspace(0.0, 1e-3, 33)
+    for coords in _clusters(3, 3):
+        print(f"dense64 k=3 N={npulse}", flush=True)
+        it = _item(coords, npulse, t / 2 if npulse == 1 else t)
+        for prec in ("fp64", "mixed"):
+            b = O.EchoBatch.from_items(ctx, [it], m=30, precision=prec)
+            b.run()
+            L, st = b.results()
+            assert (st[:, 8] & 0xFF).max() == 0
+            assert np.abs(L[0] - _oracle_cpmg(coords, npulse, t)).max() < 1e-10
+
+
+@pytest.mark.gpu
+def test_nonuniform_grid_n64_uses_krylov_and_matches_fp64(ctx):
+    t = np.sort(np.concatenate([[0.0], np.random.default_rng(1).uniform(0, 5e-4, 12)]))
+    for coords in _clusters(3, 2):
+        it = _item(coords, 1, t)
+        ref = O.EchoBatch.from_items(ctx, [it], m=30, tol=1e-12, precision="fp64")
+        ref.run()
+        mix = O.EchoBatch.from_items(ctx, [it], m=30, tol=1e-4, precision="mixed")
+        mix.run()
+        assert np.abs(ref.results()[0][0] - mix.results()[0][0]).max() < 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k,npulse", [(4, 2), (4, 4)])
 def test_cpmg_lanczos_path_matches_reference_propagator(ctx, k, npulse):
     t = np.linspace(0.0, 4e-4, 17)
     for coords in _clusters(k, 2):

@@ -95,3 +95,27 @@ def test_expv_rejects_bad_arguments(ctx):
         O.expv(ctx, T, g["c3_v"], [1e-5, 0.0], m=10)
     with pytest.raises(O.OopsError):
         O.expv(ctx, T, g["c3_v"], [0.0, 1e-5], m=41)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k", [5, 6])
+def test_hahn_batch_large_clusters_match_oracle(ctx, k):
+    """Clusters above 256 Liouville elements (CCE-5/6: n = 1024, 4096) run on
+    the single-state path inside a batch; L(t) against the oracle's two-sweep
+    Krylov restatement to 1e-10."""
+    coords = _liouville_cluster(k, seed=2)
+    par = O.cce_params(B=B111)
+    A, rho, D = O.cluster_terms(coords, par, 0)
+    Bd, _, _ = O.cluster_terms(coords, par, 2)
+    tau = np.linspace(0.0, 2e-5, 5)
+    print(f"large hahn k={k} n={A.n}", flush=True)
+    b = O.HahnBatch.from_items(ctx, [(A, Bd, rho, rho, D)], m=30, tol=1e-12, precision="fp64")
+    b.run(tau)
+    L, st = b.results()
+    assert (st[:, 8] & 0xFF).max() == 0 and st[0, 0] > 0
+    tA = P.TermTable(k, A.coeff, A.nbody, A.pos_offset, A.pos, A.dims, A.mat_offset, A.mats, A.spin_dim, A.n)
+    tB = P.TermTable(k, Bd.coeff, Bd.nbody, Bd.pos_offset, Bd.pos, Bd.dims, Bd.mat_offset, Bd.mats, Bd.spin_dim,
+                     Bd.n)
+    Lr, _, rc = P.hahn_batch([tA], [tB], [rho], [rho], [D], tau, m=30, nthreads=1)
+    assert rc == 0
+    assert np.abs(L[0] - Lr[0]).max() < 1e-10 * max(1.0, np.abs(Lr[0]).max())

@@ -1,0 +1,63 @@
+"""GPU: the serving loop (oops_server_*).
+
+A request's CCE product L(t) is the same when it runs alone and when it runs
+in a mix with other requests admitted while it is on the device (each cluster
+is computed independently; tolerance 1e-12), and equals the batch API plus the
+host reduction on the same bath."""
+import time
+
+import numpy as np
+import pytest
+
+import oops_amd as O
+
+B111 = np.ones(3) * 0.05 / np.sqrt(3.0)
+
+
+def _req(seed, radius, order, npulse, nt, field=0.05):
+    return {"seed": seed, "radius": radius, "p": 0.011, "B": list(np.ones(3) * field / np.sqrt(3.0)),
+            "cutoff": 8.0, "order": order, "npulse": npulse, "nt": nt, "t_end": 1e-3}
+
+
+@pytest.mark.gpu
+def test_request_alone_equals_request_in_the_mix(ctx):
+    target = _req(7, 22.0, 3, 2, 21)
+    s = O.Server(ctx, hbm_budget=1 << 30, max_inflight=4, tol=1e-4, threads=8)
+    rid = s.submit(target)
+    s.drain()
+    alone = s.result(rid)["L"]
+    s.close()
+    s = O.Server(ctx, hbm_budget=1 << 30, max_inflight=4, tol=1e-4, threads=8)
+    others = [_req(1, 20.0, 2, 1, 31), _req(2, 25.0, 3, 4, 17, 0.2), _req(3, 18.0, 3, 1, 41)]
+    ids = [s.submit(others[0]), s.submit(others[1])]
+    time.sleep(0.05)
+    s.poll()
+    mid = s.submit(target)  # arrives while the others are being built or run
+    ids.append(s.submit(others[2]))
+    s.drain()
+    mixed = s.result(mid)["L"]
+    for i in ids:
+        r = s.result(i)
+        assert r["clusters"] > 0 and np.all(np.isfinite(r["L"])) and abs(r["L"][0] - 1.0) < 1e-9
+    s.close()
+    print("max |alone - mix| =", np.abs(alone - mixed).max(), flush=True)
+    assert np.abs(alone - mixed).max() <= 1e-12
+
+
+@pytest.mark.gpu
+def test_served_hahn_equals_batch_and_reduction(ctx):
+    req = _req(4, 24.0, 3, 1, 26)
+    s = O.Server(ctx, hbm_budget=1 << 30, max_inflight=2, tol=1e-4, threads=8)
+    rid = s.submit(req)
+    s.drain()
+    L = s.result(rid)["L"]
+    s.close()
+    X = O.bath_diamond(3.567, 24.0, 0.011, seed=4)
+    C = O.Clusters(X, 8.0, 3)
+    nt = 26
+    tl = (1e-3 * np.arange(nt)) / (nt - 1)          # the server's grid, same rounding
+    b = C.hahn_batch(ctx, O.cce_params(B=B111), m=30, tol=1e-4, precision="mixed")
+    b.run(tl * 0.5)
+    Lc, _ = b.results()
+    want = C.reduce(Lc.reshape(-1), len(tl))["final"][:, -1]
+    assert np.abs(L - want).max() <= 1e-12

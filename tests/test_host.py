@@ -17,11 +17,18 @@ B111 = np.ones(3) * 0.05 / np.sqrt(3.0)
 def test_library_exports_every_declared_symbol():
     lib = O.lib()
     names = set()
-    for h in ("oops_expv.h", "oops_cce.h"):
+    for h in ("oops_expv.h", "oops_cce.h", "oops_serve.h"):
         txt = open(os.path.join(ROOT, "include", h)).read()
         names |= set(re.findall(r"^\s*(?:int|double|const char\*)\s+(oops_\w+)\s*\(", txt, re.M))
     assert len(names) > 20
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    # link-compatible symbols of the reference's expv library
+    txt = open(os.path.join(ROOT, "include", "oops_expv.h")).read()
+    ref = set(re.findall(r"^\s*(?:int|void)\s+((?:krylov_\w+)|(?:\w+_))\s*\(", txt, re.M))
+    assert {"main_cache_", "main_mkl_", "krylov_zgexpv", "krylov_zcooexpv", "hamvec_func3_", "hamvec_cuda3_",
+            "hamvec_cuda3_init_", "hamvec_cuda3_term_", "zgpadm_"} <= ref
+    missing = [n for n in sorted(ref) if not hasattr(lib, n)]
     assert not missing, missing
 
 
