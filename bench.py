@@ -149,15 +149,15 @@ def tuned_mixed():
     return 30, 1e-4, "default"
 
 
-def roofline(counts, st, timing, m, flops_fn=krylov_flops):
-    """Roofline of the Krylov kernel over all its launches (every bucket with
-    n > m): algorithmic flops of the executed products and Krylov steps per
+def roofline(counts, st, timing, m, flops_fn=krylov_flops, kernel="lanczos_mixed"):
+    """Roofline of one Krylov kernel over all its launches (the buckets that
+    ran it): algorithmic flops of the executed products and Krylov steps per
     launch over the average launch time (HIP events on the launch stream)."""
     base, flops, ms, nl = 0, 0.0, 0.0, 0
     by_n = {b["n"]: b for b in timing}
     for o, cnt in enumerate(counts):
         n = 4 ** (o + 1)
-        if n > m and n in by_n and cnt > 0:
+        if n in by_n and by_n[n]["kernel"] == kernel and cnt > 0:
             sl = st[base:base + cnt]
             flops += flops_fn(n, o + 1, sl[:, 1].sum() + sl[:, 5].sum(), sl[:, 0].sum() + sl[:, 4].sum(), m)
             ms += by_n[n]["ms"]
@@ -190,6 +190,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", dest="no_fp64", action="store_true",
                     help="skip the FP64 comparison (profiling runs)")
+    ap.add_argument("--serve-requests", dest="serve_requests", type=int, default=20,
+                    help="requests of the serving segment (Poisson arrivals at 50/s); 0 skips it")
     ap.add_argument("--cpu-per", dest="cpu_per", type=int, default=None)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -248,7 +250,7 @@ def main():
         err_per_order = [float(dL[order_of == o].max()) if np.any(order_of == o) else 0.0
                          for o in range(len(counts))]
         roof64 = roofline([int(np.sum(order_of == o)) for o in range(len(counts))], st64, batch64.timing(),
-                             args.m)
+                             args.m, krylov_flops, "arnoldi_fp64")
         fp64 = {"m": args.m, "tol": args.tol, "sample_clusters": int(len(sel)), "value": rate64, "seconds": el64,
                 "buckets": batch64.timing(), "failed_clusters": int(((st64[:, 8] & 0xFF) != 0).sum()),
                 "roofline_frac": roof64["frac"], "roofline_achieved": roof64["achieved"],
@@ -277,6 +279,16 @@ def main():
                    "max_abs_dL": fp64["max_abs_dL"] if fp64 else None},
         "roofline": roof,
     }
+    if args.serve_requests > 0:
+        # serving segment: Poisson arrivals at 50 req/s of the serving request mix
+        # (bath r 3-5 nm, B 0.01-0.5 T, CPMG-N N in {1,2,4,8,16}, CCE 2-4, 101-401 points)
+        from oops_amd import serve_bench
+        reqs = serve_bench.make_requests(args.serve_requests, 0, 30.0, 50.0, 2e-3, 3, 4)
+        summary, _ = serve_bench.run(ctx, reqs, 50.0, 0, 32 << 30, 8, mtol, mm, 16, log=sys.stderr)
+        summary.pop("mix", None)
+        out["serving"] = summary
+        print(f"# serving p50 {summary['p50_latency_s']:.2f}s p99 {summary['p99_latency_s']:.2f}s",
+              file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(X, C, par, tlist, args, counts)
         out["config"]["mixed_over_cpu"] = value / out["cpu_baseline"]["value"]

@@ -93,6 +93,13 @@ int oops_hamvec(oops_ctx* ctx, const oops_term_table* H, const double* v, double
 int oops_expv(oops_ctx* ctx, const oops_term_table* H, const double* v, int32_t nt, const double* ta, int32_t klim,
               int32_t m, double tol, int32_t out_mode, double* out, double* stats);
 
+/* oops_expv with a precision choice: 0 FP64 throughout (= oops_expv), 1 the
+ * basis stored in complex FP32 (half the HBM traffic of the Gram-Schmidt
+ * passes; products, sums and the small exponential stay FP64). */
+int oops_expv_ex(oops_ctx* ctx, const oops_term_table* H, const double* v, int32_t nt, const double* ta,
+                 int32_t klim, int32_t m, double tol, int32_t precision, int32_t out_mode, double* out,
+                 double* stats);
+
 /* Fortran-ABI drop-in for the reference's main_cache_ (main_cache.h:6),
  * same argument list; runs oops_expv with out_mode 0 on device 0. */
 void main_cache_(size_t* nspin, size_t* nTerm, double* coeff_lst, size_t* nbody_lst, size_t* pos_i_idx,
@@ -205,6 +212,10 @@ int oops_echo_batch_footprint(const oops_hahn_batch* b, int64_t* bytes);
  * item count and kernel time in ms (HIP events on the launch stream). */
 int oops_hahn_batch_timing(const oops_hahn_batch* b, int32_t max_buckets, int32_t* n, int32_t* count, double* ms,
                            int32_t* nbuckets);
+/* Kernel each bucket ran with in the last run: 0 dense VALU Pade (n <= m),
+ * 1 Arnoldi FP64, 2 Arnoldi mixed, 3 Lanczos mixed, 4 dense FP64-MFMA
+ * propagator (n = 64), 5 single-state path (n > 256); -1 not run. */
+int oops_hahn_batch_kernels(const oops_hahn_batch* b, int32_t max_buckets, int32_t* kind);
 int oops_hahn_batch_destroy(oops_hahn_batch* b);
 
 /* DLARAN (LAPACK 48-bit multiplicative congruential generator, the Lanczos

@@ -60,6 +60,8 @@ int oops_server_state(oops_server* s, int64_t id, int32_t* state);
  * bytes it held.  Any output may be NULL. */
 int oops_server_result(oops_server* s, int64_t id, double* L, double* latency_s, int64_t* nclusters,
                        int64_t* footprint);
+/* When the request finished building and was launched (s since submit). */
+int oops_server_times(oops_server* s, int64_t id, double* built_s, double* launched_s);
 /* Requests not yet done. */
 int oops_server_pending(oops_server* s, int32_t* pending);
 int oops_server_destroy(oops_server* s);

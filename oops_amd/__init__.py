@@ -56,10 +56,11 @@ EXPORTS = [
     "oops_bath_diamond", "oops_nv_states", "oops_cce_cluster_terms", "oops_terms_view",
     "oops_terms_sizes", "oops_terms_destroy", "oops_clusters_enumerate", "oops_clusters_count",
     "oops_clusters_get", "oops_clusters_parents", "oops_clusters_destroy", "oops_cce_reduce",
-    "oops_cce_hahn_batch_create", "oops_hamvec", "oops_expv", "oops_echo_batch_create", "oops_echo_batch_launch",
+    "oops_cce_hahn_batch_create", "oops_hamvec", "oops_expv", "oops_expv_ex", "oops_echo_batch_create", "oops_echo_batch_launch",
     "oops_echo_batch_query", "oops_echo_batch_wait", "oops_echo_batch_run", "oops_echo_batch_footprint",
     "oops_cce_echo_batch_create", "oops_server_create", "oops_server_submit", "oops_server_poll", "oops_server_state",
-    "oops_server_result", "oops_server_pending", "oops_server_destroy",
+    "oops_server_result", "oops_server_pending", "oops_server_destroy", "oops_server_times",
+    "oops_hahn_batch_kernels",
 ]
 
 
@@ -96,12 +97,20 @@ def lib():
                                          ctypes.c_void_p, ctypes.c_void_p]
         L.oops_server_pending.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
         L.oops_server_destroy.argtypes = [ctypes.c_void_p]
+        L.oops_server_times.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
         L.oops_hamvec.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.oops_expv.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
                                 ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_void_p,
                                 ctypes.c_void_p]
+        L.oops_expv_ex.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                   ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                                   ctypes.c_void_p, ctypes.c_void_p]
         _lib = L
     return _lib
+
+
+KERNEL_NAMES = {0: "dense_pade", 1: "arnoldi_fp64", 2: "arnoldi_mixed", 3: "lanczos_mixed", 4: "dense64_mfma",
+                5: "single_state", -1: None}
 
 
 def _ptr(a):
@@ -191,19 +200,23 @@ def hamvec(ctx: Context, H: TermTable, v):
     return w
 
 
-def expv(ctx: Context, H: TermTable, v, ta, m=30, klim=10, tol=1e-12, overlaps=False):
+def expv(ctx: Context, H: TermTable, v, ta, m=30, klim=10, tol=1e-12, overlaps=False, precision="fp64"):
     """exp(-i H t) v at every time of ta (MatExpVector::runInexplicitGPU,
     MatExp.cpp:367-521).  Returns (w_seq [n x nt], stats) or, with
-    overlaps=True, (<v, w(t_k)> [nt], stats)."""
+    overlaps=True, (<v, w(t_k)> [nt], stats).  precision "fp32" stores the
+    Krylov basis in complex FP32."""
     v = np.ascontiguousarray(v, np.complex128)
     ta = np.ascontiguousarray(ta, np.float64)
     if v.shape != (H.n,):
         raise ValueError(f"v must have length {H.n}")
     out = np.empty(len(ta) if overlaps else (len(ta), H.n), np.complex128)
     st = np.zeros(8)
-    rc = lib().oops_expv(ctx.h, ctypes.byref(H.c), _ptr(v), ctypes.c_int32(len(ta)), _ptr(ta), ctypes.c_int32(klim),
-                         ctypes.c_int32(m), ctypes.c_double(tol), ctypes.c_int32(1 if overlaps else 0), _ptr(out),
-                         _ptr(st))
+    if precision not in ("fp64", "fp32"):
+        raise ValueError("precision must be fp64 or fp32")
+    rc = lib().oops_expv_ex(ctx.h, ctypes.byref(H.c), _ptr(v), ctypes.c_int32(len(ta)), _ptr(ta),
+                            ctypes.c_int32(klim), ctypes.c_int32(m), ctypes.c_double(tol),
+                            ctypes.c_int32(1 if precision == "fp32" else 0), ctypes.c_int32(1 if overlaps else 0),
+                            _ptr(out), _ptr(st))
     _check(rc, ctx.h, "oops_expv")
     keys = ["nmult", "nexph", "nscale", "nstep", "nreject", "ibrkflag", "anorm", "kernel_ms"]
     stats = dict(zip(keys, st.tolist()))
@@ -273,7 +286,11 @@ class HahnBatch:
         _check(lib().oops_hahn_batch_timing(self.h, ctypes.c_int32(32), _ptr(n), _ptr(cnt), _ptr(ms),
                                             ctypes.byref(nb)), self.ctx.h, "oops_hahn_batch_timing")
         k = nb.value
-        return [{"n": int(n[i]), "count": int(cnt[i]), "ms": float(ms[i])} for i in range(k)]
+        kind = np.full(32, -1, np.int32)
+        _check(lib().oops_hahn_batch_kernels(self.h, ctypes.c_int32(32), _ptr(kind)), self.ctx.h,
+               "oops_hahn_batch_kernels")
+        return [{"n": int(n[i]), "count": int(cnt[i]), "ms": float(ms[i]), "kernel": KERNEL_NAMES.get(int(kind[i]))}
+                for i in range(k)]
 
     def close(self):
         if self.h:
@@ -410,7 +427,11 @@ class Server:
         fp = ctypes.c_int64()
         _check(lib().oops_server_result(self.h, ctypes.c_int64(rid), _ptr(L), ctypes.byref(lat), ctypes.byref(ncl),
                                         ctypes.byref(fp)), self.ctx.h, "oops_server_result")
-        return {"L": L, "latency_s": lat.value, "clusters": ncl.value, "footprint": fp.value}
+        tb, tl = ctypes.c_double(), ctypes.c_double()
+        _check(lib().oops_server_times(self.h, ctypes.c_int64(rid), ctypes.byref(tb), ctypes.byref(tl)),
+               self.ctx.h, "oops_server_times")
+        return {"L": L, "latency_s": lat.value, "clusters": ncl.value, "footprint": fp.value,
+                "built_s": tb.value, "launched_s": tl.value}
 
     def drain(self, sleep=0.002):
         import time

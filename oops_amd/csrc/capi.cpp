@@ -198,6 +198,7 @@ struct oops_hahn_batch {
         int dense_grid = 0;
         // states above 256 elements: every item on the single-state path, in turn
         bool large = false;
+        int kernel = -1;  // kind that ran last: see oops_hahn_batch_kernels
         std::vector<CompiledOp> hA, hB;
         std::vector<std::vector<double>> hu0, hy0;
         std::vector<double> hscale, hL, htau;
@@ -541,6 +542,7 @@ static KrylovArgs bucket_args(const oops_hahn_batch* b, const oops_hahn_batch::B
 // (synchronous); L and stats kept on the host.
 static int bucket_run_large(oops_hahn_batch* b, oops_hahn_batch::Bucket& B, int32_t nt, const double* tau) {
     const size_t ni = B.items.size();
+    B.kernel = 5;
     int64_t total = 0;
     for (size_t i = 0; i < ni; ++i) total += b->per_item ? B.hnt[i] : nt;
     B.hL.assign((size_t)total, 0.0);
@@ -563,12 +565,15 @@ static int bucket_launch(oops_hahn_batch* b, oops_hahn_batch::Bucket& B, const K
     const bool lanczos = b->precision == 1 && B.hermitian && B.n > b->m;
     int lrc;
     if (dense64) {
+        B.kernel = 4;
         launch_zero_counter(a.queue, st);
         lrc = launch_dense64(a, (double*)B.work.p, B.dense_grid, st);
     } else if (lanczos) {
+        B.kernel = 3;
         launch_zero_counter(a.queue, st);
         lrc = launch_hahn_lanczos(a, b->m, B.grid, st);
     } else {
+        B.kernel = B.n <= b->m ? 0 : (b->precision == 0 ? 1 : 2);
         lrc = launch_hahn_bucket(a, b->m, b->precision == 0 ? 0 : 1, B.grid, st);
     }
     if (lrc != 0)
@@ -759,6 +764,12 @@ int oops_hahn_batch_timing(const oops_hahn_batch* b, int32_t max_buckets, int32_
         if (count) count[k] = (int32_t)b->buckets[k].items.size();
         if (ms) ms[k] = b->buckets[k].ms;
     }
+    return OOPS_OK;
+}
+
+int oops_hahn_batch_kernels(const oops_hahn_batch* b, int32_t max_buckets, int32_t* kind) {
+    if (!b || !kind) return OOPS_ERR_ARG;
+    for (int k = 0; k < (int)b->buckets.size() && k < max_buckets; ++k) kind[k] = b->buckets[k].kernel;
     return OOPS_OK;
 }
 

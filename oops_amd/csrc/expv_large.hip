@@ -17,7 +17,7 @@
 //                    (grid.y = vector index), 256-thread block reductions.
 //   large_finish     sums the partials, stores h into the Hessenberg column
 //                    and the reciprocal norm for the next product.
-//   large_update     p -= sum_l h_l V_l (one read of each basis vector).
+//   large_gs_*       fused Gram-Schmidt passes over the whole basis.
 //   large_pade       Pade(6) first column of exp(t H_aug) on one workgroup.
 //   large_combine    w = beta * sum_j F_j V_j.
 #include <hip/hip_runtime.h>
@@ -48,9 +48,19 @@ __device__ __forceinline__ void large_op_load(const LargeOpArgs& o, LargeOpLds& 
 // y = alpha * H (xscale * x); if xstore, xstore = xscale * x.
 // xscale is read from device memory (the reciprocal norm of the previous
 // Arnoldi step), alpha: 0 -> +1 (norm estimate), 1 -> -i.
+// Basis element types: FP64 (double2) or FP32 (float2) storage.
+__device__ __forceinline__ void vstore(double2* p, double re, double im) { *p = make_double2(re, im); }
+__device__ __forceinline__ void vstore(float2* p, double re, double im) { *p = make_float2((float)re, (float)im); }
+__device__ __forceinline__ double2 vload(const double2* p) { return *p; }
+__device__ __forceinline__ double2 vload(const float2* p) {
+    const float2 f = *p;
+    return make_double2((double)f.x, (double)f.y);
+}
+
+template <typename VT>
 __global__ __launch_bounds__(kLT) void large_matvec(LargeOpArgs o, const double2* __restrict__ x,
                                                     const double* __restrict__ xscale_p, double2* __restrict__ y,
-                                                    double2* __restrict__ xstore, int64_t n, int alpha_mode) {
+                                                    VT* __restrict__ xstore, int64_t n, int alpha_mode) {
     __shared__ LargeOpLds L;
     large_op_load(o, L);
     const double xs = xscale_p ? *xscale_p : 1.0;
@@ -63,7 +73,7 @@ __global__ __launch_bounds__(kLT) void large_matvec(LargeOpArgs o, const double2
         y[i] = make_double2(acc.re, acc.im);
         if (xstore) {
             const double2 xv = x[i];
-            xstore[i] = make_double2(xv.x * xs, xv.y * xs);
+            vstore(xstore + i, xv.x * xs, xv.y * xs);
         }
     }
 }
@@ -205,8 +215,8 @@ __device__ __forceinline__ void gs_block_reduce(const double (&sr)[MAXJ], const 
     (void)row0;
 }
 
-template <int MAXJ>
-__global__ __launch_bounds__(kLT) void large_gs_dots(const double2* __restrict__ V, int64_t ldv,
+template <int MAXJ, typename VT>
+__global__ __launch_bounds__(kLT) void large_gs_dots(const VT* __restrict__ V, int64_t ldv,
                                                      const double2* __restrict__ p, int64_t n, int J,
                                                      double* __restrict__ part) {
     __shared__ double red[2 * MAXJ * (kLT / kWave)];
@@ -218,7 +228,7 @@ __global__ __launch_bounds__(kLT) void large_gs_dots(const double2* __restrict__
 #pragma unroll
         for (int l = 0; l < MAXJ; ++l)
             if (l <= J) {
-                const double2 a = V[(int64_t)l * ldv + i];
+                const double2 a = vload(V + (int64_t)l * ldv + i);
                 sr[l] = fma(a.x, b.x, fma(a.y, b.y, sr[l]));
                 si[l] = fma(a.x, b.y, fma(-a.y, b.x, si[l]));
             }
@@ -226,8 +236,8 @@ __global__ __launch_bounds__(kLT) void large_gs_dots(const double2* __restrict__
     gs_block_reduce<MAXJ>(sr, si, J + 1, red, part, 0);
 }
 
-template <int MAXJ>
-__global__ __launch_bounds__(kLT) void large_gs_update(const double2* __restrict__ V, int64_t ldv,
+template <int MAXJ, typename VT>
+__global__ __launch_bounds__(kLT) void large_gs_update(const VT* __restrict__ V, int64_t ldv,
                                                        double2* __restrict__ p, int64_t n, int J,
                                                        const double2* __restrict__ coef, int norm_only,
                                                        double* __restrict__ part) {
@@ -244,7 +254,7 @@ __global__ __launch_bounds__(kLT) void large_gs_update(const double2* __restrict
 #pragma unroll
         for (int l = 0; l < MAXJ; ++l)
             if (l <= J) {
-                v[l] = V[(int64_t)l * ldv + i];
+                v[l] = vload(V + (int64_t)l * ldv + i);
                 acc = cfms(mkcd(c[l].x, c[l].y), mkcd(v[l].x, v[l].y), acc);
             }
         p[i] = make_double2(acc.re, acc.im);
@@ -260,22 +270,6 @@ __global__ __launch_bounds__(kLT) void large_gs_update(const double2* __restrict
         }
     }
     gs_block_reduce<MAXJ>(sr, si, norm_only ? 1 : J + 1, red, part, 0);
-}
-
-// p -= sum_{l<=J} coef_l V_l
-__global__ __launch_bounds__(kLT) void large_update(const double2* __restrict__ V, int64_t ldv, double2* __restrict__ p,
-                                                    int64_t n, int J, const double2* __restrict__ coef) {
-    __shared__ double2 c[kLargeMaxM + 2];
-    for (int l = threadIdx.x; l <= J; l += kLT) c[l] = coef[l];
-    __syncthreads();
-    for (int64_t i = (int64_t)blockIdx.x * kLT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kLT) {
-        cd acc = mkcd(p[i].x, p[i].y);
-        for (int l = 0; l <= J; ++l) {
-            const double2 v = V[(int64_t)l * ldv + i];
-            acc = cfms(mkcd(c[l].x, c[l].y), mkcd(v.x, v.y), acc);
-        }
-        p[i] = make_double2(acc.re, acc.im);
-    }
 }
 
 // F = exp(t H(0:mx,0:mx)) e1 for the augmented Hessenberg matrix (Pade(6),
@@ -319,7 +313,8 @@ __global__ __launch_bounds__(kLT) void large_pade_full(const double2* __restrict
 }
 
 // w = beta * sum_{j<mx} F_j V_j ; partial |w|^2 per block into part
-__global__ __launch_bounds__(kLT) void large_combine(const double2* __restrict__ V, int64_t ldv, int mx,
+template <typename VT>
+__global__ __launch_bounds__(kLT) void large_combine(const VT* __restrict__ V, int64_t ldv, int mx,
                                                      const double2* __restrict__ F, double beta,
                                                      double2* __restrict__ w, int64_t n, double* __restrict__ part) {
     __shared__ double2 c[kLargeMaxM + 2];
@@ -330,7 +325,7 @@ __global__ __launch_bounds__(kLT) void large_combine(const double2* __restrict__
     for (int64_t i = (int64_t)blockIdx.x * kLT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kLT) {
         cd acc = mkcd(0.0, 0.0);
         for (int l = 0; l < mx; ++l) {
-            const double2 v = V[(int64_t)l * ldv + i];
+            const double2 v = vload(V + (int64_t)l * ldv + i);
             acc = cfma(mkcd(c[l].x, c[l].y), mkcd(v.x, v.y), acc);
         }
         w[i] = make_double2(acc.re, acc.im);
@@ -366,9 +361,14 @@ int large_grid(int64_t n, int num_cu) {
     return (int)(want < cap ? want : cap);
 }
 
-void launch_large_matvec(const LargeOpArgs& o, const double2* x, const double* xscale, double2* y, double2* xstore,
-                         int64_t n, int alpha_mode, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(large_matvec, dim3(grid), dim3(kLT), 0, s, o, x, xscale, y, xstore, n, alpha_mode);
+void launch_large_matvec(const LargeOpArgs& o, const double2* x, const double* xscale, double2* y, void* xstore,
+                         int64_t n, int alpha_mode, int grid, hipStream_t s, bool fp32) {
+    if (fp32)
+        hipLaunchKernelGGL(large_matvec<float2>, dim3(grid), dim3(kLT), 0, s, o, x, xscale, y, (float2*)xstore, n,
+                           alpha_mode);
+    else
+        hipLaunchKernelGGL(large_matvec<double2>, dim3(grid), dim3(kLT), 0, s, o, x, xscale, y, (double2*)xstore, n,
+                           alpha_mode);
 }
 
 void launch_large_dots(const double2* V, int64_t ldv, const double2* p, int64_t n, int J, int with_norm, double* part,
@@ -383,20 +383,24 @@ void launch_large_finish(const double* part, int nblk, int nrows, int jcol, int 
                        flags, break_tol);
 }
 
-void launch_large_gs_dots(const double2* V, int64_t ldv, const double2* p, int64_t n, int J, double* part, int nblk,
-                          hipStream_t s) {
-    hipLaunchKernelGGL(large_gs_dots<kLargeMaxM>, dim3(nblk), dim3(kLT), 0, s, V, ldv, p, n, J, part);
+void launch_large_gs_dots(const void* V, int64_t ldv, const double2* p, int64_t n, int J, double* part, int nblk,
+                          hipStream_t s, bool fp32) {
+    if (fp32)
+        hipLaunchKernelGGL((large_gs_dots<kLargeMaxM, float2>), dim3(nblk), dim3(kLT), 0, s, (const float2*)V, ldv, p,
+                           n, J, part);
+    else
+        hipLaunchKernelGGL((large_gs_dots<kLargeMaxM, double2>), dim3(nblk), dim3(kLT), 0, s, (const double2*)V, ldv,
+                           p, n, J, part);
 }
 
-void launch_large_gs_update(const double2* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef,
-                            int norm_only, double* part, int nblk, hipStream_t s) {
-    hipLaunchKernelGGL(large_gs_update<kLargeMaxM>, dim3(nblk), dim3(kLT), 0, s, V, ldv, p, n, J, coef, norm_only,
-                       part);
-}
-
-void launch_large_update(const double2* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef, int grid,
-                         hipStream_t s) {
-    hipLaunchKernelGGL(large_update, dim3(grid), dim3(kLT), 0, s, V, ldv, p, n, J, coef);
+void launch_large_gs_update(const void* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef,
+                            int norm_only, double* part, int nblk, hipStream_t s, bool fp32) {
+    if (fp32)
+        hipLaunchKernelGGL((large_gs_update<kLargeMaxM, float2>), dim3(nblk), dim3(kLT), 0, s, (const float2*)V, ldv,
+                           p, n, J, coef, norm_only, part);
+    else
+        hipLaunchKernelGGL((large_gs_update<kLargeMaxM, double2>), dim3(nblk), dim3(kLT), 0, s, (const double2*)V, ldv,
+                           p, n, J, coef, norm_only, part);
 }
 
 int launch_large_pade(const double2* hess, int ldh, int mx, double t, double2* F, int* status, hipStream_t s) {
@@ -432,9 +436,14 @@ int launch_large_pade_full(const double2* hess, int ldh, int mx, double t, doubl
     return 0;
 }
 
-void launch_large_combine(const double2* V, int64_t ldv, int mx, const double2* F, double beta, double2* w, int64_t n,
-                          double* part, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(large_combine, dim3(grid), dim3(kLT), 0, s, V, ldv, mx, F, beta, w, n, part);
+void launch_large_combine(const void* V, int64_t ldv, int mx, const double2* F, double beta, double2* w, int64_t n,
+                          double* part, int grid, hipStream_t s, bool fp32) {
+    if (fp32)
+        hipLaunchKernelGGL(large_combine<float2>, dim3(grid), dim3(kLT), 0, s, (const float2*)V, ldv, mx, F, beta, w,
+                           n, part);
+    else
+        hipLaunchKernelGGL(large_combine<double2>, dim3(grid), dim3(kLT), 0, s, (const double2*)V, ldv, mx, F, beta,
+                           w, n, part);
 }
 
 void launch_large_axpby(double2 a, const double2* x, double2 b, double2* y, int64_t n, int grid, hipStream_t s) {

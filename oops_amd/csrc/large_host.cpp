@@ -51,6 +51,8 @@ struct LargeRun {
     // general sparse operator (CSR) instead of the Kronecker tables
     bool csr = false;
     DevBuf rp, ci, cv;
+    // basis stored in complex FP32 (the FP32 variant; Kronecker operators only)
+    bool fp32 = false;
     int64_t n = 0;
     int m = 0, ldh = 0, grid = 0, nblk = 0;
     DevBuf V, p0, p1, w, v0, part, hess, coef, scal, flags, F, status, wpart;
@@ -60,12 +62,17 @@ struct LargeRun {
 // y = alpha Op (xscale x), alpha_mode 0: +1, 1: -i.  A Kronecker term table
 // carries the Hamiltonian H (the driver works on -iH); a CSR matrix carries
 // the generator A itself (the driver works on A, as krylov_zgexpv does).
-void matvec(LargeRun& R, const double2* x, const double* xs, double2* y, double2* xstore, int alpha_mode) {
+void matvec(LargeRun& R, const double2* x, const double* xs, double2* y, void* xstore, int alpha_mode) {
     if (R.csr)
         launch_large_csr_matvec((const int64_t*)R.rp.p, (const int32_t*)R.ci.p, (const double2*)R.cv.p, x, xs, y,
-                                xstore, R.n, alpha_mode, R.grid, R.s);
+                                (double2*)xstore, R.n, alpha_mode, R.grid, R.s);
     else
-        launch_large_matvec(R.args, x, xs, y, xstore, R.n, alpha_mode, R.grid, R.s);
+        launch_large_matvec(R.args, x, xs, y, xstore, R.n, alpha_mode, R.grid, R.s, R.fp32);
+}
+
+// Basis column j.
+void* vcol(LargeRun& R, int j) {
+    return R.fp32 ? (void*)((float2*)R.V.p + (size_t)j * R.n) : (void*)((double2*)R.V.p + (size_t)j * R.n);
 }
 int arnoldi_mode(const LargeRun& R) { return R.csr ? 0 : 1; }
 
@@ -154,7 +161,7 @@ int setup_buffers(oops_ctx* ctx, int m, LargeRun& R) {
     R.grid = large_grid(R.n, ctx->num_cu);
     R.nblk = std::min(R.grid, 512);
     const size_t vb = sizeof(double2) * (size_t)R.n;
-    rc = rc ? rc : alloc(ctx, R.V, vb * (m + 1));
+    rc = rc ? rc : alloc(ctx, R.V, (R.fp32 ? vb / 2 : vb) * (m + 1));
     rc = rc ? rc : alloc(ctx, R.p0, vb);
     rc = rc ? rc : alloc(ctx, R.p1, vb);
     rc = rc ? rc : alloc(ctx, R.w, vb);
@@ -290,7 +297,6 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
         t_new = round_step((1.0 / anorm) * std::pow(p2 / (4.0 * beta * anorm), xm));
     }
     double t_now = 0.0;
-    double2* V = (double2*)R.V.p;
     double2* hess = (double2*)R.hess.p;
     double* scal = (double*)R.scal.p;
     hipEvent_t e0, e1;
@@ -313,22 +319,24 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
             const double2* x = (const double2*)R.w.p;
             for (int j = 0; j < m; ++j) {
                 double2* p = bufs[j & 1];
-                matvec(R, x, scal, p, V + (size_t)j * n, arnoldi_mode(R));
+                matvec(R, x, scal, p, vcol(R, j), arnoldi_mode(R));
                 // CGS2: dots, update + dots, update + norm (one read of V per pass)
-                launch_large_gs_dots(V, n, p, n, j, (double*)R.part.p, R.nblk, R.s);
+                launch_large_gs_dots(R.V.p, n, p, n, j, (double*)R.part.p, R.nblk, R.s, R.fp32);
                 launch_large_finish((double*)R.part.p, R.nblk, j + 1, j, 0, hess, R.ldh, (double2*)R.coef.p, scal,
                                     (int*)R.flags.p, break_tol, R.s);
-                launch_large_gs_update(V, n, p, n, j, (double2*)R.coef.p, 0, (double*)R.part.p, R.nblk, R.s);
+                launch_large_gs_update(R.V.p, n, p, n, j, (double2*)R.coef.p, 0, (double*)R.part.p, R.nblk, R.s,
+                                       R.fp32);
                 launch_large_finish((double*)R.part.p, R.nblk, j + 1, j, 1, hess, R.ldh, (double2*)R.coef.p, scal,
                                     (int*)R.flags.p, break_tol, R.s);
-                launch_large_gs_update(V, n, p, n, j, (double2*)R.coef.p, 1, (double*)R.part.p, R.nblk, R.s);
+                launch_large_gs_update(R.V.p, n, p, n, j, (double2*)R.coef.p, 1, (double*)R.part.p, R.nblk, R.s,
+                                       R.fp32);
                 launch_large_finish((double*)R.part.p, R.nblk, 1, j, 2, hess, R.ldh, nullptr, scal, (int*)R.flags.p,
                                     break_tol, R.s);
                 x = p;
             }
             // extra product for the error estimate: V[m] = p / h(m, m-1), p' = -iH V[m]
             double2* pe = bufs[m & 1];
-            matvec(R, x, scal, pe, V + (size_t)m * n, arnoldi_mode(R));
+            matvec(R, x, scal, pe, vcol(R, m), arnoldi_mode(R));
             launch_large_dots(pe, 0, pe, n, -1, 1, (double*)R.part.p, R.nblk, R.s);
             launch_large_finish((double*)R.part.p, R.nblk, 1, 0, 3, hess, R.ldh, nullptr, scal, (int*)R.flags.p,
                                 break_tol, R.s);
@@ -385,7 +393,8 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
                 break;
             }
             mx = mb + std::max(0, k1 - 1);
-            launch_large_combine(V, n, mx, (double2*)R.F.p, beta, (double2*)R.w.p, n, (double*)R.wpart.p, R.grid, R.s);
+            launch_large_combine(R.V.p, n, mx, (double2*)R.F.p, beta, (double2*)R.w.p, n, (double*)R.wpart.p, R.grid,
+                                 R.s, R.fp32);
             hipEventRecord(e1, R.s);
             std::vector<double> wp(R.grid);
             rc = hip_check(R.ctx, hipMemcpyAsync(wp.data(), R.wpart.p, sizeof(double) * R.grid, hipMemcpyDeviceToHost, R.s),
@@ -492,12 +501,21 @@ int oops_hamvec(oops_ctx* ctx, const oops_term_table* H, const double* v, double
 
 int oops_expv(oops_ctx* ctx, const oops_term_table* H, const double* v, int32_t nt, const double* ta, int32_t klim,
               int32_t m, double tol, int32_t out_mode, double* out, double* stats) {
+    return oops_expv_ex(ctx, H, v, nt, ta, klim, m, tol, 0, out_mode, out, stats);
+}
+
+int oops_expv_ex(oops_ctx* ctx, const oops_term_table* H, const double* v, int32_t nt, const double* ta,
+                 int32_t klim, int32_t m, double tol, int32_t precision, int32_t out_mode, double* out,
+                 double* stats) {
     if (!ctx || !H || !v || nt <= 0 || !ta) return set_err(ctx, OOPS_ERR_ARG, "bad arguments");
+    if (precision != 0 && precision != 1) return set_err(ctx, OOPS_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32 basis)");
+    if (out_mode != 0 && out_mode != 1) return set_err(ctx, OOPS_ERR_ARG, "out_mode must be 0 or 1");
     if (m < 2 || m > kLargeMaxM) return set_err(ctx, OOPS_ERR_UNSUPPORTED, "m must be in [2, 40] on the large path");
     if (klim < 1 || klim > 64) return set_err(ctx, OOPS_ERR_ARG, "klim must be in [1, 64]");
     for (int k = 1; k < nt; ++k)
         if (!(ta[k] >= ta[k - 1])) return set_err(ctx, OOPS_ERR_ARG, "output times must be ascending");
     LargeRun R;
+    R.fp32 = precision == 1;
     int rc = setup(ctx, H, m, R);
     if (rc) return rc;
     if (m >= R.n) return set_err(ctx, OOPS_ERR_ARG, "the Krylov dimension must be smaller than the state dimension");

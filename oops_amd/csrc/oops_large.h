@@ -18,8 +18,9 @@ struct LargeOpArgs {
 };
 
 int large_grid(int64_t n, int num_cu);
-void launch_large_matvec(const LargeOpArgs& o, const double2* x, const double* xscale, double2* y, double2* xstore,
-                         int64_t n, int alpha_mode, int grid, hipStream_t s);
+// xstore (basis column, may be NULL) is float2 when fp32, else double2.
+void launch_large_matvec(const LargeOpArgs& o, const double2* x, const double* xscale, double2* y, void* xstore,
+                         int64_t n, int alpha_mode, int grid, hipStream_t s, bool fp32 = false);
 void launch_large_dots(const double2* V, int64_t ldv, const double2* p, int64_t n, int J, int with_norm, double* part,
                        int nblk, hipStream_t s);
 void launch_large_finish(const double* part, int nblk, int nrows, int jcol, int mode, double2* hess, int ldh,
@@ -27,15 +28,13 @@ void launch_large_finish(const double* part, int nblk, int nrows, int jcol, int 
 // Fused Gram-Schmidt passes: part_l = V_l^H p (l <= J < kLargeMaxM); and
 // p -= sum_l coef_l V_l followed by part_l = V_l^H p (norm_only = 0) or
 // part_0 = |p|^2 (norm_only = 1).  part rows are laid out as for large_dots.
-void launch_large_gs_dots(const double2* V, int64_t ldv, const double2* p, int64_t n, int J, double* part, int nblk,
-                          hipStream_t s);
-void launch_large_gs_update(const double2* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef,
-                            int norm_only, double* part, int nblk, hipStream_t s);
-void launch_large_update(const double2* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef, int grid,
-                         hipStream_t s);
+void launch_large_gs_dots(const void* V, int64_t ldv, const double2* p, int64_t n, int J, double* part, int nblk,
+                          hipStream_t s, bool fp32 = false);
+void launch_large_gs_update(const void* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef,
+                            int norm_only, double* part, int nblk, hipStream_t s, bool fp32 = false);
 int launch_large_pade(const double2* hess, int ldh, int mx, double t, double2* F, int* status, hipStream_t s);
-void launch_large_combine(const double2* V, int64_t ldv, int mx, const double2* F, double beta, double2* w, int64_t n,
-                          double* part, int grid, hipStream_t s);
+void launch_large_combine(const void* V, int64_t ldv, int mx, const double2* F, double beta, double2* w, int64_t n,
+                          double* part, int grid, hipStream_t s, bool fp32 = false);
 void launch_large_csr_matvec(const int64_t* row_ptr, const int32_t* col, const double2* val, const double2* x,
                              const double* xscale, double2* y, double2* xstore, int64_t n, int alpha_mode, int grid,
                              hipStream_t s);

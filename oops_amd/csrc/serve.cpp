@@ -51,6 +51,7 @@ struct Req {
     int64_t nclusters = 0;
     std::vector<double> L;
     double latency = 0.0;
+    double t_built = 0.0, t_launched = 0.0;  // since submit
     ~Req() {
         if (batch) oops_hahn_batch_destroy(batch);
         if (C) oops_clusters_destroy(C);
@@ -110,6 +111,7 @@ struct oops_server {
                 q->state = rc;
             } else {
                 q->state = kBuilt;
+                q->t_built = now_s() - q->t_submit;
                 built.push_back(q->id);
             }
         }
@@ -172,6 +174,7 @@ struct oops_server {
                 continue;
             }
             q->state = kRunning;
+            q->t_launched = now_s() - q->t_submit;
             used += q->footprint;
             running.push_back(q->id);
             ++launched;
@@ -282,6 +285,16 @@ int oops_server_result(oops_server* s, int64_t id, double* L, double* latency_s,
     if (latency_s) *latency_s = q.latency;
     if (nclusters) *nclusters = q.nclusters;
     if (footprint) *footprint = q.footprint;
+    return OOPS_OK;
+}
+
+int oops_server_times(oops_server* s, int64_t id, double* built_s, double* launched_s) {
+    if (!s) return OOPS_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto f = s->reqs.find(id);
+    if (f == s->reqs.end()) return OOPS_ERR_ARG;
+    if (built_s) *built_s = f->second->t_built;
+    if (launched_s) *launched_s = f->second->t_launched;
     return OOPS_OK;
 }
 

@@ -70,6 +70,8 @@ def run(ctx, requests, rate, seed, budget, inflight, tol, m, threads, log=None):
         "mean_latency_s": float(lat.mean()), "max_latency_s": float(lat.max()),
         "clusters": clusters, "clusters_per_s": clusters / wall,
         "max_footprint_bytes": int(max(r["footprint"] for r in res)),
+        "mean_build_s": float(np.mean([r["built_s"] for r in res])),
+        "mean_device_s": float(np.mean([r["latency_s"] - r["launched_s"] for r in res])),
         "mix": {"orders": [q["order"] for q in requests], "npulse": [q["npulse"] for q in requests],
                 "nt": [q["nt"] for q in requests]},
     }, res
