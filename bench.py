@@ -36,25 +36,35 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 
 
-def krylov_flops(n, k, nmult, nstep, m):
-    """Algorithmic flops: merged matvec 8 n (k s + P s^2) per product (s = 4
-    Liouville, P = k(k-1)/2 pairs) + one classical Gram-Schmidt pass per
-    Arnoldi step, 8 n m (m+1) per Krylov cycle of size m."""
+def matvec_flops(n, k):
+    """Algorithmic flops of one merged product (one s x s operator per spin,
+    one s^2 x s^2 operator per pair, s = 4 Liouville): 8 n (k s + P s^2),
+    P = k(k-1)/2.  The flip-mask kernel executes fewer: 1 + 2k + k(k-1)
+    complex multiply-adds per element (21 at k = 4, against 112 here)."""
     s = 4
     P = k * (k - 1) // 2
-    mv = 8.0 * n * (k * s + P * s * s)
-    return nmult * mv + nstep * 8.0 * n * m * (m + 1)
+    return 8.0 * n * (k * s + P * s * s)
+
+
+def krylov_flops(n, k, nmult, nstep, m):
+    """Algorithmic flops of the Arnoldi path: the products + one classical
+    Gram-Schmidt pass per Arnoldi step, 8 n m (m+1) per Krylov cycle of size m."""
+    return nmult * matvec_flops(n, k) + nstep * 8.0 * n * m * (m + 1)
 
 
 def lanczos_flops(n, k, nmult, nstep, m):
-    """Algorithmic flops of the mixed Lanczos kernel: merged matvec per
-    product + per Lanczos step one fused <v_j, p>, |p|^2 reduction and the
-    three-term recurrence update (4 complex dot/axpy passes, 32 n) + the
-    (m+1)-term combination of the update (8 n (m+1))."""
-    s = 4
-    P = k * (k - 1) // 2
-    mv = 8.0 * n * (k * s + P * s * s)
-    return nmult * mv + nstep * (32.0 * n * m + 8.0 * n * (m + 1))
+    """Algorithmic flops of the mixed Lanczos kernel: the products + per
+    Lanczos step one fused <v_j, p>, |p|^2 reduction and the three-term
+    recurrence update (4 complex dot/axpy passes, 32 n) + the (m+1)-term
+    combination of the update (8 n (m+1))."""
+    return nmult * matvec_flops(n, k) + nstep * (32.0 * n * m + 8.0 * n * (m + 1))
+
+
+# device function of each kernel kind (oops_hahn_batch_kernels), for the
+# rocprofv3 summaries
+DEVICE_NAMES = {"lanczos_slot": "cce_hahn_lanczos_slot_kernel<{m}>", "lanczos_mixed": "cce_hahn_lanczos_kernel<{m}>",
+                "arnoldi_fp64": "cce_hahn_krylov_kernel<{m}, false>", "arnoldi_mixed": "cce_hahn_krylov_kernel<{m}, true>",
+                "dense64_mfma": "cce_dense64_kernel", "dense_pade": "cce_hahn_dense_kernel"}
 
 
 def build_workload(args):
@@ -129,15 +139,15 @@ def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` on this workload, from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
     calibration), or None when no summary covers it."""
+    import glob
     import re
-    path = os.path.join(ROOT, "profiles", "cce_hahn_lanczos_cce4_r5nm_pmc.md")
-    if not os.path.exists(path):
-        return None
-    txt = open(path).read()
-    if kernel.split(" ")[0] not in txt:
-        return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+-]+)", txt)
-    return float(m.group(1)) if m else None
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_cce4_r5nm_pmc.md"))):
+        txt = open(path).read()
+        m = re.search(r"kernel:\s*(\S+)", txt)
+        t = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+-]+)", txt)
+        if m and t and m.group(1) == kernel:
+            return float(t.group(1))
+    return None
 
 
 def tuned_mixed():
@@ -233,8 +243,13 @@ def main():
     timing = batch.timing()
     value = nclu * args.steps / elapsed
 
-    roof = roofline(counts, st, timing, mm, lanczos_flops)
-    roof["kernel"] = f"cce_hahn_lanczos_kernel<{mm}>"
+    # dominant kernel: the kind with the most device time
+    per_kind = {}
+    for bkt in timing:
+        per_kind[bkt["kernel"]] = per_kind.get(bkt["kernel"], 0.0) + bkt["ms"]
+    dom = max(per_kind, key=per_kind.get)
+    roof = roofline(counts, st, timing, mm, lanczos_flops, dom)
+    roof["kernel"] = DEVICE_NAMES.get(dom, dom).format(m=mm)
     roof["traffic"] = measured_traffic(roof["kernel"])
     fp64 = None
     if batch64 is not None:

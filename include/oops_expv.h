@@ -69,6 +69,18 @@ typedef struct {
     const int64_t* spin_dim;    /* [nspin] */
 } oops_term_table;
 
+/* Flip-mask (slot) form of an operator, computed on the host: with
+ * power-of-two local dimensions every entry of a merged local operator moves
+ * element i to i ^ mask, so (H x)_i = sum_s coef[s][i] x[i ^ masks[s]].
+ * Slots [0, nslot64) change at most one spin's digits (slot 0 is mask 0),
+ * slots [nslot64, nslot64 + nslot32) change two.  With masks or coef NULL
+ * only the counts are returned; otherwise masks gets nslot64 + nslot32 values
+ * and coef (nslot64 + nslot32) x n complex values, slot-major.  No reference
+ * counterpart: it is the layout the mixed Lanczos kernel builds on the device
+ * (OOPS_ERR_CAPACITY beyond max_slots). */
+int oops_slot_form(const oops_term_table* T, int32_t max_slots, int32_t* nslot64, int32_t* nslot32, int32_t* masks,
+                   double* coef);
+
 /* --- context ------------------------------------------------------------ */
 int oops_ctx_create(int32_t device, oops_ctx** out);
 int oops_ctx_destroy(oops_ctx* ctx);
@@ -159,8 +171,11 @@ typedef struct {
  * 1: mixed — 2-body couplings applied in complex FP32 and the basis stored in
  * FP32; 1-body terms, the recurrence coefficients, re-orthogonalisation and
  * the small exponential in FP64.  Buckets whose generators are all Hermitian
- * run Lanczos with an eigendecomposition of the tridiagonal matrix, the
- * others Arnoldi + Pade.  2: mixed, always Arnoldi + Pade. */
+ * run Lanczos with a Chebyshev exponential of the tridiagonal matrix (in the
+ * flip-mask form when every operator has at most 9 one-spin and 12 two-spin
+ * flip masks: all products of 4-spin spin-1/2 Liouvillians), the others
+ * Arnoldi + Pade.  2: mixed, always Arnoldi + Pade.  3: as 1 without the
+ * flip-mask form (ELL product in the Lanczos kernel). */
 int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* items, int32_t m, int32_t klim,
                            double tol, int32_t precision, oops_hahn_batch** out);
 /* Run every bucket for the ascending sweep times tau[0..nt) (device-resident
@@ -199,6 +214,10 @@ typedef struct {
  * at offset nt_0 + .. + nt_{i-1}). */
 int oops_echo_batch_create(oops_ctx* ctx, int32_t nitems, const oops_echo_item* items, int32_t m, int32_t klim,
                            double tol, int32_t precision, oops_hahn_batch** out);
+/* Capture the batch's kernel launches into a HIP graph; later launches
+ * replay it (one submission for all buckets).  Not for batches with states
+ * above 256 elements. */
+int oops_echo_batch_capture(oops_hahn_batch* b);
 /* Asynchronous launch on the batch's stream; query without blocking; wait. */
 int oops_echo_batch_launch(oops_hahn_batch* b);
 int oops_echo_batch_query(oops_hahn_batch* b, int32_t* done);
@@ -214,7 +233,8 @@ int oops_hahn_batch_timing(const oops_hahn_batch* b, int32_t max_buckets, int32_
                            int32_t* nbuckets);
 /* Kernel each bucket ran with in the last run: 0 dense VALU Pade (n <= m),
  * 1 Arnoldi FP64, 2 Arnoldi mixed, 3 Lanczos mixed, 4 dense FP64-MFMA
- * propagator (n = 64), 5 single-state path (n > 256); -1 not run. */
+ * propagator (n = 64), 5 single-state path (n > 256), 6 Lanczos mixed in the
+ * flip-mask (slot) form; -1 not run. */
 int oops_hahn_batch_kernels(const oops_hahn_batch* b, int32_t max_buckets, int32_t* kind);
 int oops_hahn_batch_destroy(oops_hahn_batch* b);
 

@@ -60,7 +60,7 @@ EXPORTS = [
     "oops_echo_batch_query", "oops_echo_batch_wait", "oops_echo_batch_run", "oops_echo_batch_footprint",
     "oops_cce_echo_batch_create", "oops_server_create", "oops_server_submit", "oops_server_poll", "oops_server_state",
     "oops_server_result", "oops_server_pending", "oops_server_destroy", "oops_server_times",
-    "oops_hahn_batch_kernels",
+    "oops_hahn_batch_kernels", "oops_echo_batch_capture", "oops_slot_form",
 ]
 
 
@@ -110,7 +110,7 @@ def lib():
 
 
 KERNEL_NAMES = {0: "dense_pade", 1: "arnoldi_fp64", 2: "arnoldi_mixed", 3: "lanczos_mixed", 4: "dense64_mfma",
-                5: "single_state", -1: None}
+                5: "single_state", 6: "lanczos_slot", -1: None}
 
 
 def _ptr(a):
@@ -118,7 +118,7 @@ def _ptr(a):
 
 
 def _precision_code(p):
-    codes = {"fp64": 0, "mixed": 1, "mixed-arnoldi": 2}
+    codes = {"fp64": 0, "mixed": 1, "mixed-arnoldi": 2, "mixed-ell": 3}
     if p not in codes:
         raise ValueError(f"precision must be one of {sorted(codes)}")
     return codes[p]
@@ -188,6 +188,22 @@ class Context:
             self.close()
         except Exception:
             pass
+
+
+def slot_form(H: TermTable, max_slots=4096):
+    """Flip-mask form of H (host): (masks, coef, nslot64) with
+    (H x)_i = sum_s coef[s, i] x[i ^ masks[s]]; the first nslot64 masks change
+    at most one spin's digits.  The layout the mixed Lanczos kernel builds."""
+    n64, n32 = ctypes.c_int32(), ctypes.c_int32()
+    _check(lib().oops_slot_form(ctypes.byref(H.c), ctypes.c_int32(max_slots), ctypes.byref(n64), ctypes.byref(n32),
+                                None, None), None, "oops_slot_form")
+    S = n64.value + n32.value
+    n = int(np.prod(H.spin_dim))
+    masks = np.zeros(S, np.int32)
+    coef = np.zeros((S, n), np.complex128)
+    _check(lib().oops_slot_form(ctypes.byref(H.c), ctypes.c_int32(max_slots), ctypes.byref(n64), ctypes.byref(n32),
+                                _ptr(masks), _ptr(coef)), None, "oops_slot_form")
+    return masks, coef, n64.value
 
 
 def hamvec(ctx: Context, H: TermTable, v):
@@ -341,6 +357,10 @@ class EchoBatch(HahnBatch):
         b = cls(ctx, h, len(items))
         b.nts = nts
         return b
+
+    def capture(self):
+        """Record the batch's launches in a HIP graph; later runs replay it."""
+        _check(lib().oops_echo_batch_capture(self.h), self.ctx.h, "oops_echo_batch_capture")
 
     def launch(self):
         _check(lib().oops_echo_batch_launch(self.h), self.ctx.h, "oops_echo_batch_launch")

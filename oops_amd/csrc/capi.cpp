@@ -97,6 +97,28 @@ int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err
     out.ld_bits = ld;
     out.c0 = c0;
     auto stride = [&](int s) { return (int64_t)1 << (ld * (ns - 1 - s)); };
+    // flip-mask slots (slot 0 = mask 0, the diagonal)
+    std::map<int64_t, int> slot64_of, slot32_of;
+    out.mask64.push_back(0);
+    slot64_of[0] = 0;
+    auto slot_of = [&](int64_t mask) -> uint8_t {
+        int fields = 0;
+        for (int q = 0; q < ns; ++q)
+            if ((mask >> (ld * (ns - 1 - q))) & (d - 1)) ++fields;
+        auto& tab = fields <= 1 ? slot64_of : slot32_of;
+        auto& lst = fields <= 1 ? out.mask64 : out.mask32;
+        auto f = tab.find(mask);
+        int id;
+        if (f != tab.end()) {
+            id = f->second;
+        } else {
+            id = (int)lst.size();
+            tab[mask] = id;
+            lst.push_back((int32_t)mask);
+        }
+        if (id > 127) return 0;  // counted below; the slot path is then unavailable
+        return (uint8_t)(fields <= 1 ? id : (id | 128));
+    };
     auto add_local = [&](int nb, int s, int u, const std::vector<cplx>& M, int64_t rows) {
         int R = 0;
         for (int64_t r = 0; r < rows; ++r) {
@@ -115,14 +137,20 @@ int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err
             for (int64_t c = 0; c < rows; ++c) {
                 const cplx v = M[r + rows * c];
                 if (v == cplx(0.0)) continue;
-                int64_t off;
-                if (nb == 1) off = c * stride(s);
-                else off = (c / d) * stride(s) + (c % d) * stride(u);
+                int64_t off, mask;
+                if (nb == 1) {
+                    off = c * stride(s);
+                    mask = (r ^ c) * stride(s);
+                } else {
+                    off = (c / d) * stride(s) + (c % d) * stride(u);
+                    mask = ((r / d) ^ (c / d)) * stride(s) + ((r % d) ^ (c % d)) * stride(u);
+                }
                 out.val.push_back(v);
                 out.off.push_back((int32_t)off);
+                out.eslot.push_back(slot_of(mask));
                 ++cnt;
             }
-            for (; cnt < R; ++cnt) { out.val.push_back(cplx(0.0)); out.off.push_back(0); }
+            for (; cnt < R; ++cnt) { out.val.push_back(cplx(0.0)); out.off.push_back(0); out.eslot.push_back(0); }
         }
         out.locs.push_back(lo);
         out.max_row = std::max(out.max_row, R);
@@ -131,6 +159,9 @@ int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err
         if (!single[s].empty()) add_local(1, s, s, single[s], d);
     out.ent2_first = (int32_t)out.val.size();
     for (auto& kv : pair) add_local(2, kv.first.first, kv.first.second, kv.second, d * d);
+    out.nslot64 = (int)out.mask64.size();
+    out.nslot32 = (int)out.mask32.size();
+    if (out.nslot64 > 128 || out.nslot32 > 128) out.nslot64 = -1;
     // Hermitian check of every merged local operator (sufficient for H = H^dagger)
     auto is_herm = [](const std::vector<cplx>& M, int64_t rows) {
         double mx = 0.0;
@@ -196,6 +227,9 @@ struct oops_hahn_batch {
         bool uniform = true;    // every item's times are k * dt from 0 (dense propagator powers apply)
         DevBuf work;            // dense64 workspace
         int dense_grid = 0;
+        DevBuf eslot, smask, slotws;  // slot form of the mixed Lanczos kernel
+        bool slots = true;            // every operator fits kSlot64 / kSlot32
+        int slot_grid = 0;
         // states above 256 elements: every item on the single-state path, in turn
         bool large = false;
         int kernel = -1;  // kind that ran last: see oops_hahn_batch_kernels
@@ -217,7 +251,11 @@ struct oops_hahn_batch {
     hipStream_t stream = nullptr;
     std::vector<hipEvent_t> ev;
     bool launched = false;
+    hipGraph_t graph = nullptr;        // captured launch sequence (oops_echo_batch_capture)
+    hipGraphExec_t graph_exec = nullptr;
     ~oops_hahn_batch() {
+        if (graph_exec) hipGraphExecDestroy(graph_exec);
+        if (graph) hipGraphDestroy(graph);
         for (auto& e : ev) hipEventDestroy(e);
         if (stream) hipStreamDestroy(stream);
     }
@@ -278,8 +316,11 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
                        double tol, int32_t precision, oops_hahn_batch** out) {
     if (!ctx || !out || nitems < 0 || (nitems > 0 && !items)) return set_err(ctx, OOPS_ERR_ARG, "bad arguments");
     *out = nullptr;
-    if (precision < 0 || precision > 2)
-        return set_err(ctx, OOPS_ERR_ARG, "precision must be 0 (FP64), 1 (mixed) or 2 (mixed, Arnoldi)");
+    if (precision < 0 || precision > 3)
+        return set_err(ctx, OOPS_ERR_ARG,
+                       "precision must be 0 (FP64), 1 (mixed), 2 (mixed, Arnoldi) or 3 (mixed, ELL product)");
+    const bool ell_only = precision == 3;   // mixed Lanczos without the slot form
+    if (ell_only) precision = 1;
     if (m != 20 && m != 30) return set_err(ctx, OOPS_ERR_UNSUPPORTED, "Krylov dimension must be 20 or 30");
     if (klim < 1 || klim > 16) return set_err(ctx, OOPS_ERR_ARG, "klim must be in [1, 16]");
     std::unique_ptr<oops_hahn_batch> b(new oops_hahn_batch());
@@ -303,6 +344,8 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
         std::vector<int32_t> opC, opD, npulse, nt;
         std::vector<int64_t> tau_off, L_off;
         std::vector<double> tau;
+        std::vector<uint8_t> eslot;
+        std::vector<int32_t> smask;
     };
     b->per_item = per_item;
     std::vector<Host> hosts;
@@ -410,6 +453,14 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
             H.loc.insert(H.loc.end(), C.locs.begin(), C.locs.end());
             for (auto& v : C.val) { H.val.push_back(v.real()); H.val.push_back(v.imag()); }
             H.off.insert(H.off.end(), C.off.begin(), C.off.end());
+            H.eslot.insert(H.eslot.end(), C.eslot.begin(), C.eslot.end());
+            if (C.nslot64 < 0 || C.nslot64 > kSlot64 || C.nslot32 > kSlot32) b->buckets[bk].slots = false;
+            for (int q = 0; q < kSlotMaskStride; ++q) {
+                int32_t mk = 0;
+                if (q < kSlot64 && q < (int)C.mask64.size()) mk = C.mask64[q];
+                if (q >= kSlot64 && q - kSlot64 < kSlot32 && q - kSlot64 < (int)C.mask32.size()) mk = C.mask32[q - kSlot64];
+                H.smask.push_back(mk);
+            }
         }
         H.voff.push_back((int64_t)(H.u0.size() / 2));
         for (int64_t i = 0; i < 2 * n; ++i) {
@@ -475,6 +526,17 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
         const bool dense = B.n <= m;
         const int per_cu = dense ? 4 : 2;
         B.grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu * per_cu));
+        B.slots = B.slots && !ell_only && precision == 1 && B.hermitian && !dense && B.n <= 256 && (m == 20 || m == 30);
+        if (B.slots) {
+            // two workgroups per CU; 2 (Hahn) or 4 (CPMG-N) coefficient tables each
+            B.slot_grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu * 2));
+            rc = rc ? rc : dev_upload(ctx, B.eslot, H.eslot);
+            rc = rc ? rc : dev_upload(ctx, B.smask, H.smask);
+            const size_t bytes = sizeof(double) * 2 * (size_t)kSlotOpD2 * 4 * (size_t)B.slot_grid;
+            rc = rc ? rc : hip_check(ctx, hipMalloc(&B.slotws.p, bytes), "hipMalloc slot tables");
+            if (!rc) B.slotws.bytes = bytes;
+            if (rc) return rc;
+        }
         if (B.n == 64) {
             // one workgroup per CU (the kernel holds 152 KB of LDS)
             B.dense_grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu));
@@ -487,7 +549,7 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
     for (auto& B : b->buckets)
         for (DevBuf* d : {&B.hdr, &B.loc, &B.val, &B.off, &B.opA, &B.opB, &B.opC, &B.opD, &B.voff, &B.u0, &B.y0,
                           &B.scale, &B.rnd, &B.L, &B.stats, &B.queue, &B.npulse, &B.nt, &B.tau_off, &B.L_off, &B.tau,
-                          &B.work})
+                          &B.work, &B.eslot, &B.smask, &B.slotws})
             b->footprint += (int64_t)d->bytes;
     if (per_item) {
         if (hipStreamCreate(&b->stream) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipStreamCreate");
@@ -520,6 +582,9 @@ static KrylovArgs bucket_args(const oops_hahn_batch* b, const oops_hahn_batch::B
     a.L = (double*)B.L.p;
     a.stats = (int32_t*)B.stats.p;
     a.queue = (int32_t*)B.queue.p;
+    a.eslot = (const uint8_t*)B.eslot.p;
+    a.smask = (const int32_t*)B.smask.p;
+    a.slotws = (double2*)B.slotws.p;
     a.nitems = (int32_t)B.items.size();
     a.n = B.n;
     a.klim = b->klim;
@@ -568,6 +633,10 @@ static int bucket_launch(oops_hahn_batch* b, oops_hahn_batch::Bucket& B, const K
         B.kernel = 4;
         launch_zero_counter(a.queue, st);
         lrc = launch_dense64(a, (double*)B.work.p, B.dense_grid, st);
+    } else if (lanczos && B.slots && B.slotws.p) {
+        B.kernel = 6;
+        launch_zero_counter(a.queue, st);
+        lrc = launch_hahn_lanczos_slot(a, b->m, B.slot_grid, st);
     } else if (lanczos) {
         B.kernel = 3;
         launch_zero_counter(a.queue, st);
@@ -602,10 +671,40 @@ int oops_echo_batch_create(oops_ctx* ctx, int32_t nitems, const oops_echo_item* 
     return batch_create(ctx, nitems, nitems > 0 && items ? in.data() : nullptr, true, m, klim, tol, precision, out);
 }
 
+int oops_echo_batch_capture(oops_hahn_batch* b) {
+    if (!b || !b->per_item) return OOPS_ERR_ARG;
+    oops_ctx* ctx = b->ctx;
+    for (auto& B : b->buckets)
+        if (B.large) return set_err(ctx, OOPS_ERR_UNSUPPORTED, "batches with states above 256 elements run on the host loop");
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
+    if (b->graph_exec) return OOPS_OK;
+    int rc = hip_check(ctx, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    if (rc) return rc;
+    int lrc = OOPS_OK;
+    for (auto& B : b->buckets) {
+        const KrylovArgs a = bucket_args(b, B, 0, nullptr);
+        lrc = bucket_launch(b, B, a, b->stream, B.uniform);
+        if (lrc) break;
+    }
+    hipGraph_t g = nullptr;
+    rc = hip_check(ctx, hipStreamEndCapture(b->stream, &g), "hipStreamEndCapture");
+    if (lrc) { if (g) hipGraphDestroy(g); return lrc; }
+    if (rc) return rc;
+    b->graph = g;
+    return hip_check(ctx, hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+}
+
 int oops_echo_batch_launch(oops_hahn_batch* b) {
     if (!b || !b->per_item) return OOPS_ERR_ARG;
     oops_ctx* ctx = b->ctx;
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
+    if (b->graph_exec) {
+        hipEventRecord(b->ev[0], b->stream);
+        int rc = hip_check(ctx, hipGraphLaunch(b->graph_exec, b->stream), "hipGraphLaunch");
+        hipEventRecord(b->ev[1], b->stream);
+        b->launched = rc == OOPS_OK;
+        return rc;
+    }
     for (size_t bk = 0; bk < b->buckets.size(); ++bk) {
         auto& B = b->buckets[bk];
         if (B.large) {
@@ -638,6 +737,12 @@ int oops_echo_batch_wait(oops_hahn_batch* b) {
     if (!b || !b->per_item || !b->launched) return OOPS_ERR_ARG;
     int rc = hip_check(b->ctx, hipStreamSynchronize(b->stream), "hipStreamSynchronize");
     if (rc) return rc;
+    if (b->graph_exec) {  // one interval for the whole replayed graph
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
+        for (auto& B : b->buckets) B.ms = ms / std::max<size_t>(1, b->buckets.size());
+        return OOPS_OK;
+    }
     for (size_t bk = 0; bk < b->buckets.size(); ++bk) {
         float ms = 0.f;
         hipEventElapsedTime(&ms, b->ev[2 * bk], b->ev[2 * bk + 1]);
@@ -770,6 +875,44 @@ int oops_hahn_batch_timing(const oops_hahn_batch* b, int32_t max_buckets, int32_
 int oops_hahn_batch_kernels(const oops_hahn_batch* b, int32_t max_buckets, int32_t* kind) {
     if (!b || !kind) return OOPS_ERR_ARG;
     for (int k = 0; k < (int)b->buckets.size() && k < max_buckets; ++k) kind[k] = b->buckets[k].kernel;
+    return OOPS_OK;
+}
+
+int oops_slot_form(const oops_term_table* T, int32_t max_slots, int32_t* nslot64, int32_t* nslot32, int32_t* masks,
+                   double* coef) {
+    if (!T || !nslot64 || !nslot32) return OOPS_ERR_ARG;
+    CompiledOp C;
+    std::string err;
+    int rc = compile_operator(T, C, err, 1 << 20);
+    if (rc) return rc;
+    if (C.nslot64 < 0) return OOPS_ERR_CAPACITY;
+    *nslot64 = C.nslot64;
+    *nslot32 = C.nslot32;
+    const int S = C.nslot64 + C.nslot32;
+    if (!masks || !coef) return OOPS_OK;   // counts only
+    if (S > max_slots) return OOPS_ERR_CAPACITY;
+    for (int s = 0; s < C.nslot64; ++s) masks[s] = C.mask64[s];
+    for (int s = 0; s < C.nslot32; ++s) masks[C.nslot64 + s] = C.mask32[s];
+    const int64_t n = C.n;
+    std::vector<std::complex<double>> tab((size_t)S * n, 0.0);
+    const int ld = C.ld_bits;
+    const int64_t lmask = (1LL << ld) - 1;
+    const int top = C.nspin - 1;
+    // the same accumulation as the device's per-element table build
+    for (int64_t i = 0; i < n; ++i) {
+        tab[i] += C.c0;
+        for (const DevOpLocal& lo : C.locs) {
+            int64_t row = (i >> (ld * (top - lo.s))) & lmask;
+            if (lo.nb == 2) row = (row << ld) | ((i >> (ld * (top - lo.t))) & lmask);
+            const int64_t e0 = lo.ent + row * lo.R;
+            for (int e = 0; e < lo.R; ++e) {
+                const uint8_t sl = C.eslot[e0 + e];
+                const int slot = (sl & 128) ? C.nslot64 + (sl & 127) : sl;
+                tab[(size_t)slot * n + i] += C.val[e0 + e];
+            }
+        }
+    }
+    std::memcpy(coef, tab.data(), sizeof(double) * 2 * tab.size());
     return OOPS_OK;
 }
 

@@ -199,7 +199,7 @@ __device__ __forceinline__ void advance(KrylovSmem<M>& S, BlockRed<kNT>& br, Int
                 else { err_loc = p1; g.xm = 1.0 / (double)(M - 1); }
             }
             if (k1 != 0 && err_loc > delta * t_step * tol) {
-                t_step = gamma * t_step * pow(t_step * tol / err_loc, g.xm);
+                t_step = gamma * t_step * ool_pow(t_step * tol / err_loc, g.xm);
                 t_step = round_step2(t_step);
                 ++g.nrej;
                 __syncthreads();
@@ -221,7 +221,7 @@ __device__ __forceinline__ void advance(KrylovSmem<M>& S, BlockRed<kNT>& br, Int
             const auto tb = block_sum1<1, kNT>(br, [&](int) { return tid < n ? cabs2(g.w) : 0.0; });
             g.beta = sqrt(tb.get(0));
         }
-        g.t_new = gamma * t_step * pow(t_step * tol / err_loc, g.xm);
+        g.t_new = gamma * t_step * ool_pow(t_step * tol / err_loc, g.xm);
         g.t_new = round_step2(g.t_new);
         g.t_now += t_step;
         OOPS_TOC(g, tupd, tu0);

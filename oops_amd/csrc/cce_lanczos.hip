@@ -130,8 +130,8 @@ __device__ __forceinline__ void lanczos_all(LzCtx<M>& c, Basis<M, true>& V, std:
 //   (X x)_M = -beta_m x_{M-1},   (X x)_{M+1} = -i x_M.
 // Long steps are split into ns equal substeps of tau r / ns <= 40.
 // Returns the number of series terms summed.
-template <int M>
-__device__ int cheb_coeffs(LanczosSmem<M>& S, int mb, int k1, double tau, double beta_m) {
+template <int M, typename SM>
+__device__ int cheb_coeffs(SM& S, int mb, int k1, double tau, double beta_m) {
     const int tid = threadIdx.x;
     int terms = 0;
     if (tid < kWave) {
@@ -164,7 +164,7 @@ __device__ int cheb_coeffs(LanczosSmem<M>& S, int mb, int k1, double tau, double
         const double x = xfull / ns;
         // Bessel J_k(x), k < K, by backward recurrence (Miller), normalised
         // with J_0 + 2 sum J_2k = 1; every lane computes the same values.
-        int K = (int)ceil(x + 10.0 * cbrt(fmax(x, 1.0)) + 20.0);
+        int K = (int)ceil(x + 10.0 * ool_cbrt(fmax(x, 1.0)) + 20.0);
         if (K > kBesMax - 2) K = kBesMax - 2;
         if (x == 0.0) {
             if (j == 0) S.bes[0] = 1.0;
@@ -190,9 +190,7 @@ __device__ int cheb_coeffs(LanczosSmem<M>& S, int mb, int k1, double tau, double
         }
         // series on the vector, ns times
         cd v = mkcd(j == 0 ? 1.0 : 0.0, 0.0);
-        double sn, cs;
-        sincos(c * (tau / ns), &sn, &cs);   // e^{i c tau / ns}
-        const cd ph = mkcd(cs, sn);
+        const cd ph = ool_expi(c * (tau / ns));   // e^{i c tau / ns}
         for (int sub = 0; sub < ns; ++sub) {
             auto apply = [&](cd t) -> cd {
                 const double pr = __shfl_up(t.re, 1, kWave), pi = __shfl_up(t.im, 1, kWave);
@@ -267,7 +265,7 @@ __device__ __forceinline__ void advance_lz(LanczosSmem<M>& S, BlockRed<kNT>& br,
                 else { err_loc = p1; g.xm = 1.0 / (double)(M - 1); }
             }
             if (k1 != 0 && err_loc > delta * t_step * tol) {
-                t_step = gamma * t_step * pow(t_step * tol / err_loc, g.xm);
+                t_step = gamma * t_step * ool_pow(t_step * tol / err_loc, g.xm);
                 t_step = round_step2(t_step);
                 ++g.nrej;
                 if (!(t_step < 1e300) || g.nrej > max_steps) { g.fail |= 4; return; }
@@ -288,7 +286,7 @@ __device__ __forceinline__ void advance_lz(LanczosSmem<M>& S, BlockRed<kNT>& br,
             const auto tb = block_sum1<1, kNT>(br, [&](int) { return tid < g.n ? cabs2(g.w) : 0.0; });
             g.beta = sqrt(tb.get(0));
         }
-        g.t_new = gamma * t_step * pow(t_step * tol / err_loc, g.xm);
+        g.t_new = gamma * t_step * ool_pow(t_step * tol / err_loc, g.xm);
         g.t_new = round_step2(g.t_new);
         // land exactly on t_out when the step was clamped to it (segment ends)
         if (t_step >= t_out - g.t_now) g.t_now = t_out;
@@ -384,6 +382,350 @@ __global__ __launch_bounds__(kNT) void cce_hahn_lanczos_kernel(KrylovArgs a) {
         }
         __syncthreads();
     }
+}
+
+// ---------------------------------------------------------------------------
+// Slot form.  With power-of-two local dimensions the local digits of an
+// element index are bit fields, so every entry of a merged local operator
+// moves element i to i ^ mask, with a mask fixed by the entry's local row and
+// column.  Summing all entries of all local operators that share a mask gives
+//   (A x)_i = sum_s c_s(i) x[i ^ mask_s],
+// 21 terms per element for a 4-spin spin-1/2 Liouvillian instead of 64 ELL
+// entries (4 one-body rows of 4, 6 two-body rows of 8).  Each thread holds
+// its c_s(i) in registers (FP64 for masks that change at most one spin's
+// digits, where the large one-body terms land; FP32 for two-spin flips), so
+// one product is 21 LDS reads of x and no table reads.  The per-element
+// coefficient tables of a cluster's operators are built once per cluster in
+// a per-workgroup global buffer and reloaded into registers when the active
+// generator changes.  The Lanczos basis v_1..v_M lives in LDS (thread i owns
+// column i), which leaves registers for the coefficients and fits two
+// workgroups per CU.
+
+struct SlotOp {
+    cd c[kSlot64];
+    float2 f[kSlot32];
+    int mc[kSlot64];   // masks (uniform)
+    int mf[kSlot32];
+    int n;
+
+    __device__ __forceinline__ void load(const double2* __restrict__ tab, const int32_t* __restrict__ masks, int i,
+                                         int n_) {
+        n = n_;
+        // opaque column index: keeps the 2 x 21 per-slot addresses of every
+        // call site from being hoisted (and held) across the kernel's loops
+        asm volatile("" : "+v"(i));
+        const float2* t32 = reinterpret_cast<const float2*>(tab + kSlot64 * 256);
+#pragma unroll
+        for (int s = 0; s < kSlot64; ++s) {
+            const double2 v = tab[s * 256 + i];
+            c[s] = mkcd(v.x, v.y);
+            mc[s] = masks[s];
+        }
+#pragma unroll
+        for (int s = 0; s < kSlot32; ++s) {
+            f[s] = t32[s * 256 + i];
+            mf[s] = masks[kSlot64 + s];
+        }
+    }
+
+    // element i of A x with x staged in LDS (FP64 copy xs, FP32 copy xs32).
+    // Three rounds of 3 FP64 + 4 FP32 gathers bound the registers in flight;
+    // the empty asm keeps the gather addresses from being hoisted out of the
+    // caller's loops (21 more live registers otherwise).
+    __device__ __forceinline__ cd apply(const cd* __restrict__ xs, const float2* __restrict__ xs32, int i) const {
+        static_assert(kSlot64 == 9 && kSlot32 == 12, "gather rounds assume 9 + 12 slots");
+        if (i >= n) return mkcd(0.0, 0.0);
+        int ii = i;
+        asm volatile("" : "+v"(ii));
+        cd acc = mkcd(0.0, 0.0);
+        float ar = 0.f, ai = 0.f;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            cd x64[3];
+            float2 x32[4];
+#pragma unroll
+            for (int t = 0; t < 3; ++t) x64[t] = xs[ii ^ mc[3 * r + t]];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) x32[t] = xs32[ii ^ mf[4 * r + t]];
+#pragma unroll
+            for (int t = 0; t < 3; ++t) acc = cfma(c[3 * r + t], x64[t], acc);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const float2 cf = f[4 * r + t];
+                ar = fmaf(cf.x, x32[t].x, fmaf(-cf.y, x32[t].y, ar));
+                ai = fmaf(cf.x, x32[t].y, fmaf(cf.y, x32[t].x, ai));
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        return mkcd(acc.re + (double)ar, acc.im + (double)ai);
+    }
+};
+
+// Coefficient table of operator `op` for this thread's element: FP64 slots
+// as double2 [kSlot64][256], then FP32 slots as float2 [kSlot32][256].  Every
+// thread writes and reads only its own column.
+__device__ __noinline__ void slot_build(const KrylovArgs& a, int op, double2* __restrict__ tab, int i) {
+    const DevOpHeader h = a.hdr[op];
+    asm volatile("" : "+v"(i));
+    float2* t32 = reinterpret_cast<float2*>(tab + kSlot64 * 256);
+    for (int s = 0; s < kSlot64; ++s) tab[s * 256 + i] = make_double2(0.0, 0.0);
+    for (int s = 0; s < kSlot32; ++s) t32[s * 256 + i] = make_float2(0.f, 0.f);
+    if (i >= h.n) return;
+    tab[i] = make_double2(h.c0re, h.c0im);   // slot 0: mask 0
+    const int ld = h.ld_bits;
+    const int lmask = (1 << ld) - 1;
+    const int top = h.nspin - 1;
+    for (int L = 0; L < h.nloc; ++L) {
+        const DevOpLocal lo = a.loc[h.loc_off + L];
+        int row = (i >> (ld * (top - lo.s))) & lmask;
+        if (lo.nb == 2) row = (row << ld) | ((i >> (ld * (top - lo.t))) & lmask);
+        const int e0 = h.ent_off + lo.ent + row * lo.R;
+        for (int e = 0; e < lo.R; ++e) {
+            const double2 v = a.val[e0 + e];
+            const int sl = a.eslot[e0 + e];
+            // this thread's own column: read-modify-write in program order
+            if (sl & 128) {
+                float2 t = t32[(sl & 127) * 256 + i];
+                t.x += (float)v.x;
+                t.y += (float)v.y;
+                t32[(sl & 127) * 256 + i] = t;
+            } else {
+                double2 t = tab[sl * 256 + i];
+                t.x += v.x;
+                t.y += v.y;
+                tab[sl * 256 + i] = t;
+            }
+        }
+    }
+}
+
+template <int M>
+struct LzSlotSmem {
+    float2 V[M][kNT];                   // v_1..v_M (FP32), column i owned by thread i
+    cd xs[256];
+    float2 xs32[256];
+    double red[3 * (kNT / kWave) * 64];
+    double d[M + 2];
+    double e[M + 2];
+    cd F[M + 2];
+    double bes[kBesMax];
+    double tdiag[16];
+    double toff[16];
+    double scal[16];
+    int ints[8];
+};
+
+template <int M>
+__device__ __forceinline__ void advance_slot(LzSlotSmem<M>& S, BlockRed<kNT>& br, IntegState& g, const SlotOp& so,
+                                             double t_out, int max_steps, double tol) {
+    const int tid = threadIdx.x;
+    const double delta = 1.2, gamma = 0.9;
+    while (g.t_now < t_out) {
+        if (g.nstep >= max_steps) { g.fail |= 1; break; }
+        ++g.nstep;
+        const cd v0 = cscale(g.w, 1.0 / g.beta);
+        cd vcur = v0, vprev = mkcd(0.0, 0.0);
+        double bprev = 0.0;
+        int k1 = 2, mb = M;
+        bool brk = false;
+        double t_step = fmin(t_out - g.t_now, g.t_new);
+#pragma unroll 1
+        for (int j = 0; j < M; ++j) {
+            OOPS_TIC(tm0);
+            stage_x<true>(S.xs, S.xs32, tid, g.n, vcur);
+            __syncthreads();
+            cd p = so.apply(S.xs, S.xs32, tid);
+            ++g.nmult;
+            p = mkcd(p.re - bprev * vprev.re, p.im - bprev * vprev.im);
+            OOPS_TOC(g, tmv, tm0);
+            OOPS_TIC(tr0);
+            const cd vj = vcur;
+            const auto r1 = block_sum1<4, kNT>(br, [&](int k) -> double {
+                if (k == 0) return fma(vj.re, p.re, vj.im * p.im);
+                if (k == 1) return fma(vj.re, p.im, -vj.im * p.re);
+                if (k == 2) return cabs2(p);
+                return 0.0;
+            });
+            const cd h = mkcd(r1.get(0), r1.get(1));
+            p = cfms(h, vj, p);
+            const double alpha = h.re;
+            const double beta = sqrt(fmax(r1.get(2) - cabs2(h), 0.0));
+            OOPS_TOC(g, tred, tr0);
+            if (tid == 0) {
+                S.d[j] = alpha;
+                S.e[j] = beta;
+            }
+            if (beta <= 1.0e-7) {
+                k1 = 0;
+                g.brk = 1;
+                mb = j + 1;
+                t_step = t_out - g.t_now;
+                brk = true;
+                break;
+            }
+            vprev = vj;
+            vcur = cscale(p, 1.0 / beta);
+            S.V[j][tid] = make_float2((float)vcur.re, (float)vcur.im);
+            bprev = beta;
+        }
+        double avnorm = 0.0;
+        if (!brk) {
+            stage_x<true>(S.xs, S.xs32, tid, g.n, vcur);
+            __syncthreads();
+            const cd p = so.apply(S.xs, S.xs32, tid);
+            ++g.nmult;
+            const auto tn = block_sum1<1, kNT>(br, [&](int) { return cabs2(p); });
+            avnorm = sqrt(tn.get(0));
+        }
+        __syncthreads();
+        OOPS_TIC(te0);
+        double err_loc = 0.0;
+        for (;;) {
+            ++g.nexph;
+            g.nscale += cheb_coeffs<M>(S, mb, k1, t_step, bprev);
+            if (k1 == 0) {
+                err_loc = tol;
+            } else {
+                const double p1 = cabs(S.F[M]) * g.beta;
+                const double p2 = cabs(S.F[M + 1]) * g.beta * avnorm;
+                if (p1 > 10.0 * p2) { err_loc = p2; g.xm = 1.0 / (double)M; }
+                else if (p1 > p2) { err_loc = (p1 * p2) / (p1 - p2); g.xm = 1.0 / (double)M; }
+                else { err_loc = p1; g.xm = 1.0 / (double)(M - 1); }
+            }
+            if (k1 != 0 && err_loc > delta * t_step * tol) {
+                t_step = gamma * t_step * ool_pow(t_step * tol / err_loc, g.xm);
+                t_step = round_step2(t_step);
+                ++g.nrej;
+                if (!(t_step < 1e300) || g.nrej > max_steps) { g.fail |= 4; return; }
+                continue;
+            }
+            break;
+        }
+        OOPS_TOC(g, texp, te0);
+        OOPS_TIC(tu0);
+        // w = beta * (sum_{j<mb} F_j v_j + [k1] F_M v_M)
+        const int mx = mb + (k1 - 1 > 0 ? k1 - 1 : 0);
+        cd acc = cmul(v0, S.F[0]);
+#pragma unroll 1
+        for (int j = 1; j < mx; ++j) {
+            const float2 f = S.V[j - 1][tid];
+            acc = cfma(mkcd((double)f.x, (double)f.y), S.F[j], acc);
+        }
+        g.w = cscale(acc, g.beta);
+        {
+            const auto tb = block_sum1<1, kNT>(br, [&](int) { return tid < g.n ? cabs2(g.w) : 0.0; });
+            g.beta = sqrt(tb.get(0));
+        }
+        g.t_new = gamma * t_step * ool_pow(t_step * tol / err_loc, g.xm);
+        g.t_new = round_step2(g.t_new);
+        if (t_step >= t_out - g.t_now) g.t_now = t_out;
+        else g.t_now += t_step;
+        OOPS_TOC(g, tupd, tu0);
+    }
+}
+
+template <int M>
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void cce_hahn_lanczos_slot_kernel(KrylovArgs a) {
+    __shared__ LzSlotSmem<M> S;
+    const int tid = threadIdx.x;
+    BlockRed<kNT> br(S.red);
+    const double tol = a.tol;
+    const int max_steps = a.max_steps;
+    double2* const ws = a.slotws + (int64_t)blockIdx.x * 4 * kSlotOpD2;
+    for (;;) {
+        if (tid == 0) S.ints[7] = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int item = S.ints[7];
+        __syncthreads();
+        if (item >= a.nitems) break;
+        const int64_t voff = a.vec_off[item];
+        const ItemSeq q = item_seq(a, item);
+        const bool cpmg = q.npulse > 1;
+        const int n = a.n;
+        // operator indices (uniform): A, -B^dagger, and for CPMG-N B, -A^dagger
+        const int qA = __builtin_amdgcn_readfirstlane(a.item_opA[item]);
+        const int qB = __builtin_amdgcn_readfirstlane(a.item_opB[item]);
+        const int qC = cpmg ? __builtin_amdgcn_readfirstlane(a.item_opC[item]) : qA;
+        const int qD = cpmg ? __builtin_amdgcn_readfirstlane(a.item_opD[item]) : qB;
+        slot_build(a, qA, ws, tid);
+        slot_build(a, qB, ws + kSlotOpD2, tid);
+        if (cpmg) {
+            slot_build(a, qC, ws + 2 * kSlotOpD2, tid);
+            slot_build(a, qD, ws + 3 * kSlotOpD2, tid);
+        }
+        IntegState gA, gB;
+        const cd u0 = tid < n ? mkcd(a.u0[voff + tid].x, a.u0[voff + tid].y) : mkcd(0.0, 0.0);
+        const cd y0 = tid < n ? mkcd(a.y0[voff + tid].x, a.y0[voff + tid].y) : mkcd(0.0, 0.0);
+        const NormWork nw{S.xs, S.xs32, S.tdiag, S.toff, S.scal};
+        SlotOp so;
+        so.load(ws, a.smask + (int64_t)qA * kSlotMaskStride, tid, n);
+        init_integ_mv<true>(nw, n, [&](int i) { return so.apply(S.xs, S.xs32, i); }, br, gA, u0, a.rnd, a.klim,
+                            tol, M);
+        so.load(ws + kSlotOpD2, a.smask + (int64_t)qB * kSlotMaskStride, tid, n);
+        init_integ_mv<true>(nw, n, [&](int i) { return so.apply(S.xs, S.xs32, i); }, br, gB, y0, a.rnd, a.klim,
+                            tol, M);
+        const double beta0A = gA.beta, beta0B = gB.beta;
+        const double scale = a.scale[item];
+        const int h = cpmg_split(q.npulse);
+        for (int k = 0; k < q.nt; ++k) {
+            for (int s = 0; s < 2; ++s) {
+                IntegState& g = gA;   // active sweep; the states swap after each sweep
+                int nseg = 1;
+                if (cpmg) {
+                    g.w = s ? y0 : u0;
+                    g.beta = s ? beta0B : beta0A;
+                    g.t_now = 0.0;
+                    nseg = s == 0 ? h : q.npulse + 1 - h;
+                }
+                for (int sg = 0; sg < nseg; ++sg) {
+                    int opi = s;
+                    double target = q.tau[k];
+                    if (cpmg) {
+                        const int j = s == 0 ? sg : q.npulse - sg;
+                        opi = s == 0 ? ((j & 1) ? 2 : 0) : ((j & 1) ? 1 : 3);
+                        target = g.t_now + cpmg_fraction(j, q.npulse) * q.tau[k];
+                    }
+                    const int qo = opi == 0 ? qA : (opi == 1 ? qB : (opi == 2 ? qC : qD));
+                    so.load(ws + opi * kSlotOpD2, a.smask + (int64_t)qo * kSlotMaskStride, tid, n);
+                    advance_slot<M>(S, br, g, so, target, max_steps, tol);
+                }
+                const IntegState tmp = gA;
+                gA = gB;
+                gB = tmp;
+            }
+            const auto tl = block_sum1<1, kNT>(br, [&](int) {
+                return fma(gB.w.re, gA.w.re, gB.w.im * gA.w.im);
+            });
+            if (tid == 0) q.L[k] = scale * tl.get(0);
+            if ((gA.fail | gB.fail) != 0) break;
+        }
+        if (tid == 0) {
+            int32_t* st = a.stats + (int64_t)item * kStatsPerItem;
+            st[0] = gA.nstep; st[1] = gA.nmult; st[2] = gA.nrej; st[3] = gA.nexph;
+            st[4] = gB.nstep; st[5] = gB.nmult; st[6] = gB.nrej; st[7] = gB.nexph;
+            st[8] = gA.fail | (gB.fail << 4) | (gA.brk << 8) | (gB.brk << 9);
+            st[9] = gA.nscale + gB.nscale;
+            reinterpret_cast<double*>(st + 10)[0] = gA.anorm;
+            reinterpret_cast<double*>(st + 12)[0] = gB.anorm;
+            st[14] = 0; st[15] = 0;
+#ifdef OOPS_PHASE_TIMERS
+            st[16] = (int)((gA.tmv + gB.tmv) >> 10);
+            st[17] = (int)((gA.tred + gB.tred) >> 10);
+            st[18] = (int)((gA.texp + gB.texp) >> 10);
+            st[19] = (int)((gA.tupd + gB.tupd) >> 10);
+#endif
+        }
+        __syncthreads();
+    }
+}
+
+int launch_hahn_lanczos_slot(const KrylovArgs& a, int m, int grid, hipStream_t stream) {
+    if (a.n > 256 || !a.slotws || !a.eslot || !a.smask) return -11;
+    if (m == 20) hipLaunchKernelGGL((cce_hahn_lanczos_slot_kernel<20>), dim3(grid), dim3(kNT), 0, stream, a);
+    else if (m == 30) hipLaunchKernelGGL((cce_hahn_lanczos_slot_kernel<30>), dim3(grid), dim3(kNT), 0, stream, a);
+    else return -12;
+    return (int)hipGetLastError();
 }
 
 int launch_hahn_lanczos(const KrylovArgs& a, int m, int grid, hipStream_t stream) {

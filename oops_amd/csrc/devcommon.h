@@ -20,6 +20,20 @@ struct cd {
 };
 
 __device__ __forceinline__ cd mkcd(double a, double b) { cd r; r.re = a; r.im = b; return r; }
+
+// Scalar FP64 library functions called out of line.  Inlined into a kernel
+// with long-lived loops, their polynomial constants (two VGPRs each) are
+// hoisted to the kernel entry and stay live everywhere; these are called a
+// few times per Krylov step at most.
+static __device__ __noinline__ double ool_pow(double x, double y) { return pow(x, y); }
+static __device__ __noinline__ double ool_log(double x) { return log(x); }
+static __device__ __noinline__ double ool_log10(double x) { return log10(x); }
+static __device__ __noinline__ double ool_cbrt(double x) { return cbrt(x); }
+static __device__ __noinline__ cd ool_expi(double x) {   // (cos x, sin x)
+    double sn, cs;
+    sincos(x, &sn, &cs);
+    return mkcd(cs, sn);
+}
 __device__ __forceinline__ cd cadd(cd a, cd b) { return mkcd(a.re + b.re, a.im + b.im); }
 __device__ __forceinline__ cd csub(cd a, cd b) { return mkcd(a.re - b.re, a.im - b.im); }
 __device__ __forceinline__ cd cscale(cd a, double s) { return mkcd(a.re * s, a.im * s); }
@@ -373,7 +387,7 @@ __device__ int lds_pade6(const cd* H, int ldh, int m, double t, cd* w0, cd* w1, 
     const double hnorm = fabs(t * block_max1<NT>(rs, ws));
     if (hnorm == 0.0) return -1;
     if (!(hnorm < 1e300)) return -3;  // non-finite input
-    ns = (int)(log(hnorm) / log(2.0)) + 2;
+    ns = (int)(ool_log(hnorm) / 0.6931471805599453) + 2;
     if (ns < 0) ns = 0;
     const double scale = t / ldexp(1.0, ns);
     // Pade(6) coefficients by the recurrence c_k = c_{k-1} (7-k) / (k (13-k))
@@ -530,7 +544,7 @@ __device__ int pade6_rational(const cd* H, int ld, int m, double t, cd* W0, cd* 
     const double hnorm = fabs(t * scal[0]);
     if (hnorm == 0.0) return -1;
     if (!(hnorm < 1e300)) return -3;
-    int ns = (int)(log(hnorm) / log(2.0)) + 2;
+    int ns = (int)(ool_log(hnorm) / 0.6931471805599453) + 2;
     if (ns < 0) ns = 0;
     const double scale = t / ldexp(1.0, ns);
     double c[7];
@@ -704,7 +718,7 @@ __device__ int pade6_first_column3(const cd* H, int ld, int m, double t, cd* W0,
     const double hnorm = fabs(t * scal[0]);
     if (hnorm == 0.0) return -1;
     if (!(hnorm < 1e300)) return -3;
-    int ns = (int)(log(hnorm) / log(2.0)) + 2;
+    int ns = (int)(ool_log(hnorm) / 0.6931471805599453) + 2;
     if (ns < 0) ns = 0;
     const double scale = t / ldexp(1.0, ns);
     double c[7];

@@ -35,6 +35,14 @@ struct CompiledOp {
     std::vector<DevOpLocal> locs;
     std::vector<std::complex<double>> val;
     std::vector<int32_t> off;
+    // Flip-mask slots: every entry moves element i to i ^ mask (the local
+    // digits are bit fields).  Slot 0 is mask 0; FP64-class slots have masks
+    // that change at most one spin's digits, FP32-class slots the others.
+    // eslot[e] = slot of entry e (bit 7 set: FP32 class); nslot64 < 0 when
+    // either class exceeds 127 slots.
+    std::vector<uint8_t> eslot;
+    int nslot64 = 0, nslot32 = 0;
+    std::vector<int32_t> mask64, mask32;
 };
 
 // Owning device allocation.

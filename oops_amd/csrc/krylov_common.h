@@ -54,7 +54,7 @@ struct OpSrc {
 __device__ __forceinline__ double round_step2(double t) {
     if (!(t > 0.0) || isinf(t)) return __builtin_huge_val();
     const double sqr1 = 0.31622776601683794;  // sqrt(0.1)
-    const double x = log10(t) - sqr1;
+    const double x = ool_log10(t) - sqr1;
     const int k = (int)(x >= 0.0 ? floor(x + 0.5) : -floor(-x + 0.5)) - 1;
     double p = 1.0;
     const int a = k < 0 ? -k : k;
@@ -78,6 +78,12 @@ __device__ __forceinline__ cd matvec_elem(const OpLds& o, const cd* xs, const fl
     if (i >= o.hdr.n) return mkcd(0.0, 0.0);
     if (MIXED) return op_apply_elem_mixed(i, xs, xs32, o.hdr, o.locs, o.vals, o.offs);
     return op_apply_elem(i, xs, o.hdr, o.locs, o.vals, o.offs);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int mask = 32; mask >= 1; mask >>= 1) v += __shfl_xor(v, mask, kWave);
+    return v;
 }
 
 __device__ __forceinline__ double wave_max(double v) {
@@ -143,11 +149,12 @@ __device__ __forceinline__ void op_load_lds(const OpSrc& a, int op, OpLds& o) {
 // ----------------------------------------------------------------------------
 // Norm estimate: k Lanczos steps on +H from the DLARAN start vector, then
 // power iteration on the k x k tridiagonal (reference :210-265).
-template <bool MIXED>
-__device__ double lanczos_anorm(const NormWork& S, BlockRed<kNT>& br, const OpLds& o, const double* rnd, int klim,
-                                double tol) {
+// mv(i) returns element i of the product with the vector staged in S.xs /
+// S.xs32.
+template <bool MIXED, typename MV>
+__device__ double lanczos_anorm_mv(const NormWork& S, BlockRed<kNT>& br, int n, MV&& mv, const double* rnd,
+                                   int klim, double tol) {
     const int tid = threadIdx.x;
-    const int n = o.hdr.n;
     int k = klim;
     if (k > n) k = n;
     double* tdiag = S.tdiag;
@@ -163,7 +170,7 @@ __device__ double lanczos_anorm(const NormWork& S, BlockRed<kNT>& br, const OpLd
     for (int j = 1; j <= k; ++j) {
         stage_x<MIXED>(S.xs, S.xs32, tid, n, q);
         __syncthreads();
-        cd w = matvec_elem<MIXED>(o, S.xs, S.xs32, tid);
+        cd w = mv(tid);
         const auto ta = block_sum1<2, kNT>(br, [&](int c) {
             return c == 0 ? fma(q.re, w.re, q.im * w.im) : fma(q.re, w.im, -q.im * w.re);
         });
@@ -181,55 +188,71 @@ __device__ double lanczos_anorm(const NormWork& S, BlockRed<kNT>& br, const OpLd
         bjm = bj;
     }
     __syncthreads();
-    if (tid == 0) {
-        // power iteration on T (k x k symmetric tridiagonal; entries not set stay 0)
-        double qr[16], wr[16];
-        for (int j = 0; j < k; ++j) qr[j] = rnd[2 * n + j];
+    if (tid < kWave) {
+        // power iteration on T (k x k symmetric tridiagonal; entries not set
+        // stay 0), one row per lane of wave 0
+        const int j = tid;
+        const bool row = j < k;
+        const double dj = row ? tdiag[j] : 0.0;
+        const double lo = (row && j > 0) ? toff[j - 1] : 0.0;
+        const double up = (row && j + 1 < k) ? toff[j] : 0.0;
+        double qj = row ? rnd[2 * n + j] : 0.0;
         double cjm = 0.0, cj = 0.0;
         for (int it = 0; it < 10000; ++it) {
-            for (int i = 0; i < k; ++i) {
-                double s = tdiag[i] * qr[i];
-                if (i > 0) s += toff[i - 1] * qr[i - 1];
-                if (i + 1 < k) s += toff[i] * qr[i + 1];
-                wr[i] = s;
-            }
-            cj = 0.0;
-            for (int i = 0; i < k; ++i) cj += qr[i] * wr[i];
+            const double qm1 = __shfl_up(qj, 1, kWave);
+            const double qp1 = __shfl_down(qj, 1, kWave);
+            double wj = dj * qj;
+            wj += lo * qm1;
+            wj += up * qp1;
+            cj = wave_sum(qj * wj);
             if (fabs(1.0 - cjm / cj) < tol) break;
             cjm = cj;
-            double nr = 0.0;
-            for (int i = 0; i < k; ++i) nr += wr[i] * wr[i];
-            cj = sqrt(nr);
-            for (int i = 0; i < k; ++i) qr[i] = wr[i] / cj;
+            cj = sqrt(wave_sum(wj * wj));
+            qj = wj / cj;
         }
-        S.scal[0] = fabs(cj) + fabs(bj * qr[k - 1]);
+        const double qlast = __shfl(qj, k - 1, kWave);
+        if (j == 0) S.scal[0] = fabs(cj) + fabs(bj * qlast);
     }
     __syncthreads();
     const double an = S.scal[0];
     __syncthreads();
     return an;
 }
+
+template <bool MIXED>
+__device__ double lanczos_anorm(const NormWork& S, BlockRed<kNT>& br, const OpLds& o, const double* rnd, int klim,
+                                double tol) {
+    return lanczos_anorm_mv<MIXED>(S, br, o.hdr.n, [&](int i) { return matvec_elem<MIXED>(o, S.xs, S.xs32, i); },
+                                   rnd, klim, tol);
+}
 // Start one integrator: operator o (already in LDS), start vector w0 (this
 // thread's element), norm estimate and Expokit's first step size.
-template <bool MIXED>
-__device__ void init_integ(const NormWork& S, const OpLds& o, BlockRed<kNT>& br, IntegState& g, cd w0,
-                           const double* rnd, int klim, double tol, int m) {
+template <bool MIXED, typename MV>
+__device__ void init_integ_mv(const NormWork& S, int n, MV&& mv, BlockRed<kNT>& br, IntegState& g, cd w0,
+                              const double* rnd, int klim, double tol, int m) {
     const int tid = threadIdx.x;
-    g.n = o.hdr.n;
-    g.w = tid < o.hdr.n ? w0 : mkcd(0.0, 0.0);
-    g.anorm = lanczos_anorm<MIXED>(S, br, o, rnd, klim, tol);
+    g.n = n;
+    g.w = tid < n ? w0 : mkcd(0.0, 0.0);
+    g.anorm = lanczos_anorm_mv<MIXED>(S, br, n, mv, rnd, klim, tol);
     const auto tb = block_sum1<1, kNT>(br, [&](int) { return cabs2(g.w); });
     g.beta = sqrt(tb.get(0));
     g.t_now = 0.0;
     g.xm = 1.0 / (double)m;
     g.rndoff = 2.220446049250313e-16 * g.anorm;
-    const double p2 = tol * pow((m + 1) / 2.72, (double)(m + 1)) * sqrt(2.0 * 3.14 * (m + 1));
-    g.t_new = (1.0 / g.anorm) * pow(p2 / (4.0 * g.beta * g.anorm), g.xm);
+    const double p2 = tol * ool_pow((m + 1) / 2.72, (double)(m + 1)) * sqrt(2.0 * 3.14 * (m + 1));
+    g.t_new = (1.0 / g.anorm) * ool_pow(p2 / (4.0 * g.beta * g.anorm), g.xm);
     g.t_new = round_step2(g.t_new);
     g.nstep = g.nmult = g.nrej = g.nexph = g.nscale = g.brk = g.fail = 0;
 #ifdef OOPS_PHASE_TIMERS
     g.tmv = g.tred = g.texp = g.tupd = 0;
 #endif
+}
+
+template <bool MIXED>
+__device__ void init_integ(const NormWork& S, const OpLds& o, BlockRed<kNT>& br, IntegState& g, cd w0,
+                           const double* rnd, int klim, double tol, int m) {
+    init_integ_mv<MIXED>(S, o.hdr.n, [&](int i) { return matvec_elem<MIXED>(o, S.xs, S.xs32, i); }, br, g, w0, rnd,
+                         klim, tol, m);
 }
 
 // ----------------------------------------------------------------------------

@@ -12,6 +12,14 @@ constexpr int kDenseMaxEnt = 1024;  // ELL entries per operator on the dense pat
 constexpr int kKrylovMaxEnt = 1024; // ELL entries per operator on the batched Krylov path
 constexpr int kDenseNMAX = 16;      // largest state handled by the dense path
 constexpr int kStatsPerItem = 24;   // int32 slots of per-cluster counters
+// Flip-mask (slot) form of the product on the mixed Lanczos path: per element
+// kSlot64 FP64 coefficients (masks changing at most one spin's digits; for
+// spin-1/2 Liouvillians of k <= 4 spins: the diagonal and 2k ket/bra flips)
+// and kSlot32 FP32 coefficients (two-spin flips: 2 k(k-1)/2 <= 12).
+constexpr int kSlot64 = 9;
+constexpr int kSlot32 = 12;
+constexpr int kSlotMaskStride = 32;                            // int32 masks per operator
+constexpr int kSlotOpD2 = kSlot64 * 256 + kSlot32 * 256 / 2;   // double2 per operator table (n <= 256)
 
 // One bucket of Hahn-echo clusters sharing the state dimension n.
 struct KrylovArgs {
@@ -36,6 +44,9 @@ struct KrylovArgs {
     double* L;                 // nitems x nt
     int32_t* stats;            // nitems x kStatsPerItem
     int32_t* queue;            // work counter (zeroed by the launcher)
+    const uint8_t* eslot;      // slot of every ELL entry (Lanczos slot kernel), or NULL
+    const int32_t* smask;      // kSlotMaskStride masks per operator: kSlot64 FP64-class, then kSlot32
+    double2* slotws;           // per-workgroup coefficient tables: 4 x kSlotOpD2 double2 per workgroup
     int32_t nitems;
     int32_t n;
     int32_t nt;
@@ -49,6 +60,10 @@ int launch_hahn_bucket(const KrylovArgs& a, int m, int precision, int grid, hipS
 // Mixed-precision Lanczos kernel for buckets whose generators are all Hermitian
 // (n > m); the caller zeroes the work counter first.
 int launch_hahn_lanczos(const KrylovArgs& a, int m, int grid, hipStream_t stream);
+// Slot form of the Lanczos kernel (two workgroups per CU): every operator of
+// the bucket has at most kSlot64 / kSlot32 slots; a.slotws holds
+// grid * 4 * kSlotOpD2 double2.  The caller zeroes the work counter first.
+int launch_hahn_lanczos_slot(const KrylovArgs& a, int m, int grid, hipStream_t stream);
 void launch_zero_counter(int32_t* counter, hipStream_t stream);
 // Dense FP64-MFMA propagator kernel for n = 64 buckets whose items all have
 // uniform time grids starting at 0 (Hahn echo or CPMG-N); `work` holds

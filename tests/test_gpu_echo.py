@@ -97,6 +97,7 @@ def test_cpmg_lanczos_path_matches_reference_propagator(ctx, k, npulse):
         b = O.EchoBatch.from_items(ctx, [_item(coords, npulse, t)], m=30, tol=1e-6, precision="mixed")
         b.run()
         L, st = b.results()
+        assert b.timing()[0]["kernel"] == "lanczos_slot"
         assert (st[:, 8] & 0xFF).max() == 0
         assert np.abs(L[0] - _oracle_cpmg(coords, npulse, t)).max() < 1e-6
 
@@ -136,3 +137,31 @@ def test_echo_npulse1_equals_hahn_batch(ctx):
     Lh, _ = h.results()
     for i in range(len(items)):
         assert np.abs(Le[i] - Lh[i]).max() <= 1e-13
+
+
+@pytest.mark.gpu
+def test_graph_replay_matches_eager_launch(ctx):
+    """A batch whose bucket launches are captured in a HIP graph gives, on
+    every replay, the eager batch's L(t); eager runs of another batch and
+    synchronisations sit between the replays."""
+    t = np.linspace(0.0, 6e-4, 25)
+    items = []
+    for k, npulse in ((1, 1), (2, 2), (3, 4), (3, 1), (4, 2)):
+        for coords in _clusters(k, 2):
+            items.append(_item(coords, npulse, t / 2 if npulse == 1 else t))
+    eager = O.EchoBatch.from_items(ctx, items, m=30, tol=1e-4, precision="mixed")
+    graph = O.EchoBatch.from_items(ctx, items, m=30, tol=1e-4, precision="mixed")
+    graph.capture()
+    for rep in range(3):
+        print(f"graph replay {rep}", flush=True)
+        eager.run()
+        Le, _ = eager.results()
+        te = sum(r["ms"] for r in eager.timing())
+        graph.run()
+        Lg, st = graph.results()
+        tg = sum(r["ms"] for r in graph.timing())
+        assert (st[:, 8] & 0xFF).max() == 0
+        for i in range(len(items)):
+            assert np.array_equal(Lg[i], Le[i]), (rep, i)
+        # the replay did the work (not stale results of an earlier replay)
+        assert tg > 0.3 * te, (tg, te)

@@ -1,0 +1,117 @@
+"""GPU: the flip-mask (slot) form of the mixed Lanczos kernel.
+
+For a 4-spin spin-1/2 Liouvillian every merged local operator entry moves
+element i to i ^ mask; the slot kernel sums the entries per mask into
+per-element coefficients (9 FP64 one-spin masks, 12 FP32 two-spin masks) and
+applies 21 gathers per element instead of the 64 ELL entries.  It must give
+the same L_c(t) as the FP64 path to 1e-6 (the mixed-precision bar) and agree
+with the ELL form of the same Lanczos kernel (precision "mixed-ell"), for
+Hahn echoes and CPMG-N, on n = 256 and n = 64 buckets.  (CPMG-N on n = 256
+against the oracle's dense propagator: test_gpu_echo.py, which runs this
+kernel too.)"""
+import numpy as np
+import pytest
+
+import oops_amd as O
+from oracle import physics as P
+
+B111 = np.ones(3) * 0.05 / np.sqrt(3.0)
+
+
+def _sel(C, order, count):
+    counts = C.counts()
+    base = int(sum(counts[:order - 1]))
+    return base + np.linspace(0, counts[order - 1] - 1, count).astype(np.int64)
+
+
+def _kinds(batch):
+    return {r["n"]: r["kernel"] for r in batch.timing()}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("m,tol", [(30, 1e-3), (20, 3e-4)])
+def test_slot_kernel_matches_fp64_and_ell(ctx, m, tol):
+    X = O.bath_diamond(3.567, 30.0, 0.011, seed=5)
+    C = O.Clusters(X, 8.0, 4)
+    par = O.cce_params(B=B111)
+    sel = _sel(C, 4, 48)
+    tau = np.linspace(0.0, 2.5e-3, 101) / 2
+    print(f"slot m={m} tol={tol} clusters={len(sel)}", flush=True)
+    out = {}
+    for prec in ("fp64", "mixed", "mixed-ell"):
+        b = C.hahn_batch(ctx, par, m=30 if prec == "fp64" else m, tol=1e-12 if prec == "fp64" else tol,
+                         precision=prec, select=sel)
+        b.run(tau)
+        L, st = b.results()
+        assert (st[:, 8] & 0xFF).max() == 0
+        out[prec] = (L, _kinds(b))
+        b.close()
+    assert out["mixed"][1][256] == "lanczos_slot"
+    assert out["mixed-ell"][1][256] == "lanczos_mixed"
+    e64 = np.abs(out["mixed"][0] - out["fp64"][0]).max()
+    eell = np.abs(out["mixed"][0] - out["mixed-ell"][0]).max()
+    print(f"max |dL| vs fp64 {e64:.3e}, vs ell {eell:.3e}", flush=True)
+    assert e64 <= 1e-6
+    assert eell <= 1e-6
+
+
+def _clusters(k, count, seed=3):
+    X = O.bath_diamond(3.567, 25.0, 0.02, seed=seed)
+    C = O.Clusters(X, 7.0, k)
+    cl = C.get(k - 1)
+    return [X[list(r)] for r in cl[np.linspace(0, len(cl) - 1, count).astype(int)]]
+
+
+def _item(coords, npulse, t):
+    par = O.cce_params(B=B111)
+    A, rho, D = O.cluster_terms(coords, par, 0)
+    Bd, _, _ = O.cluster_terms(coords, par, 2)
+    it = {"opA": A, "opB": Bd, "u0": rho, "y0": rho, "scale": D, "npulse": npulse, "t": t}
+    if npulse > 1:
+        it["opC"], _, _ = O.cluster_terms(coords, par, 1)
+        it["opD"], _, _ = O.cluster_terms(coords, par, 5)
+    return it
+
+
+def _oracle_cpmg(coords, npulse, t):
+    """L(t_k) from the oracle's dense piecewise propagator, one time at a time
+    (any grid)."""
+    nv = P.NVCenter((0, 0, 0), B111)
+    prob = P.cluster_problem([P.Spin(c) for c in coords], B111, nv)
+    n = int(np.prod(prob["ldims"]))
+    ops = P.riffle(P.full(prob["lvA"], n), P.full(prob["lvB"], n), npulse)
+    seg = P.pulse_interval(npulse)
+    if t[0] == 0.0 and np.allclose(np.diff(t), t[1], rtol=1e-12, atol=0.0):
+        st = P.piecewise_dense(ops, seg, t[1], prob["rho"], len(t))   # uniform grid: powers of one step
+        return np.real(prob["D"] * (prob["rho"].conj() @ st))
+    out = []
+    for tk in t:
+        x = P.piecewise_dense(ops, seg, tk, prob["rho"], 2)[:, 1] if tk > 0 else prob["rho"]
+        out.append(np.real(prob["D"] * (prob["rho"].conj() @ x)))
+    return np.array(out)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k,npulse", [(4, 2), (4, 8), (3, 4)])
+def test_slot_kernel_cpmg_matches_reference_propagator(ctx, k, npulse):
+    items, coords = [], _clusters(k, 2 if k == 4 else 3)
+    # a non-uniform grid keeps n = 64 items off the dense propagator path
+    tt = 1e-3 * np.linspace(0.0, 1.0, 21) ** (1.0 if k == 4 else 1.5)
+    for c in coords:
+        items.append(_item(c, npulse, tt))
+    print(f"slot cpmg k={k} N={npulse}", flush=True)
+    res = {}
+    for prec in ("mixed", "mixed-ell"):
+        b = O.EchoBatch.from_items(ctx, items, m=30, tol=1e-5, precision=prec)
+        b.run()
+        L, st = b.results()
+        assert (st[:, 8] & 0xFF).max() == 0
+        res[prec] = (L, _kinds(b))
+    n = 4 ** k
+    assert res["mixed"][1][n] == "lanczos_slot"
+    assert res["mixed-ell"][1][n] == "lanczos_mixed"
+    for i, c in enumerate(coords):
+        assert np.abs(res["mixed"][0][i] - res["mixed-ell"][0][i]).max() < 1e-6
+        if k == 3:   # n = 64: the oracle's dense propagator per output time is cheap
+            ref = _oracle_cpmg(c, npulse, tt)
+            assert np.abs(res["mixed"][0][i] - ref).max() < 1e-6
