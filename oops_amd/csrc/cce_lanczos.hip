@@ -36,8 +36,6 @@
 
 namespace oops {
 
-constexpr int kBesMax = 128;  // Bessel coefficients per substep
-
 template <int M>
 struct LanczosSmem {
     OpLds op[4];                        // lvA, -lvB^dagger, lvB, -lvA^dagger stay resident
@@ -47,7 +45,6 @@ struct LanczosSmem {
     double d[M + 2];                    // diagonal of T
     double e[M + 2];                    // off-diagonal of T (e[j] couples j and j+1)
     cd F[M + 2];                        // Krylov coefficients of the step
-    double bes[kBesMax];                // Bessel coefficients J_k(tau r / ns)
     double tdiag[16];
     double toff[16];
     double scal[16];
@@ -122,16 +119,35 @@ __device__ __forceinline__ void lanczos_all(LzCtx<M>& c, Basis<M, true>& V, std:
     (lanczos_step<M, Js>(c, V), ...);
 }
 
+// Neighbour exchange across the wave (DPP wave shifts, 64-bit as two halves):
+// lane j receives lane j-1's value (shr) or lane j+1's (shl); 0 at the ends.
+__device__ __forceinline__ double wave_shr1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x138, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double wave_shl1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x130, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // Coefficients of one trial step tau: F = exp(tau H_aug) e1 for the
 // augmented matrix of the Lanczos tridiagonal T (mb x mb) plus, with the
 // correction (k1 = 2), rows M (coupling -i beta_m to row M-1) and M+1 (1 from
 // row M).  Wave 0 runs the Chebyshev-Bessel series on X = -i H_aug:
 //   (X x)_j = -(d_j x_j + e_{j-1} x_{j-1} + e_j x_{j+1})   j < mb
 //   (X x)_M = -beta_m x_{M-1},   (X x)_{M+1} = -i x_M.
-// Long steps are split into ns equal substeps of tau r / ns <= 40.
-// Returns the number of series terms summed.
+// Lane j owns row j; neighbours come by DPP wave shifts.  Long steps are split
+// into ns equal substeps of tau r / ns <= 40.  J_k(x) for k < 128 lives in the
+// lanes (J_k in lane k of `bl`, J_{64+k} in lane k of `bh`) and is read with
+// readlane; the series stops at the first k > x with |J_k| < 1e-18 (the tail
+// decays faster than geometrically there).  Returns the number of terms.
 template <int M, typename SM>
 __device__ int cheb_coeffs(SM& S, int mb, int k1, double tau, double beta_m) {
+    static_assert(M + 2 <= kWave, "one lane per row of the augmented matrix");
     const int tid = threadIdx.x;
     int terms = 0;
     if (tid < kWave) {
@@ -158,62 +174,80 @@ __device__ int cheb_coeffs(SM& S, int mb, int k1, double tau, double beta_m) {
         const double sup = -ej * ir;   // times x_{j+1}
         const double slo = j < mb ? -ejm * ir : 0.0;
         const double sM = (k1 && j == M) ? -beta_m * ir : 0.0;   // row M: times x_{M-1}
-        const bool rowM1 = k1 && j == M + 1;                      // row M+1: -i x_M / r
+        const double sM1 = (k1 && j == M + 1) ? ir : 0.0;        // row M+1: -i x_M / r
         const double xfull = tau * r;
         const int ns = xfull > 40.0 ? (int)ceil(xfull / 40.0) : 1;
         const double x = xfull / ns;
-        // Bessel J_k(x), k < K, by backward recurrence (Miller), normalised
-        // with J_0 + 2 sum J_2k = 1; every lane computes the same values.
-        int K = (int)ceil(x + 10.0 * ool_cbrt(fmax(x, 1.0)) + 20.0);
-        if (K > kBesMax - 2) K = kBesMax - 2;
+        // Bessel J_k(x) by backward recurrence (Miller) from N, normalised with
+        // J_0 + 2 sum J_2k = 1; every lane runs the recurrence and keeps its k.
+        double bl = 0.0, bh = 0.0;
+        int K = 1;
         if (x == 0.0) {
-            if (j == 0) S.bes[0] = 1.0;
-            for (int k = 1 + j; k < K; k += kWave) S.bes[k] = 0.0;
+            bl = j == 0 ? 1.0 : 0.0;
         } else {
-            const int N = K + 12;
+            int N = (int)ceil(x + 10.0 * ool_cbrt(fmax(x, 1.0)) + 32.0);
+            if (N > 2 * kWave) N = 2 * kWave;
             double jp = 0.0, jc = 1e-280, norm = 0.0;
             const double tx = 2.0 / x;
             for (int k = N; k >= 1; --k) {
                 const double jm = (double)k * tx * jc - jp;   // J_{k-1}
                 jp = jc;
                 jc = jm;
-                if (k - 1 < K && j == 0) S.bes[k - 1] = jm;
+                if (k - 1 == j) bl = jm;
+                if (k - 1 == j + kWave) bh = jm;
                 if (((k - 1) & 1) == 0) norm += (k - 1 == 0 ? 1.0 : 2.0) * jm;
-                if (fabs(jc) > 1e250) {   // rescale the stored tail
-                    for (int q = k - 1; q < K && j == 0; ++q) S.bes[q] *= 1e-250;
+                if (fabs(jc) > 1e250) {   // rescale everything kept so far
+                    bl *= 1e-250; bh *= 1e-250;
                     jp *= 1e-250; jc *= 1e-250; norm *= 1e-250;
                 }
             }
             const double inorm = 1.0 / norm;
-            // wave-synchronous: lane 0 wrote bes; every lane scales its share
-            for (int k = j; k < K; k += kWave) S.bes[k] *= inorm;
+            bl *= inorm;
+            bh *= inorm;
+            const unsigned long long sl = __ballot((double)j > x && fabs(bl) < 1e-18);
+            const unsigned long long sh = __ballot((double)(j + kWave) > x && fabs(bh) < 1e-18);
+            K = sl ? __ffsll((long long)sl) - 1 : (sh ? kWave + __ffsll((long long)sh) - 1 : N);
+            if (K < 2) K = 2;
         }
-        // series on the vector, ns times
+        auto bes = [&](int k) -> double { return k < kWave ? lane_val(bl, k) : lane_val(bh, k - kWave); };
+        auto apply = [&](cd t) -> cd {
+            const double pr = wave_shr1(t.re), pi = wave_shr1(t.im);
+            const double nr = wave_shl1(t.re), ni = wave_shl1(t.im);
+            cd y = mkcd(fma(sdiag, t.re, fma(slo, pr, sup * nr)), fma(sdiag, t.im, fma(slo, pi, sup * ni)));
+            y = mkcd(fma(sM, pr, fma(sM1, pi, y.re)), fma(sM, pi, fma(-sM1, pr, y.im)));
+            return y;
+        };
         cd v = mkcd(j == 0 ? 1.0 : 0.0, 0.0);
         const cd ph = ool_expi(c * (tau / ns));   // e^{i c tau / ns}
         for (int sub = 0; sub < ns; ++sub) {
-            auto apply = [&](cd t) -> cd {
-                const double pr = __shfl_up(t.re, 1, kWave), pi = __shfl_up(t.im, 1, kWave);
-                const double nr = __shfl_down(t.re, 1, kWave), ni = __shfl_down(t.im, 1, kWave);
-                cd y = mkcd(sdiag * t.re + slo * pr + sup * nr, sdiag * t.im + slo * pi + sup * ni);
-                if (sM != 0.0) y = mkcd(y.re + sM * pr, y.im + sM * pi);
-                if (rowM1) y = mkcd(y.re + pi * ir, y.im - pr * ir);   // -i x_M / r
-                return y;
-            };
+            // acc = J0 T0 v + sum_{k>=1} 2 i^k J_k T_k v, T_{k+1} = 2 S T_k - T_{k-1}
             cd tm = v;            // T_0(S) v
             cd tk = apply(v);     // T_1(S) v
-            // acc = J0 T0 + 2 i J1 T1 + sum_k 2 i^k J_k T_k
-            cd acc = cscale(tm, S.bes[0]);
-            acc = cadd(acc, cscale(cmulmi(cmulmi(cmulmi(tk))), 2.0 * S.bes[1]));  // i = (-i)^3
-            cd ik = mkcd(0.0, 1.0);   // i^k
-            for (int k = 2; k < K; ++k) {
+            cd acc = cscale(tm, bes(0));
+            {
+                const double b1 = 2.0 * bes(1);
+                acc = mkcd(acc.re - b1 * tk.im, acc.im + b1 * tk.re);   // i^1
+            }
+            // step k with i^k = (1, i, -1, -i)[R]
+            auto step = [&](int k, int R) {
                 const cd y = apply(tk);
                 const cd tn = mkcd(2.0 * y.re - tm.re, 2.0 * y.im - tm.im);
                 tm = tk;
                 tk = tn;
-                ik = mkcd(-ik.im, ik.re);   // times i
-                acc = cadd(acc, cscale(cmul(ik, tk), 2.0 * S.bes[k]));
+                const double bk = 2.0 * bes(k);
+                if (R == 0) acc = mkcd(fma(bk, tk.re, acc.re), fma(bk, tk.im, acc.im));
+                else if (R == 1) acc = mkcd(fma(-bk, tk.im, acc.re), fma(bk, tk.re, acc.im));
+                else if (R == 2) acc = mkcd(fma(-bk, tk.re, acc.re), fma(-bk, tk.im, acc.im));
+                else acc = mkcd(fma(bk, tk.im, acc.re), fma(-bk, tk.re, acc.im));
+            };
+            int k = 2;
+            for (; k + 3 < K; k += 4) {
+                step(k, 2);
+                step(k + 1, 3);
+                step(k + 2, 0);
+                step(k + 3, 1);
             }
+            for (; k < K; ++k) step(k, k & 3);
             v = cmul(ph, acc);
             if (j >= rows) v = mkcd(0.0, 0.0);
         }
@@ -508,7 +542,6 @@ struct LzSlotSmem {
     double d[M + 2];
     double e[M + 2];
     cd F[M + 2];
-    double bes[kBesMax];
     double tdiag[16];
     double toff[16];
     double scal[16];
