@@ -81,6 +81,14 @@ typedef struct {
 int oops_slot_form(const oops_term_table* T, int32_t max_slots, int32_t* nslot64, int32_t* nslot32, int32_t* masks,
                    double* coef);
 
+/* Pass plan of the tiled product the large path uses for Kronecker
+ * operators of 2^19 elements and more (*npasses = 0: the flat ELL product).
+ * mode 1 with v, w: w = H v on the host through that plan (the kernels'
+ * tile-to-index mapping, checked to be a bijection in every pass: status
+ * OOPS_ERR_NUMERIC otherwise); mode 2: w = H v through the flat ELL tables.
+ * No reference counterpart (host check of the device layout). */
+int oops_tile_plan(const oops_term_table* T, int32_t* npasses, int32_t mode, const double* v, double* w);
+
 /* --- context ------------------------------------------------------------ */
 int oops_ctx_create(int32_t device, oops_ctx** out);
 int oops_ctx_destroy(oops_ctx* ctx);

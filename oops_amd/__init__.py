@@ -60,7 +60,7 @@ EXPORTS = [
     "oops_echo_batch_query", "oops_echo_batch_wait", "oops_echo_batch_run", "oops_echo_batch_footprint",
     "oops_cce_echo_batch_create", "oops_server_create", "oops_server_submit", "oops_server_poll", "oops_server_state",
     "oops_server_result", "oops_server_pending", "oops_server_destroy", "oops_server_times",
-    "oops_hahn_batch_kernels", "oops_echo_batch_capture", "oops_slot_form",
+    "oops_hahn_batch_kernels", "oops_echo_batch_capture", "oops_slot_form", "oops_tile_plan",
 ]
 
 
@@ -204,6 +204,22 @@ def slot_form(H: TermTable, max_slots=4096):
     _check(lib().oops_slot_form(ctypes.byref(H.c), ctypes.c_int32(max_slots), ctypes.byref(n64), ctypes.byref(n32),
                                 _ptr(masks), _ptr(coef)), None, "oops_slot_form")
     return masks, coef, n64.value
+
+
+def tile_plan(H: TermTable, v=None, flat=False):
+    """Number of passes of the large path's tiled product for H (0: flat ELL
+    product); with v, also H v computed on the host through that plan (or
+    through the flat ELL tables with flat=True)."""
+    n = ctypes.c_int32()
+    if v is None:
+        _check(lib().oops_tile_plan(ctypes.byref(H.c), ctypes.byref(n), ctypes.c_int32(0), None, None), None,
+               "oops_tile_plan")
+        return n.value
+    v = np.ascontiguousarray(v, np.complex128)
+    w = np.zeros_like(v)
+    _check(lib().oops_tile_plan(ctypes.byref(H.c), ctypes.byref(n), ctypes.c_int32(2 if flat else 1), _ptr(v),
+                                _ptr(w)), None, "oops_tile_plan")
+    return n.value, w
 
 
 def hamvec(ctx: Context, H: TermTable, v):

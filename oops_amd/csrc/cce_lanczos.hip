@@ -49,6 +49,9 @@ struct LanczosSmem {
     double toff[16];
     double scal[16];
     int ints[8];
+#ifdef OOPS_PHASE_TIMERS
+    unsigned long long tbes;   // cycles of the Chebyshev set-up (interval, Bessel values)
+#endif
 };
 
 template <int M>
@@ -151,6 +154,9 @@ __device__ int cheb_coeffs(SM& S, int mb, int k1, double tau, double beta_m) {
     const int tid = threadIdx.x;
     int terms = 0;
     if (tid < kWave) {
+#ifdef OOPS_PHASE_TIMERS
+        const unsigned long long tb0 = clock64();
+#endif
         const int j = tid;
         const int rows = k1 ? M + 2 : mb;
         const double dj = j < mb ? S.d[j] : 0.0;
@@ -169,14 +175,18 @@ __device__ int cheb_coeffs(SM& S, int mb, int k1, double tau, double beta_m) {
         const double c = 0.5 * (xlo + xhi);
         const double r = fmax(0.5 * (xhi - xlo), 1e-300);
         const double ir = 1.0 / r;
-        // rows of S = (X - c) / r owned by this lane
-        const double sdiag = j < mb ? (-dj - c) * ir : (j < rows ? -c * ir : 0.0);
-        const double sup = -ej * ir;   // times x_{j+1}
-        const double slo = j < mb ? -ejm * ir : 0.0;
-        const double sM = (k1 && j == M) ? -beta_m * ir : 0.0;   // row M: times x_{M-1}
-        const double sM1 = (k1 && j == M + 1) ? ir : 0.0;        // row M+1: -i x_M / r
+        // rows of 2S = 2(X - c) / r owned by this lane (the factor 2 of the
+        // Chebyshev recurrence folded in): diagonal, the coupling to x_{j+1}
+        // and a complex coupling to x_{j-1} (T inside, -beta_m in row M,
+        // -i in row M+1)
+        const double sdiag = 2.0 * (j < mb ? (-dj - c) * ir : (j < rows ? -c * ir : 0.0));
+        const double sup = -2.0 * ej * ir;
+        const double lo_re = 2.0 * (j < mb ? -ejm * ir : ((k1 && j == M) ? -beta_m * ir : 0.0));
+        const double lo_im = (k1 && j == M + 1) ? -2.0 * ir : 0.0;
         const double xfull = tau * r;
-        const int ns = xfull > 40.0 ? (int)ceil(xfull / 40.0) : 1;
+        // substeps of tau r / ns <= 64 (the Bessel values below stay within
+        // 2e-15 of J_k for x <= 64 with at most 128 recurrence steps)
+        const int ns = xfull > 64.0 ? (int)ceil(xfull / 64.0) : 1;
         const double x = xfull / ns;
         // Bessel J_k(x) by backward recurrence (Miller) from N, normalised with
         // J_0 + 2 sum J_2k = 1; every lane runs the recurrence and keeps its k.
@@ -209,32 +219,37 @@ __device__ int cheb_coeffs(SM& S, int mb, int k1, double tau, double beta_m) {
             K = sl ? __ffsll((long long)sl) - 1 : (sh ? kWave + __ffsll((long long)sh) - 1 : N);
             if (K < 2) K = 2;
         }
+#ifdef OOPS_PHASE_TIMERS
+        if (j == 0) S.tbes += clock64() - tb0;
+#endif
+        // 2 J_k for k >= 1 (J_0 as is): the series coefficients
+        if (j != 0) bl *= 2.0;
+        bh *= 2.0;
         auto bes = [&](int k) -> double { return k < kWave ? lane_val(bl, k) : lane_val(bh, k - kWave); };
-        auto apply = [&](cd t) -> cd {
+        auto apply2 = [&](cd t) -> cd {   // 2 S t
             const double pr = wave_shr1(t.re), pi = wave_shr1(t.im);
             const double nr = wave_shl1(t.re), ni = wave_shl1(t.im);
-            cd y = mkcd(fma(sdiag, t.re, fma(slo, pr, sup * nr)), fma(sdiag, t.im, fma(slo, pi, sup * ni)));
-            y = mkcd(fma(sM, pr, fma(sM1, pi, y.re)), fma(sM, pi, fma(-sM1, pr, y.im)));
-            return y;
+            return mkcd(fma(sdiag, t.re, fma(lo_re, pr, fma(-lo_im, pi, sup * nr))),
+                        fma(sdiag, t.im, fma(lo_re, pi, fma(lo_im, pr, sup * ni))));
         };
         cd v = mkcd(j == 0 ? 1.0 : 0.0, 0.0);
         const cd ph = ool_expi(c * (tau / ns));   // e^{i c tau / ns}
         for (int sub = 0; sub < ns; ++sub) {
             // acc = J0 T0 v + sum_{k>=1} 2 i^k J_k T_k v, T_{k+1} = 2 S T_k - T_{k-1}
             cd tm = v;            // T_0(S) v
-            cd tk = apply(v);     // T_1(S) v
+            cd tk = cscale(apply2(v), 0.5);   // T_1(S) v
             cd acc = cscale(tm, bes(0));
             {
-                const double b1 = 2.0 * bes(1);
+                const double b1 = bes(1);
                 acc = mkcd(acc.re - b1 * tk.im, acc.im + b1 * tk.re);   // i^1
             }
             // step k with i^k = (1, i, -1, -i)[R]
             auto step = [&](int k, int R) {
-                const cd y = apply(tk);
-                const cd tn = mkcd(2.0 * y.re - tm.re, 2.0 * y.im - tm.im);
+                const cd y = apply2(tk);
+                const cd tn = mkcd(y.re - tm.re, y.im - tm.im);
                 tm = tk;
                 tk = tn;
-                const double bk = 2.0 * bes(k);
+                const double bk = bes(k);
                 if (R == 0) acc = mkcd(fma(bk, tk.re, acc.re), fma(bk, tk.im, acc.im));
                 else if (R == 1) acc = mkcd(fma(-bk, tk.im, acc.re), fma(bk, tk.re, acc.im));
                 else if (R == 2) acc = mkcd(fma(-bk, tk.re, acc.re), fma(-bk, tk.im, acc.im));
@@ -546,6 +561,9 @@ struct LzSlotSmem {
     double toff[16];
     double scal[16];
     int ints[8];
+#ifdef OOPS_PHASE_TIMERS
+    unsigned long long tbes;   // cycles of the Chebyshev set-up (interval, Bessel values)
+#endif
 };
 
 template <int M>
@@ -573,7 +591,7 @@ __device__ __forceinline__ void advance_slot(LzSlotSmem<M>& S, BlockRed<kNT>& br
             OOPS_TOC(g, tmv, tm0);
             OOPS_TIC(tr0);
             const cd vj = vcur;
-            const auto r1 = block_sum1<4, kNT>(br, [&](int k) -> double {
+            const auto r1 = block_sum_lanes<3, kNT>(br, [&](int k) -> double {
                 if (k == 0) return fma(vj.re, p.re, vj.im * p.im);
                 if (k == 1) return fma(vj.re, p.im, -vj.im * p.re);
                 if (k == 2) return cabs2(p);
@@ -607,7 +625,7 @@ __device__ __forceinline__ void advance_slot(LzSlotSmem<M>& S, BlockRed<kNT>& br
             __syncthreads();
             const cd p = so.apply(S.xs, S.xs32, tid);
             ++g.nmult;
-            const auto tn = block_sum1<1, kNT>(br, [&](int) { return cabs2(p); });
+            const auto tn = block_sum_lanes<1, kNT>(br, [&](int) { return cabs2(p); });
             avnorm = sqrt(tn.get(0));
         }
         __syncthreads();
@@ -646,7 +664,7 @@ __device__ __forceinline__ void advance_slot(LzSlotSmem<M>& S, BlockRed<kNT>& br
         }
         g.w = cscale(acc, g.beta);
         {
-            const auto tb = block_sum1<1, kNT>(br, [&](int) { return tid < g.n ? cabs2(g.w) : 0.0; });
+            const auto tb = block_sum_lanes<1, kNT>(br, [&](int) { return tid < g.n ? cabs2(g.w) : 0.0; });
             g.beta = sqrt(tb.get(0));
         }
         g.t_new = gamma * t_step * ool_pow(t_step * tol / err_loc, g.xm);
@@ -666,6 +684,9 @@ void cce_hahn_lanczos_slot_kernel(KrylovArgs a) {
     const double tol = a.tol;
     const int max_steps = a.max_steps;
     double2* const ws = a.slotws + (int64_t)blockIdx.x * 4 * kSlotOpD2;
+#ifdef OOPS_PHASE_TIMERS
+    if (tid == 0) S.tbes = 0;
+#endif
     for (;;) {
         if (tid == 0) S.ints[7] = atomicAdd(a.queue, 1);
         __syncthreads();
@@ -727,7 +748,7 @@ void cce_hahn_lanczos_slot_kernel(KrylovArgs a) {
                 gA = gB;
                 gB = tmp;
             }
-            const auto tl = block_sum1<1, kNT>(br, [&](int) {
+            const auto tl = block_sum_lanes<1, kNT>(br, [&](int) {
                 return fma(gB.w.re, gA.w.re, gB.w.im * gA.w.im);
             });
             if (tid == 0) q.L[k] = scale * tl.get(0);
@@ -747,6 +768,8 @@ void cce_hahn_lanczos_slot_kernel(KrylovArgs a) {
             st[17] = (int)((gA.tred + gB.tred) >> 10);
             st[18] = (int)((gA.texp + gB.texp) >> 10);
             st[19] = (int)((gA.tupd + gB.tupd) >> 10);
+            st[20] = (int)(S.tbes >> 10);
+            S.tbes = 0;
 #endif
         }
         __syncthreads();

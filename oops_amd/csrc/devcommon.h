@@ -222,6 +222,72 @@ __device__ __forceinline__ BlockTotals<NT, P> block_sum1(BlockRed<NT>& br, Gen g
     return t;
 }
 
+// Wave sum of a double without LDS traffic: DPP row_mirror, row_half_mirror
+// and two quad permutations sum each 16-lane row (every lane of the row ends
+// with the row total, all lanes adding in an order that gives bit-identical
+// results), then gfx950's permlane16/permlane32 swaps add the rows.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+template <bool HALF32>
+__device__ __forceinline__ double swap_add_f64(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = (int)b, hi = (int)(b >> 32);
+    const auto l = HALF32 ? __builtin_amdgcn_permlane32_swap(lo, lo, false, false)
+                          : __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto h = HALF32 ? __builtin_amdgcn_permlane32_swap(hi, hi, false, false)
+                          : __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    const double e = __longlong_as_double(((long long)h[0] << 32) | (unsigned)l[0]);
+    const double o = __longlong_as_double(((long long)h[1] << 32) | (unsigned)l[1]);
+    return e + o;
+}
+
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+    v += dpp_f64<0x140>(v);   // row_mirror: lane i + lane 15-i
+    v += dpp_f64<0x141>(v);   // row_half_mirror: lane i + lane 7-i (within 8)
+    v += dpp_f64<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dpp_f64<0xB1>(v);    // quad_perm [1,0,3,2]
+    v = swap_add_f64<false>(v);
+    return swap_add_f64<true>(v);
+}
+
+// Block sums of P values with one barrier; every lane gets all P totals in
+// registers (bit-identical across the block).
+template <int P>
+struct LaneTotals {
+    double v[P];
+    __device__ __forceinline__ double get(int k) const { return v[k]; }
+};
+
+template <int P, int NT, typename Gen>
+__device__ __forceinline__ LaneTotals<P> block_sum_lanes(BlockRed<NT>& br, Gen gen) {
+    constexpr int NW = NT / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    LaneTotals<P> t;
+#pragma unroll
+    for (int k = 0; k < P; ++k) t.v[k] = wave_sum_dpp(gen(k));
+    double* part = br.ws + br.slot * NW * 64;
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+        if (lane == k) part[wave * P + k] = t.v[k];
+    br.slot ^= 1;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+        double s = part[k];
+#pragma unroll
+        for (int w = 1; w < NW; ++w) s += part[w * P + k];
+        t.v[k] = s;
+    }
+    return t;
+}
+
 template <int P, int NT>
 __device__ __forceinline__ void block_sum(double (&v)[P], double* ws, double* out) {
     block_sum_gen<P, NT>([&](int k) { return v[k]; }, ws, out);

@@ -78,6 +78,72 @@ __global__ __launch_bounds__(kLT) void large_matvec(LargeOpArgs o, const double2
     }
 }
 
+// Tiled product: workgroup b owns the tile whose block-spin digits are the
+// digits of b; tile element e (tile-spin digits) sits at global index
+// base(b) | scatter(e).  The tile is read once (coalesced in runs of the
+// lowest spin's digits, which every pass keeps in G), the pass's local
+// operators are applied from LDS, and y is written (pass 0, which also
+// stores the scaled basis vector) or accumulated.
+struct LargeTileLds {
+    cd xs[1 << kLargeTileBits];
+    DevOpHeader hdr;
+    DevOpLocal locs[kLargeTileMaxLocals];
+    int32_t map[kLargeTileMapStride];
+};
+
+template <typename VT>
+__global__ __launch_bounds__(kLT) void large_tile_matvec(LargeTilePass tp, const double2* __restrict__ x,
+                                                         const double* __restrict__ xscale_p,
+                                                         double2* __restrict__ y, VT* __restrict__ xstore,
+                                                         int alpha_mode, int accumulate) {
+    __shared__ LargeTileLds S;
+    const int tid = threadIdx.x;
+    if (tid == 0) S.hdr = *tp.op.hdr;
+    if (tid < kLargeTileMapStride) S.map[tid] = tp.map[tid];
+    const int nloc = tp.op.hdr->nloc;
+    for (int k = tid; k < nloc; k += kLT) S.locs[k] = tp.op.loc[k];
+    __syncthreads();
+    const int ld = tp.ld, ng = tp.ng, nb = tp.nb;
+    const int64_t lmask = (1 << ld) - 1;
+    const int tsize = 1 << (ld * ng);
+    int64_t base = 0;
+    {
+        int64_t b = blockIdx.x;
+        for (int q = nb - 1; q >= 0; --q) {
+            base |= (b & lmask) << S.map[16 + q];
+            b >>= ld;
+        }
+    }
+    auto gidx = [&](int e) -> int64_t {
+        int64_t g = base;
+        for (int q = ng - 1; q >= 0; --q) {
+            g |= ((int64_t)e & lmask) << S.map[q];
+            e >>= ld;
+        }
+        return g;
+    };
+    const double xsc = xscale_p ? *xscale_p : 1.0;
+    for (int e = tid; e < tsize; e += kLT) {
+        const int64_t g = gidx(e);
+        const double2 xv = x[g];
+        S.xs[e] = mkcd(xv.x, xv.y);
+        if (xstore) vstore(xstore + g, xv.x * xsc, xv.y * xsc);
+    }
+    __syncthreads();
+    const cd* vals = reinterpret_cast<const cd*>(tp.op.val);
+    for (int e = tid; e < tsize; e += kLT) {
+        cd acc = op_apply_elem(e, S.xs, S.hdr, S.locs, vals, tp.op.off);
+        acc = cscale(acc, xsc);
+        if (alpha_mode == 1) acc = cmulmi(acc);
+        const int64_t g = gidx(e);
+        if (accumulate) {
+            const double2 yo = y[g];
+            acc = mkcd(acc.re + yo.x, acc.im + yo.y);
+        }
+        y[g] = make_double2(acc.re, acc.im);
+    }
+}
+
 // Same product for a general sparse matrix in CSR form (row_ptr[n+1], col,
 // val): y = alpha * A (xscale * x), alpha_mode 0: +1, 1: -i.
 __global__ __launch_bounds__(kLT) void large_csr_matvec(const int64_t* __restrict__ row_ptr,
@@ -369,6 +435,16 @@ void launch_large_matvec(const LargeOpArgs& o, const double2* x, const double* x
     else
         hipLaunchKernelGGL(large_matvec<double2>, dim3(grid), dim3(kLT), 0, s, o, x, xscale, y, (double2*)xstore, n,
                            alpha_mode);
+}
+
+void launch_large_tile_matvec(const LargeTilePass& tp, const double2* x, const double* xscale, double2* y,
+                              void* xstore, int alpha_mode, int accumulate, hipStream_t s, bool fp32) {
+    if (fp32)
+        hipLaunchKernelGGL(large_tile_matvec<float2>, dim3(tp.ntiles), dim3(kLT), 0, s, tp, x, xscale, y,
+                           (float2*)xstore, alpha_mode, accumulate);
+    else
+        hipLaunchKernelGGL(large_tile_matvec<double2>, dim3(tp.ntiles), dim3(kLT), 0, s, tp, x, xscale, y,
+                           (double2*)xstore, alpha_mode, accumulate);
 }
 
 void launch_large_dots(const double2* V, int64_t ldv, const double2* p, int64_t n, int J, int with_norm, double* part,

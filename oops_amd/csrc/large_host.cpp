@@ -56,8 +56,155 @@ struct LargeRun {
     int64_t n = 0;
     int m = 0, ldh = 0, grid = 0, nblk = 0;
     DevBuf V, p0, p1, w, v0, part, hess, coef, scal, flags, F, status, wpart;
+    // tiled product (large Kronecker operators): one entry per pass
+    struct Tile {
+        DevBuf hdr, loc, val, off, map;
+        LargeTilePass pass{};
+    };
+    std::vector<Tile> tiles;
     int rc = 0;
 };
+
+// Pass plan of the tiled product.  The lowest spins filling kLargeTileBits
+// bits form pass 0; the other passes hold every spin outside that set (the
+// "outer" spins), the lowest spin (coalesced runs) and a chunk of the
+// remaining inner spins, so that every pair (outer, inner) and every local
+// operator among the outer spins lies inside some pass.  Each local operator
+// goes to the first pass containing its spins.  Returns false (flat ELL
+// product) when the state is small or the outer spins do not fit a tile.
+struct TilePlan {
+    std::vector<int> G;   // spins of the tile, slowest first
+    CompiledOp op;        // the pass's local operators in tile digits
+};
+
+static bool plan_tiles(const CompiledOp& op, std::vector<TilePlan>& plan) {
+    const int ns = op.nspin, ld = op.ld_bits;
+    if (ld < 1 || ns * ld < 19 || ns * ld > 30) return false;
+    const int tmax = kLargeTileBits / ld;
+    const int nouter = ns - tmax;
+    if (nouter < 1 || nouter + 2 > tmax || nouter > 32) return false;
+    std::vector<std::vector<int>> G;
+    {
+        std::vector<int> g0;
+        for (int s = nouter; s < ns; ++s) g0.push_back(s);
+        G.push_back(g0);
+        const int chunk = tmax - nouter - 1;
+        for (int s0 = nouter; s0 < ns - 1; s0 += chunk) {
+            std::vector<int> g;
+            for (int s = 0; s < nouter; ++s) g.push_back(s);
+            for (int s = s0; s < std::min(s0 + chunk, ns - 1); ++s) g.push_back(s);
+            g.push_back(ns - 1);
+            G.push_back(g);
+        }
+    }
+    const int64_t lmask = (1LL << ld) - 1;
+    auto gpos = [&](int s) { return ld * (ns - 1 - s); };
+    plan.assign(G.size(), TilePlan{});
+    std::vector<int> done(op.locs.size(), 0);
+    for (size_t p = 0; p < G.size(); ++p) {
+        const std::vector<int>& g = G[p];
+        std::vector<int> tq(ns, -1);
+        for (size_t q = 0; q < g.size(); ++q) tq[g[q]] = (int)q;
+        const int ng = (int)g.size();
+        auto tpos = [&](int q) { return ld * (ng - 1 - q); };
+        CompiledOp& T = plan[p].op;
+        plan[p].G = g;
+        T.n = 1LL << (ld * ng);
+        T.nspin = ng;
+        T.ld_bits = ld;
+        T.c0 = p == 0 ? op.c0 : std::complex<double>(0.0, 0.0);
+        for (int pass_nb = 1; pass_nb <= 2; ++pass_nb) {   // 1-body entries first
+            if (pass_nb == 2) T.ent2_first = (int32_t)T.val.size();
+            for (size_t L = 0; L < op.locs.size(); ++L) {
+                const DevOpLocal& lo = op.locs[L];
+                if (done[L] || lo.nb != pass_nb) continue;
+                if (tq[lo.s] < 0 || (lo.nb == 2 && tq[lo.t] < 0)) continue;
+                done[L] = 1;
+                DevOpLocal t = lo;
+                t.s = (int16_t)tq[lo.s];
+                t.t = (int16_t)(lo.nb == 2 ? tq[lo.t] : tq[lo.s]);
+                t.ent = (int32_t)T.val.size();
+                const int64_t nent = (int64_t)lo.rows * lo.R;
+                for (int64_t e = 0; e < nent; ++e) {
+                    const int64_t off = op.off[lo.ent + e];
+                    int64_t o = ((off >> gpos(lo.s)) & lmask) << tpos(t.s);
+                    if (lo.nb == 2) o |= ((off >> gpos(lo.t)) & lmask) << tpos(t.t);
+                    T.val.push_back(op.val[lo.ent + e]);
+                    T.off.push_back((int32_t)o);
+                }
+                T.locs.push_back(t);
+                T.max_row = std::max(T.max_row, (int)lo.R);
+            }
+        }
+        if ((int)T.locs.size() > kLargeTileMaxLocals) return false;
+    }
+    for (int d : done)
+        if (!d) return false;
+    // every global index appears in exactly one (tile, element) of every pass
+    const int64_t n = op.n;
+    for (const TilePlan& tp : plan) {
+        int64_t bits = 0;
+        for (int s : tp.G) bits |= lmask << gpos(s);
+        if ((int64_t)tp.op.n * (n / tp.op.n) != n) return false;
+        if (__builtin_popcountll((unsigned long long)bits) != ld * (int)tp.G.size()) return false;
+    }
+    return true;
+}
+
+template <typename T>
+int upload(oops_ctx* ctx, DevBuf& b, const std::vector<T>& v);
+
+static int setup_tiles(oops_ctx* ctx, LargeRun& R) {
+    std::vector<TilePlan> plan;
+    if (!plan_tiles(R.op, plan)) return OOPS_OK;
+    const int ns = R.op.nspin, ld = R.op.ld_bits;
+    R.tiles.resize(plan.size());
+    for (size_t p = 0; p < plan.size(); ++p) {
+        const CompiledOp& T = plan[p].op;
+        LargeRun::Tile& t = R.tiles[p];
+        DevOpHeader h;
+        std::memset(&h, 0, sizeof(h));
+        h.n = (int32_t)T.n;
+        h.nspin = T.nspin;
+        h.ld_bits = T.ld_bits;
+        h.nloc = (int32_t)T.locs.size();
+        h.nent = (int32_t)T.val.size();
+        h.max_row = T.max_row;
+        h.c0re = T.c0.real();
+        h.c0im = T.c0.imag();
+        h.ent2_first = T.ent2_first;
+        std::vector<double> vals;
+        for (auto& z : T.val) { vals.push_back(z.real()); vals.push_back(z.imag()); }
+        if (vals.empty()) { vals.push_back(0.0); vals.push_back(0.0); }
+        std::vector<int32_t> offs(T.off);
+        if (offs.empty()) offs.push_back(0);
+        std::vector<DevOpLocal> locs(T.locs);
+        if (locs.empty()) locs.push_back(DevOpLocal{});
+        std::vector<int32_t> map(kLargeTileMapStride, 0);
+        std::vector<int> in(ns, 0);
+        const std::vector<int>& g = plan[p].G;
+        for (size_t q = 0; q < g.size(); ++q) { map[q] = ld * (ns - 1 - g[q]); in[g[q]] = 1; }
+        int nb = 0;
+        for (int s2 = 0; s2 < ns; ++s2)
+            if (!in[s2]) map[16 + nb++] = ld * (ns - 1 - s2);
+        int rc = upload(ctx, t.hdr, std::vector<DevOpHeader>{h});
+        rc = rc ? rc : upload(ctx, t.loc, locs);
+        rc = rc ? rc : upload(ctx, t.val, vals);
+        rc = rc ? rc : upload(ctx, t.off, offs);
+        rc = rc ? rc : upload(ctx, t.map, map);
+        if (rc) return rc;
+        t.pass.op.hdr = (const DevOpHeader*)t.hdr.p;
+        t.pass.op.loc = (const DevOpLocal*)t.loc.p;
+        t.pass.op.val = (const double2*)t.val.p;
+        t.pass.op.off = (const int32_t*)t.off.p;
+        t.pass.map = (const int32_t*)t.map.p;
+        t.pass.ng = (int32_t)g.size();
+        t.pass.nb = nb;
+        t.pass.ld = ld;
+        t.pass.ntiles = (int32_t)(R.op.n >> (ld * (int)g.size()));
+    }
+    return OOPS_OK;
+}
 
 // y = alpha Op (xscale x), alpha_mode 0: +1, 1: -i.  A Kronecker term table
 // carries the Hamiltonian H (the driver works on -iH); a CSR matrix carries
@@ -66,6 +213,10 @@ void matvec(LargeRun& R, const double2* x, const double* xs, double2* y, void* x
     if (R.csr)
         launch_large_csr_matvec((const int64_t*)R.rp.p, (const int32_t*)R.ci.p, (const double2*)R.cv.p, x, xs, y,
                                 (double2*)xstore, R.n, alpha_mode, R.grid, R.s);
+    else if (!R.tiles.empty())
+        for (size_t p = 0; p < R.tiles.size(); ++p)
+            launch_large_tile_matvec(R.tiles[p].pass, x, xs, y, p == 0 ? xstore : nullptr, alpha_mode, p > 0 ? 1 : 0,
+                                     R.s, R.fp32);
     else
         launch_large_matvec(R.args, x, xs, y, xstore, R.n, alpha_mode, R.grid, R.s, R.fp32);
 }
@@ -137,6 +288,8 @@ int setup_compiled(oops_ctx* ctx, int m, LargeRun& R) {
     R.args.loc = (const DevOpLocal*)R.loc.p;
     R.args.val = (const double2*)R.val.p;
     R.args.off = (const int32_t*)R.off.p;
+    rc = setup_tiles(ctx, R);
+    if (rc) return rc;
     return setup_buffers(ctx, m, R);
 }
 
@@ -483,7 +636,86 @@ int large_hahn_item(oops_ctx* ctx, const CompiledOp& A, const CompiledOp& B, con
 
 }  // namespace oops
 
+// Host evaluation of the compiled product (ELL rows of the merged local
+// operators), element i of y = H x over a state of op.n elements whose index
+// is taken through `gidx` (identity for the flat product, the tile scatter for
+// a pass).
+template <typename Index>
+static std::complex<double> host_elem(const CompiledOp& op, int64_t i, const std::complex<double>* x, Index gidx) {
+    const int ld = op.ld_bits;
+    const int64_t lmask = (1LL << ld) - 1;
+    const int top = op.nspin - 1;
+    std::complex<double> acc = op.c0 * x[gidx(i)];
+    for (const DevOpLocal& lo : op.locs) {
+        const int shs = ld * (top - lo.s);
+        int64_t row = (i >> shs) & lmask;
+        int64_t base = i - (row << shs);
+        if (lo.nb == 2) {
+            const int sht = ld * (top - lo.t);
+            const int64_t lt = (i >> sht) & lmask;
+            base -= lt << sht;
+            row = (row << ld) | lt;
+        }
+        for (int e = 0; e < lo.R; ++e) {
+            const int64_t k = lo.ent + row * lo.R + e;
+            acc += op.val[k] * x[gidx(base + op.off[k])];
+        }
+    }
+    return acc;
+}
+
 extern "C" {
+
+int oops_tile_plan(const oops_term_table* T, int32_t* npasses, int32_t mode, const double* v, double* w) {
+    if (!T || !npasses) return OOPS_ERR_ARG;
+    CompiledOp op;
+    std::string err;
+    int rc = compile_operator(T, op, err, kLargeMaxLocals);
+    if (rc) return rc;
+    std::vector<TilePlan> plan;
+    const bool tiled = plan_tiles(op, plan);
+    *npasses = tiled ? (int32_t)plan.size() : 0;
+    if (mode == 0 || !v || !w) return OOPS_OK;
+    typedef std::complex<double> cplx;
+    const cplx* x = reinterpret_cast<const cplx*>(v);
+    cplx* y = reinterpret_cast<cplx*>(w);
+    const int64_t n = op.n;
+    if (mode == 2 || !tiled) {
+        for (int64_t i = 0; i < n; ++i) y[i] = host_elem(op, i, x, [](int64_t g) { return g; });
+        return OOPS_OK;
+    }
+    // the kernel's mapping: tile b, element e -> base(b) | scatter(e)
+    const int ns = op.nspin, ld = op.ld_bits;
+    const int64_t lmask = (1LL << ld) - 1;
+    std::vector<uint8_t> seen(n);
+    for (size_t p = 0; p < plan.size(); ++p) {
+        const std::vector<int>& g = plan[p].G;
+        std::vector<int> in(ns, 0), blk;
+        for (int s2 : g) in[s2] = 1;
+        for (int s2 = 0; s2 < ns; ++s2)
+            if (!in[s2]) blk.push_back(s2);
+        const int ng = (int)g.size(), nb = (int)blk.size();
+        const int64_t tsize = 1LL << (ld * ng), ntiles = n >> (ld * ng);
+        std::fill(seen.begin(), seen.end(), 0);
+        for (int64_t b = 0; b < ntiles; ++b) {
+            int64_t base = 0, bb = b;
+            for (int q = nb - 1; q >= 0; --q) { base |= (bb & lmask) << (ld * (ns - 1 - blk[q])); bb >>= ld; }
+            auto scatter = [&](int64_t e) {
+                int64_t gi = base;
+                for (int q = ng - 1; q >= 0; --q) { gi |= (e & lmask) << (ld * (ns - 1 - g[q])); e >>= ld; }
+                return gi;
+            };
+            for (int64_t e = 0; e < tsize; ++e) {
+                const int64_t gi = scatter(e);
+                if (gi < 0 || gi >= n || seen[gi]) return OOPS_ERR_NUMERIC;   // not a bijection
+                seen[gi] = 1;
+                const cplx a = host_elem(plan[p].op, e, x, scatter);
+                y[gi] = p == 0 ? a : y[gi] + a;
+            }
+        }
+    }
+    return OOPS_OK;
+}
 
 int oops_hamvec(oops_ctx* ctx, const oops_term_table* H, const double* v, double* w) {
     if (!ctx || !H || !v || !w) return OOPS_ERR_ARG;
@@ -493,7 +725,7 @@ int oops_hamvec(oops_ctx* ctx, const oops_term_table* H, const double* v, double
     R.s = nullptr;
     rc = hip_check(ctx, hipMemcpy(R.p0.p, v, sizeof(double2) * R.n, hipMemcpyHostToDevice), "H2D v");
     if (rc) return rc;
-    launch_large_matvec(R.args, (const double2*)R.p0.p, nullptr, (double2*)R.p1.p, nullptr, R.n, 0, R.grid, nullptr);
+    matvec(R, (const double2*)R.p0.p, nullptr, (double2*)R.p1.p, nullptr, 0);
     rc = hip_check(ctx, hipGetLastError(), "matvec launch");
     rc = rc ? rc : hip_check(ctx, hipMemcpy(w, R.p1.p, sizeof(double2) * R.n, hipMemcpyDeviceToHost), "D2H w");
     return rc;

@@ -17,6 +17,28 @@ struct LargeOpArgs {
     const int32_t* off;
 };
 
+// Tiled product for large states: the vector is cut into tiles of
+// 2^kLargeTileBits elements whose index bits are the digits of a set G of
+// spins (the rest of the index fixed per workgroup); a tile is staged in LDS
+// and every local operator acting inside G is applied there.  A few passes
+// with different G (each one launch) cover all local operators; the first
+// writes y, the others add to it.
+constexpr int kLargeTileBits = 12;
+constexpr int kLargeTileMapStride = 48;  // int32 per pass: [0,16) tile spins' bit positions, [16,48) block spins'
+constexpr int kLargeTileMaxLocals = 256; // local operators of one pass (LDS)
+
+struct LargeTilePass {
+    LargeOpArgs op;       // tile operator (local operators of this pass, offsets in tile digits)
+    const int32_t* map;   // kLargeTileMapStride ints (device)
+    int32_t ng;           // tile spins
+    int32_t nb;           // block spins
+    int32_t ld;           // bits per local digit
+    int32_t ntiles;       // 2^(ld * nb)
+};
+
+void launch_large_tile_matvec(const LargeTilePass& tp, const double2* x, const double* xscale, double2* y,
+                              void* xstore, int alpha_mode, int accumulate, hipStream_t s, bool fp32 = false);
+
 int large_grid(int64_t n, int num_cu);
 // xstore (basis column, may be NULL) is float2 when fp32, else double2.
 void launch_large_matvec(const LargeOpArgs& o, const double2* x, const double* xscale, double2* y, void* xstore,

@@ -37,6 +37,22 @@ def test_hamvec_against_explicit_kron(ctx):
 
 
 @pytest.mark.gpu
+def test_tiled_product_matches_flat_product(ctx):
+    """The tiled product of the large path (10-spin Liouvillian, n = 2^20,
+    6 passes) against the flat ELL product evaluated on the host from the
+    same compiled tables (pinned by the Kronecker fixtures above)."""
+    rng = np.random.default_rng(0)
+    X = np.array([p for p in rng.uniform(-15, 15, (400, 3)) if np.linalg.norm(p) <= 15.0][:10])
+    T, _, _ = O.cluster_terms(X, O.cce_params(B=np.ones(3) * 0.05 / np.sqrt(3.0)), 0)
+    assert O.tile_plan(T) == 6
+    v = rng.standard_normal(T.n) + 1j * rng.standard_normal(T.n)
+    print(f"tiled hamvec n={T.n}", flush=True)
+    got = O.hamvec(ctx, T, v)
+    _, want = O.tile_plan(T, v, flat=True)
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+
+
+@pytest.mark.gpu
 def test_expv_against_scipy_fixture(ctx):
     g = np.load(os.path.join(GOLD, "expv_cases.npz"), allow_pickle=False)
     for c in range(int(g["ncases"])):
