@@ -57,6 +57,16 @@ typedef struct {
 int oops_bath_diamond(double a, double radius, double p, uint64_t seed, int32_t shape, double* coords,
                       int32_t max_n, int32_t* n_out);
 
+/* Spin-coordinate files (cSpinSourceFromFile::generate and
+ * cSpinSource::export_coordinates, SpinSource.cpp:20-79): first the spin
+ * count, then one line per spin "isotope x y z" (Angstrom).  Reading writes
+ * up to max_n spins (coords x,y,z; isotope names NUL-terminated in 16-byte
+ * slots, may be NULL) and the count in *n_out (OOPS_ERR_CAPACITY when the file
+ * holds more).  Writing uses the reference's format: tab-separated, default
+ * six significant digits. */
+int oops_read_xyz(const char* path, int32_t max_n, double* coords, char* isotopes, int32_t* n_out);
+int oops_write_xyz(const char* path, int32_t n, const double* coords, const char* isotopes);
+
 /* NV eigenvalues (rad/s, ascending) and eigenvectors (3x3 complex,
  * column-major, interleaved). */
 int oops_nv_states(const double* pos, const double* B, double* evals, double* evecs);

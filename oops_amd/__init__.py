@@ -60,7 +60,7 @@ EXPORTS = [
     "oops_echo_batch_query", "oops_echo_batch_wait", "oops_echo_batch_run", "oops_echo_batch_footprint",
     "oops_cce_echo_batch_create", "oops_server_create", "oops_server_submit", "oops_server_poll", "oops_server_state",
     "oops_server_result", "oops_server_pending", "oops_server_destroy", "oops_server_times",
-    "oops_hahn_batch_kernels", "oops_echo_batch_capture", "oops_slot_form", "oops_tile_plan",
+    "oops_hahn_batch_kernels", "oops_echo_batch_capture", "oops_slot_form", "oops_tile_plan", "oops_read_xyz", "oops_write_xyz",
 ]
 
 
@@ -220,6 +220,32 @@ def tile_plan(H: TermTable, v=None, flat=False):
     _check(lib().oops_tile_plan(ctypes.byref(H.c), ctypes.byref(n), ctypes.c_int32(2 if flat else 1), _ptr(v),
                                 _ptr(w)), None, "oops_tile_plan")
     return n.value, w
+
+
+def read_xyz(path, max_n=1 << 20):
+    """Spin coordinates from a coordinate file (count, then "isotope x y z"
+    lines): (coords [n, 3] in Angstrom, isotope names)."""
+    n = ctypes.c_int32()
+    coords = np.zeros((max_n, 3))
+    iso = ctypes.create_string_buffer(16 * max_n)
+    _check(lib().oops_read_xyz(os.fsencode(path), ctypes.c_int32(max_n), _ptr(coords), iso, ctypes.byref(n)), None,
+           "oops_read_xyz")
+    raw = iso.raw[:16 * n.value]
+    names = [raw[16 * i:16 * (i + 1)].split(b"\0", 1)[0].decode() for i in range(n.value)]
+    return coords[:n.value].copy(), names
+
+
+def write_xyz(path, coords, isotopes=None):
+    """Write coordinates in the same format (the reference's export layout)."""
+    coords = np.ascontiguousarray(coords, np.float64).reshape(-1, 3)
+    n = len(coords)
+    iso = None
+    if isotopes is not None:
+        iso = ctypes.create_string_buffer(16 * n)
+        for i, name in enumerate(isotopes):
+            b = name.encode()[:15]
+            iso[16 * i:16 * i + len(b)] = b
+    _check(lib().oops_write_xyz(os.fsencode(path), ctypes.c_int32(n), _ptr(coords), iso), None, "oops_write_xyz")
 
 
 def hamvec(ctx: Context, H: TermTable, v):

@@ -5,12 +5,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <thread>
+#include <atomic>
 #include <cmath>
 #include <complex>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -24,7 +27,11 @@ using oops::DevOpLocal;
 namespace oops {
 
 int set_err(oops_ctx* ctx, int code, const std::string& msg) {
-    if (ctx) ctx->err = msg;
+    static std::mutex mu;   // host worker threads (large items, serving builders) share a context
+    if (ctx) {
+        std::lock_guard<std::mutex> lock(mu);
+        ctx->err = msg;
+    }
     return code;
 }
 
@@ -612,16 +619,34 @@ static int bucket_run_large(oops_hahn_batch* b, oops_hahn_batch::Bucket& B, int3
     for (size_t i = 0; i < ni; ++i) total += b->per_item ? B.hnt[i] : nt;
     B.hL.assign((size_t)total, 0.0);
     B.hstats.assign(ni * kStatsPerItem, 0);
-    int64_t off = 0;
-    for (size_t i = 0; i < ni; ++i) {
-        const int nti = b->per_item ? B.hnt[i] : nt;
-        const double* ti = b->per_item ? B.htau.data() + B.htau_off[i] : tau;
-        int rc = large_hahn_item(b->ctx, B.hA[i], B.hB[i], B.hu0[i].data(), B.hy0[i].data(), B.hscale[i], nti, ti,
-                                 b->m, b->klim, b->tol, B.hL.data() + off, B.hstats.data() + i * kStatsPerItem);
-        if (rc) return rc;
-        off += nti;
-    }
-    return OOPS_OK;
+    std::vector<int64_t> off(ni + 1, 0);
+    for (size_t i = 0; i < ni; ++i) off[i + 1] = off[i] + (b->per_item ? B.hnt[i] : nt);
+    // items run concurrently on host threads, each on its own stream (the
+    // per-step host synchronisation of one item overlaps the kernels of the
+    // others); device memory per item ~ n (m + 3 + nt) complex
+    const double per_item = 16.0 * (double)B.n * (b->m + 3 + (double)(off[ni] / std::max<size_t>(ni, 1)));
+    const int nthreads = (int)std::max(1.0, std::min({8.0, (double)ni, 16e9 / per_item}));
+    std::atomic<size_t> next{0};
+    std::atomic<int> first_rc{0};
+    auto worker = [&]() {
+        if (hipSetDevice(b->ctx->device) != hipSuccess) { first_rc = OOPS_ERR_HIP; return; }
+        for (size_t i; (i = next++) < ni && first_rc.load() == 0;) {
+            const int nti = b->per_item ? B.hnt[i] : nt;
+            const double* ti = b->per_item ? B.htau.data() + B.htau_off[i] : tau;
+            const int rc = large_hahn_item(b->ctx, B.hA[i], B.hB[i], B.hu0[i].data(), B.hy0[i].data(), B.hscale[i],
+                                           nti, ti, b->m, b->klim, b->tol, B.hL.data() + off[i],
+                                           B.hstats.data() + i * kStatsPerItem);
+            if (rc) {
+                int zero = 0;
+                first_rc.compare_exchange_strong(zero, rc);
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& th : pool) th.join();
+    return first_rc.load();
 }
 
 static int bucket_launch(oops_hahn_batch* b, oops_hahn_batch::Bucket& B, const KrylovArgs& a, hipStream_t st,

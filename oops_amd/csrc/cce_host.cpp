@@ -22,6 +22,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <memory>
 #include <set>
@@ -458,6 +459,42 @@ struct oops_clusters {
 };
 
 extern "C" {
+
+int oops_read_xyz(const char* path, int32_t max_n, double* coords, char* isotopes, int32_t* n_out) {
+    if (!path || !n_out || max_n < 0 || (max_n > 0 && !coords)) return OOPS_ERR_ARG;
+    std::ifstream f(path);
+    if (!f) return OOPS_ERR_ARG;
+    long long nb = -1;
+    if (!(f >> nb) || nb < 0) return OOPS_ERR_ARG;
+    *n_out = (int32_t)nb;
+    if (nb > max_n) return OOPS_ERR_CAPACITY;
+    for (long long i = 0; i < nb; ++i) {
+        std::string iso;
+        double x, y, z;
+        if (!(f >> iso >> x >> y >> z)) return OOPS_ERR_ARG;
+        coords[3 * i] = x;
+        coords[3 * i + 1] = y;
+        coords[3 * i + 2] = z;
+        if (isotopes) {
+            char* slot = isotopes + 16 * i;
+            std::memset(slot, 0, 16);
+            std::memcpy(slot, iso.data(), std::min<size_t>(iso.size(), 15));
+        }
+    }
+    return OOPS_OK;
+}
+
+int oops_write_xyz(const char* path, int32_t n, const double* coords, const char* isotopes) {
+    if (!path || n < 0 || (n > 0 && !coords)) return OOPS_ERR_ARG;
+    std::ofstream f(path);
+    if (!f) return OOPS_ERR_ARG;
+    f << n << std::endl;
+    for (int32_t i = 0; i < n; ++i) {
+        f << (isotopes ? std::string(isotopes + 16 * i, strnlen(isotopes + 16 * i, 16)) : std::string("13C")) << "\t";
+        f << coords[3 * i] << "\t" << coords[3 * i + 1] << "\t" << coords[3 * i + 2] << std::endl;
+    }
+    return f ? OOPS_OK : OOPS_ERR_ARG;
+}
 
 int oops_bath_diamond(double a, double radius, double p, uint64_t seed, int32_t shape, double* coords,
                       int32_t max_n, int32_t* n_out) {

@@ -48,6 +48,7 @@ struct LanczosSmem {
     double tdiag[16];
     double toff[16];
     double scal[16];
+    double bes[2 * kWave];             // Bessel values J_k of the Chebyshev series
     int ints[8];
 #ifdef OOPS_PHASE_TIMERS
     unsigned long long tbes;   // cycles of the Chebyshev set-up (interval, Bessel values)
@@ -165,11 +166,8 @@ __device__ int cheb_coeffs(SM& S, int mb, int k1, double tau, double beta_m) {
         // Gershgorin interval of T, then of X = -T (+ 0 for the augmentation)
         double lo = j < mb ? dj - fabs(ejm) - fabs(ej) : 1e300;
         double hi = j < mb ? dj + fabs(ejm) + fabs(ej) : -1e300;
-#pragma unroll
-        for (int mask = 32; mask >= 1; mask >>= 1) {
-            lo = fmin(lo, __shfl_xor(lo, mask, kWave));
-            hi = fmax(hi, __shfl_xor(hi, mask, kWave));
-        }
+        lo = wave_reduce_dpp(lo, [](double a, double b) { return fmin(a, b); });
+        hi = wave_reduce_dpp(hi, [](double a, double b) { return fmax(a, b); });
         double xlo = -hi, xhi = -lo;
         if (k1) { xlo = fmin(xlo, 0.0); xhi = fmax(xhi, 0.0); }
         const double c = 0.5 * (xlo + xhi);
@@ -189,31 +187,47 @@ __device__ int cheb_coeffs(SM& S, int mb, int k1, double tau, double beta_m) {
         const int ns = xfull > 64.0 ? (int)ceil(xfull / 64.0) : 1;
         const double x = xfull / ns;
         // Bessel J_k(x) by backward recurrence (Miller) from N, normalised with
-        // J_0 + 2 sum J_2k = 1; every lane runs the recurrence and keeps its k.
+        // J_0 + 2 sum J_2k = 1.  Every lane runs the recurrence (four steps per
+        // iteration, the parity of k fixed); lane 0 stores J_k in LDS and every
+        // lane then keeps J_j (bl) and J_{64+j} (bh).
         double bl = 0.0, bh = 0.0;
-        int K = 1;
-        if (x == 0.0) {
-            bl = j == 0 ? 1.0 : 0.0;
+        int K = 2;
+        if (x < 1e-12) {   // exp(i x s) = 1 + i x s to double precision
+            bl = j == 0 ? 1.0 : (j == 1 ? 0.5 * x : 0.0);
         } else {
-            int N = (int)ceil(x + 10.0 * ool_cbrt(fmax(x, 1.0)) + 32.0);
-            if (N > 2 * kWave) N = 2 * kWave;
-            double jp = 0.0, jc = 1e-280, norm = 0.0;
+            const float xf = (float)fmax(x, 1.0);
+            int N = x < 1e-3 ? 20 : (int)ceilf((float)x + 10.0f * __expf(__logf(xf) * (1.0f / 3.0f)) + 32.0f);
+            N = min((N + 3) & ~3, 2 * kWave);
+            double jp = 0.0, jc = 1e-280, even = 0.0;
             const double tx = 2.0 / x;
-            for (int k = N; k >= 1; --k) {
-                const double jm = (double)k * tx * jc - jp;   // J_{k-1}
-                jp = jc;
-                jc = jm;
-                if (k - 1 == j) bl = jm;
-                if (k - 1 == j + kWave) bh = jm;
-                if (((k - 1) & 1) == 0) norm += (k - 1 == 0 ? 1.0 : 2.0) * jm;
-                if (fabs(jc) > 1e250) {   // rescale everything kept so far
-                    bl *= 1e-250; bh *= 1e-250;
-                    jp *= 1e-250; jc *= 1e-250; norm *= 1e-250;
+            for (int k = N; k >= 4; k -= 4) {
+                const double j1 = (double)k * tx * jc - jp;          // J_{k-1}
+                const double j2 = (double)(k - 1) * tx * j1 - jc;    // J_{k-2}
+                const double j3 = (double)(k - 2) * tx * j2 - j1;    // J_{k-3}
+                const double j4 = (double)(k - 3) * tx * j3 - j2;    // J_{k-4}
+                if (j == 0) {
+                    S.bes[k - 1] = j1;
+                    S.bes[k - 2] = j2;
+                    S.bes[k - 3] = j3;
+                    S.bes[k - 4] = j4;
+                }
+                even += j2 + j4;
+                jp = j3;
+                jc = j4;
+                if (fabs(jc) > 1e150) {   // rescale everything so far (four steps grow < 1e130)
+                    jp *= 1e-150;
+                    jc *= 1e-150;
+                    even *= 1e-150;
+                    if (j == 0)
+                        for (int q = k - 4; q < N; ++q) S.bes[q] *= 1e-150;
                 }
             }
-            const double inorm = 1.0 / norm;
-            bl *= inorm;
-            bh *= inorm;
+            const double inorm = 1.0 / (2.0 * even - jc);   // jc = J_0
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            bl = j < N ? S.bes[j] * inorm : 0.0;
+            bh = j + kWave < N ? S.bes[j + kWave] * inorm : 0.0;
             const unsigned long long sl = __ballot((double)j > x && fabs(bl) < 1e-18);
             const unsigned long long sh = __ballot((double)(j + kWave) > x && fabs(bh) < 1e-18);
             K = sl ? __ffsll((long long)sl) - 1 : (sh ? kWave + __ffsll((long long)sh) - 1 : N);
@@ -560,6 +574,7 @@ struct LzSlotSmem {
     double tdiag[16];
     double toff[16];
     double scal[16];
+    double bes[2 * kWave];             // Bessel values J_k of the Chebyshev series
     int ints[8];
 #ifdef OOPS_PHASE_TIMERS
     unsigned long long tbes;   // cycles of the Chebyshev set-up (interval, Bessel values)

@@ -247,6 +247,28 @@ __device__ __forceinline__ double swap_add_f64(double v) {
     return e + o;
 }
 
+// Generic wave reduction with the same DPP/permlane pattern (op must be
+// commutative and associative on the values: min, max).
+template <typename Op>
+__device__ __forceinline__ double wave_reduce_dpp(double v, Op op) {
+    v = op(v, dpp_f64<0x140>(v));
+    v = op(v, dpp_f64<0x141>(v));
+    v = op(v, dpp_f64<0x4E>(v));
+    v = op(v, dpp_f64<0xB1>(v));
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const long long b = __double_as_longlong(v);
+        const int lo = (int)b, hi = (int)(b >> 32);
+        const auto l = h ? __builtin_amdgcn_permlane32_swap(lo, lo, false, false)
+                         : __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+        const auto u = h ? __builtin_amdgcn_permlane32_swap(hi, hi, false, false)
+                         : __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+        v = op(__longlong_as_double(((long long)u[0] << 32) | (unsigned)l[0]),
+               __longlong_as_double(((long long)u[1] << 32) | (unsigned)l[1]));
+    }
+    return v;
+}
+
 __device__ __forceinline__ double wave_sum_dpp(double v) {
     v += dpp_f64<0x140>(v);   // row_mirror: lane i + lane 15-i
     v += dpp_f64<0x141>(v);   // row_half_mirror: lane i + lane 7-i (within 8)

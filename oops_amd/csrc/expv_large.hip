@@ -84,15 +84,19 @@ __global__ __launch_bounds__(kLT) void large_matvec(LargeOpArgs o, const double2
 // lowest spin's digits, which every pass keeps in G), the pass's local
 // operators are applied from LDS, and y is written (pass 0, which also
 // stores the scaled basis vector) or accumulated.
+constexpr int kTT = 1024;   // threads of the tiled product (four elements each)
+
 struct LargeTileLds {
     cd xs[1 << kLargeTileBits];
+    cd vals[kLargeTileMaxEnt];
+    int32_t offs[kLargeTileMaxEnt];
     DevOpHeader hdr;
     DevOpLocal locs[kLargeTileMaxLocals];
     int32_t map[kLargeTileMapStride];
 };
 
 template <typename VT>
-__global__ __launch_bounds__(kLT) void large_tile_matvec(LargeTilePass tp, const double2* __restrict__ x,
+__global__ __launch_bounds__(kTT) void large_tile_matvec(LargeTilePass tp, const double2* __restrict__ x,
                                                          const double* __restrict__ xscale_p,
                                                          double2* __restrict__ y, VT* __restrict__ xstore,
                                                          int alpha_mode, int accumulate) {
@@ -101,7 +105,13 @@ __global__ __launch_bounds__(kLT) void large_tile_matvec(LargeTilePass tp, const
     if (tid == 0) S.hdr = *tp.op.hdr;
     if (tid < kLargeTileMapStride) S.map[tid] = tp.map[tid];
     const int nloc = tp.op.hdr->nloc;
-    for (int k = tid; k < nloc; k += kLT) S.locs[k] = tp.op.loc[k];
+    const int nent = tp.op.hdr->nent;
+    for (int k = tid; k < nloc; k += kTT) S.locs[k] = tp.op.loc[k];
+    for (int k = tid; k < nent; k += kTT) {
+        const double2 v = tp.op.val[k];
+        S.vals[k] = mkcd(v.x, v.y);
+        S.offs[k] = tp.op.off[k];
+    }
     __syncthreads();
     const int ld = tp.ld, ng = tp.ng, nb = tp.nb;
     const int64_t lmask = (1 << ld) - 1;
@@ -123,16 +133,37 @@ __global__ __launch_bounds__(kLT) void large_tile_matvec(LargeTilePass tp, const
         return g;
     };
     const double xsc = xscale_p ? *xscale_p : 1.0;
-    for (int e = tid; e < tsize; e += kLT) {
+    for (int e = tid; e < tsize; e += kTT) {
         const int64_t g = gidx(e);
         const double2 xv = x[g];
         S.xs[e] = mkcd(xv.x, xv.y);
         if (xstore) vstore(xstore + g, xv.x * xsc, xv.y * xsc);
     }
     __syncthreads();
-    const cd* vals = reinterpret_cast<const cd*>(tp.op.val);
-    for (int e = tid; e < tsize; e += kLT) {
-        cd acc = op_apply_elem(e, S.xs, S.hdr, S.locs, vals, tp.op.off);
+    const int top = ng - 1;
+    const int lm = (1 << ld) - 1;
+    for (int e = tid; e < tsize; e += kTT) {
+        // the pass's local operators, tables entry-major (entry e of row r at
+        // ent + e * rows + r)
+        cd acc = cmul(mkcd(S.hdr.c0re, S.hdr.c0im), S.xs[e]);
+        for (int L = 0; L < S.hdr.nloc; ++L) {
+            const DevOpLocal lo = S.locs[L];
+            const int shs = ld * (top - lo.s);
+            const int ls = (e >> shs) & lm;
+            int base = e - (ls << shs);
+            int row = ls;
+            if (lo.nb == 2) {
+                const int sht = ld * (top - lo.t);
+                const int lt = (e >> sht) & lm;
+                base -= lt << sht;
+                row = (ls << ld) | lt;
+            }
+            const int k0 = lo.ent + row;
+            for (int q = 0; q < lo.R; ++q) {
+                const int k = k0 + q * lo.rows;
+                acc = cfma(S.vals[k], S.xs[base + S.offs[k]], acc);
+            }
+        }
         acc = cscale(acc, xsc);
         if (alpha_mode == 1) acc = cmulmi(acc);
         const int64_t g = gidx(e);
@@ -214,16 +245,27 @@ __global__ __launch_bounds__(kLT) void large_dots(const double2* __restrict__ V,
 //           (0 after a breakdown), scal[1] = h
 //   mode 3: norm (row 0 of part) into scal[2] (the extra product's norm)
 // One block.
-__global__ __launch_bounds__(kLT) void large_finish(const double* __restrict__ part, int nblk, int nrows, int jcol,
-                                                    int mode, double2* __restrict__ hess, int ldh,
-                                                    double2* __restrict__ coef, double* __restrict__ scal,
-                                                    int* __restrict__ flags, double break_tol) {
-    for (int l = threadIdx.x; l < nrows; l += kLT) {
+constexpr int kFinishThreads = 1024;
+
+// One wave per row: the lanes sum strided partials (independent loads), then
+// a wave reduction; 16 rows at a time.
+__global__ __launch_bounds__(kFinishThreads) void large_finish(const double* __restrict__ part, int nblk, int nrows,
+                                                               int jcol, int mode, double2* __restrict__ hess, int ldh,
+                                                               double2* __restrict__ coef, double* __restrict__ scal,
+                                                               int* __restrict__ flags, double break_tol) {
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    for (int l = wave; l < nrows; l += kFinishThreads / kWave) {
         double a = 0.0, b = 0.0;
-        for (int k = 0; k < nblk; ++k) {
+        for (int k = lane; k < nblk; k += kWave) {
             a += part[((int64_t)l * nblk + k) * 2];
             b += part[((int64_t)l * nblk + k) * 2 + 1];
         }
+#pragma unroll
+        for (int mask = 32; mask >= 1; mask >>= 1) {
+            a += __shfl_xor(a, mask, kWave);
+            b += __shfl_xor(b, mask, kWave);
+        }
+        if (lane != 0) continue;
         if (mode == 2) {
             const double h = sqrt(a);
             const bool broke = flags[0] != 0 || h <= break_tol;
@@ -440,10 +482,10 @@ void launch_large_matvec(const LargeOpArgs& o, const double2* x, const double* x
 void launch_large_tile_matvec(const LargeTilePass& tp, const double2* x, const double* xscale, double2* y,
                               void* xstore, int alpha_mode, int accumulate, hipStream_t s, bool fp32) {
     if (fp32)
-        hipLaunchKernelGGL(large_tile_matvec<float2>, dim3(tp.ntiles), dim3(kLT), 0, s, tp, x, xscale, y,
+        hipLaunchKernelGGL(large_tile_matvec<float2>, dim3(tp.ntiles), dim3(kTT), 0, s, tp, x, xscale, y,
                            (float2*)xstore, alpha_mode, accumulate);
     else
-        hipLaunchKernelGGL(large_tile_matvec<double2>, dim3(tp.ntiles), dim3(kLT), 0, s, tp, x, xscale, y,
+        hipLaunchKernelGGL(large_tile_matvec<double2>, dim3(tp.ntiles), dim3(kTT), 0, s, tp, x, xscale, y,
                            (double2*)xstore, alpha_mode, accumulate);
 }
 
@@ -455,8 +497,8 @@ void launch_large_dots(const double2* V, int64_t ldv, const double2* p, int64_t 
 
 void launch_large_finish(const double* part, int nblk, int nrows, int jcol, int mode, double2* hess, int ldh,
                          double2* coef, double* scal, int* flags, double break_tol, hipStream_t s) {
-    hipLaunchKernelGGL(large_finish, dim3(1), dim3(kLT), 0, s, part, nblk, nrows, jcol, mode, hess, ldh, coef, scal,
-                       flags, break_tol);
+    hipLaunchKernelGGL(large_finish, dim3(1), dim3(kFinishThreads), 0, s, part, nblk, nrows, jcol, mode, hess, ldh,
+                       coef, scal, flags, break_tol);
 }
 
 void launch_large_gs_dots(const void* V, int64_t ldv, const double2* p, int64_t n, int J, double* part, int nblk,

@@ -124,19 +124,22 @@ static bool plan_tiles(const CompiledOp& op, std::vector<TilePlan>& plan) {
                 t.s = (int16_t)tq[lo.s];
                 t.t = (int16_t)(lo.nb == 2 ? tq[lo.t] : tq[lo.s]);
                 t.ent = (int32_t)T.val.size();
-                const int64_t nent = (int64_t)lo.rows * lo.R;
-                for (int64_t e = 0; e < nent; ++e) {
-                    const int64_t off = op.off[lo.ent + e];
-                    int64_t o = ((off >> gpos(lo.s)) & lmask) << tpos(t.s);
-                    if (lo.nb == 2) o |= ((off >> gpos(lo.t)) & lmask) << tpos(t.t);
-                    T.val.push_back(op.val[lo.ent + e]);
-                    T.off.push_back((int32_t)o);
-                }
+                // entry-major (entry e of every row, then e + 1): lanes whose
+                // rows differ read consecutive table slots (no LDS bank conflicts)
+                for (int64_t e = 0; e < lo.R; ++e)
+                    for (int64_t row = 0; row < lo.rows; ++row) {
+                        const int64_t k = lo.ent + row * lo.R + e;
+                        const int64_t off = op.off[k];
+                        int64_t o = ((off >> gpos(lo.s)) & lmask) << tpos(t.s);
+                        if (lo.nb == 2) o |= ((off >> gpos(lo.t)) & lmask) << tpos(t.t);
+                        T.val.push_back(op.val[k]);
+                        T.off.push_back((int32_t)o);
+                    }
                 T.locs.push_back(t);
                 T.max_row = std::max(T.max_row, (int)lo.R);
             }
         }
-        if ((int)T.locs.size() > kLargeTileMaxLocals) return false;
+        if ((int)T.locs.size() > kLargeTileMaxLocals || (int)T.val.size() > kLargeTileMaxEnt) return false;
     }
     for (int d : done)
         if (!d) return false;
@@ -641,7 +644,8 @@ int large_hahn_item(oops_ctx* ctx, const CompiledOp& A, const CompiledOp& B, con
 // is taken through `gidx` (identity for the flat product, the tile scatter for
 // a pass).
 template <typename Index>
-static std::complex<double> host_elem(const CompiledOp& op, int64_t i, const std::complex<double>* x, Index gidx) {
+static std::complex<double> host_elem(const CompiledOp& op, int64_t i, const std::complex<double>* x, Index gidx,
+                                      bool entry_major = false) {
     const int ld = op.ld_bits;
     const int64_t lmask = (1LL << ld) - 1;
     const int top = op.nspin - 1;
@@ -657,7 +661,7 @@ static std::complex<double> host_elem(const CompiledOp& op, int64_t i, const std
             row = (row << ld) | lt;
         }
         for (int e = 0; e < lo.R; ++e) {
-            const int64_t k = lo.ent + row * lo.R + e;
+            const int64_t k = entry_major ? lo.ent + (int64_t)e * lo.rows + row : lo.ent + row * lo.R + e;
             acc += op.val[k] * x[gidx(base + op.off[k])];
         }
     }
@@ -709,7 +713,7 @@ int oops_tile_plan(const oops_term_table* T, int32_t* npasses, int32_t mode, con
                 const int64_t gi = scatter(e);
                 if (gi < 0 || gi >= n || seen[gi]) return OOPS_ERR_NUMERIC;   // not a bijection
                 seen[gi] = 1;
-                const cplx a = host_elem(plan[p].op, e, x, scatter);
+                const cplx a = host_elem(plan[p].op, e, x, scatter, true);
                 y[gi] = p == 0 ? a : y[gi] + a;
             }
         }

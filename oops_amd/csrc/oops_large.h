@@ -26,6 +26,7 @@ struct LargeOpArgs {
 constexpr int kLargeTileBits = 12;
 constexpr int kLargeTileMapStride = 48;  // int32 per pass: [0,16) tile spins' bit positions, [16,48) block spins'
 constexpr int kLargeTileMaxLocals = 256; // local operators of one pass (LDS)
+constexpr int kLargeTileMaxEnt = 2560;   // ELL entries of one pass (LDS)
 
 struct LargeTilePass {
     LargeOpArgs op;       // tile operator (local operators of this pass, offsets in tile digits)

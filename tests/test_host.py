@@ -164,3 +164,23 @@ def test_tile_plan_product_equals_flat_product():
     assert np.abs(wt - wf).max() <= 1e-13 * np.abs(wf).max()
     T9, _, _ = O.cluster_terms(X[:9], par, 0)
     assert O.tile_plan(T9) == 0
+
+
+def test_coordinate_files_round_trip(tmp_path):
+    """Coordinate files in the reference's layout (SpinSource.cpp:20-79):
+    RoyCoord.xyz reads back 548 13C spins with the coordinates the survey's
+    counts are computed from, and writing the reference's exported file
+    (dat_example/output/c.xyz, 100 spins) reproduces it byte for byte."""
+    X, iso = O.read_xyz(os.path.join(ROOT, "tests", "golden", "RoyCoord.xyz"))
+    assert X.shape == (548, 3) and set(iso) == {"13C"}
+    rows = [ln.split() for ln in open(os.path.join(ROOT, "tests", "golden", "RoyCoord.xyz")).read().split("\n")[1:]
+            if ln.strip()]
+    assert np.array_equal(X, np.array([[float(v) for v in r[1:]] for r in rows]))
+    src = os.path.join(ROOT, "tests", "golden", "c_export.xyz")
+    Y, iso2 = O.read_xyz(src)
+    assert Y.shape == (100, 3) and set(iso2) == {"E"}
+    out = tmp_path / "c.xyz"
+    O.write_xyz(str(out), Y, iso2)
+    assert open(out, "rb").read() == open(src, "rb").read()
+    with pytest.raises(O.OopsError):
+        O.read_xyz(src, max_n=10)
