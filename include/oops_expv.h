@@ -242,7 +242,8 @@ int oops_hahn_batch_timing(const oops_hahn_batch* b, int32_t max_buckets, int32_
 /* Kernel each bucket ran with in the last run: 0 dense VALU Pade (n <= m),
  * 1 Arnoldi FP64, 2 Arnoldi mixed, 3 Lanczos mixed, 4 dense FP64-MFMA
  * propagator (n = 64), 5 single-state path (n > 256), 6 Lanczos mixed in the
- * flip-mask (slot) form; -1 not run. */
+ * flip-mask (slot) form, 7 Lanczos mixed for n = 1024 / 4096 (one 1024-thread
+ * workgroup per cluster); -1 not run. */
 int oops_hahn_batch_kernels(const oops_hahn_batch* b, int32_t max_buckets, int32_t* kind);
 int oops_hahn_batch_destroy(oops_hahn_batch* b);
 

@@ -110,7 +110,7 @@ def lib():
 
 
 KERNEL_NAMES = {0: "dense_pade", 1: "arnoldi_fp64", 2: "arnoldi_mixed", 3: "lanczos_mixed", 4: "dense64_mfma",
-                5: "single_state", 6: "lanczos_slot", -1: None}
+                5: "single_state", 6: "lanczos_slot", 7: "lanczos_medium", -1: None}
 
 
 def _ptr(a):
@@ -623,6 +623,28 @@ class Clusters:
         lst = np.zeros(max(int(off[-1]), 1), np.int32)
         lib().oops_clusters_parents(self.h, ctypes.c_int32(order), _ptr(off), _ptr(lst))
         return [lst[off[i]:off[i + 1]].tolist() for i in range(c)]
+
+    def export_npz(self, path, L, time_list):
+        """Write a CCE result in the variables of the reference's result file
+        (CCE::export_mat_file, cce.cpp:190-230) as an .npz instead of a MATLAB
+        file: CCE{i} and CCE{i}_tilder (nt x clusters of order i+1: L_C(t) and
+        the reduced L~_C(t)), final_result_each_order, final_result (nt x
+        max order) and time_list.  L: every cluster's L_C(t), cluster-major."""
+        t = np.ascontiguousarray(time_list, np.float64)
+        nt = len(t)
+        L = np.ascontiguousarray(L, np.float64).reshape(-1, nt)
+        r = self.reduce(L.reshape(-1), nt)
+        til = r["tilde"].reshape(-1, nt)
+        out, base = {}, 0
+        for i, c in enumerate(self.counts()):
+            out[f"CCE{i}"] = L[base:base + c].T.copy()
+            out[f"CCE{i}_tilder"] = til[base:base + c].T.copy()
+            base += c
+        out["final_result_each_order"] = r["each"]
+        out["final_result"] = r["final"]
+        out["time_list"] = t.reshape(nt, 1)
+        np.savez(path, **out)
+        return out
 
     def reduce(self, L, nt):
         """CCE reduction: returns dict(tilde, final, each, log_abs, sign)."""

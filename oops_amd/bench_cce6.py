@@ -11,6 +11,7 @@ device time, the CCE-6 coherence and, on the n <= 256 clusters, the largest
 """
 import json
 import sys
+import threading
 import time
 
 import numpy as np
@@ -28,8 +29,18 @@ def main():
     b = C.hahn_batch(ctx, par, m=30, tol=1e-3, precision="mixed")
     setup = time.perf_counter() - t0
     t1 = time.perf_counter()
+    done = threading.Event()
+
+    def progress():   # the batch run is one blocking call; report that it is alive
+        while not done.wait(30.0):
+            print(f"# running {time.perf_counter() - t1:.0f}s", file=sys.stderr, flush=True)
+
+    th = threading.Thread(target=progress, daemon=True)
+    th.start()
     b.run(tau)
+    done.set()
     run_s = time.perf_counter() - t1
+    print(f"# batch of {int(sum(counts))} clusters in {run_s:.1f}s", file=sys.stderr, flush=True)
     L, st = b.results()
     timing = b.timing()
     small = int(sum(counts[:4]))

@@ -235,6 +235,9 @@ struct oops_hahn_batch {
         DevBuf work;            // dense64 workspace
         int dense_grid = 0;
         DevBuf eslot, smask, slotws;  // slot form of the mixed Lanczos kernel
+        bool medium = false;          // n = 1024 / 4096 on the batched medium kernel
+        DevBuf medws;                 // its FP32 Lanczos bases (one per workgroup)
+        int medium_grid = 0;
         bool slots = true;            // every operator fits kSlot64 / kSlot32
         int slot_grid = 0;
         // states above 256 elements: every item on the single-state path, in turn
@@ -393,22 +396,32 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
         const bool dense = n <= m;
         if (dense && n > kDenseNMAX)
             return set_err(ctx, OOPS_ERR_UNSUPPORTED, "item " + std::to_string(it) + ": n <= m needs n <= 16");
-        const bool large = !dense && n > 256;
+        // n = 1024 / 4096 with mixed precision and Hermitian generators: the
+        // batched medium kernel; other states above 256 elements: the
+        // single-state path, item by item
+        auto fits_medium = [&](const CompiledOp& o) {
+            return o.hermitian && (int)o.val.size() <= kMediumMaxEnt && (int)o.locs.size() <= kMaxLocals;
+        };
+        const bool medium = !dense && (n == 1024 || n == 4096) && precision == 1 && fits_medium(A) &&
+                            fits_medium(B) && (!cpmg || (fits_medium(Cc) && fits_medium(Dd)));
+        const bool large = !dense && n > 256 && !medium;
         if (large && cpmg)
             return set_err(ctx, OOPS_ERR_UNSUPPORTED, "item " + std::to_string(it) + ": CPMG-N needs n <= 256");
         if (large && (A.locs.size() > 1024 || B.locs.size() > 1024))
             return set_err(ctx, OOPS_ERR_CAPACITY, "item " + std::to_string(it) + ": too many local operators");
-        const int cap = dense ? kDenseMaxEnt : emax;
+        const int cap = dense ? kDenseMaxEnt : (medium ? kMediumMaxEnt : emax);
         if (!large && ((int)A.val.size() > cap || (int)B.val.size() > cap ||
                        (cpmg && ((int)Cc.val.size() > cap || (int)Dd.val.size() > cap))))
             return set_err(ctx, OOPS_ERR_CAPACITY, "item " + std::to_string(it) + ": operator table exceeds LDS capacity");
-        auto f = bucket_of.find(n);
+        const int64_t key = 2 * n + (medium ? 1 : 0);
+        auto f = bucket_of.find(key);
         int bk;
         if (f == bucket_of.end()) {
             bk = (int)b->buckets.size();
-            bucket_of[n] = bk;
+            bucket_of[key] = bk;
             b->buckets.emplace_back();
             b->buckets.back().n = (int)n;
+            b->buckets.back().medium = medium;
             hosts.emplace_back();
         } else {
             bk = f->second;
@@ -533,7 +546,14 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
         const bool dense = B.n <= m;
         const int per_cu = dense ? 4 : 2;
         B.grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu * per_cu));
-        B.slots = B.slots && !ell_only && precision == 1 && B.hermitian && !dense && B.n <= 256 && (m == 20 || m == 30);
+        if (B.medium) {
+            B.medium_grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu));
+            const size_t bytes = sizeof(float) * 2 * (size_t)medium_workspace_float2(B.medium_grid, m, B.n);
+            rc = hip_check(ctx, hipMalloc(&B.medws.p, bytes), "hipMalloc medium workspace");
+            if (rc) return rc;
+            B.medws.bytes = bytes;
+        }
+        B.slots = B.slots && !B.medium && !ell_only && precision == 1 && B.hermitian && !dense && B.n <= 256 && (m == 20 || m == 30);
         if (B.slots) {
             // two workgroups per CU; 2 (Hahn) or 4 (CPMG-N) coefficient tables each
             B.slot_grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu * 2));
@@ -556,7 +576,7 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
     for (auto& B : b->buckets)
         for (DevBuf* d : {&B.hdr, &B.loc, &B.val, &B.off, &B.opA, &B.opB, &B.opC, &B.opD, &B.voff, &B.u0, &B.y0,
                           &B.scale, &B.rnd, &B.L, &B.stats, &B.queue, &B.npulse, &B.nt, &B.tau_off, &B.L_off, &B.tau,
-                          &B.work, &B.eslot, &B.smask, &B.slotws})
+                          &B.work, &B.eslot, &B.smask, &B.slotws, &B.medws})
             b->footprint += (int64_t)d->bytes;
     if (per_item) {
         if (hipStreamCreate(&b->stream) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipStreamCreate");
@@ -658,6 +678,10 @@ static int bucket_launch(oops_hahn_batch* b, oops_hahn_batch::Bucket& B, const K
         B.kernel = 4;
         launch_zero_counter(a.queue, st);
         lrc = launch_dense64(a, (double*)B.work.p, B.dense_grid, st);
+    } else if (B.medium && B.medws.p) {
+        B.kernel = 7;
+        launch_zero_counter(a.queue, st);
+        lrc = launch_hahn_medium(a, (float2*)B.medws.p, b->m, B.medium_grid, st);
     } else if (lanczos && B.slots && B.slotws.p) {
         B.kernel = 6;
         launch_zero_counter(a.queue, st);

@@ -1,0 +1,452 @@
+// Mixed-precision batched echo kernel for states of 1024 and 4096 elements
+// (5 and 6 spin-1/2 Liouville spins): one 1024-thread workgroup per cluster,
+// thread t owning elements t + 1024 q (q < E = n / 1024).  The product input
+// (FP64 and FP32 copies) and the ELL tables of the active generator are held
+// whole in LDS (148 KB at n = 4096); the FP32 Lanczos basis v_1..v_M lives in a
+// per-workgroup device buffer (each thread reads and writes only its own
+// elements).  The Krylov scheme is the one of the n <= 256 Lanczos kernels
+// (cce_lanczos.hip): Expokit's step control (zgexpv_cache.f:418-661) with
+// Lanczos for Hermitian generators, one fused block reduction per step, the
+// Chebyshev exponential of the augmented tridiagonal matrix, and the
+// reference's norm estimate (k Lanczos steps from the DLARAN vector, power
+// iteration on the tridiagonal, hamvec_zgexpv_w_mkl_profile.f:210-265).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cheb.h"
+#include "krylov_common.h"
+
+namespace oops {
+
+namespace {
+
+constexpr int kMT = 1024;
+constexpr int kMW = kMT / kWave;
+
+struct MedOpLds {
+    DevOpHeader hdr;
+    DevOpLocal locs[kMaxLocals];
+    alignas(16) cd vals[kMediumMaxEnt];
+    alignas(16) int32_t offs[kMediumMaxEnt];
+};
+
+template <int M, int E>
+struct MedSmem {
+    cd xs[kMT * E];
+    float2 xs32[kMT * E];
+    MedOpLds op;
+    double red[2 * kMW * 4];
+    double d[M + 2];
+    double e[M + 2];
+    cd F[M + 2];
+    double scal[16];
+    double bes[2 * kWave];
+    int ints[8];
+#ifdef OOPS_PHASE_TIMERS
+    unsigned long long tbes;
+#endif
+};
+
+struct MedState {
+    double beta, t_now, t_new, anorm, xm;
+    int nstep, nmult, nrej, nexph, nscale, brk, fail;
+};
+
+// Generator `op` of the bucket into LDS (2-body values as complex FP32).
+__device__ void med_load(const KrylovArgs& a, int op, MedOpLds& o) {
+    const DevOpHeader h = a.hdr[op];
+    if (threadIdx.x == 0) o.hdr = h;
+    for (int k = threadIdx.x; k < h.nloc; k += kMT) o.locs[k] = a.loc[h.loc_off + k];
+    float2* v32 = reinterpret_cast<float2*>(o.vals + h.ent2_first);
+    for (int k = threadIdx.x; k < h.nent; k += kMT) {
+        const double2 v = a.val[h.ent_off + k];
+        if (k >= h.ent2_first) v32[k - h.ent2_first] = make_float2((float)v.x, (float)v.y);
+        else o.vals[k] = mkcd(v.x, v.y);
+        o.offs[k] = a.off[h.ent_off + k];
+    }
+    __syncthreads();
+}
+
+template <int M, int E>
+__device__ __forceinline__ void med_stage(MedSmem<M, E>& S, const cd (&x)[E]) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const int i = threadIdx.x + kMT * q;
+        S.xs[i] = x[q];
+        S.xs32[i] = make_float2((float)x[q].re, (float)x[q].im);
+    }
+}
+
+template <int M, int E>
+__device__ __forceinline__ cd med_product(const MedSmem<M, E>& S, int q) {
+    const int i = threadIdx.x + kMT * q;
+    return op_apply_elem_mixed(i, S.xs, S.xs32, S.op.hdr, S.op.locs, S.op.vals, S.op.offs);
+}
+
+// Norm estimate of the generator in LDS: k Lanczos steps on +H from the
+// DLARAN start vector, then power iteration on the k x k tridiagonal.
+template <int M, int E>
+__device__ double med_anorm(MedSmem<M, E>& S, BlockRed<kMT>& br, const double* rnd, int n, int klim, double tol) {
+    const int tid = threadIdx.x;
+    const int k = klim < n ? klim : n;
+    if (tid < 16) { S.scal[tid] = 0.0; }
+    cd q[E], qm[E], w[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const int i = tid + kMT * r;
+        q[r] = mkcd(rnd[2 * i], rnd[2 * i + 1]);
+        qm[r] = mkcd(0.0, 0.0);
+    }
+    {
+        const auto t0 = block_sum_lanes<1, kMT, 4>(br, [&](int) {
+            double s = 0.0;
+#pragma unroll
+            for (int r = 0; r < E; ++r) s += cabs2(q[r]);
+            return s;
+        });
+        const double inv = 1.0 / sqrt(t0.get(0));
+#pragma unroll
+        for (int r = 0; r < E; ++r) q[r] = cscale(q[r], inv);
+    }
+    double* tdiag = S.d;   // the tridiagonal of the estimate (before any Krylov step uses d, e)
+    double* toff = S.e;
+    if (tid < M + 2) { tdiag[tid] = 0.0; toff[tid] = 0.0; }
+    double bjm = 0.0, bj = 0.0;
+#pragma unroll 1
+    for (int j = 1; j <= k; ++j) {
+        med_stage<M, E>(S, q);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < E; ++r) w[r] = med_product<M, E>(S, r);
+        const auto ta = block_sum_lanes<2, kMT, 4>(br, [&](int c) {
+            double s = 0.0;
+#pragma unroll
+            for (int r = 0; r < E; ++r)
+                s += c == 0 ? fma(q[r].re, w[r].re, q[r].im * w[r].im) : fma(q[r].re, w[r].im, -q[r].im * w[r].re);
+            return s;
+        });
+        const cd aj = mkcd(ta.get(0), ta.get(1));
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            w[r] = cfms(aj, q[r], w[r]);
+            w[r] = mkcd(w[r].re - bjm * qm[r].re, w[r].im - bjm * qm[r].im);
+        }
+        const auto tb = block_sum_lanes<1, kMT, 4>(br, [&](int) {
+            double s = 0.0;
+#pragma unroll
+            for (int r = 0; r < E; ++r) s += cabs2(w[r]);
+            return s;
+        });
+        bj = sqrt(tb.get(0));
+        if (tid == 0) {
+            tdiag[j - 1] = aj.re;
+            if (j < k) toff[j - 1] = bj;
+        }
+        if (j == k) break;
+        if (bj < tol) break;
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            qm[r] = q[r];
+            q[r] = cscale(w[r], 1.0 / bj);
+        }
+        bjm = bj;
+        __syncthreads();   // the product of the next step overwrites xs
+    }
+    __syncthreads();
+    if (tid < kWave) {
+        // power iteration on T (k x k symmetric tridiagonal), one row per lane
+        const int j = tid;
+        const double dj = j < k ? tdiag[j] : 0.0;
+        const double lo = (j > 0 && j < k) ? toff[j - 1] : 0.0;
+        const double up = (j + 1 < k) ? toff[j] : 0.0;
+        double qj = j < k ? rnd[2 * n + j] : 0.0;
+        double cjm = 0.0, cj = 0.0;
+        for (int it = 0; it < 10000; ++it) {
+            const double qm1 = __shfl_up(qj, 1, kWave);
+            const double qp1 = __shfl_down(qj, 1, kWave);
+            double wj = dj * qj;
+            wj += (j > 0 ? lo * qm1 : 0.0);
+            wj += up * qp1;
+            cj = wave_sum(qj * wj);
+            if (fabs(1.0 - cjm / cj) < tol) break;
+            cjm = cj;
+            cj = sqrt(wave_sum(wj * wj));
+            qj = wj / cj;
+        }
+        const double qlast = __shfl(qj, k - 1, kWave);
+        if (j == 0) S.scal[0] = fabs(cj) + fabs(bj * qlast);
+    }
+    __syncthreads();
+    const double an = S.scal[0];
+    __syncthreads();
+    return an;
+}
+
+template <int M, int E>
+__device__ void med_init(MedSmem<M, E>& S, BlockRed<kMT>& br, MedState& g, cd (&w)[E], const double* rnd, int n,
+                         int klim, double tol) {
+    g.anorm = med_anorm<M, E>(S, br, rnd, n, klim, tol);
+    const auto tb = block_sum_lanes<1, kMT, 4>(br, [&](int) {
+        double s = 0.0;
+#pragma unroll
+        for (int r = 0; r < E; ++r) s += cabs2(w[r]);
+        return s;
+    });
+    g.beta = sqrt(tb.get(0));
+    g.t_now = 0.0;
+    g.xm = 1.0 / (double)M;
+    const double p2 = tol * ool_pow((M + 1) / 2.72, (double)(M + 1)) * sqrt(2.0 * 3.14 * (M + 1));
+    g.t_new = (1.0 / g.anorm) * ool_pow(p2 / (4.0 * g.beta * g.anorm), g.xm);
+    g.t_new = round_step2(g.t_new);
+    g.nstep = g.nmult = g.nrej = g.nexph = g.nscale = g.brk = g.fail = 0;
+}
+
+template <int M, int E>
+__device__ void med_advance(MedSmem<M, E>& S, BlockRed<kMT>& br, MedState& g, cd (&w)[E], float2* __restrict__ Vg,
+                            int n, double t_out, int max_steps, double tol) {
+    const int tid = threadIdx.x;
+    const double delta = 1.2, gamma = 0.9;
+    while (g.t_now < t_out) {
+        if (g.nstep >= max_steps) { g.fail |= 1; break; }
+        ++g.nstep;
+        cd v0[E], vcur[E], vprev[E], p[E];
+        const double ib = 1.0 / g.beta;
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            v0[r] = cscale(w[r], ib);
+            vcur[r] = v0[r];
+            vprev[r] = mkcd(0.0, 0.0);
+        }
+        double bprev = 0.0;
+        int k1 = 2, mb = M;
+        bool brk = false;
+        double t_step = fmin(t_out - g.t_now, g.t_new);
+#pragma unroll 1
+        for (int j = 0; j < M; ++j) {
+            med_stage<M, E>(S, vcur);
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < E; ++r) {
+                p[r] = med_product<M, E>(S, r);
+                p[r] = mkcd(p[r].re - bprev * vprev[r].re, p[r].im - bprev * vprev[r].im);
+            }
+            ++g.nmult;
+            const auto r1 = block_sum_lanes<3, kMT, 4>(br, [&](int c) -> double {
+                double s = 0.0;
+#pragma unroll
+                for (int r = 0; r < E; ++r) {
+                    if (c == 0) s += fma(vcur[r].re, p[r].re, vcur[r].im * p[r].im);
+                    else if (c == 1) s += fma(vcur[r].re, p[r].im, -vcur[r].im * p[r].re);
+                    else s += cabs2(p[r]);
+                }
+                return s;
+            });
+            const cd h = mkcd(r1.get(0), r1.get(1));
+            const double alpha = h.re;
+            const double beta = sqrt(fmax(r1.get(2) - cabs2(h), 0.0));
+            if (tid == 0) {
+                S.d[j] = alpha;
+                S.e[j] = beta;
+            }
+            if (beta <= 1.0e-7) {
+                k1 = 0;
+                g.brk = 1;
+                mb = j + 1;
+                t_step = t_out - g.t_now;
+                brk = true;
+                break;
+            }
+            const double inv = 1.0 / beta;
+#pragma unroll
+            for (int r = 0; r < E; ++r) {
+                const cd pr = cfms(h, vcur[r], p[r]);
+                vprev[r] = vcur[r];
+                vcur[r] = cscale(pr, inv);
+                Vg[(int64_t)j * n + tid + kMT * r] = make_float2((float)vcur[r].re, (float)vcur[r].im);
+            }
+            bprev = beta;
+        }
+        double avnorm = 0.0;
+        if (!brk) {
+            med_stage<M, E>(S, vcur);
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < E; ++r) p[r] = med_product<M, E>(S, r);
+            ++g.nmult;
+            const auto tn = block_sum_lanes<1, kMT, 4>(br, [&](int) {
+                double s = 0.0;
+#pragma unroll
+                for (int r = 0; r < E; ++r) s += cabs2(p[r]);
+                return s;
+            });
+            avnorm = sqrt(tn.get(0));
+        }
+        __syncthreads();
+        double err_loc = 0.0;
+        for (;;) {
+            ++g.nexph;
+            g.nscale += cheb_coeffs<M>(S, mb, k1, t_step, bprev);
+            if (k1 == 0) {
+                err_loc = tol;
+            } else {
+                const double p1 = cabs(S.F[M]) * g.beta;
+                const double p2 = cabs(S.F[M + 1]) * g.beta * avnorm;
+                if (p1 > 10.0 * p2) { err_loc = p2; g.xm = 1.0 / (double)M; }
+                else if (p1 > p2) { err_loc = (p1 * p2) / (p1 - p2); g.xm = 1.0 / (double)M; }
+                else { err_loc = p1; g.xm = 1.0 / (double)(M - 1); }
+            }
+            if (k1 != 0 && err_loc > delta * t_step * tol) {
+                t_step = gamma * t_step * ool_pow(t_step * tol / err_loc, g.xm);
+                t_step = round_step2(t_step);
+                ++g.nrej;
+                if (!(t_step < 1e300) || g.nrej > max_steps) { g.fail |= 4; return; }
+                continue;
+            }
+            break;
+        }
+        // w = beta * (sum_{j<mb} F_j v_j + [k1] F_M v_M)
+        const int mx = mb + (k1 - 1 > 0 ? k1 - 1 : 0);
+        const cd F0 = S.F[0];
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            cd acc = cmul(v0[r], F0);
+#pragma unroll 1
+            for (int j = 1; j < mx; ++j) {
+                const float2 f = Vg[(int64_t)(j - 1) * n + tid + kMT * r];
+                acc = cfma(mkcd((double)f.x, (double)f.y), S.F[j], acc);
+            }
+            w[r] = cscale(acc, g.beta);
+        }
+        {
+            const auto tb = block_sum_lanes<1, kMT, 4>(br, [&](int) {
+                double s = 0.0;
+#pragma unroll
+                for (int r = 0; r < E; ++r) s += cabs2(w[r]);
+                return s;
+            });
+            g.beta = sqrt(tb.get(0));
+        }
+        g.t_new = gamma * t_step * ool_pow(t_step * tol / err_loc, g.xm);
+        g.t_new = round_step2(g.t_new);
+        if (t_step >= t_out - g.t_now) g.t_now = t_out;
+        else g.t_now += t_step;
+    }
+}
+
+template <int M, int E>
+__global__ __launch_bounds__(kMT) void cce_hahn_medium_kernel(KrylovArgs a, float2* medws) {
+    __shared__ MedSmem<M, E> S;
+    const int tid = threadIdx.x;
+    BlockRed<kMT> br(S.red);
+    const double tol = a.tol;
+    const int max_steps = a.max_steps;
+    const int n = kMT * E;
+    float2* const Vg = medws + (int64_t)blockIdx.x * M * n;
+    for (;;) {
+        if (tid == 0) S.ints[7] = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int item = S.ints[7];
+        __syncthreads();
+        if (item >= a.nitems) break;
+        const int64_t voff = a.vec_off[item];
+        const ItemSeq q = item_seq(a, item);
+        const bool cpmg = q.npulse > 1;
+        const int qA = a.item_opA[item], qB = a.item_opB[item];
+        const int qC = cpmg ? a.item_opC[item] : qA, qD = cpmg ? a.item_opD[item] : qB;
+        cd wA[E], wB[E];
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const int i = tid + kMT * r;
+            wA[r] = mkcd(a.u0[voff + i].x, a.u0[voff + i].y);
+            wB[r] = mkcd(a.y0[voff + i].x, a.y0[voff + i].y);
+        }
+        MedState gA, gB;
+        med_load(a, qA, S.op);
+        med_init<M, E>(S, br, gA, wA, a.rnd, n, a.klim, tol);
+        med_load(a, qB, S.op);
+        med_init<M, E>(S, br, gB, wB, a.rnd, n, a.klim, tol);
+        const double beta0A = gA.beta, beta0B = gB.beta;
+        const double scale = a.scale[item];
+        const int h = cpmg_split(q.npulse);
+        int loaded = qB;
+        for (int k = 0; k < q.nt; ++k) {
+            for (int s = 0; s < 2; ++s) {
+                // the active sweep is always gA / wA; the two swap after each
+                // sweep (register moves, no dynamically selected storage)
+                int nseg = 1;
+                if (cpmg) {   // restart from rho0 on this side
+                    const double2* src = s ? a.y0 : a.u0;
+#pragma unroll
+                    for (int r = 0; r < E; ++r) wA[r] = mkcd(src[voff + tid + kMT * r].x, src[voff + tid + kMT * r].y);
+                    gA.beta = s ? beta0B : beta0A;
+                    gA.t_now = 0.0;
+                    nseg = s == 0 ? h : q.npulse + 1 - h;
+                }
+                for (int sg = 0; sg < nseg; ++sg) {
+                    int opi = s;
+                    double target = q.tau[k];
+                    if (cpmg) {
+                        const int j = s == 0 ? sg : q.npulse - sg;
+                        opi = s == 0 ? ((j & 1) ? 2 : 0) : ((j & 1) ? 1 : 3);
+                        target = gA.t_now + cpmg_fraction(j, q.npulse) * q.tau[k];
+                    }
+                    const int qo = opi == 0 ? qA : (opi == 1 ? qB : (opi == 2 ? qC : qD));
+                    if (qo != loaded) {
+                        __syncthreads();
+                        med_load(a, qo, S.op);
+                        loaded = qo;
+                    }
+                    med_advance<M, E>(S, br, gA, wA, Vg, n, target, max_steps, tol);
+                }
+                const MedState tg = gA;
+                gA = gB;
+                gB = tg;
+#pragma unroll
+                for (int r = 0; r < E; ++r) {
+                    const cd tw = wA[r];
+                    wA[r] = wB[r];
+                    wB[r] = tw;
+                }
+            }
+            const auto tl = block_sum_lanes<1, kMT, 4>(br, [&](int) {
+                double s2 = 0.0;
+#pragma unroll
+                for (int r = 0; r < E; ++r) s2 += fma(wB[r].re, wA[r].re, wB[r].im * wA[r].im);
+                return s2;
+            });
+            if (tid == 0) q.L[k] = scale * tl.get(0);
+            if ((gA.fail | gB.fail) != 0) break;
+        }
+        if (tid == 0) {
+            int32_t* st = a.stats + (int64_t)item * kStatsPerItem;
+            st[0] = gA.nstep; st[1] = gA.nmult; st[2] = gA.nrej; st[3] = gA.nexph;
+            st[4] = gB.nstep; st[5] = gB.nmult; st[6] = gB.nrej; st[7] = gB.nexph;
+            st[8] = gA.fail | (gB.fail << 4) | (gA.brk << 8) | (gB.brk << 9);
+            st[9] = gA.nscale + gB.nscale;
+            reinterpret_cast<double*>(st + 10)[0] = gA.anorm;
+            reinterpret_cast<double*>(st + 12)[0] = gB.anorm;
+            st[14] = 0; st[15] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+int64_t medium_workspace_float2(int grid, int m, int n) { return (int64_t)grid * m * n; }
+
+int launch_hahn_medium(const KrylovArgs& a, float2* work, int m, int grid, hipStream_t stream) {
+    if (!work || (a.n != 1024 && a.n != 4096)) return -11;
+    if (a.n == 1024) {
+        if (m == 20) hipLaunchKernelGGL((cce_hahn_medium_kernel<20, 1>), dim3(grid), dim3(kMT), 0, stream, a, work);
+        else if (m == 30) hipLaunchKernelGGL((cce_hahn_medium_kernel<30, 1>), dim3(grid), dim3(kMT), 0, stream, a, work);
+        else return -12;
+    } else {
+        if (m == 20) hipLaunchKernelGGL((cce_hahn_medium_kernel<20, 4>), dim3(grid), dim3(kMT), 0, stream, a, work);
+        else if (m == 30) hipLaunchKernelGGL((cce_hahn_medium_kernel<30, 4>), dim3(grid), dim3(kMT), 0, stream, a, work);
+        else return -12;
+    }
+    return (int)hipGetLastError();
+}
+
+}  // namespace oops

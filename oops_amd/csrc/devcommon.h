@@ -286,15 +286,19 @@ struct LaneTotals {
     __device__ __forceinline__ double get(int k) const { return v[k]; }
 };
 
-template <int P, int NT, typename Gen>
+// STRIDE: doubles per wave in a partial slot (64 matches block_sum1's
+// layout, so the two may alternate on one BlockRed; a kernel that only uses
+// this reduction may pass P-sized slots: 2 * NW * STRIDE doubles of LDS).
+template <int P, int NT, int STRIDE = 64, typename Gen>
 __device__ __forceinline__ LaneTotals<P> block_sum_lanes(BlockRed<NT>& br, Gen gen) {
+    static_assert(P <= STRIDE, "slot too small");
     constexpr int NW = NT / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     LaneTotals<P> t;
 #pragma unroll
     for (int k = 0; k < P; ++k) t.v[k] = wave_sum_dpp(gen(k));
-    double* part = br.ws + br.slot * NW * 64;
+    double* part = br.ws + br.slot * NW * STRIDE;
 #pragma unroll
     for (int k = 0; k < P; ++k)
         if (lane == k) part[wave * P + k] = t.v[k];

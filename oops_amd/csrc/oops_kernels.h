@@ -16,6 +16,7 @@ constexpr int kStatsPerItem = 24;   // int32 slots of per-cluster counters
 // kSlot64 FP64 coefficients (masks changing at most one spin's digits; for
 // spin-1/2 Liouvillians of k <= 4 spins: the diagonal and 2k ket/bra flips)
 // and kSlot32 FP32 coefficients (two-spin flips: 2 k(k-1)/2 <= 12).
+constexpr int kMediumMaxEnt = 2560;  // ELL entries per operator on the n = 1024 / 4096 kernel
 constexpr int kSlot64 = 9;
 constexpr int kSlot32 = 12;
 constexpr int kSlotMaskStride = 32;                            // int32 masks per operator
@@ -64,6 +65,12 @@ int launch_hahn_lanczos(const KrylovArgs& a, int m, int grid, hipStream_t stream
 // the bucket has at most kSlot64 / kSlot32 slots; a.slotws holds
 // grid * 4 * kSlotOpD2 double2.  The caller zeroes the work counter first.
 int launch_hahn_lanczos_slot(const KrylovArgs& a, int m, int grid, hipStream_t stream);
+// Mixed Lanczos kernel for n = 1024 and 4096 (1024 threads per cluster, the
+// product input and the active generator's tables in LDS, the FP32 basis in
+// `work`: medium_workspace_float2(grid, m, n) float2).  Hermitian generators,
+// at most kMediumMaxEnt ELL entries and kMaxLocals local operators each.
+int launch_hahn_medium(const KrylovArgs& a, float2* work, int m, int grid, hipStream_t stream);
+int64_t medium_workspace_float2(int grid, int m, int n);
 void launch_zero_counter(int32_t* counter, hipStream_t stream);
 // Dense FP64-MFMA propagator kernel for n = 64 buckets whose items all have
 // uniform time grids starting at 0 (Hahn echo or CPMG-N); `work` holds

@@ -184,3 +184,23 @@ def test_coordinate_files_round_trip(tmp_path):
     assert open(out, "rb").read() == open(src, "rb").read()
     with pytest.raises(O.OopsError):
         O.read_xyz(src, max_n=10)
+
+
+def test_cce_result_export(tmp_path):
+    """The result file carries the reference's variables (cce.cpp:190-230):
+    per order the L_C(t) and reduced L~_C(t) matrices (nt x clusters), the
+    per-order and cumulative products and the time list."""
+    X, _ = O.read_xyz(os.path.join(ROOT, "tests", "golden", "RoyCoord.xyz"))
+    C = O.Clusters(X, 6.0, 3)
+    nt = 7
+    rng = np.random.default_rng(0)
+    L = 1.0 - 0.01 * rng.random((sum(C.counts()), nt))
+    t = np.linspace(0.0, 1e-3, nt)
+    path = tmp_path / "cce.npz"
+    C.export_npz(str(path), L, t)
+    z = np.load(path, allow_pickle=False)
+    assert [z[f"CCE{i}"].shape for i in range(3)] == [(nt, c) for c in C.counts()]
+    assert z["final_result"].shape == (nt, 3) and z["time_list"].shape == (nt, 1)
+    r = C.reduce(L.reshape(-1), nt)
+    assert np.array_equal(z["final_result"], r["final"])
+    assert np.array_equal(z["CCE1"], L[C.counts()[0]:C.counts()[0] + C.counts()[1]].T)
