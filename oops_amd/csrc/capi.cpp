@@ -240,6 +240,8 @@ struct oops_hahn_batch {
         int medium_grid = 0;
         bool slots = true;            // every operator fits kSlot64 / kSlot32
         int slot_grid = 0;
+        DevBuf sweepws;               // slot kernel: sweep-A states at every output time, per workgroup
+        int sweep_nt = 0;             // output times per workgroup that sweepws holds
         // states above 256 elements: every item on the single-state path, in turn
         bool large = false;
         int kernel = -1;  // kind that ran last: see oops_hahn_batch_kernels
@@ -321,6 +323,8 @@ int oops_dlaran_fill(int32_t* iseed, int64_t count, double* out) {
 }
 
 }  // extern "C"
+
+static int sweep_workspace(oops_ctx* ctx, oops_hahn_batch::Bucket& B, int ntmax);
 
 int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool per_item, int32_t m, int32_t klim,
                        double tol, int32_t precision, oops_hahn_batch** out) {
@@ -547,7 +551,8 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
         const int per_cu = dense ? 4 : 2;
         B.grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu * per_cu));
         if (B.medium) {
-            B.medium_grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu));
+            // one workgroup per CU; Hahn items run as two units (one per sweep)
+            B.medium_grid = std::max(1, std::min(2 * (int)B.items.size(), ctx->num_cu));
             const size_t bytes = sizeof(float) * 2 * (size_t)medium_workspace_float2(B.medium_grid, m, B.n);
             rc = hip_check(ctx, hipMalloc(&B.medws.p, bytes), "hipMalloc medium workspace");
             if (rc) return rc;
@@ -564,6 +569,13 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
             if (!rc) B.slotws.bytes = bytes;
             if (rc) return rc;
         }
+        if (per_item && (B.slots || B.medium)) {
+            int ntmax = 0;
+            for (size_t i = 0; i < H.nt.size(); ++i)
+                if (H.npulse[i] == 1) ntmax = std::max(ntmax, (int)H.nt[i]);
+            rc = sweep_workspace(ctx, B, ntmax);
+            if (rc) return rc;
+        }
         if (B.n == 64) {
             // one workgroup per CU (the kernel holds 152 KB of LDS)
             B.dense_grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu));
@@ -576,7 +588,7 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
     for (auto& B : b->buckets)
         for (DevBuf* d : {&B.hdr, &B.loc, &B.val, &B.off, &B.opA, &B.opB, &B.opC, &B.opD, &B.voff, &B.u0, &B.y0,
                           &B.scale, &B.rnd, &B.L, &B.stats, &B.queue, &B.npulse, &B.nt, &B.tau_off, &B.L_off, &B.tau,
-                          &B.work, &B.eslot, &B.smask, &B.slotws, &B.medws})
+                          &B.work, &B.eslot, &B.smask, &B.slotws, &B.sweepws, &B.medws})
             b->footprint += (int64_t)d->bytes;
     if (per_item) {
         if (hipStreamCreate(&b->stream) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipStreamCreate");
@@ -585,6 +597,29 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
             if (hipEventCreate(&e) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipEventCreate");
     }
     *out = b.release();
+    return OOPS_OK;
+}
+
+// Hahn-sweep state stores, kept when already large enough.  Slot kernel: the
+// sweep-A state (256 complex FP64) at ntmax output times per workgroup.
+// Medium kernel: both sweeps' states of every item at ntmax times (items x 2 x
+// n x ntmax complex FP64); above kMediumStoreMax bytes the bucket runs without
+// a store (both sweeps of an item in one workgroup, a step cut at every output).
+constexpr size_t kMediumStoreMax = size_t(32) << 30;
+static int sweep_workspace(oops_ctx* ctx, oops_hahn_batch::Bucket& B, int ntmax) {
+    if (ntmax <= 0) return OOPS_OK;
+    size_t bytes = 0;
+    if (B.slots && B.slot_grid > 0) bytes = sizeof(double) * 2 * 256 * (size_t)ntmax * (size_t)B.slot_grid;
+    else if (B.medium) bytes = sizeof(double) * 2 * 2 * (size_t)B.n * (size_t)ntmax * B.items.size();
+    else return OOPS_OK;
+    if (ntmax <= B.sweep_nt && B.sweepws.p) return OOPS_OK;
+    B.sweepws.release();
+    B.sweep_nt = 0;
+    if (B.medium && bytes > kMediumStoreMax) return OOPS_OK;
+    const int rc = hip_check(ctx, hipMalloc(&B.sweepws.p, bytes), "hipMalloc sweep states");
+    if (rc) return rc;
+    B.sweepws.bytes = bytes;
+    B.sweep_nt = ntmax;
     return OOPS_OK;
 }
 
@@ -612,6 +647,8 @@ static KrylovArgs bucket_args(const oops_hahn_batch* b, const oops_hahn_batch::B
     a.eslot = (const uint8_t*)B.eslot.p;
     a.smask = (const int32_t*)B.smask.p;
     a.slotws = (double2*)B.slotws.p;
+    a.sweepws = (double2*)B.sweepws.p;
+    a.ws_nt = B.sweep_nt;
     a.nitems = (int32_t)B.items.size();
     a.n = B.n;
     a.klim = b->klim;
@@ -841,6 +878,8 @@ int oops_hahn_batch_run(oops_hahn_batch* b, int32_t nt, const double* tau) {
         rc = hip_check(ctx, hipMalloc(&B.L.p, sizeof(double) * B.items.size() * nt), "hipMalloc L");
         if (rc) break;
         B.L.bytes = sizeof(double) * B.items.size() * nt;
+        rc = sweep_workspace(ctx, B, nt);
+        if (rc) break;
         const KrylovArgs a = bucket_args(b, B, nt, (const double*)b->tau.p);
         hipEventRecord(ev[2 * bk], ctx->stream);
         rc = bucket_launch(b, B, a, ctx->stream, uniform);

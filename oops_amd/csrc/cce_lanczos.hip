@@ -418,12 +418,49 @@ struct LzSlotSmem {
 #endif
 };
 
+// Output side of a sweep.  mode 0: advance to t_end only (CPMG segments).
+// mode 1: store the state at every output time tau[k] in ob (kNT elements per
+// time).  mode 2: at every tau[k] take Re <stored state, state> into L[k].
+struct SweepOut {
+    const double* tau;
+    double2* ob;
+    double* L;
+    double scale;
+    int nt;
+    int mode;
+};
+
+template <int M>
+__device__ __forceinline__ void emit_state(BlockRed<kNT>& br, const SweepOut& o, int k, const cd& w) {
+    const int tid = threadIdx.x;
+    double2* const p = o.ob + (int64_t)k * kNT + tid;
+    if (o.mode == 1) {
+        *p = make_double2(w.re, w.im);
+    } else {
+        const double2 u = *p;
+        const auto t = block_sum_lanes<1, kNT>(br, [&](int) { return fma(w.re, u.x, w.im * u.y); });
+        if (tid == 0) o.L[k] = o.scale * t.get(0);
+    }
+}
+
+// One sweep of Expokit's integrator (zgexpv_cache.f:440-661) with the Lanczos
+// recurrence of a Hermitian generator.  With output times (mode 1 / 2) the
+// steps are not cut at the outputs: a step runs to min(t_new, t_end) under the
+// usual error control, and every output time inside it is evaluated from the
+// same Krylov basis, w(tau_k) = beta V exp((tau_k - t_now) H_aug) e_1, one
+// small exponential each.  The FP64 path keeps the reference's schedule (a
+// step ends at every output time); the mixed path only has to meet its
+// tolerance on L(t), and sparing the clamped steps halves its Krylov work on
+// output grids finer than the natural step.
 template <int M>
 __device__ __forceinline__ void advance_slot(LzSlotSmem<M>& S, BlockRed<kNT>& br, IntegState& g, const SlotOp& so,
-                                             double t_out, int max_steps, double tol) {
+                                             double t_end, const SweepOut& o, int max_steps, double tol) {
     const int tid = threadIdx.x;
     const double delta = 1.2, gamma = 0.9;
-    while (g.t_now < t_out) {
+    int k = 0;
+    if (o.mode)
+        for (; k < o.nt && o.tau[k] <= g.t_now; ++k) emit_state<M>(br, o, k, g.w);
+    while (g.t_now < t_end) {
         if (g.nstep >= max_steps) { g.fail |= 1; break; }
         ++g.nstep;
         const cd v0 = cscale(g.w, 1.0 / g.beta);
@@ -431,7 +468,7 @@ __device__ __forceinline__ void advance_slot(LzSlotSmem<M>& S, BlockRed<kNT>& br
         double bprev = 0.0;
         int k1 = 2, mb = M;
         bool brk = false;
-        double t_step = fmin(t_out - g.t_now, g.t_new);
+        double t_step = fmin(t_end - g.t_now, g.t_new);
 #pragma unroll 1
         for (int j = 0; j < M; ++j) {
             OOPS_TIC(tm0);
@@ -462,7 +499,7 @@ __device__ __forceinline__ void advance_slot(LzSlotSmem<M>& S, BlockRed<kNT>& br
                 k1 = 0;
                 g.brk = 1;
                 mb = j + 1;
-                t_step = t_out - g.t_now;
+                t_step = t_end - g.t_now;
                 brk = true;
                 break;
             }
@@ -500,6 +537,7 @@ __device__ __forceinline__ void advance_slot(LzSlotSmem<M>& S, BlockRed<kNT>& br
                 t_step = round_step2(t_step);
                 ++g.nrej;
                 if (!(t_step < 1e300) || g.nrej > max_steps) { g.fail |= 4; return; }
+                __syncthreads();   // every thread has read F before the next attempt rewrites it
                 continue;
             }
             break;
@@ -508,21 +546,36 @@ __device__ __forceinline__ void advance_slot(LzSlotSmem<M>& S, BlockRed<kNT>& br
         OOPS_TIC(tu0);
         // w = beta * (sum_{j<mb} F_j v_j + [k1] F_M v_M)
         const int mx = mb + (k1 - 1 > 0 ? k1 - 1 : 0);
-        cd acc = cmul(v0, S.F[0]);
+        auto combine = [&]() -> cd {
+            cd acc = cmul(v0, S.F[0]);
 #pragma unroll 1
-        for (int j = 1; j < mx; ++j) {
-            const float2 f = S.V[j - 1][tid];
-            acc = cfma(mkcd((double)f.x, (double)f.y), S.F[j], acc);
+            for (int j = 1; j < mx; ++j) {
+                const float2 f = S.V[j - 1][tid];
+                acc = cfma(mkcd((double)f.x, (double)f.y), S.F[j], acc);
+            }
+            return cscale(acc, g.beta);
+        };
+        const cd wnew = combine();
+        const double t1 = t_step >= t_end - g.t_now ? t_end : g.t_now + t_step;
+        if (o.mode) {
+            // output times inside the step, from the same basis
+            for (; k < o.nt && o.tau[k] < t1; ++k) {
+                __syncthreads();   // every thread has read F
+                ++g.nexph;
+                g.nscale += cheb_coeffs<M>(S, mb, k1, o.tau[k] - g.t_now, bprev);
+                emit_state<M>(br, o, k, combine());
+            }
         }
-        g.w = cscale(acc, g.beta);
+        g.w = wnew;
         {
             const auto tb = block_sum_lanes<1, kNT>(br, [&](int) { return tid < g.n ? cabs2(g.w) : 0.0; });
             g.beta = sqrt(tb.get(0));
         }
         g.t_new = gamma * t_step * ool_pow(t_step * tol / err_loc, g.xm);
         g.t_new = round_step2(g.t_new);
-        if (t_step >= t_out - g.t_now) g.t_now = t_out;
-        else g.t_now += t_step;
+        g.t_now = t1;
+        if (o.mode)
+            for (; k < o.nt && o.tau[k] <= g.t_now; ++k) emit_state<M>(br, o, k, g.w);
         OOPS_TOC(g, tupd, tu0);
     }
 }
@@ -564,6 +617,7 @@ void cce_hahn_lanczos_slot_kernel(KrylovArgs a) {
         const cd u0 = tid < n ? mkcd(a.u0[voff + tid].x, a.u0[voff + tid].y) : mkcd(0.0, 0.0);
         const cd y0 = tid < n ? mkcd(a.y0[voff + tid].x, a.y0[voff + tid].y) : mkcd(0.0, 0.0);
         const NormWork nw{S.xs, S.xs32, S.tdiag, S.toff, S.scal};
+        const double scale = a.scale[item];
         SlotOp so;
         so.load(ws, a.smask + (int64_t)qA * kSlotMaskStride, tid, n);
         init_integ_mv<true>(nw, n, [&](int i) { return so.apply(S.xs, S.xs32, i); }, br, gA, u0, a.rnd, a.klim,
@@ -572,9 +626,14 @@ void cce_hahn_lanczos_slot_kernel(KrylovArgs a) {
         init_integ_mv<true>(nw, n, [&](int i) { return so.apply(S.xs, S.xs32, i); }, br, gB, y0, a.rnd, a.klim,
                             tol, M);
         const double beta0A = gA.beta, beta0B = gB.beta;
-        const double scale = a.scale[item];
         const int h = cpmg_split(q.npulse);
-        for (int k = 0; k < q.nt; ++k) {
+        // Hahn echo with a state store: sweep A once over every output time
+        // (states stored), then sweep B taking <state_B, state_A> at the same
+        // times.  Otherwise (CPMG-N) one pair of sweeps per output time.
+        const bool hahn = !cpmg && a.sweepws && q.nt > 0 && q.nt <= a.ws_nt;
+        SweepOut o{q.tau, a.sweepws + (int64_t)blockIdx.x * kNT * a.ws_nt, q.L, scale, q.nt, 0};
+        const int nouter = hahn ? 1 : q.nt;
+        for (int k = 0; k < nouter; ++k) {
             for (int s = 0; s < 2; ++s) {
                 IntegState& g = gA;   // active sweep; the states swap after each sweep
                 int nseg = 1;
@@ -584,9 +643,13 @@ void cce_hahn_lanczos_slot_kernel(KrylovArgs a) {
                     g.t_now = 0.0;
                     nseg = s == 0 ? h : q.npulse + 1 - h;
                 }
+                if (hahn) {
+                    o.mode = 1 + s;
+                    if (s == 1 && gB.fail != 0) nseg = 0;   // sweep A failed: no stored states
+                }
                 for (int sg = 0; sg < nseg; ++sg) {
                     int opi = s;
-                    double target = q.tau[k];
+                    double target = hahn ? q.tau[q.nt - 1] : q.tau[k];
                     if (cpmg) {
                         const int j = s == 0 ? sg : q.npulse - sg;
                         opi = s == 0 ? ((j & 1) ? 2 : 0) : ((j & 1) ? 1 : 3);
@@ -594,17 +657,19 @@ void cce_hahn_lanczos_slot_kernel(KrylovArgs a) {
                     }
                     const int qo = opi == 0 ? qA : (opi == 1 ? qB : (opi == 2 ? qC : qD));
                     so.load(ws + opi * kSlotOpD2, a.smask + (int64_t)qo * kSlotMaskStride, tid, n);
-                    advance_slot<M>(S, br, g, so, target, max_steps, tol);
+                    advance_slot<M>(S, br, g, so, target, o, max_steps, tol);
                 }
                 const IntegState tmp = gA;
                 gA = gB;
                 gB = tmp;
             }
-            const auto tl = block_sum_lanes<1, kNT>(br, [&](int) {
-                return fma(gB.w.re, gA.w.re, gB.w.im * gA.w.im);
-            });
-            if (tid == 0) q.L[k] = scale * tl.get(0);
-            if ((gA.fail | gB.fail) != 0) break;
+            if (!hahn) {
+                const auto tl = block_sum_lanes<1, kNT>(br, [&](int) {
+                    return fma(gB.w.re, gA.w.re, gB.w.im * gA.w.im);
+                });
+                if (tid == 0) q.L[k] = scale * tl.get(0);
+                if ((gA.fail | gB.fail) != 0) break;
+            }
         }
         if (tid == 0) {
             int32_t* st = a.stats + (int64_t)item * kStatsPerItem;

@@ -201,11 +201,28 @@ __device__ void med_init(MedSmem<M, E>& S, BlockRed<kMT>& br, MedState& g, cd (&
     g.nstep = g.nmult = g.nrej = g.nexph = g.nscale = g.brk = g.fail = 0;
 }
 
+// Stores the state at output time k (ob: n complex FP64 per output time).
+template <int E>
+__device__ __forceinline__ void med_emit(double2* ob, int n, int k, const cd (&x)[E]) {
+#pragma unroll
+    for (int r = 0; r < E; ++r)
+        ob[(int64_t)k * n + threadIdx.x + kMT * r] = make_double2(x[r].re, x[r].im);
+}
+
+// One sweep to t_out.  With an output store (ob != NULL) the steps are not cut
+// at the output times otau[0..ont): every output inside a step is evaluated
+// from that step's Krylov basis, w(tau_k) = beta V exp((tau_k - t_now) H_aug) e_1,
+// and stored; the step itself runs to min(t_new, t_out) under Expokit's error
+// control.  Without it the step ends at t_out as in the reference.
 template <int M, int E>
 __device__ void med_advance(MedSmem<M, E>& S, BlockRed<kMT>& br, MedState& g, cd (&w)[E], float2* __restrict__ Vg,
-                            int n, double t_out, int max_steps, double tol) {
+                            int n, double t_out, int max_steps, double tol, const double* otau, int ont,
+                            double2* ob) {
     const int tid = threadIdx.x;
     const double delta = 1.2, gamma = 0.9;
+    int k = 0;
+    if (ob)
+        for (; k < ont && otau[k] <= g.t_now; ++k) med_emit<E>(ob, n, k, w);
     while (g.t_now < t_out) {
         if (g.nstep >= max_steps) { g.fail |= 1; break; }
         ++g.nstep;
@@ -300,23 +317,40 @@ __device__ void med_advance(MedSmem<M, E>& S, BlockRed<kMT>& br, MedState& g, cd
                 t_step = round_step2(t_step);
                 ++g.nrej;
                 if (!(t_step < 1e300) || g.nrej > max_steps) { g.fail |= 4; return; }
+                __syncthreads();   // every thread has read F before the next attempt rewrites it
                 continue;
             }
             break;
         }
         // w = beta * (sum_{j<mb} F_j v_j + [k1] F_M v_M)
         const int mx = mb + (k1 - 1 > 0 ? k1 - 1 : 0);
-        const cd F0 = S.F[0];
+        auto combine = [&](cd (&x)[E]) {
+            const cd F0 = S.F[0];
 #pragma unroll
-        for (int r = 0; r < E; ++r) {
-            cd acc = cmul(v0[r], F0);
+            for (int r = 0; r < E; ++r) {
+                cd acc = cmul(v0[r], F0);
 #pragma unroll 1
-            for (int j = 1; j < mx; ++j) {
-                const float2 f = Vg[(int64_t)(j - 1) * n + tid + kMT * r];
-                acc = cfma(mkcd((double)f.x, (double)f.y), S.F[j], acc);
+                for (int j = 1; j < mx; ++j) {
+                    const float2 f = Vg[(int64_t)(j - 1) * n + tid + kMT * r];
+                    acc = cfma(mkcd((double)f.x, (double)f.y), S.F[j], acc);
+                }
+                x[r] = cscale(acc, g.beta);
             }
-            w[r] = cscale(acc, g.beta);
+        };
+        const double t1 = t_step >= t_out - g.t_now ? t_out : g.t_now + t_step;
+        combine(p);   // the state at the end of the step
+        if (ob) {
+            // output times inside the step, from the same basis (vcur is free here)
+            for (; k < ont && otau[k] < t1; ++k) {
+                __syncthreads();   // every thread has read F
+                ++g.nexph;
+                g.nscale += cheb_coeffs<M>(S, mb, k1, otau[k] - g.t_now, bprev);
+                combine(vcur);
+                med_emit<E>(ob, n, k, vcur);
+            }
         }
+#pragma unroll
+        for (int r = 0; r < E; ++r) w[r] = p[r];
         {
             const auto tb = block_sum_lanes<1, kMT, 4>(br, [&](int) {
                 double s = 0.0;
@@ -328,11 +362,17 @@ __device__ void med_advance(MedSmem<M, E>& S, BlockRed<kMT>& br, MedState& g, cd
         }
         g.t_new = gamma * t_step * ool_pow(t_step * tol / err_loc, g.xm);
         g.t_new = round_step2(g.t_new);
-        if (t_step >= t_out - g.t_now) g.t_now = t_out;
-        else g.t_now += t_step;
+        g.t_now = t1;
+        if (ob)
+            for (; k < ont && otau[k] <= g.t_now; ++k) med_emit<E>(ob, n, k, w);
     }
 }
 
+// Hahn items with a state store (a.sweepws) run as two work units, one per
+// sweep: unit 2i + s integrates sweep s of item i over every output time and
+// stores its states; medium_dot_kernel then forms L(t).  CPMG-N items (and
+// every item without a store) run both sweeps in one unit, as in the n <= 256
+// kernels.
 template <int M, int E>
 __global__ __launch_bounds__(kMT) void cce_hahn_medium_kernel(KrylovArgs a, float2* medws) {
     __shared__ MedSmem<M, E> S;
@@ -342,35 +382,62 @@ __global__ __launch_bounds__(kMT) void cce_hahn_medium_kernel(KrylovArgs a, floa
     const int max_steps = a.max_steps;
     const int n = kMT * E;
     float2* const Vg = medws + (int64_t)blockIdx.x * M * n;
+    const bool split = a.sweepws != nullptr;
+    const int nunits = split ? 2 * a.nitems : a.nitems;
     for (;;) {
         if (tid == 0) S.ints[7] = atomicAdd(a.queue, 1);
         __syncthreads();
-        const int item = S.ints[7];
+        const int unit = S.ints[7];
         __syncthreads();
-        if (item >= a.nitems) break;
+        if (unit >= nunits) break;
+        const int item = split ? unit >> 1 : unit;
+        const int side = split ? unit & 1 : 0;
         const int64_t voff = a.vec_off[item];
         const ItemSeq q = item_seq(a, item);
         const bool cpmg = q.npulse > 1;
+        const bool hahn = split && !cpmg && q.nt > 0 && q.nt <= a.ws_nt;
+        if (split && side == 1 && !hahn) continue;   // unit 0 runs both sweeps of this item
         const int qA = a.item_opA[item], qB = a.item_opB[item];
         const int qC = cpmg ? a.item_opC[item] : qA, qD = cpmg ? a.item_opD[item] : qB;
         cd wA[E], wB[E];
-#pragma unroll
-        for (int r = 0; r < E; ++r) {
-            const int i = tid + kMT * r;
-            wA[r] = mkcd(a.u0[voff + i].x, a.u0[voff + i].y);
-            wB[r] = mkcd(a.y0[voff + i].x, a.y0[voff + i].y);
-        }
         MedState gA, gB;
-        med_load(a, qA, S.op);
-        med_init<M, E>(S, br, gA, wA, a.rnd, n, a.klim, tol);
-        med_load(a, qB, S.op);
-        med_init<M, E>(S, br, gB, wB, a.rnd, n, a.klim, tol);
-        const double beta0A = gA.beta, beta0B = gB.beta;
+        int loaded;
+        double beta0A = 0.0, beta0B = 0.0;
+        if (hahn) {
+            // this unit's sweep only, in the active slot gA / wA
+            const double2* src = side ? a.y0 : a.u0;
+#pragma unroll
+            for (int r = 0; r < E; ++r) {
+                const int i = tid + kMT * r;
+                wA[r] = mkcd(src[voff + i].x, src[voff + i].y);
+                wB[r] = mkcd(0.0, 0.0);
+            }
+            loaded = side ? qB : qA;
+            med_load(a, loaded, S.op);
+            med_init<M, E>(S, br, gA, wA, a.rnd, n, a.klim, tol);
+            gB = gA;
+        } else {
+#pragma unroll
+            for (int r = 0; r < E; ++r) {
+                const int i = tid + kMT * r;
+                wA[r] = mkcd(a.u0[voff + i].x, a.u0[voff + i].y);
+                wB[r] = mkcd(a.y0[voff + i].x, a.y0[voff + i].y);
+            }
+            med_load(a, qA, S.op);
+            med_init<M, E>(S, br, gA, wA, a.rnd, n, a.klim, tol);
+            med_load(a, qB, S.op);
+            med_init<M, E>(S, br, gB, wB, a.rnd, n, a.klim, tol);
+            loaded = qB;
+            beta0A = gA.beta;
+            beta0B = gB.beta;
+        }
         const double scale = a.scale[item];
         const int h = cpmg_split(q.npulse);
-        int loaded = qB;
-        for (int k = 0; k < q.nt; ++k) {
-            for (int s = 0; s < 2; ++s) {
+        double2* const ob = hahn ? a.sweepws + ((int64_t)item * 2 + side) * a.ws_nt * n : nullptr;
+        const int nouter = hahn ? 1 : q.nt;
+        const int s0 = hahn ? side : 0, s1 = hahn ? side + 1 : 2;
+        for (int k = 0; k < nouter; ++k) {
+            for (int s = s0; s < s1; ++s) {
                 // the active sweep is always gA / wA; the two swap after each
                 // sweep (register moves, no dynamically selected storage)
                 int nseg = 1;
@@ -384,7 +451,7 @@ __global__ __launch_bounds__(kMT) void cce_hahn_medium_kernel(KrylovArgs a, floa
                 }
                 for (int sg = 0; sg < nseg; ++sg) {
                     int opi = s;
-                    double target = q.tau[k];
+                    double target = hahn ? q.tau[q.nt - 1] : q.tau[k];
                     if (cpmg) {
                         const int j = s == 0 ? sg : q.npulse - sg;
                         opi = s == 0 ? ((j & 1) ? 2 : 0) : ((j & 1) ? 1 : 3);
@@ -396,38 +463,81 @@ __global__ __launch_bounds__(kMT) void cce_hahn_medium_kernel(KrylovArgs a, floa
                         med_load(a, qo, S.op);
                         loaded = qo;
                     }
-                    med_advance<M, E>(S, br, gA, wA, Vg, n, target, max_steps, tol);
+                    med_advance<M, E>(S, br, gA, wA, Vg, n, target, max_steps, tol, q.tau, q.nt, ob);
                 }
-                const MedState tg = gA;
-                gA = gB;
-                gB = tg;
+                if (!hahn) {
+                    const MedState tg = gA;
+                    gA = gB;
+                    gB = tg;
 #pragma unroll
-                for (int r = 0; r < E; ++r) {
-                    const cd tw = wA[r];
-                    wA[r] = wB[r];
-                    wB[r] = tw;
+                    for (int r = 0; r < E; ++r) {
+                        const cd tw = wA[r];
+                        wA[r] = wB[r];
+                        wB[r] = tw;
+                    }
                 }
             }
-            const auto tl = block_sum_lanes<1, kMT, 4>(br, [&](int) {
-                double s2 = 0.0;
+            if (!hahn) {
+                const auto tl = block_sum_lanes<1, kMT, 4>(br, [&](int) {
+                    double s2 = 0.0;
 #pragma unroll
-                for (int r = 0; r < E; ++r) s2 += fma(wB[r].re, wA[r].re, wB[r].im * wA[r].im);
-                return s2;
-            });
-            if (tid == 0) q.L[k] = scale * tl.get(0);
-            if ((gA.fail | gB.fail) != 0) break;
+                    for (int r = 0; r < E; ++r) s2 += fma(wB[r].re, wA[r].re, wB[r].im * wA[r].im);
+                    return s2;
+                });
+                if (tid == 0) q.L[k] = scale * tl.get(0);
+                if ((gA.fail | gB.fail) != 0) break;
+            }
         }
         if (tid == 0) {
             int32_t* st = a.stats + (int64_t)item * kStatsPerItem;
-            st[0] = gA.nstep; st[1] = gA.nmult; st[2] = gA.nrej; st[3] = gA.nexph;
-            st[4] = gB.nstep; st[5] = gB.nmult; st[6] = gB.nrej; st[7] = gB.nexph;
-            st[8] = gA.fail | (gB.fail << 4) | (gA.brk << 8) | (gB.brk << 9);
-            st[9] = gA.nscale + gB.nscale;
-            reinterpret_cast<double*>(st + 10)[0] = gA.anorm;
-            reinterpret_cast<double*>(st + 12)[0] = gB.anorm;
-            st[14] = 0; st[15] = 0;
+            if (hahn) {
+                // each sweep its own fields; medium_dot_kernel merges 14 / 15 into 8 / 9
+                st[4 * side + 0] = gA.nstep; st[4 * side + 1] = gA.nmult;
+                st[4 * side + 2] = gA.nrej; st[4 * side + 3] = gA.nexph;
+                reinterpret_cast<double*>(st + 10 + 2 * side)[0] = gA.anorm;
+                if (side == 0) { st[8] = gA.fail | (gA.brk << 8); st[9] = gA.nscale; }
+                else { st[14] = (gA.fail << 4) | (gA.brk << 9); st[15] = gA.nscale; }
+            } else {
+                st[0] = gA.nstep; st[1] = gA.nmult; st[2] = gA.nrej; st[3] = gA.nexph;
+                st[4] = gB.nstep; st[5] = gB.nmult; st[6] = gB.nrej; st[7] = gB.nexph;
+                st[8] = gA.fail | (gB.fail << 4) | (gA.brk << 8) | (gB.brk << 9);
+                st[9] = gA.nscale + gB.nscale;
+                reinterpret_cast<double*>(st + 10)[0] = gA.anorm;
+                reinterpret_cast<double*>(st + 12)[0] = gB.anorm;
+                st[14] = 0; st[15] = 0;
+            }
         }
         __syncthreads();
+    }
+}
+
+// L(t_k) = scale Re <state_B(t_k), state_A(t_k)> for the items whose sweeps
+// ran as separate units (one workgroup per item, one wave per output time),
+// and the merge of their per-sweep status fields.
+__global__ __launch_bounds__(256) void medium_dot_kernel(KrylovArgs a) {
+    const int item = blockIdx.x;
+    const ItemSeq q = item_seq(a, item);
+    if (q.npulse > 1 || q.nt <= 0 || q.nt > a.ws_nt) return;
+    const int n = a.n;
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const double2* A = a.sweepws + (int64_t)item * 2 * a.ws_nt * n;
+    const double2* B = A + (int64_t)a.ws_nt * n;
+    const double scale = a.scale[item];
+    for (int k = wv; k < q.nt; k += 256 / kWave) {
+        double s = 0.0;
+        for (int i = lane; i < n; i += kWave) {
+            const double2 u = A[(int64_t)k * n + i], v = B[(int64_t)k * n + i];
+            s = fma(v.x, u.x, fma(v.y, u.y, s));
+        }
+        s = wave_sum_dpp(s);
+        if (lane == 0) q.L[k] = scale * s;
+    }
+    if (threadIdx.x == 0) {
+        int32_t* st = a.stats + (int64_t)item * kStatsPerItem;
+        st[8] |= st[14];
+        st[9] += st[15];
+        st[14] = 0;
+        st[15] = 0;
     }
 }
 
@@ -437,6 +547,7 @@ int64_t medium_workspace_float2(int grid, int m, int n) { return (int64_t)grid *
 
 int launch_hahn_medium(const KrylovArgs& a, float2* work, int m, int grid, hipStream_t stream) {
     if (!work || (a.n != 1024 && a.n != 4096)) return -11;
+    if (a.sweepws && a.ws_nt <= 0) return -13;
     if (a.n == 1024) {
         if (m == 20) hipLaunchKernelGGL((cce_hahn_medium_kernel<20, 1>), dim3(grid), dim3(kMT), 0, stream, a, work);
         else if (m == 30) hipLaunchKernelGGL((cce_hahn_medium_kernel<30, 1>), dim3(grid), dim3(kMT), 0, stream, a, work);
@@ -446,6 +557,7 @@ int launch_hahn_medium(const KrylovArgs& a, float2* work, int m, int grid, hipSt
         else if (m == 30) hipLaunchKernelGGL((cce_hahn_medium_kernel<30, 4>), dim3(grid), dim3(kMT), 0, stream, a, work);
         else return -12;
     }
+    if (a.sweepws) hipLaunchKernelGGL(medium_dot_kernel, dim3(a.nitems), dim3(256), 0, stream, a);
     return (int)hipGetLastError();
 }
 

@@ -48,12 +48,14 @@ struct KrylovArgs {
     const uint8_t* eslot;      // slot of every ELL entry (Lanczos slot kernel), or NULL
     const int32_t* smask;      // kSlotMaskStride masks per operator: kSlot64 FP64-class, then kSlot32
     double2* slotws;           // per-workgroup coefficient tables: 4 x kSlotOpD2 double2 per workgroup
+    double2* sweepws;          // Hahn state store, or NULL: slot kernel 256 x ws_nt per workgroup (sweep A);
+                               // medium kernel n x ws_nt per item and sweep
     int32_t nitems;
     int32_t n;
     int32_t nt;
     int32_t klim;
     int32_t max_steps;
-    int32_t pad;
+    int32_t ws_nt;             // output times per workgroup that sweepws holds
     double tol;
 };
 

@@ -59,9 +59,20 @@ struct Req {
 };
 
 // Device bytes a request will hold, from its operators' term counts (an upper
-// bound of the merged ELL tables: one d x d block per spin, one d^2 x d^2 per pair).
+// bound of the merged ELL tables: one d x d block per spin, one d^2 x d^2 per pair),
+// plus the Hahn-sweep state store of the 256-element Lanczos bucket (256 complex
+// FP64 per output time for each of at most 512 workgroups).
 int64_t estimate_bytes(const CceItems& it) {
-    int64_t b = 0;
+    int64_t b = 0, n256 = 0, nt256 = 0;
+    for (const EchoIn& e : it.items) {
+        int64_t dim = 1;
+        for (int s = 0; s < e.opA->nspin; ++s) dim *= e.opA->spin_dim[s];
+        if (dim > 64 && dim <= 256 && e.npulse == 1) {
+            ++n256;
+            nt256 = std::max<int64_t>(nt256, e.nt);
+        }
+    }
+    b += std::min<int64_t>(n256, 512) * 256 * 16 * nt256;
     for (const EchoIn& e : it.items) {
         const oops_term_table* ops[4] = {e.opA, e.opB, e.opC, e.opD};
         int64_t n = 1;

@@ -80,3 +80,46 @@ def test_medium_kernel_cpmg_matches_reference_propagator(ctx):
             v = expm_multiply(-1j * seg[j] * tk * op, v)
         Lr.append(np.real(prob["D"] * (prob["rho"].conj() @ v)))
     assert np.abs(L[0] - np.array(Lr)).max() < 1e-6
+
+
+@pytest.mark.gpu
+def test_medium_hahn_units_and_cpmg_in_one_bucket(ctx):
+    """Hahn items with a state store run as one work unit per sweep (output
+    times inside a Krylov step evaluated from its basis, L(t) formed by a
+    second kernel); a CPMG-N item in the same bucket runs both sides in one
+    unit.  Hahn items on fine and irregular grids match the FP64 single-state
+    path to 1e-6, and the CPMG item gives the same L as when it runs alone."""
+    coords = _clusters(5, 3)
+    par = O.cce_params(B=B111)
+    rng = np.random.default_rng(4)
+    grids = [np.linspace(0.0, 1e-3, 161) / 2, np.sort(np.concatenate([[0.0, 0.0], rng.uniform(0.0, 5e-4, 30)]))]
+    hahn = []
+    for c, g in zip(coords[:2], grids):
+        A, rho, D = O.cluster_terms(c, par, 0)
+        Bd, _, _ = O.cluster_terms(c, par, 2)
+        hahn.append({"opA": A, "opB": Bd, "u0": rho, "y0": rho, "scale": D, "npulse": 1, "t": g})
+    A, rho, D = O.cluster_terms(coords[2], par, 0)
+    Bd, _, _ = O.cluster_terms(coords[2], par, 2)
+    Cc, _, _ = O.cluster_terms(coords[2], par, 1)
+    Dd, _, _ = O.cluster_terms(coords[2], par, 5)
+    cp = {"opA": A, "opB": Bd, "opC": Cc, "opD": Dd, "u0": rho, "y0": rho, "scale": D, "npulse": 2,
+          "t": np.linspace(0.0, 4e-4, 5)}
+    print("medium units: hahn grids", [len(g) for g in grids], "+ cpmg N=2", flush=True)
+    mixed = O.EchoBatch.from_items(ctx, hahn + [cp], m=30, tol=1e-3, precision="mixed")
+    mixed.run()
+    Lm, st = mixed.results()
+    assert mixed.timing()[0]["kernel"] == "lanczos_medium"
+    assert (st[:, 8] & 0xFF).max() == 0
+    assert (st[:, 14:16] == 0).all()
+    ref = O.EchoBatch.from_items(ctx, hahn, m=30, tol=1e-12, precision="fp64")
+    ref.run()
+    L64, st64 = ref.results()
+    for i in range(2):
+        e = np.abs(Lm[i] - L64[i]).max()
+        print(f"hahn item {i}: max |dL| {e:.2e}, mixed steps {st[i, 0]}/{st[i, 4]}, fp64 steps {st64[i, 0]}",
+              flush=True)
+        assert e <= 1e-6
+    alone = O.EchoBatch.from_items(ctx, [cp], m=30, tol=1e-3, precision="mixed")
+    alone.run()
+    La, _ = alone.results()
+    assert np.abs(Lm[2] - La[0]).max() <= 1e-12

@@ -115,3 +115,65 @@ def test_slot_kernel_cpmg_matches_reference_propagator(ctx, k, npulse):
         if k == 3:   # n = 64: the oracle's dense propagator per output time is cheap
             ref = _oracle_cpmg(c, npulse, tt)
             assert np.abs(res["mixed"][0][i] - ref).max() < 1e-6
+
+
+@pytest.mark.gpu
+def test_slot_hahn_outputs_inside_steps(ctx):
+    """Mixed Hahn sweeps evaluate every output time that falls inside a Krylov
+    step from that step's basis (sweep A stores its states, sweep B takes the
+    inner products).  On a grid finer than the natural step they take fewer
+    steps than output times, while the FP64 path (the reference's schedule,
+    zgexpv_cache.f:440-661) ends a step at every output time; L(t) agrees to
+    1e-6 on fine, irregular, repeated, single and offset grids."""
+    rng = np.random.default_rng(11)
+    coords = _clusters(4, 4)
+    irregular = np.sort(np.concatenate([[0.0, 0.0], rng.uniform(0.0, 1.25e-3, 58)]))
+    irregular[30] = irregular[29]
+    grids = [np.linspace(0.0, 2.5e-3, 801) / 2, irregular, np.array([7e-4]), np.linspace(1e-4, 1.2e-3, 37)]
+    items = []
+    for c, g in zip(coords, grids):
+        it = _item(c, 1, g)
+        items.append(it)
+    print("slot hahn grids", [len(g) for g in grids], flush=True)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        b = O.EchoBatch.from_items(ctx, items, m=30, tol=1e-12 if prec == "fp64" else 1e-3, precision=prec)
+        b.run()
+        L, st = b.results()
+        assert (st[:, 8] & 0xFF).max() == 0
+        res[prec] = (L, st, _kinds(b))
+    assert res["mixed"][2][256] == "lanczos_slot"
+    for i, g in enumerate(grids):
+        e = np.abs(np.asarray(res["mixed"][0][i]) - np.asarray(res["fp64"][0][i])).max()
+        print(f"grid {i} ({len(g)} times): max |dL| {e:.3e}, mixed steps A/B {res['mixed'][1][i, 0]}/"
+              f"{res['mixed'][1][i, 4]}, fp64 steps A {res['fp64'][1][i, 0]}", flush=True)
+        assert e <= 1e-6
+    # the fine grid: 800 intervals of 1.56 us; the FP64 schedule takes at least
+    # one step per interval, the mixed sweeps far fewer
+    assert res["fp64"][1][0, 0] >= 800
+    assert res["mixed"][1][0, 0] < 600 and res["mixed"][1][0, 4] < 600
+
+
+@pytest.mark.gpu
+def test_slot_hahn_rerun_with_longer_grid(ctx):
+    """A uniform-time batch run again with more output times grows its sweep
+    state store and still matches the FP64 path."""
+    X = O.bath_diamond(3.567, 30.0, 0.011, seed=5)
+    C = O.Clusters(X, 8.0, 4)
+    par = O.cce_params(B=B111)
+    sel = _sel(C, 4, 24)
+    b = C.hahn_batch(ctx, par, m=30, tol=1e-3, precision="mixed", select=sel)
+    b.run(np.linspace(0.0, 1e-3, 51) / 2)
+    tau = np.linspace(0.0, 2.5e-3, 251) / 2
+    b.run(tau)
+    L, st = b.results()
+    assert (st[:, 8] & 0xFF).max() == 0
+    assert _kinds(b)[256] == "lanczos_slot"
+    b.close()
+    r = C.hahn_batch(ctx, par, m=30, tol=1e-12, precision="fp64", select=sel)
+    r.run(tau)
+    L64, _ = r.results()
+    r.close()
+    e = np.abs(L - L64).max()
+    print(f"rerun max |dL| {e:.3e}", flush=True)
+    assert e <= 1e-6
