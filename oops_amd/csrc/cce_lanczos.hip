@@ -558,11 +558,14 @@ __device__ __forceinline__ void advance_slot(LzSlotSmem<M>& S, BlockRed<kNT>& br
         const cd wnew = combine();
         const double t1 = t_step >= t_end - g.t_now ? t_end : g.t_now + t_step;
         if (o.mode) {
-            // output times inside the step, from the same basis
-            for (; k < o.nt && o.tau[k] < t1; ++k) {
+            // output times inside the step, from the same basis: F walks from
+            // e_1 at t_now to each output time in turn (one interval each)
+            double tp = g.t_now;
+            for (int q = 0; k < o.nt && o.tau[k] < t1; ++k, ++q) {
                 __syncthreads();   // every thread has read F
                 ++g.nexph;
-                g.nscale += cheb_coeffs<M>(S, mb, k1, o.tau[k] - g.t_now, bprev);
+                g.nscale += cheb_coeffs<M>(S, mb, k1, o.tau[k] - tp, bprev, q > 0);
+                tp = o.tau[k];
                 emit_state<M>(br, o, k, combine());
             }
         }

@@ -340,11 +340,14 @@ __device__ void med_advance(MedSmem<M, E>& S, BlockRed<kMT>& br, MedState& g, cd
         const double t1 = t_step >= t_out - g.t_now ? t_out : g.t_now + t_step;
         combine(p);   // the state at the end of the step
         if (ob) {
-            // output times inside the step, from the same basis (vcur is free here)
-            for (; k < ont && otau[k] < t1; ++k) {
+            // output times inside the step, from the same basis (vcur is free
+            // here): F walks from e_1 at t_now to each output time in turn
+            double tp = g.t_now;
+            for (int q = 0; k < ont && otau[k] < t1; ++k, ++q) {
                 __syncthreads();   // every thread has read F
                 ++g.nexph;
-                g.nscale += cheb_coeffs<M>(S, mb, k1, otau[k] - g.t_now, bprev);
+                g.nscale += cheb_coeffs<M>(S, mb, k1, otau[k] - tp, bprev, q > 0);
+                tp = otau[k];
                 combine(vcur);
                 med_emit<E>(ob, n, k, vcur);
             }

@@ -36,8 +36,10 @@ __device__ __forceinline__ double wave_shl1(double v) {
 // lanes (J_k in lane k of `bl`, J_{64+k} in lane k of `bh`) and is read with
 // readlane; the series stops at the first k > x with |J_k| < 1e-18 (the tail
 // decays faster than geometrically there).  Returns the number of terms.
+// from_f: start from the vector in F (exp(tau H_aug) F, e.g. to walk a step's
+// output times one interval at a time) instead of e_1.
 template <int M, typename SM>
-__device__ int cheb_coeffs(SM& S, int mb, int k1, double tau, double beta_m) {
+__device__ int cheb_coeffs(SM& S, int mb, int k1, double tau, double beta_m, bool from_f = false) {
     static_assert(M + 2 <= kWave, "one lane per row of the augmented matrix");
     const int tid = threadIdx.x;
     int terms = 0;
@@ -147,6 +149,10 @@ __device__ int cheb_coeffs(SM& S, int mb, int k1, double tau, double beta_m) {
         };
         const double sgn = imh ? 1.0 : -1.0;
         double v = lane == 0 ? 1.0 : 0.0;          // e1 (real part of row 0)
+        if (from_f) {   // this lane's component of F (row M+1 carried as i x_{M+1})
+            const cd f = j < rows ? S.F[j] : mkcd(0.0, 0.0);
+            v = (k1 && j == M + 1) ? (imh ? f.re : -f.im) : (imh ? f.im : f.re);
+        }
         const cd ph = ool_expi(c * (tau / ns));    // e^{i c tau / ns}
         for (int sub = 0; sub < ns; ++sub) {
             // sum_k c_k i^k T_k v = E + i O with E the even, O the odd terms

@@ -177,3 +177,32 @@ def test_slot_hahn_rerun_with_longer_grid(ctx):
     e = np.abs(L - L64).max()
     print(f"rerun max |dL| {e:.3e}", flush=True)
     assert e <= 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("field", [0.01, 0.45])
+def test_slot_hahn_low_and_high_field(ctx, field):
+    """At 0.01 T one Krylov step spans many output times; at 0.45 T the steps
+    are shorter than the output spacing and clusters far from the centre reach
+    a happy breakdown (the step then covers every remaining output time).
+    Both must stay within 1e-6 of the FP64 path."""
+    X = O.bath_diamond(3.567, 40.0, 0.011, seed=8)
+    C = O.Clusters(X, 8.0, 4)
+    par = O.cce_params(B=np.ones(3) * field / np.sqrt(3.0))
+    sel = _sel(C, 4, 32)
+    tau = np.linspace(0.0, 2e-3, 151) / 2
+    out = {}
+    for prec in ("fp64", "mixed"):
+        b = C.hahn_batch(ctx, par, m=30, tol=1e-12 if prec == "fp64" else 1e-3, precision=prec, select=sel)
+        b.run(tau)
+        L, st = b.results()
+        assert (st[:, 8] & 0xFF).max() == 0
+        out[prec] = (L, st, _kinds(b))
+        b.close()
+    assert out["mixed"][2][256] == "lanczos_slot"
+    st = out["mixed"][1]
+    e = np.abs(out["mixed"][0] - out["fp64"][0]).max()
+    print(f"B={field}: max |dL| {e:.3e}, mixed steps/sweep {st[:, 0].mean():.1f}, breakdowns "
+          f"{int(((st[:, 8] >> 8) & 3).astype(bool).sum())}, fp64 steps/sweep {out['fp64'][1][:, 0].mean():.1f}",
+          flush=True)
+    assert e <= 1e-6
