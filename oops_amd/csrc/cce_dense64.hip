@@ -8,6 +8,10 @@
 //                                          B for odd j, applied once at the
 //                                          ends (fraction 1/(2N)), twice inside)
 //   L(t_k) = Re(scale <y0, s(t_k)>)
+// A Hahn echo (N = 1) needs no powers: L(t_k) = Re(scale <b_k, a_k>) with
+// a_k = U_A a_{k-1}, b_k = exp(-i opB dt) b_{k-1} (opB = -lvB^dagger, so
+// exp(-i opB dt) = U_B^H), one 64 x 64 matrix-vector product per side and
+// output time; the powers are formed for CPMG-N, whose segments interleave.
 // All complex products are FP64 MFMA (v_mfma_f64_16x16x4f64): complex
 // C = A B as four real products, one 16x16 output tile per (row block,
 // column block), 16 rows per wave.  The exponential is a degree-18 Taylor
@@ -39,6 +43,7 @@ constexpr int64_t kWsDoubles = kWsMats * kPlanes * kDD;
 struct Dense64Smem {
     double pst[2][4][2][kD * 16];  // [A/B][wave][re/im][k*16 + i]: rows 16w+i of P_g, column k
     double s[2][2][kD];            // [ping-pong][re/im]
+    double hb[2][2][kD];           // Hahn echo: the bra-side state [ping-pong][re/im]
     double y0[2][kD];
     OpLds op;
     double red[8];
@@ -219,6 +224,10 @@ __global__ __launch_bounds__(kNT) void cce_dense64_kernel(KrylovArgs a, double* 
         // half of the sequence lasts tau_k = k dt; CPMG-N: times are t_k and the
         // end segments last t_k / (2N).
         const double delta = N == 1 ? dt : dt / (2.0 * N);
+        // Hahn echo: L(t_k) = Re(scale <(U_B^k)^H y0, U_A^k u0>) with
+        // (U_B)^H = exp(-i opB dt) (opB = -lvB^dagger), so both sides advance by
+        // one matrix-vector product per output time instead of a power
+        const bool hahn_vec = N == 1;
         int fail = 0;
         // exponentials of lvA and lvB (Hahn items carry -lvB^dagger: lvB = -(opB)^H)
         for (int g = 0; g < 2; ++g) {
@@ -227,7 +236,7 @@ __global__ __launch_bounds__(kNT) void cce_dense64_kernel(KrylovArgs a, double* 
             __syncthreads();
             const int sq = dense_expm(S, delta, ws, g == 0 ? UAr : UBr, g == 0 ? UAi : UBi);
             if (sq < 0) fail = 2;
-            if (g == 1 && N == 1) {
+            if (g == 1 && N == 1 && !hahn_vec) {
                 // exp(-i lvB d) with lvB = -(opB)^H equals exp(-i (-opB) d)^H ... computed
                 // here as exp(+i opB^H d): conjugate-transpose of exp(-i opB d).
                 for (int idx = tid; idx < kDD; idx += kNT) {
@@ -244,6 +253,73 @@ __global__ __launch_bounds__(kNT) void cce_dense64_kernel(KrylovArgs a, double* 
             }
             __syncthreads();
         }
+        const double scale = a.scale[item];
+        if (hahn_vec) {
+            // U_A and exp(-i opB dt) into the power slots (rows 16w+i, column k)
+            for (int g = 0; g < 2; ++g) {
+                const double* ur = g == 0 ? UAr : UBr;
+                const double* ui = g == 0 ? UAi : UBi;
+                for (int idx = l; idx < kD * 16; idx += kWave) {
+                    const int k = idx / 16, i = idx % 16;
+                    S.pst[g][w][0][idx] = ur[(16 * w + i) * kD + k];
+                    S.pst[g][w][1][idx] = ui[(16 * w + i) * kD + k];
+                }
+            }
+            if (tid < kD) {
+                S.s[0][0][tid] = a.u0[voff + tid].x;
+                S.s[0][1][tid] = a.u0[voff + tid].y;
+                S.hb[0][0][tid] = a.y0[voff + tid].x;
+                S.hb[0][1][tid] = a.y0[voff + tid].y;
+            }
+            __syncthreads();
+            int cur = 0;
+            for (int k = 0; k < q.nt && !fail; ++k) {
+                if (k > 0) {
+                    // ket s <- U_A s and bra hb <- exp(-i opB dt) hb, rows 16w..16w+15
+                    const int i = l & 15, kb = (l >> 4) * 16;
+                    double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+#pragma unroll 4
+                    for (int kk = kb; kk < kb + 16; ++kk) {
+                        const double mr = S.pst[0][w][0][kk * 16 + i], mi = S.pst[0][w][1][kk * 16 + i];
+                        const double xr = S.s[cur][0][kk], xi = S.s[cur][1][kk];
+                        ar = fma(mr, xr, fma(-mi, xi, ar));
+                        ai = fma(mr, xi, fma(mi, xr, ai));
+                        const double nr = S.pst[1][w][0][kk * 16 + i], ni = S.pst[1][w][1][kk * 16 + i];
+                        const double yr = S.hb[cur][0][kk], yi = S.hb[cur][1][kk];
+                        br = fma(nr, yr, fma(-ni, yi, br));
+                        bi = fma(nr, yi, fma(ni, yr, bi));
+                    }
+#pragma unroll
+                    for (int m = 16; m <= 32; m <<= 1) {
+                        ar += __shfl_xor(ar, m, kWave);
+                        ai += __shfl_xor(ai, m, kWave);
+                        br += __shfl_xor(br, m, kWave);
+                        bi += __shfl_xor(bi, m, kWave);
+                    }
+                    if (l < 16) {
+                        S.s[cur ^ 1][0][16 * w + i] = ar;
+                        S.s[cur ^ 1][1][16 * w + i] = ai;
+                        S.hb[cur ^ 1][0][16 * w + i] = br;
+                        S.hb[cur ^ 1][1][16 * w + i] = bi;
+                    }
+                    __syncthreads();
+                    cur ^= 1;
+                }
+                if (tid < kWave) {
+                    double v = fma(S.hb[cur][0][tid], S.s[cur][0][tid], S.hb[cur][1][tid] * S.s[cur][1][tid]);
+#pragma unroll
+                    for (int mask = 32; mask >= 1; mask >>= 1) v += __shfl_xor(v, mask, kWave);
+                    if (tid == 0) q.L[k] = scale * v;
+                }
+            }
+            if (tid == 0) {
+                int32_t* st = a.stats + (int64_t)item * kStatsPerItem;
+                for (int z = 0; z < kStatsPerItem; ++z) st[z] = 0;
+                st[8] = fail;
+            }
+            __syncthreads();
+            continue;
+        }
         // P_A = P_B = I (each wave its 16 rows), s = u0, y0
         for (int g = 0; g < 2; ++g)
             for (int idx = l; idx < kD * 16; idx += kWave) {
@@ -256,7 +332,6 @@ __global__ __launch_bounds__(kNT) void cce_dense64_kernel(KrylovArgs a, double* 
             S.y0[1][tid] = a.y0[voff + tid].y;
         }
         __syncthreads();
-        const double scale = a.scale[item];
         for (int k = 0; k < q.nt && !fail; ++k) {
             if (k > 0) {
                 // P_g <- P_g U_g for both generators (rows of this wave only)

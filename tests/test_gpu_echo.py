@@ -77,6 +77,25 @@ def test_dense64_mfma_path_matches_reference_propagator(ctx, npulse):
 
 
 @pytest.mark.gpu
+def test_dense64_hahn_long_grid_matches_reference_propagator(ctx):
+    """Hahn echo on the n = 64 path advances each side by one matrix-vector
+    product per output time; over the 401-point 0-5 ms grid of the CCE-4
+    workload it stays within 1e-10 of the oracle's powers of the step
+    propagator (PiecewiseFullMatrixVectorEvolution)."""
+    t = np.linspace(0.0, 5e-3, 401)
+    for coords in _clusters(3, 2):
+        it = _item(coords, 1, t / 2)
+        b = O.EchoBatch.from_items(ctx, [it], m=30, precision="fp64")
+        b.run()
+        L, st = b.results()
+        assert b.timing()[0]["kernel"] == "dense64_mfma"
+        assert (st[:, 8] & 0xFF).max() == 0
+        e = np.abs(L[0] - _oracle_cpmg(coords, 1, t)).max()
+        print(f"dense64 hahn 401 points: max |dL| {e:.2e}", flush=True)
+        assert e < 1e-10
+
+
+@pytest.mark.gpu
 def test_nonuniform_grid_n64_uses_krylov_and_matches_fp64(ctx):
     t = np.sort(np.concatenate([[0.0], np.random.default_rng(1).uniform(0, 5e-4, 12)]))
     for coords in _clusters(3, 2):
