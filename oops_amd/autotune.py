@@ -8,7 +8,7 @@ tol = 1e-12, the reference's settings).  The tuning sample is a CCE-4
 enumeration of a bath drawn from a different seed than the benchmark's, so
 the benchmark checks the choice on data it was not tuned on.
 
-    python -m oops_amd.autotune [--per-order 128] [--out oops_amd/tuning.json]
+    python -m oops_amd.autotune [--per-order 512] [--out oops_amd/tuning.json]
 """
 import argparse
 import json
@@ -44,11 +44,11 @@ def main(argv=None):
     ap.add_argument("--field", type=float, default=0.05)
     ap.add_argument("--t-end", dest="t_end", type=float, default=5e-3)
     ap.add_argument("--nt", type=int, default=401)
-    ap.add_argument("--per-order", dest="per", type=int, default=128)
+    ap.add_argument("--per-order", dest="per", type=int, default=512)
     ap.add_argument("--budget", type=float, default=1e-6, help="max |L_mixed - L_fp64| allowed")
     ap.add_argument("--safety", type=float, default=2.0)
     ap.add_argument("--ms", default="20,30")
-    ap.add_argument("--tols", default="1e-3,3e-4,1e-4,1e-5,1e-6")
+    ap.add_argument("--tols", default="1e-2,3e-3,1e-3,3e-4,1e-4,1e-5")
     ap.add_argument("--out", default=os.path.join(HERE, "tuning.json"))
     a = ap.parse_args(argv)
 

@@ -519,10 +519,40 @@ __device__ __forceinline__ void advance_slot(LzSlotSmem<M>& S, BlockRed<kNT>& br
         }
         __syncthreads();
         OOPS_TIC(te0);
-        double err_loc = 0.0;
+        // w = beta * (sum_{j<mb} F_j v_j + [k1] F_M v_M)
+        const int mx = mb + (k1 - 1 > 0 ? k1 - 1 : 0);
+        auto combine = [&]() -> cd {
+            cd acc = cmul(v0, S.F[0]);
+#pragma unroll 1
+            for (int j = 1; j < mx; ++j) {
+                const float2 f = S.V[j - 1][tid];
+                acc = cfma(mkcd((double)f.x, (double)f.y), S.F[j], acc);
+            }
+            return cscale(acc, g.beta);
+        };
+        // Trial step: F walks from e_1 through the output times inside the
+        // step (each evaluated and emitted from this basis) to the step end,
+        // where Expokit's error estimate decides.  After a rejection the walk
+        // restarts for the shorter step; outputs emitted beyond its end are
+        // emitted again by the steps that reach them.
+        double err_loc = 0.0, t1 = t_end;
+        int kend = k;
         for (;;) {
+            t1 = t_step >= t_end - g.t_now ? t_end : g.t_now + t_step;
+            double tp = g.t_now;
+            int q = 0;
+            if (o.mode) {
+                for (kend = k; kend < o.nt && o.tau[kend] < t1; ++kend, ++q) {
+                    if (q > 0) __syncthreads();   // every thread has read F
+                    ++g.nexph;
+                    g.nscale += cheb_coeffs<M>(S, mb, k1, o.tau[kend] - tp, bprev, q > 0);
+                    tp = o.tau[kend];
+                    emit_state<M>(br, o, kend, combine());
+                }
+                if (q > 0) __syncthreads();
+            }
             ++g.nexph;
-            g.nscale += cheb_coeffs<M>(S, mb, k1, t_step, bprev);
+            g.nscale += cheb_coeffs<M>(S, mb, k1, t1 - tp, bprev, q > 0);
             if (k1 == 0) {
                 err_loc = tol;
             } else {
@@ -542,33 +572,10 @@ __device__ __forceinline__ void advance_slot(LzSlotSmem<M>& S, BlockRed<kNT>& br
             }
             break;
         }
+        if (o.mode) k = kend;
         OOPS_TOC(g, texp, te0);
         OOPS_TIC(tu0);
-        // w = beta * (sum_{j<mb} F_j v_j + [k1] F_M v_M)
-        const int mx = mb + (k1 - 1 > 0 ? k1 - 1 : 0);
-        auto combine = [&]() -> cd {
-            cd acc = cmul(v0, S.F[0]);
-#pragma unroll 1
-            for (int j = 1; j < mx; ++j) {
-                const float2 f = S.V[j - 1][tid];
-                acc = cfma(mkcd((double)f.x, (double)f.y), S.F[j], acc);
-            }
-            return cscale(acc, g.beta);
-        };
         const cd wnew = combine();
-        const double t1 = t_step >= t_end - g.t_now ? t_end : g.t_now + t_step;
-        if (o.mode) {
-            // output times inside the step, from the same basis: F walks from
-            // e_1 at t_now to each output time in turn (one interval each)
-            double tp = g.t_now;
-            for (int q = 0; k < o.nt && o.tau[k] < t1; ++k, ++q) {
-                __syncthreads();   // every thread has read F
-                ++g.nexph;
-                g.nscale += cheb_coeffs<M>(S, mb, k1, o.tau[k] - tp, bprev, q > 0);
-                tp = o.tau[k];
-                emit_state<M>(br, o, k, combine());
-            }
-        }
         g.w = wnew;
         {
             const auto tb = block_sum_lanes<1, kNT>(br, [&](int) { return tid < g.n ? cabs2(g.w) : 0.0; });
