@@ -299,10 +299,45 @@ __device__ void med_advance(MedSmem<M, E>& S, BlockRed<kMT>& br, MedState& g, cd
             avnorm = sqrt(tn.get(0));
         }
         __syncthreads();
-        double err_loc = 0.0;
+        // w = beta * (sum_{j<mb} F_j v_j + [k1] F_M v_M)
+        const int mx = mb + (k1 - 1 > 0 ? k1 - 1 : 0);
+        auto combine = [&](cd (&x)[E]) {
+            const cd F0 = S.F[0];
+#pragma unroll
+            for (int r = 0; r < E; ++r) {
+                cd acc = cmul(v0[r], F0);
+#pragma unroll 1
+                for (int j = 1; j < mx; ++j) {
+                    const float2 f = Vg[(int64_t)(j - 1) * n + tid + kMT * r];
+                    acc = cfma(mkcd((double)f.x, (double)f.y), S.F[j], acc);
+                }
+                x[r] = cscale(acc, g.beta);
+            }
+        };
+        // Trial step: F walks from e_1 through the output times inside the
+        // step (each evaluated from this basis and stored; vcur is free here)
+        // to the step end, where Expokit's error estimate decides.  After a
+        // rejection the walk restarts for the shorter step; states stored
+        // beyond its end are stored again by the steps that reach them.
+        double err_loc = 0.0, t1 = t_out;
+        int kend = k;
         for (;;) {
+            t1 = t_step >= t_out - g.t_now ? t_out : g.t_now + t_step;
+            double tp = g.t_now;
+            int q = 0;
+            if (ob) {
+                for (kend = k; kend < ont && otau[kend] < t1; ++kend, ++q) {
+                    if (q > 0) __syncthreads();   // every thread has read F
+                    ++g.nexph;
+                    g.nscale += cheb_coeffs<M>(S, mb, k1, otau[kend] - tp, bprev, q > 0);
+                    tp = otau[kend];
+                    combine(vcur);
+                    med_emit<E>(ob, n, kend, vcur);
+                }
+                if (q > 0) __syncthreads();
+            }
             ++g.nexph;
-            g.nscale += cheb_coeffs<M>(S, mb, k1, t_step, bprev);
+            g.nscale += cheb_coeffs<M>(S, mb, k1, t1 - tp, bprev, q > 0);
             if (k1 == 0) {
                 err_loc = tol;
             } else {
@@ -322,36 +357,8 @@ __device__ void med_advance(MedSmem<M, E>& S, BlockRed<kMT>& br, MedState& g, cd
             }
             break;
         }
-        // w = beta * (sum_{j<mb} F_j v_j + [k1] F_M v_M)
-        const int mx = mb + (k1 - 1 > 0 ? k1 - 1 : 0);
-        auto combine = [&](cd (&x)[E]) {
-            const cd F0 = S.F[0];
-#pragma unroll
-            for (int r = 0; r < E; ++r) {
-                cd acc = cmul(v0[r], F0);
-#pragma unroll 1
-                for (int j = 1; j < mx; ++j) {
-                    const float2 f = Vg[(int64_t)(j - 1) * n + tid + kMT * r];
-                    acc = cfma(mkcd((double)f.x, (double)f.y), S.F[j], acc);
-                }
-                x[r] = cscale(acc, g.beta);
-            }
-        };
-        const double t1 = t_step >= t_out - g.t_now ? t_out : g.t_now + t_step;
+        if (ob) k = kend;
         combine(p);   // the state at the end of the step
-        if (ob) {
-            // output times inside the step, from the same basis (vcur is free
-            // here): F walks from e_1 at t_now to each output time in turn
-            double tp = g.t_now;
-            for (int q = 0; k < ont && otau[k] < t1; ++k, ++q) {
-                __syncthreads();   // every thread has read F
-                ++g.nexph;
-                g.nscale += cheb_coeffs<M>(S, mb, k1, otau[k] - tp, bprev, q > 0);
-                tp = otau[k];
-                combine(vcur);
-                med_emit<E>(ob, n, k, vcur);
-            }
-        }
 #pragma unroll
         for (int r = 0; r < E; ++r) w[r] = p[r];
         {
