@@ -113,9 +113,15 @@ int oops_hamvec(oops_ctx* ctx, const oops_term_table* H, const double* v, double
 int oops_expv(oops_ctx* ctx, const oops_term_table* H, const double* v, int32_t nt, const double* ta, int32_t klim,
               int32_t m, double tol, int32_t out_mode, double* out, double* stats);
 
-/* oops_expv with a precision choice: 0 FP64 throughout (= oops_expv), 1 the
- * basis stored in complex FP32 (half the HBM traffic of the Gram-Schmidt
- * passes; products, sums and the small exponential stay FP64). */
+/* oops_expv with options in `precision`.  Bit 0: the basis stored in complex
+ * FP32 (half the HBM traffic of the Gram-Schmidt passes; products, sums and
+ * the small exponential stay FP64); clear: FP64 throughout (0 = oops_expv).
+ * Bit 1: output times inside steps.  The steps run to min(t_new, ta[nt-1])
+ * under Expokit's error control instead of ending at every output time
+ * (zgexpv_cache.f:418-440), and every output time inside a step is evaluated
+ * from that step's basis.  Fewer Krylov cycles on grids finer than the
+ * natural step; results agree with the reference's schedule to the
+ * tolerance, not step for step.  Other bits: OOPS_ERR_ARG. */
 int oops_expv_ex(oops_ctx* ctx, const oops_term_table* H, const double* v, int32_t nt, const double* ta,
                  int32_t klim, int32_t m, double tol, int32_t precision, int32_t out_mode, double* out,
                  double* stats);

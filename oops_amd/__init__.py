@@ -258,11 +258,14 @@ def hamvec(ctx: Context, H: TermTable, v):
     return w
 
 
-def expv(ctx: Context, H: TermTable, v, ta, m=30, klim=10, tol=1e-12, overlaps=False, precision="fp64"):
+def expv(ctx: Context, H: TermTable, v, ta, m=30, klim=10, tol=1e-12, overlaps=False, precision="fp64",
+         steps="reference"):
     """exp(-i H t) v at every time of ta (MatExpVector::runInexplicitGPU,
     MatExp.cpp:367-521).  Returns (w_seq [n x nt], stats) or, with
     overlaps=True, (<v, w(t_k)> [nt], stats).  precision "fp32" stores the
-    Krylov basis in complex FP32."""
+    Krylov basis in complex FP32.  steps "reference": a step ends at every
+    output time (Expokit's schedule); "inside": steps run past output times
+    and evaluate the ones they contain from their own basis."""
     v = np.ascontiguousarray(v, np.complex128)
     ta = np.ascontiguousarray(ta, np.float64)
     if v.shape != (H.n,):
@@ -271,9 +274,12 @@ def expv(ctx: Context, H: TermTable, v, ta, m=30, klim=10, tol=1e-12, overlaps=F
     st = np.zeros(8)
     if precision not in ("fp64", "fp32"):
         raise ValueError("precision must be fp64 or fp32")
+    if steps not in ("reference", "inside"):
+        raise ValueError("steps must be reference or inside")
+    flags = (1 if precision == "fp32" else 0) | (2 if steps == "inside" else 0)
     rc = lib().oops_expv_ex(ctx.h, ctypes.byref(H.c), _ptr(v), ctypes.c_int32(len(ta)), _ptr(ta),
                             ctypes.c_int32(klim), ctypes.c_int32(m), ctypes.c_double(tol),
-                            ctypes.c_int32(1 if precision == "fp32" else 0), ctypes.c_int32(1 if overlaps else 0),
+                            ctypes.c_int32(flags), ctypes.c_int32(1 if overlaps else 0),
                             _ptr(out), _ptr(st))
     _check(rc, ctx.h, "oops_expv")
     keys = ["nmult", "nexph", "nscale", "nstep", "nreject", "ibrkflag", "anorm", "kernel_ms"]

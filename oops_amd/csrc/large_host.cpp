@@ -56,6 +56,7 @@ struct LargeRun {
     int64_t n = 0;
     int m = 0, ldh = 0, grid = 0, nblk = 0;
     DevBuf V, p0, p1, w, v0, part, hess, coef, scal, flags, F, status, wpart;
+    DevBuf wk, wkpart;   // a state at an output time inside a step (interior-output mode)
     // tiled product (large Kronecker operators): one entry per pass
     struct Tile {
         DevBuf hdr, loc, val, off, map;
@@ -428,8 +429,13 @@ struct Counters {
 // The Expokit integration.  out_mode 0: w_seq (n x nt) to the host;
 // 1: overlaps <v0, w(t_k)> to the host (2 doubles per time); 2: w(t_k) into
 // the device array dev (n x nt); 3: overlaps <w(t_k), dev(:, k)> to the host.
+// interior: the steps are not cut at the output times (they run to
+// min(t_new, last output time) under Expokit's error control) and every output
+// time inside a step is evaluated from that step's basis (one small
+// exponential and one combination each).  Off: the reference's schedule, a
+// step ends at every output time (zgexpv_cache.f:418-440).
 int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int klim, double tol, int out_mode,
-             double* out, Counters& C, double2* dev = nullptr) {
+             double* out, Counters& C, double2* dev = nullptr, bool interior = false) {
     const int m = R.m;
     const int64_t n = R.n;
     const double delta = 1.2, gamma = 0.9, break_tol = 1.0e-7;
@@ -459,9 +465,40 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
     hipEventCreate(&e0);
     hipEventCreate(&e1);
     float ms_acc = 0.0f;
-    for (int tt = 0; tt < nt; ++tt) {
-        const double t_out = ta[tt];
-        while (t_now < t_out) {
+    if (interior) {
+        rc = alloc(R.ctx, R.wk, sizeof(double2) * (size_t)n);
+        rc = rc ? rc : alloc(R.ctx, R.wkpart, sizeof(double) * (size_t)R.grid);
+        if (rc) return rc;
+    }
+    // the output action at time index tt on the state wv
+    auto emit = [&](int tt, const double2* wv) -> int {
+        int e = OOPS_OK;
+        if (out_mode == 2) {
+            e = hip_check(R.ctx, hipMemcpyAsync(dev + (size_t)n * tt, wv, sizeof(double2) * n, hipMemcpyDeviceToDevice,
+                                                R.s), "D2D w");
+        } else if (out_mode == 3) {
+            cplx ov;
+            e = dev_dot(R, wv, dev + (size_t)n * tt, ov);
+            out[2 * tt] = ov.real();
+            out[2 * tt + 1] = ov.imag();
+        } else if (out) {
+            if (out_mode == 0) {
+                e = hip_check(R.ctx, hipMemcpyAsync(out + 2 * (size_t)n * tt, wv, sizeof(double2) * n,
+                                                    hipMemcpyDeviceToHost, R.s), "D2H w");
+                e = e ? e : hip_check(R.ctx, hipStreamSynchronize(R.s), "sync w");
+            } else {
+                cplx ov;
+                e = dev_dot(R, (const double2*)R.v0.p, wv, ov);
+                out[2 * tt] = ov.real();
+                out[2 * tt + 1] = ov.imag();
+            }
+        }
+        return e;
+    };
+    int tt = 0;   // next output time
+    // one Krylov step from t_now toward t_out
+    auto step = [&](double t_out) -> int {
+        {
             if (C.nstep >= max_steps) return set_err(R.ctx, OOPS_ERR_NUMERIC, "step limit reached");
             ++C.nstep;
             double t_step = std::min(t_out - t_now, t_new);
@@ -560,34 +597,57 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
             float ms = 0.f;
             hipEventElapsedTime(&ms, e0, e1);
             ms_acc += ms;
+            if (interior && tt < nt && ta[tt] < (t_step >= t_out - t_now ? t_out : t_now + t_step)) {
+                // output times inside the step, from the same basis (their
+                // exponentials, combinations and overlaps count as device time)
+                const double t1 = t_step >= t_out - t_now ? t_out : t_now + t_step;
+                hipEventRecord(e0, R.s);
+                for (; tt < nt && ta[tt] < t1; ++tt) {
+                    ++C.nexph;
+                    if (launch_large_pade(hess, R.ldh, mb + k1, ta[tt] - t_now, (double2*)R.F.p, (int*)R.status.p,
+                                          R.s) != 0)
+                        return set_err(R.ctx, OOPS_ERR_HIP, "Pade kernel launch");
+                    launch_large_combine(R.V.p, n, mx, (double2*)R.F.p, beta, (double2*)R.wk.p, n,
+                                         (double*)R.wkpart.p, R.grid, R.s, R.fp32);
+                    int st = 0;
+                    rc = hip_check(R.ctx, hipMemcpyAsync(&st, R.status.p, sizeof(int), hipMemcpyDeviceToHost, R.s),
+                                   "D2H st");
+                    rc = rc ? rc : emit(tt, (const double2*)R.wk.p);
+                    rc = rc ? rc : hip_check(R.ctx, hipStreamSynchronize(R.s), "sync interior");
+                    if (rc) return rc;
+                    if (st < 0) return set_err(R.ctx, OOPS_ERR_NUMERIC, "small exponential failed");
+                    C.nscale += st;
+                }
+                hipEventRecord(e1, R.s);
+                rc = hip_check(R.ctx, hipEventSynchronize(e1), "sync interior");
+                if (rc) return rc;
+                float mi = 0.f;
+                hipEventElapsedTime(&mi, e0, e1);
+                ms_acc += mi;
+            }
             double s2 = 0.0;
             for (double v : wp) s2 += v;
             beta = std::sqrt(s2);
             t_new = round_step(gamma * t_step * std::pow(t_step * tol / err_loc, xm));
-            t_now += t_step;
+            if (interior && t_step >= t_out - t_now) t_now = t_out;
+            else t_now += t_step;
         }
-        if (out_mode == 2) {
-            rc = hip_check(R.ctx, hipMemcpyAsync(dev + (size_t)n * tt, R.w.p, sizeof(double2) * n,
-                                                 hipMemcpyDeviceToDevice, R.s), "D2D w");
-            if (rc) return rc;
-        } else if (out_mode == 3) {
-            cplx ov;
-            rc = dev_dot(R, (const double2*)R.w.p, dev + (size_t)n * tt, ov);
-            if (rc) return rc;
-            out[2 * tt] = ov.real();
-            out[2 * tt + 1] = ov.imag();
-        } else if (out) {
-            if (out_mode == 0) {
-                rc = hip_check(R.ctx, hipMemcpyAsync(out + 2 * (size_t)n * tt, R.w.p, sizeof(double2) * n,
-                                                     hipMemcpyDeviceToHost, R.s), "D2H w");
-                rc = rc ? rc : hip_check(R.ctx, hipStreamSynchronize(R.s), "sync w");
-            } else {
-                cplx ov;
-                rc = dev_dot(R, (const double2*)R.v0.p, (const double2*)R.w.p, ov);
-                out[2 * tt] = ov.real();
-                out[2 * tt + 1] = ov.imag();
-            }
-            if (rc) return rc;
+        return OOPS_OK;
+    };
+    if (interior) {
+        const double t_last = ta[nt - 1];
+        for (; tt < nt && ta[tt] <= t_now; ++tt)
+            if ((rc = emit(tt, (const double2*)R.w.p)) != 0) return rc;
+        while (t_now < t_last) {
+            if ((rc = step(t_last)) != 0) return rc;
+            for (; tt < nt && ta[tt] <= t_now; ++tt)
+                if ((rc = emit(tt, (const double2*)R.w.p)) != 0) return rc;
+        }
+    } else {
+        for (tt = 0; tt < nt; ++tt) {
+            while (t_now < ta[tt])
+                if ((rc = step(ta[tt])) != 0) return rc;
+            if ((rc = emit(tt, (const double2*)R.w.p)) != 0) return rc;
         }
     }
     hipEventDestroy(e0);
@@ -744,20 +804,21 @@ int oops_expv_ex(oops_ctx* ctx, const oops_term_table* H, const double* v, int32
                  int32_t klim, int32_t m, double tol, int32_t precision, int32_t out_mode, double* out,
                  double* stats) {
     if (!ctx || !H || !v || nt <= 0 || !ta) return set_err(ctx, OOPS_ERR_ARG, "bad arguments");
-    if (precision != 0 && precision != 1) return set_err(ctx, OOPS_ERR_ARG, "precision must be 0 (FP64) or 1 (FP32 basis)");
+    if (precision & ~3)
+        return set_err(ctx, OOPS_ERR_ARG, "precision: bit 0 FP32 basis, bit 1 output times inside steps; no other bits");
     if (out_mode != 0 && out_mode != 1) return set_err(ctx, OOPS_ERR_ARG, "out_mode must be 0 or 1");
     if (m < 2 || m > kLargeMaxM) return set_err(ctx, OOPS_ERR_UNSUPPORTED, "m must be in [2, 40] on the large path");
     if (klim < 1 || klim > 64) return set_err(ctx, OOPS_ERR_ARG, "klim must be in [1, 64]");
     for (int k = 1; k < nt; ++k)
         if (!(ta[k] >= ta[k - 1])) return set_err(ctx, OOPS_ERR_ARG, "output times must be ascending");
     LargeRun R;
-    R.fp32 = precision == 1;
+    R.fp32 = (precision & 1) != 0;
     int rc = setup(ctx, H, m, R);
     if (rc) return rc;
     if (m >= R.n) return set_err(ctx, OOPS_ERR_ARG, "the Krylov dimension must be smaller than the state dimension");
     hipStreamCreate(&R.s);
     Counters C;
-    rc = run_expv(R, v, nt, ta, klim, tol, out_mode, out, C);
+    rc = run_expv(R, v, nt, ta, klim, tol, out_mode, out, C, nullptr, (precision & 2) != 0);
     hipStreamDestroy(R.s);
     if (stats) {
         stats[0] = (double)C.nmult;
