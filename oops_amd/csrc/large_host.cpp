@@ -495,6 +495,41 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
         }
         return e;
     };
+    // one Arnoldi cycle: m products with CGS2 and the extra product of the
+    // error estimate, all on R.s
+    hipGraphExec_t arn_exec = nullptr;
+    bool arn_eager = false;
+    struct ExecGuard {
+        hipGraphExec_t* e;
+        ~ExecGuard() { if (*e) hipGraphExecDestroy(*e); }
+    } arn_guard{&arn_exec};
+    auto arnoldi_launches = [&]() {
+        double2* bufs[2] = {(double2*)R.p0.p, (double2*)R.p1.p};
+        const double2* x = (const double2*)R.w.p;
+        for (int j = 0; j < m; ++j) {
+            double2* p = bufs[j & 1];
+            matvec(R, x, scal, p, vcol(R, j), arnoldi_mode(R));
+            // CGS2: dots, update + dots, update + norm (one read of V per pass)
+            launch_large_gs_dots(R.V.p, n, p, n, j, (double*)R.part.p, R.nblk, R.s, R.fp32);
+            launch_large_finish((double*)R.part.p, R.nblk, j + 1, j, 0, hess, R.ldh, (double2*)R.coef.p, scal,
+                                (int*)R.flags.p, break_tol, R.s);
+            launch_large_gs_update(R.V.p, n, p, n, j, (double2*)R.coef.p, 0, (double*)R.part.p, R.nblk, R.s,
+                                   R.fp32);
+            launch_large_finish((double*)R.part.p, R.nblk, j + 1, j, 1, hess, R.ldh, (double2*)R.coef.p, scal,
+                                (int*)R.flags.p, break_tol, R.s);
+            launch_large_gs_update(R.V.p, n, p, n, j, (double2*)R.coef.p, 1, (double*)R.part.p, R.nblk, R.s,
+                                   R.fp32);
+            launch_large_finish((double*)R.part.p, R.nblk, 1, j, 2, hess, R.ldh, nullptr, scal, (int*)R.flags.p,
+                                break_tol, R.s);
+            x = p;
+        }
+        // extra product for the error estimate: V[m] = p / h(m, m-1), p' = -iH V[m]
+        double2* pe = bufs[m & 1];
+        matvec(R, x, scal, pe, vcol(R, m), arnoldi_mode(R));
+        launch_large_dots(pe, 0, pe, n, -1, 1, (double*)R.part.p, R.nblk, R.s);
+        launch_large_finish((double*)R.part.p, R.nblk, 1, 0, 3, hess, R.ldh, nullptr, scal, (int*)R.flags.p,
+                            break_tol, R.s);
+    };
     int tt = 0;   // next output time
     // one Krylov step from t_now toward t_out
     auto step = [&](double t_out) -> int {
@@ -508,31 +543,24 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
             (void)hipMemsetAsync(R.flags.p, 0, sizeof(int) * 4, R.s);
             const double inv_beta = 1.0 / beta;
             (void)hipMemcpyAsync(scal, &inv_beta, sizeof(double), hipMemcpyHostToDevice, R.s);
-            double2* bufs[2] = {(double2*)R.p0.p, (double2*)R.p1.p};
-            const double2* x = (const double2*)R.w.p;
-            for (int j = 0; j < m; ++j) {
-                double2* p = bufs[j & 1];
-                matvec(R, x, scal, p, vcol(R, j), arnoldi_mode(R));
-                // CGS2: dots, update + dots, update + norm (one read of V per pass)
-                launch_large_gs_dots(R.V.p, n, p, n, j, (double*)R.part.p, R.nblk, R.s, R.fp32);
-                launch_large_finish((double*)R.part.p, R.nblk, j + 1, j, 0, hess, R.ldh, (double2*)R.coef.p, scal,
-                                    (int*)R.flags.p, break_tol, R.s);
-                launch_large_gs_update(R.V.p, n, p, n, j, (double2*)R.coef.p, 0, (double*)R.part.p, R.nblk, R.s,
-                                       R.fp32);
-                launch_large_finish((double*)R.part.p, R.nblk, j + 1, j, 1, hess, R.ldh, (double2*)R.coef.p, scal,
-                                    (int*)R.flags.p, break_tol, R.s);
-                launch_large_gs_update(R.V.p, n, p, n, j, (double2*)R.coef.p, 1, (double*)R.part.p, R.nblk, R.s,
-                                       R.fp32);
-                launch_large_finish((double*)R.part.p, R.nblk, 1, j, 2, hess, R.ldh, nullptr, scal, (int*)R.flags.p,
-                                    break_tol, R.s);
-                x = p;
+            if (!arn_exec) {
+                // the Arnoldi cycle's launches are the same every step (every
+                // per-step value lives in device memory): capture them once
+                // and replay the graph (the memsets and the 1/beta copy above
+                // stay outside it); launch eagerly if capture fails
+                hipGraph_t g = nullptr;
+                if (hipStreamBeginCapture(R.s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+                    arnoldi_launches();
+                    if (hipStreamEndCapture(R.s, &g) == hipSuccess && g &&
+                        hipGraphInstantiate(&arn_exec, g, nullptr, nullptr, 0) != hipSuccess)
+                        arn_exec = nullptr;
+                    if (g) hipGraphDestroy(g);
+                }
+                (void)hipGetLastError();
+                if (!arn_exec) arn_eager = true;
             }
-            // extra product for the error estimate: V[m] = p / h(m, m-1), p' = -iH V[m]
-            double2* pe = bufs[m & 1];
-            matvec(R, x, scal, pe, vcol(R, m), arnoldi_mode(R));
-            launch_large_dots(pe, 0, pe, n, -1, 1, (double*)R.part.p, R.nblk, R.s);
-            launch_large_finish((double*)R.part.p, R.nblk, 1, 0, 3, hess, R.ldh, nullptr, scal, (int*)R.flags.p,
-                                break_tol, R.s);
+            if (arn_eager) arnoldi_launches();
+            else if ((rc = hip_check(R.ctx, hipGraphLaunch(arn_exec, R.s), "hipGraphLaunch arnoldi")) != 0) return rc;
             int fl[4];
             double sc[4];
             rc = hip_check(R.ctx, hipMemcpyAsync(fl, R.flags.p, sizeof(int) * 4, hipMemcpyDeviceToHost, R.s), "D2H");
