@@ -46,7 +46,7 @@ def main(argv=None):
     ap.add_argument("--nt", type=int, default=401)
     ap.add_argument("--per-order", dest="per", type=int, default=512)
     ap.add_argument("--budget", type=float, default=1e-6, help="max |L_mixed - L_fp64| allowed")
-    ap.add_argument("--safety", type=float, default=2.0)
+    ap.add_argument("--safety", type=float, default=2.5)
     ap.add_argument("--ms", default="20,30")
     ap.add_argument("--tols", default="1e-2,3e-3,1e-3,3e-4,1e-4,1e-5")
     ap.add_argument("--out", default=os.path.join(HERE, "tuning.json"))
