@@ -236,6 +236,9 @@ struct oops_hahn_batch {
         int dense_grid = 0;
         DevBuf eslot, smask, slotws;  // slot form of the mixed Lanczos kernel
         bool medium = false;          // n = 1024 / 4096 on the batched medium kernel
+        bool medslot = true;          // medium: every operator fits kMedSlot64 / kMedSlot32 (flip-mask form)
+        int ms64 = 1, ms32 = 0;       // medium flip-mask slots per generator (bucket maxima)
+        DevBuf medmask, medtab;       // medium flip-mask masks and per-workgroup coefficient tables
         DevBuf medws;                 // its FP32 Lanczos bases (one per workgroup)
         int medium_grid = 0;
         bool slots = true;            // every operator fits kSlot64 / kSlot32
@@ -360,6 +363,7 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
         std::vector<double> tau;
         std::vector<uint8_t> eslot;
         std::vector<int32_t> smask;
+        std::vector<int32_t> mmask;   // kMedMaskStride masks per operator (medium flip-mask form)
     };
     b->per_item = per_item;
     std::vector<Host> hosts;
@@ -479,6 +483,18 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
             H.off.insert(H.off.end(), C.off.begin(), C.off.end());
             H.eslot.insert(H.eslot.end(), C.eslot.begin(), C.eslot.end());
             if (C.nslot64 < 0 || C.nslot64 > kSlot64 || C.nslot32 > kSlot32) b->buckets[bk].slots = false;
+            if (C.nslot64 < 1 || C.nslot64 > kMedSlot64 || C.nslot32 > kMedSlot32) {
+                b->buckets[bk].medslot = false;
+            } else {
+                b->buckets[bk].ms64 = std::max(b->buckets[bk].ms64, C.nslot64);
+                b->buckets[bk].ms32 = std::max(b->buckets[bk].ms32, C.nslot32);
+            }
+            for (int q = 0; q < kMedMaskStride; ++q) {
+                int32_t mk = 0;
+                if (q < kMedSlot64 && C.nslot64 > 0 && q < C.nslot64) mk = C.mask64[q];
+                if (q >= kMedSlot64 && q - kMedSlot64 < C.nslot32) mk = C.mask32[q - kMedSlot64];
+                H.mmask.push_back(mk);
+            }
             for (int q = 0; q < kSlotMaskStride; ++q) {
                 int32_t mk = 0;
                 if (q < kSlot64 && q < (int)C.mask64.size()) mk = C.mask64[q];
@@ -557,6 +573,17 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
             rc = hip_check(ctx, hipMalloc(&B.medws.p, bytes), "hipMalloc medium workspace");
             if (rc) return rc;
             B.medws.bytes = bytes;
+            B.medslot = B.medslot && !ell_only;
+            if (B.medslot) {
+                // flip-mask tables: per workgroup four generators of ms64 FP64 + ms32 FP32 slots
+                const int64_t gstride = (int64_t)B.ms64 * B.n + ((int64_t)B.ms32 * B.n + 1) / 2;
+                const size_t tb = sizeof(double) * 2 * 4 * (size_t)gstride * (size_t)B.medium_grid;
+                rc = dev_upload(ctx, B.eslot, H.eslot);
+                rc = rc ? rc : dev_upload(ctx, B.medmask, H.mmask);
+                rc = rc ? rc : hip_check(ctx, hipMalloc(&B.medtab.p, tb), "hipMalloc medium tables");
+                if (rc) return rc;
+                B.medtab.bytes = tb;
+            }
         }
         B.slots = B.slots && !B.medium && !ell_only && precision == 1 && B.hermitian && !dense && B.n <= 256 && (m == 20 || m == 30);
         if (B.slots) {
@@ -588,7 +615,7 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
     for (auto& B : b->buckets)
         for (DevBuf* d : {&B.hdr, &B.loc, &B.val, &B.off, &B.opA, &B.opB, &B.opC, &B.opD, &B.voff, &B.u0, &B.y0,
                           &B.scale, &B.rnd, &B.L, &B.stats, &B.queue, &B.npulse, &B.nt, &B.tau_off, &B.L_off, &B.tau,
-                          &B.work, &B.eslot, &B.smask, &B.slotws, &B.sweepws, &B.medws})
+                          &B.work, &B.eslot, &B.smask, &B.slotws, &B.sweepws, &B.medws, &B.medmask, &B.medtab})
             b->footprint += (int64_t)d->bytes;
     if (per_item) {
         if (hipStreamCreate(&b->stream) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipStreamCreate");
@@ -649,6 +676,12 @@ static KrylovArgs bucket_args(const oops_hahn_batch* b, const oops_hahn_batch::B
     a.slotws = (double2*)B.slotws.p;
     a.sweepws = (double2*)B.sweepws.p;
     a.ws_nt = B.sweep_nt;
+    if (B.medium && B.medtab.p) {
+        a.medmask = (const int32_t*)B.medmask.p;
+        a.medtab = (double2*)B.medtab.p;
+        a.ms64 = B.ms64;
+        a.ms32 = B.ms32;
+    }
     a.nitems = (int32_t)B.items.size();
     a.n = B.n;
     a.klim = b->klim;

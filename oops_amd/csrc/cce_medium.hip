@@ -42,6 +42,11 @@ struct MedSmem {
     double scal[16];
     double bes[2 * kWave];
     int ints[8];
+    // flip-mask form: the active generator's coefficient tables and masks
+    const double2* tab64;
+    const float2* tab32;
+    int ns64, ns32, slot;
+    int mask[kMedMaskStride];
 #ifdef OOPS_PHASE_TIMERS
     unsigned long long tbes;
 #endif
@@ -77,10 +82,120 @@ __device__ __forceinline__ void med_stage(MedSmem<M, E>& S, const cd (&x)[E]) {
     }
 }
 
+// Element i = threadIdx.x + 1024 q of A x, x staged in LDS.  Flip-mask form:
+// (A x)_i = sum_s c_s(i) x[i ^ mask_s], coefficients streamed from the
+// workgroup's device tables (FP64 for one-spin masks, FP32 for two-spin
+// masks); otherwise the merged ELL tables in LDS.
 template <int M, int E>
 __device__ __forceinline__ cd med_product(const MedSmem<M, E>& S, int q) {
     const int i = threadIdx.x + kMT * q;
+    if (S.slot) {
+        const int n = kMT * E;
+        const double2* __restrict__ t64 = S.tab64;
+        const float2* __restrict__ t32 = S.tab32;
+        cd acc = mkcd(0.0, 0.0);
+        const int n64 = S.ns64, n32 = S.ns32;
+#pragma unroll 4
+        for (int s = 0; s < n64; ++s) {
+            const double2 c = t64[(int64_t)s * n + i];
+            acc = cfma(mkcd(c.x, c.y), S.xs[i ^ S.mask[s]], acc);
+        }
+        float ar = 0.f, ai = 0.f;
+#pragma unroll 4
+        for (int s = 0; s < n32; ++s) {
+            const float2 c = t32[(int64_t)s * n + i];
+            const float2 x = S.xs32[i ^ S.mask[kMedSlot64 + s]];
+            ar = fmaf(c.x, x.x, fmaf(-c.y, x.y, ar));
+            ai = fmaf(c.x, x.y, fmaf(c.y, x.x, ai));
+        }
+        return mkcd(acc.re + (double)ar, acc.im + (double)ai);
+    }
     return op_apply_elem_mixed(i, S.xs, S.xs32, S.op.hdr, S.op.locs, S.op.vals, S.op.offs);
+}
+
+// All E elements of this thread.  The flip-mask form walks the slots once
+// for the E elements (E independent coefficient loads per slot in flight).
+template <int M, int E>
+__device__ __forceinline__ void med_product_all(const MedSmem<M, E>& S, cd (&y)[E]) {
+    if (!S.slot) {
+#pragma unroll
+        for (int r = 0; r < E; ++r) y[r] = med_product<M, E>(S, r);
+        return;
+    }
+    const int n = kMT * E;
+    const int i0 = threadIdx.x;
+    const double2* __restrict__ t64 = S.tab64;
+    const float2* __restrict__ t32 = S.tab32;
+    const int n64 = S.ns64, n32 = S.ns32;
+    cd acc[E];
+    float ar[E], ai[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) { acc[r] = mkcd(0.0, 0.0); ar[r] = 0.f; ai[r] = 0.f; }
+#pragma unroll 2
+    for (int s = 0; s < n64; ++s) {
+        const int mk = S.mask[s];
+        double2 c[E];
+#pragma unroll
+        for (int r = 0; r < E; ++r) c[r] = t64[(int64_t)s * n + i0 + kMT * r];
+#pragma unroll
+        for (int r = 0; r < E; ++r) acc[r] = cfma(mkcd(c[r].x, c[r].y), S.xs[(i0 + kMT * r) ^ mk], acc[r]);
+    }
+#pragma unroll 2
+    for (int s = 0; s < n32; ++s) {
+        const int mk = S.mask[kMedSlot64 + s];
+        float2 c[E];
+#pragma unroll
+        for (int r = 0; r < E; ++r) c[r] = t32[(int64_t)s * n + i0 + kMT * r];
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const float2 x = S.xs32[(i0 + kMT * r) ^ mk];
+            ar[r] = fmaf(c[r].x, x.x, fmaf(-c[r].y, x.y, ar[r]));
+            ai[r] = fmaf(c[r].x, x.y, fmaf(c[r].y, x.x, ai[r]));
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < E; ++r) y[r] = mkcd(acc[r].re + (double)ar[r], acc[r].im + (double)ai[r]);
+}
+
+// Coefficient tables of operator `op` for this thread's elements (FP64 slots
+// [ms64][n] then FP32 slots [ms32][n]); every thread writes and reads only its
+// own columns.  Slot 0 is mask 0 (the diagonal, with the constant term).
+template <int E>
+__device__ __noinline__ void med_slot_build(const KrylovArgs& a, int op, double2* __restrict__ tab) {
+    const DevOpHeader h = a.hdr[op];
+    const int n = kMT * E;
+    float2* t32 = reinterpret_cast<float2*>(tab + (int64_t)a.ms64 * n);
+    const int ld = h.ld_bits;
+    const int lmask = (1 << ld) - 1;
+    const int top = h.nspin - 1;
+    for (int q = 0; q < E; ++q) {
+        int i = threadIdx.x + kMT * q;
+        asm volatile("" : "+v"(i));
+        for (int s = 0; s < a.ms64; ++s) tab[(int64_t)s * n + i] = make_double2(0.0, 0.0);
+        for (int s = 0; s < a.ms32; ++s) t32[(int64_t)s * n + i] = make_float2(0.f, 0.f);
+        tab[i] = make_double2(h.c0re, h.c0im);
+        for (int L = 0; L < h.nloc; ++L) {
+            const DevOpLocal lo = a.loc[h.loc_off + L];
+            int row = (i >> (ld * (top - lo.s))) & lmask;
+            if (lo.nb == 2) row = (row << ld) | ((i >> (ld * (top - lo.t))) & lmask);
+            const int e0 = h.ent_off + lo.ent + row * lo.R;
+            for (int e = 0; e < lo.R; ++e) {
+                const double2 v = a.val[e0 + e];
+                const int sl = a.eslot[e0 + e];
+                if (sl & 128) {
+                    float2 t = t32[(int64_t)(sl & 127) * n + i];
+                    t.x += (float)v.x;
+                    t.y += (float)v.y;
+                    t32[(int64_t)(sl & 127) * n + i] = t;
+                } else {
+                    double2 t = tab[(int64_t)sl * n + i];
+                    t.x += v.x;
+                    t.y += v.y;
+                    tab[(int64_t)sl * n + i] = t;
+                }
+            }
+        }
+    }
 }
 
 // Norm estimate of the generator in LDS: k Lanczos steps on +H from the
@@ -242,11 +357,9 @@ __device__ void med_advance(MedSmem<M, E>& S, BlockRed<kMT>& br, MedState& g, cd
         for (int j = 0; j < M; ++j) {
             med_stage<M, E>(S, vcur);
             __syncthreads();
+            med_product_all<M, E>(S, p);
 #pragma unroll
-            for (int r = 0; r < E; ++r) {
-                p[r] = med_product<M, E>(S, r);
-                p[r] = mkcd(p[r].re - bprev * vprev[r].re, p[r].im - bprev * vprev[r].im);
-            }
+            for (int r = 0; r < E; ++r) p[r] = mkcd(p[r].re - bprev * vprev[r].re, p[r].im - bprev * vprev[r].im);
             ++g.nmult;
             const auto r1 = block_sum_lanes<3, kMT, 4>(br, [&](int c) -> double {
                 double s = 0.0;
@@ -287,8 +400,7 @@ __device__ void med_advance(MedSmem<M, E>& S, BlockRed<kMT>& br, MedState& g, cd
         if (!brk) {
             med_stage<M, E>(S, vcur);
             __syncthreads();
-#pragma unroll
-            for (int r = 0; r < E; ++r) p[r] = med_product<M, E>(S, r);
+            med_product_all<M, E>(S, p);
             ++g.nmult;
             const auto tn = block_sum_lanes<1, kMT, 4>(br, [&](int) {
                 double s = 0.0;
@@ -394,6 +506,27 @@ __global__ __launch_bounds__(kMT) void cce_hahn_medium_kernel(KrylovArgs a, floa
     float2* const Vg = medws + (int64_t)blockIdx.x * M * n;
     const bool split = a.sweepws != nullptr;
     const int nunits = split ? 2 * a.nitems : a.nitems;
+    // flip-mask form: four generator tables per workgroup (A, B, and C, D of CPMG-N)
+    const bool slot = a.medtab != nullptr;
+    const int64_t gstride = (int64_t)a.ms64 * n + ((int64_t)a.ms32 * n + 1) / 2;   // double2 units
+    double2* const wtab = slot ? a.medtab + (int64_t)blockIdx.x * 4 * gstride : nullptr;
+    // make generator qo (table index opi) the active one; ends with a barrier
+    auto activate = [&](int opi, int qo) {
+        if (slot) {
+            if (tid == 0) {
+                S.tab64 = wtab + opi * gstride;
+                S.tab32 = reinterpret_cast<const float2*>(wtab + opi * gstride + (int64_t)a.ms64 * n);
+                S.ns64 = a.ms64;
+                S.ns32 = a.ms32;
+                S.slot = 1;
+            }
+            if (tid < kMedMaskStride) S.mask[tid] = a.medmask[(int64_t)qo * kMedMaskStride + tid];
+            __syncthreads();
+        } else {
+            if (tid == 0) S.slot = 0;
+            med_load(a, qo, S.op);
+        }
+    };
     for (;;) {
         if (tid == 0) S.ints[7] = atomicAdd(a.queue, 1);
         __syncthreads();
@@ -409,6 +542,18 @@ __global__ __launch_bounds__(kMT) void cce_hahn_medium_kernel(KrylovArgs a, floa
         if (split && side == 1 && !hahn) continue;   // unit 0 runs both sweeps of this item
         const int qA = a.item_opA[item], qB = a.item_opB[item];
         const int qC = cpmg ? a.item_opC[item] : qA, qD = cpmg ? a.item_opD[item] : qB;
+        if (slot) {   // this unit's generator tables (each thread its own columns)
+            if (hahn) {
+                med_slot_build<E>(a, side ? qB : qA, wtab + side * gstride);
+            } else {
+                med_slot_build<E>(a, qA, wtab);
+                med_slot_build<E>(a, qB, wtab + gstride);
+                if (cpmg) {
+                    med_slot_build<E>(a, qC, wtab + 2 * gstride);
+                    med_slot_build<E>(a, qD, wtab + 3 * gstride);
+                }
+            }
+        }
         cd wA[E], wB[E];
         MedState gA, gB;
         int loaded;
@@ -423,7 +568,7 @@ __global__ __launch_bounds__(kMT) void cce_hahn_medium_kernel(KrylovArgs a, floa
                 wB[r] = mkcd(0.0, 0.0);
             }
             loaded = side ? qB : qA;
-            med_load(a, loaded, S.op);
+            activate(side, loaded);
             med_init<M, E>(S, br, gA, wA, a.rnd, n, a.klim, tol);
             gB = gA;
         } else {
@@ -433,9 +578,10 @@ __global__ __launch_bounds__(kMT) void cce_hahn_medium_kernel(KrylovArgs a, floa
                 wA[r] = mkcd(a.u0[voff + i].x, a.u0[voff + i].y);
                 wB[r] = mkcd(a.y0[voff + i].x, a.y0[voff + i].y);
             }
-            med_load(a, qA, S.op);
+            activate(0, qA);
             med_init<M, E>(S, br, gA, wA, a.rnd, n, a.klim, tol);
-            med_load(a, qB, S.op);
+            __syncthreads();
+            activate(1, qB);
             med_init<M, E>(S, br, gB, wB, a.rnd, n, a.klim, tol);
             loaded = qB;
             beta0A = gA.beta;
@@ -470,7 +616,7 @@ __global__ __launch_bounds__(kMT) void cce_hahn_medium_kernel(KrylovArgs a, floa
                     const int qo = opi == 0 ? qA : (opi == 1 ? qB : (opi == 2 ? qC : qD));
                     if (qo != loaded) {
                         __syncthreads();
-                        med_load(a, qo, S.op);
+                        activate(opi, qo);
                         loaded = qo;
                     }
                     med_advance<M, E>(S, br, gA, wA, Vg, n, target, max_steps, tol, q.tau, q.nt, ob);

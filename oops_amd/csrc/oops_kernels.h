@@ -17,6 +17,11 @@ constexpr int kStatsPerItem = 24;   // int32 slots of per-cluster counters
 // spin-1/2 Liouvillians of k <= 4 spins: the diagonal and 2k ket/bra flips)
 // and kSlot32 FP32 coefficients (two-spin flips: 2 k(k-1)/2 <= 12).
 constexpr int kMediumMaxEnt = 2560;  // ELL entries per operator on the n = 1024 / 4096 kernel
+// flip-mask form of the n = 1024 / 4096 kernel: per operator kMedMaskStride
+// masks, kMedSlot64 FP64-class (one spin flipped) then kMedSlot32 FP32-class
+constexpr int kMedSlot64 = 16;
+constexpr int kMedSlot32 = 48;
+constexpr int kMedMaskStride = kMedSlot64 + kMedSlot32;
 constexpr int kSlot64 = 9;
 constexpr int kSlot32 = 12;
 constexpr int kSlotMaskStride = 32;                            // int32 masks per operator
@@ -57,6 +62,11 @@ struct KrylovArgs {
     int32_t max_steps;
     int32_t ws_nt;             // output times per workgroup that sweepws holds
     double tol;
+    // flip-mask form of the n = 1024 / 4096 kernel (NULL medtab: ELL tables in LDS)
+    const int32_t* medmask;    // kMedMaskStride masks per operator
+    double2* medtab;           // per workgroup 4 generators x (ms64 FP64 + ms32 FP32 slots) x n coefficients
+    int32_t ms64;              // slots per generator in the bucket (at most kMedSlot64 / kMedSlot32)
+    int32_t ms32;
 };
 
 int launch_hahn_bucket(const KrylovArgs& a, int m, int precision, int grid, hipStream_t stream);

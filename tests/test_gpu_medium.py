@@ -123,3 +123,24 @@ def test_medium_hahn_units_and_cpmg_in_one_bucket(ctx):
     alone.run()
     La, _ = alone.results()
     assert np.abs(Lm[2] - La[0]).max() <= 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k", [5, 6])
+def test_medium_flip_mask_form_matches_ell_form(ctx, k):
+    """The n = 1024 / 4096 kernel's flip-mask product (per-element coefficient
+    tables, 11 + 20 masks at k = 5, 13 + 30 at k = 6) against its merged-ELL
+    product (precision "mixed-ell"): L(t) within 1e-6."""
+    items = [_hahn_items(c) for c in _clusters(k, 2)]
+    tau = np.linspace(0.0, 1e-3, 41) / 2
+    out = {}
+    for prec in ("mixed", "mixed-ell"):
+        b = O.HahnBatch.from_items(ctx, items, m=30, tol=1e-3, precision=prec)
+        b.run(tau)
+        L, st = b.results()
+        assert b.timing()[0]["kernel"] == "lanczos_medium"
+        assert (st[:, 8] & 0xFF).max() == 0
+        out[prec] = L
+    e = np.abs(out["mixed"] - out["mixed-ell"]).max()
+    print(f"medium k={k} flip-mask vs ELL: max |dL| {e:.2e}", flush=True)
+    assert e <= 1e-6

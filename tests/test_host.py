@@ -118,12 +118,14 @@ def test_precision_and_arguments_rejected_without_gpu_calls():
         O._precision_code("fp16")
 
 
-@pytest.mark.parametrize("k,which", [(4, 0), (4, 2), (4, 1), (4, 5), (3, 0), (2, 0)])
+@pytest.mark.parametrize("k,which", [(4, 0), (4, 2), (4, 1), (4, 5), (3, 0), (2, 0), (5, 0), (5, 2)])
 def test_slot_form_reproduces_the_operator(k, which):
-    """The flip-mask form the mixed Lanczos kernel builds (per-element
+    """The flip-mask form the mixed Lanczos kernels build (per-element
     coefficient of every mask) reassembles the oracle's explicit Liouvillian;
-    a 4-spin spin-1/2 Liouvillian has 9 one-spin masks (diagonal, a ket or bra
-    flip per spin) and 12 two-spin masks (ket-ket and bra-bra pair flips)."""
+    a k-spin spin-1/2 Liouvillian has 1 + 2k one-spin masks (diagonal, a ket
+    or bra flip per spin) and k(k-1) two-spin masks (ket-ket and bra-bra pair
+    flips): 9 + 12 at k = 4 (n = 256 kernel), 11 + 20 at k = 5 and 13 + 30 at
+    k = 6 (n = 1024 / 4096 kernel, at most 16 + 48)."""
     X = O.bath_diamond(3.567, 25.0, 0.02, seed=3)
     C = O.Clusters(X, 7.0, k)
     cl = C.get(k - 1)
