@@ -131,7 +131,7 @@ __device__ __forceinline__ void med_product_all(const MedSmem<M, E>& S, cd (&y)[
     float ar[E], ai[E];
 #pragma unroll
     for (int r = 0; r < E; ++r) { acc[r] = mkcd(0.0, 0.0); ar[r] = 0.f; ai[r] = 0.f; }
-#pragma unroll 2
+#pragma unroll 4
     for (int s = 0; s < n64; ++s) {
         const int mk = S.mask[s];
         double2 c[E];
@@ -140,7 +140,7 @@ __device__ __forceinline__ void med_product_all(const MedSmem<M, E>& S, cd (&y)[
 #pragma unroll
         for (int r = 0; r < E; ++r) acc[r] = cfma(mkcd(c[r].x, c[r].y), S.xs[(i0 + kMT * r) ^ mk], acc[r]);
     }
-#pragma unroll 2
+#pragma unroll 4
     for (int s = 0; s < n32; ++s) {
         const int mk = S.mask[kMedSlot64 + s];
         float2 c[E];
