@@ -32,9 +32,9 @@ __device__ __forceinline__ double wave_shl1(double v) {
 //   (X x)_j = -(d_j x_j + e_{j-1} x_{j-1} + e_j x_{j+1})   j < mb
 //   (X x)_M = -beta_m x_{M-1},   (X x)_{M+1} = -i x_M.
 // Lane j owns row j; neighbours come by DPP wave shifts.  Long steps are split
-// into ns equal substeps of tau r / ns <= 40.  J_k(x) for k < 128 lives in the
+// into ns equal substeps of tau r / ns <= 64.  J_k(x) for k < 128 lives in the
 // lanes (J_k in lane k of `bl`, J_{64+k} in lane k of `bh`) and is read with
-// readlane; the series stops at the first k > x with |J_k| < 1e-18 (the tail
+// readlane; the series stops at the first k > x with |J_k| < 1e-15 (the tail
 // decays faster than geometrically there).  Returns the number of terms.
 // from_f: start from the vector in F (exp(tau H_aug) F, e.g. to walk a step's
 // output times one interval at a time) instead of e_1.
@@ -124,8 +124,8 @@ __device__ int cheb_coeffs(SM& S, int mb, int k1, double tau, double beta_m, boo
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             bl = lane < N ? S.bes[lane] * inorm : 0.0;
             bh = lane + kWave < N ? S.bes[lane + kWave] * inorm : 0.0;
-            const unsigned long long sl = __ballot((double)lane > x && fabs(bl) < 1e-18);
-            const unsigned long long sh = __ballot((double)(lane + kWave) > x && fabs(bh) < 1e-18);
+            const unsigned long long sl = __ballot((double)lane > x && fabs(bl) < 1e-15);
+            const unsigned long long sh = __ballot((double)(lane + kWave) > x && fabs(bh) < 1e-15);
             K = sl ? __ffsll((long long)sl) - 1 : (sh ? kWave + __ffsll((long long)sh) - 1 : N);
             if (K < 2) K = 2;
         }
