@@ -447,11 +447,13 @@ __device__ __forceinline__ void emit_state(BlockRed<kNT>& br, const SweepOut& o,
 // recurrence of a Hermitian generator.  With output times (mode 1 / 2) the
 // steps are not cut at the outputs: a step runs to min(t_new, t_end) under the
 // usual error control, and every output time inside it is evaluated from the
-// same Krylov basis, w(tau_k) = beta V exp((tau_k - t_now) H_aug) e_1, one
-// small exponential each.  The FP64 path keeps the reference's schedule (a
-// step ends at every output time); the mixed path only has to meet its
-// tolerance on L(t), and sparing the clamped steps halves its Krylov work on
-// output grids finer than the natural step.
+// same Krylov basis, w(tau_k) = beta V exp((tau_k - t_now) H_aug) e_1, the
+// small exponential walking from one output time to the next and on to the
+// step end.  The FP64 path keeps the reference's schedule (a step ends at
+// every output time); the mixed path only has to meet its tolerance on L(t),
+// and sparing the clamped steps halves its Krylov work on output grids finer
+// than the natural step.  Mode 2 reads the states mode 1 stored for the same
+// output times.
 template <int M>
 __device__ __forceinline__ void advance_slot(LzSlotSmem<M>& S, BlockRed<kNT>& br, IntegState& g, const SlotOp& so,
                                              double t_end, const SweepOut& o, int max_steps, double tol) {
