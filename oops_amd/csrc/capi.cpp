@@ -268,11 +268,13 @@ struct oops_hahn_batch {
     bool launched = false;
     hipGraph_t graph = nullptr;        // captured launch sequence (oops_echo_batch_capture)
     hipGraphExec_t graph_exec = nullptr;
+    std::vector<hipStream_t> bstreams; // oops_hahn_batch_run: one stream per bucket (buckets overlap)
     ~oops_hahn_batch() {
         if (graph_exec) hipGraphExecDestroy(graph_exec);
         if (graph) hipGraphDestroy(graph);
         for (auto& e : ev) hipEventDestroy(e);
         if (stream) hipStreamDestroy(stream);
+        for (auto& q : bstreams) if (q) hipStreamDestroy(q);
     }
 };
 
@@ -897,27 +899,44 @@ int oops_hahn_batch_run(oops_hahn_batch* b, int32_t nt, const double* tau) {
     for (int k = 2; k < nt && uniform; ++k) uniform = std::fabs(tau[k] - k * tau[1]) <= 1e-12 * tau[nt - 1];
     std::vector<hipEvent_t> ev(2 * b->buckets.size());
     for (auto& e : ev) hipEventCreate(&e);
-    // largest buckets first so the long Krylov work starts early
-    for (size_t bk = 0; bk < b->buckets.size(); ++bk) {
+    // every bucket on its own stream, so that buckets that cannot fill the
+    // device (few large clusters) overlap; allocations first, then the
+    // launches, then the host-driven buckets of large states
+    if (b->bstreams.size() != b->buckets.size()) {
+        for (auto& q : b->bstreams) if (q) hipStreamDestroy(q);
+        b->bstreams.assign(b->buckets.size(), nullptr);
+        for (auto& q : b->bstreams)
+            if (!rc) rc = hip_check(ctx, hipStreamCreate(&q), "hipStreamCreate");
+    }
+    for (size_t bk = 0; bk < b->buckets.size() && !rc; ++bk) {
         auto& B = b->buckets[bk];
-        if (B.large) {
-            hipEventRecord(ev[2 * bk], ctx->stream);
-            rc = bucket_run_large(b, B, nt, tau);
-            hipEventRecord(ev[2 * bk + 1], ctx->stream);
-            if (rc) break;
-            continue;
-        }
+        if (B.large) continue;
         B.L.release();
         rc = hip_check(ctx, hipMalloc(&B.L.p, sizeof(double) * B.items.size() * nt), "hipMalloc L");
         if (rc) break;
         B.L.bytes = sizeof(double) * B.items.size() * nt;
         rc = sweep_workspace(ctx, B, nt);
-        if (rc) break;
+    }
+    for (size_t bk = 0; bk < b->buckets.size() && !rc; ++bk) {
+        auto& B = b->buckets[bk];
+        if (B.large) continue;
+        hipStream_t st = b->bstreams[bk];
         const KrylovArgs a = bucket_args(b, B, nt, (const double*)b->tau.p);
+        hipEventRecord(ev[2 * bk], st);
+        rc = bucket_launch(b, B, a, st, uniform);
+        hipEventRecord(ev[2 * bk + 1], st);
+    }
+    for (size_t bk = 0; bk < b->buckets.size() && !rc; ++bk) {
+        auto& B = b->buckets[bk];
+        if (!B.large) continue;
         hipEventRecord(ev[2 * bk], ctx->stream);
-        rc = bucket_launch(b, B, a, ctx->stream, uniform);
+        rc = bucket_run_large(b, B, nt, tau);
         hipEventRecord(ev[2 * bk + 1], ctx->stream);
-        if (rc) break;
+    }
+    for (auto& q : b->bstreams) {
+        if (!q) continue;
+        const hipError_t e = hipStreamSynchronize(q);
+        if (!rc) rc = hip_check(ctx, e, "hipStreamSynchronize bucket");
     }
     hipError_t se = hipStreamSynchronize(ctx->stream);
     if (!rc) rc = hip_check(ctx, se, "hipStreamSynchronize");
