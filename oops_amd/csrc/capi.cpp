@@ -632,10 +632,13 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
 // Hahn-sweep state stores, kept when already large enough.  Slot kernel: the
 // sweep-A state (256 complex FP64) at ntmax output times per workgroup.
 // Medium kernel: both sweeps' states of every item at ntmax times (items x 2 x
-// n x ntmax complex FP64); above kMediumStoreMax bytes the bucket runs without
-// a store (both sweeps of an item in one workgroup, a step cut at every output).
-constexpr size_t kMediumStoreMax = size_t(32) << 30;
+// n x ntmax complex FP64).  Above kStoreMax bytes, or when the allocation
+// fails, the bucket runs without a store (both sweeps of an item in one
+// workgroup, a step cut at every output time); results are the same to the
+// tolerance.
+constexpr size_t kStoreMax = size_t(32) << 30;
 static int sweep_workspace(oops_ctx* ctx, oops_hahn_batch::Bucket& B, int ntmax) {
+    (void)ctx;
     if (ntmax <= 0) return OOPS_OK;
     size_t bytes = 0;
     if (B.slots && B.slot_grid > 0) bytes = sizeof(double) * 2 * 256 * (size_t)ntmax * (size_t)B.slot_grid;
@@ -644,9 +647,12 @@ static int sweep_workspace(oops_ctx* ctx, oops_hahn_batch::Bucket& B, int ntmax)
     if (ntmax <= B.sweep_nt && B.sweepws.p) return OOPS_OK;
     B.sweepws.release();
     B.sweep_nt = 0;
-    if (B.medium && bytes > kMediumStoreMax) return OOPS_OK;
-    const int rc = hip_check(ctx, hipMalloc(&B.sweepws.p, bytes), "hipMalloc sweep states");
-    if (rc) return rc;
+    if (bytes > kStoreMax) return OOPS_OK;
+    if (hipMalloc(&B.sweepws.p, bytes) != hipSuccess) {
+        B.sweepws.p = nullptr;
+        (void)hipGetLastError();
+        return OOPS_OK;
+    }
     B.sweepws.bytes = bytes;
     B.sweep_nt = ntmax;
     return OOPS_OK;
