@@ -11,6 +11,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -640,6 +641,12 @@ constexpr size_t kStoreMax = size_t(32) << 30;
 static int sweep_workspace(oops_ctx* ctx, oops_hahn_batch::Bucket& B, int ntmax) {
     (void)ctx;
     if (ntmax <= 0) return OOPS_OK;
+    if (const char* e = std::getenv("OOPS_SWEEP_STORE"))   // "0": never use a store
+        if (e[0] == '0') {
+            B.sweepws.release();
+            B.sweep_nt = 0;
+            return OOPS_OK;
+        }
     size_t bytes = 0;
     if (B.slots && B.slot_grid > 0) bytes = sizeof(double) * 2 * 256 * (size_t)ntmax * (size_t)B.slot_grid;
     else if (B.medium) bytes = sizeof(double) * 2 * 2 * (size_t)B.n * (size_t)ntmax * B.items.size();

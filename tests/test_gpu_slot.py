@@ -206,3 +206,50 @@ def test_slot_hahn_low_and_high_field(ctx, field):
           f"{int(((st[:, 8] >> 8) & 3).astype(bool).sum())}, fp64 steps/sweep {out['fp64'][1][:, 0].mean():.1f}",
           flush=True)
     assert e <= 1e-6
+
+
+@pytest.mark.gpu
+def test_hahn_without_state_store_matches_with_store(ctx, monkeypatch):
+    """OOPS_SWEEP_STORE=0 (also the fallback when the store would exceed
+    32 GB or cannot be allocated): the Hahn sweeps interleave per output time
+    with a step cut at every output, on the n = 256 and n = 1024 kernels; L(t)
+    agrees with the store-based sweeps to 1e-6."""
+    X = O.bath_diamond(3.567, 30.0, 0.011, seed=5)
+    C = O.Clusters(X, 8.0, 4)
+    par = O.cce_params(B=B111)
+    sel = _sel(C, 4, 16)
+    tau = np.linspace(0.0, 1e-3, 81) / 2
+    out = {}
+    for store in ("1", "0"):
+        monkeypatch.setenv("OOPS_SWEEP_STORE", store)
+        b = C.hahn_batch(ctx, par, m=30, tol=1e-3, precision="mixed", select=sel)
+        b.run(tau)
+        L, st = b.results()
+        assert (st[:, 8] & 0xFF).max() == 0
+        out[store] = (L, st[:, 0].sum())
+        b.close()
+    e = np.abs(out["1"][0] - out["0"][0]).max()
+    print(f"store vs no store: max |dL| {e:.2e}, steps {out['1'][1]} vs {out['0'][1]}", flush=True)
+    assert e <= 1e-6
+    assert out["1"][1] < out["0"][1]
+    # n = 1024 (medium kernel units vs both sweeps in one workgroup)
+    X5 = O.bath_diamond(3.567, 25.0, 0.011, seed=2)
+    C5 = O.Clusters(X5, 6.0, 5)
+    cl = C5.get(4)
+    items = []
+    for r in cl[np.linspace(0, len(cl) - 1, 2).astype(int)]:
+        A, rho, D = O.cluster_terms(X5[list(r)], par, 0)
+        Bd, _, _ = O.cluster_terms(X5[list(r)], par, 2)
+        items.append((A, Bd, rho, rho, D))
+    res = {}
+    for store in ("1", "0"):
+        monkeypatch.setenv("OOPS_SWEEP_STORE", store)
+        b = O.HahnBatch.from_items(ctx, items, m=30, tol=1e-3, precision="mixed")
+        b.run(tau)
+        L, st = b.results()
+        assert b.timing()[0]["kernel"] == "lanczos_medium"
+        assert (st[:, 8] & 0xFF).max() == 0
+        res[store] = L
+    e5 = np.abs(res["1"] - res["0"]).max()
+    print(f"medium store vs no store: max |dL| {e5:.2e}", flush=True)
+    assert e5 <= 1e-6
