@@ -55,6 +55,9 @@ struct MedSmem {
 struct MedState {
     double beta, t_now, t_new, anorm, xm;
     int nstep, nmult, nrej, nexph, nscale, brk, fail;
+#ifdef OOPS_PHASE_TIMERS
+    unsigned long long tmv, tred, texp, tupd;   // cycles: products, reductions, small exponentials, updates
+#endif
 };
 
 // Generator `op` of the bucket into LDS (2-body values as complex FP32).
@@ -314,6 +317,9 @@ __device__ void med_init(MedSmem<M, E>& S, BlockRed<kMT>& br, MedState& g, cd (&
     g.t_new = (1.0 / g.anorm) * ool_pow(p2 / (4.0 * g.beta * g.anorm), g.xm);
     g.t_new = round_step2(g.t_new);
     g.nstep = g.nmult = g.nrej = g.nexph = g.nscale = g.brk = g.fail = 0;
+#ifdef OOPS_PHASE_TIMERS
+    g.tmv = g.tred = g.texp = g.tupd = 0;
+#endif
 }
 
 // Stores the state at output time k (ob: n complex FP64 per output time).
@@ -355,12 +361,15 @@ __device__ void med_advance(MedSmem<M, E>& S, BlockRed<kMT>& br, MedState& g, cd
         double t_step = fmin(t_out - g.t_now, g.t_new);
 #pragma unroll 1
         for (int j = 0; j < M; ++j) {
+            OOPS_TIC(tm0);
             med_stage<M, E>(S, vcur);
             __syncthreads();
             med_product_all<M, E>(S, p);
 #pragma unroll
             for (int r = 0; r < E; ++r) p[r] = mkcd(p[r].re - bprev * vprev[r].re, p[r].im - bprev * vprev[r].im);
             ++g.nmult;
+            OOPS_TOC(g, tmv, tm0);
+            OOPS_TIC(tr0);
             const auto r1 = block_sum_lanes<3, kMT, 4>(br, [&](int c) -> double {
                 double s = 0.0;
 #pragma unroll
@@ -374,6 +383,7 @@ __device__ void med_advance(MedSmem<M, E>& S, BlockRed<kMT>& br, MedState& g, cd
             const cd h = mkcd(r1.get(0), r1.get(1));
             const double alpha = h.re;
             const double beta = sqrt(fmax(r1.get(2) - cabs2(h), 0.0));
+            OOPS_TOC(g, tred, tr0);
             if (tid == 0) {
                 S.d[j] = alpha;
                 S.e[j] = beta;
@@ -411,6 +421,7 @@ __device__ void med_advance(MedSmem<M, E>& S, BlockRed<kMT>& br, MedState& g, cd
             avnorm = sqrt(tn.get(0));
         }
         __syncthreads();
+        OOPS_TIC(te0);
         // w = beta * (sum_{j<mb} F_j v_j + [k1] F_M v_M)
         const int mx = mb + (k1 - 1 > 0 ? k1 - 1 : 0);
         auto combine = [&](cd (&x)[E]) {
@@ -470,6 +481,8 @@ __device__ void med_advance(MedSmem<M, E>& S, BlockRed<kMT>& br, MedState& g, cd
             break;
         }
         if (ob) k = kend;
+        OOPS_TOC(g, texp, te0);
+        OOPS_TIC(tu0);
         combine(p);   // the state at the end of the step
 #pragma unroll
         for (int r = 0; r < E; ++r) w[r] = p[r];
@@ -487,6 +500,7 @@ __device__ void med_advance(MedSmem<M, E>& S, BlockRed<kMT>& br, MedState& g, cd
         g.t_now = t1;
         if (ob)
             for (; k < ont && otau[k] <= g.t_now; ++k) med_emit<E>(ob, n, k, w);
+        OOPS_TOC(g, tupd, tu0);
     }
 }
 
@@ -646,6 +660,14 @@ __global__ __launch_bounds__(kMT) void cce_hahn_medium_kernel(KrylovArgs a, floa
         }
         if (tid == 0) {
             int32_t* st = a.stats + (int64_t)item * kStatsPerItem;
+#ifdef OOPS_PHASE_TIMERS
+            if (!hahn || side == 0) {   // the unit's own sweep (sweep A of an item)
+                st[16] = (int)(gA.tmv >> 10);
+                st[17] = (int)(gA.tred >> 10);
+                st[18] = (int)(gA.texp >> 10);
+                st[19] = (int)(gA.tupd >> 10);
+            }
+#endif
             if (hahn) {
                 // each sweep its own fields; medium_dot_kernel merges 14 / 15 into 8 / 9
                 st[4 * side + 0] = gA.nstep; st[4 * side + 1] = gA.nmult;
