@@ -184,3 +184,27 @@ def test_graph_replay_matches_eager_launch(ctx):
             assert np.array_equal(Lg[i], Le[i]), (rep, i)
         # the replay did the work (not stale results of an earlier replay)
         assert tg > 0.3 * te, (tg, te)
+
+
+@pytest.mark.gpu
+def test_graph_replay_of_medium_bucket_matches_eager_launch(ctx):
+    """Five-spin items (n = 1024): a captured batch holds the medium kernel's
+    sweep units and the inner-product kernel (Hahn) and a two-sided unit
+    (CPMG-N); every replay, with eager runs between them, equals the eager
+    batch's L(t)."""
+    t = np.linspace(0.0, 4e-4, 17)
+    items = [_item(c, 1, t / 2) for c in _clusters(5, 2)] + [_item(_clusters(5, 3)[2], 2, t)]
+    eager = O.EchoBatch.from_items(ctx, items, m=30, tol=1e-4, precision="mixed")
+    graph = O.EchoBatch.from_items(ctx, items, m=30, tol=1e-4, precision="mixed")
+    graph.capture()
+    for rep in range(3):
+        print(f"medium graph replay {rep}", flush=True)
+        eager.run()
+        Le, _ = eager.results()
+        assert eager.timing()[0]["kernel"] == "lanczos_medium"
+        graph.run()
+        Lg, st = graph.results()
+        assert (st[:, 8] & 0xFF).max() == 0
+        assert (st[:, 14:16] == 0).all()
+        for i in range(len(items)):
+            assert np.array_equal(Lg[i], Le[i]), (rep, i)
