@@ -15,6 +15,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <cstdlib>
 #include <limits>
 #include <memory>
 #include <mutex>
@@ -499,6 +500,8 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
     // error estimate, all on R.s
     hipGraphExec_t arn_exec = nullptr;
     bool arn_eager = false;
+    if (const char* e = std::getenv("OOPS_LARGE_GRAPH"))   // "0": launch the cycle eagerly
+        arn_eager = e[0] == '0';
     struct ExecGuard {
         hipGraphExec_t* e;
         ~ExecGuard() { if (*e) hipGraphExecDestroy(*e); }
@@ -543,7 +546,7 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
             (void)hipMemsetAsync(R.flags.p, 0, sizeof(int) * 4, R.s);
             const double inv_beta = 1.0 / beta;
             (void)hipMemcpyAsync(scal, &inv_beta, sizeof(double), hipMemcpyHostToDevice, R.s);
-            if (!arn_exec) {
+            if (!arn_exec && !arn_eager) {
                 // the Arnoldi cycle's launches are the same every step (every
                 // per-step value lives in device memory): capture them once
                 // and replay the graph (the memsets and the 1/beta copy above

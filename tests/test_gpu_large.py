@@ -165,3 +165,27 @@ def test_expv_outputs_inside_steps_match_reference_schedule(ctx, precision):
     wr, _ = O.expv(ctx, T, v, t2, m=30, tol=1e-12, precision=precision)
     wi, _ = O.expv(ctx, T, v, t2, m=30, tol=1e-12, precision=precision, steps="inside")
     assert np.abs(wi - wr).max() <= bar   # unit-norm states
+
+
+@pytest.mark.gpu
+def test_expv_graph_replay_equals_eager_launches(ctx, monkeypatch):
+    """The large path replays each Arnoldi cycle from a HIP graph captured once
+    per call; OOPS_LARGE_GRAPH=0 launches the same kernels eagerly.  Same
+    kernels, same order: identical results, on FP64 and FP32 bases, over
+    several calls."""
+    rng = np.random.default_rng(5)
+    X = np.array([p for p in rng.uniform(-15, 15, (600, 3)) if np.linalg.norm(p) <= 15.0][:11])
+    T, _, _ = O.cluster_terms(X, O.cce_params(B=B111), 3)
+    v = rng.standard_normal(T.n) + 1j * rng.standard_normal(T.n)
+    v /= np.linalg.norm(v)
+    tl = np.arange(31) * 1e-6
+    for precision in ("fp64", "fp32"):
+        got = {}
+        for flag in ("1", "0", "1"):
+            monkeypatch.setenv("OOPS_LARGE_GRAPH", flag)
+            ov, st = O.expv(ctx, T, v, tl, m=30, tol=1e-12, overlaps=True, precision=precision)
+            got.setdefault(flag, []).append(ov)
+        print(f"graph vs eager n={T.n} {precision}: max diff {np.abs(got['1'][0] - got['0'][0]).max():.1e}",
+              flush=True)
+        assert np.array_equal(got["1"][0], got["0"][0])
+        assert np.array_equal(got["1"][0], got["1"][1])
