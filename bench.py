@@ -206,7 +206,9 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build targets one GPU; --gpus must be 1 "
-                                   "(clusters shard across GPUs by design, see DESIGN.md)"}))
+                                   "(clusters are independent: across GPUs each rank would take a "
+                                   "cost-balanced slice of every bucket and one all-reduce of the "
+                                   "nt-point log L(t) would combine them)"}))
         sys.exit(2)
 
     import oops_amd as O

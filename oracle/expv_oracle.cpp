@@ -22,7 +22,7 @@
 //                   multiple output times :339-517), entered the way
 //                   MatExpVector::runInexplicit does (src/source/math/MatExp.cpp:216-365)
 //
-// Decisions where the reference is undefined (see DESIGN.md "Reference quirks"):
+// Decisions where the reference is undefined:
 //   * sgn = +1 (the reference reads an uninitialised `t`, :16,:320);
 //   * the Lanczos step subtracts bjm*qjm with a purely real alpha (the
 //     reference passes a REAL*8 to ZAXPY's COMPLEX*16 alpha, :233);
