@@ -198,7 +198,9 @@ int oops_hahn_batch_run(oops_hahn_batch* b, int32_t nt, const double* tau);
 /* Copy results: L [nitems x nt] (row per item, item order of create), stats
  * [nitems x 24] int32 per-cluster counters: [0..3] steps, products, rejections,
  * small exponentials of sweep A, [4..7] the same for sweep B, [8] failure flags,
- * [9] exponential substeps, [10..13] the two norm estimates (doubles).  Either may be NULL. */
+ * [9] exponential substeps, [10..13] the two norm estimates (doubles).  Either may be NULL.
+ * L of a failed cluster (flags & 0xFF nonzero) is NaN where its sweeps
+ * stopped before writing it (result buffers start as NaN). */
 int oops_hahn_batch_results(oops_hahn_batch* b, double* L, int32_t* stats);
 /* --- pulse sequences with per-item time grids ------------------------------
  * One cluster under a Hahn echo (npulse = 1) or CPMG-N (npulse = N >= 2,
@@ -245,6 +247,10 @@ int oops_echo_batch_footprint(const oops_hahn_batch* b, int64_t* bytes);
  * item count and kernel time in ms (HIP events on the launch stream). */
 int oops_hahn_batch_timing(const oops_hahn_batch* b, int32_t max_buckets, int32_t* n, int32_t* count, double* ms,
                            int32_t* nbuckets);
+/* Device time of the last run (ms): one interval from the first launch to
+ * the completion of every bucket.  Buckets run concurrently on their own
+ * streams, so this is less than the sum of the per-bucket times. */
+int oops_hahn_batch_elapsed(const oops_hahn_batch* b, double* ms);
 /* Kernel each bucket ran with in the last run: 0 dense VALU Pade (n <= m),
  * 1 Arnoldi FP64, 2 Arnoldi mixed, 3 Lanczos mixed, 4 dense FP64-MFMA
  * propagator (n = 64), 5 single-state path (n > 256), 6 Lanczos mixed in the

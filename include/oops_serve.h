@@ -62,6 +62,10 @@ int oops_server_result(oops_server* s, int64_t id, double* L, double* latency_s,
                        int64_t* footprint);
 /* When the request finished building and was launched (s since submit). */
 int oops_server_times(oops_server* s, int64_t id, double* built_s, double* launched_s);
+/* Clusters of a completed request whose integration failed (step limit or
+ * step-size collapse: failure flags in their counters).  A request with
+ * failed clusters ends in state OOPS_ERR_NUMERIC and has no result. */
+int oops_server_failed(oops_server* s, int64_t id, int64_t* failed);
 /* Requests not yet done. */
 int oops_server_pending(oops_server* s, int32_t* pending);
 int oops_server_destroy(oops_server* s);

@@ -61,6 +61,7 @@ EXPORTS = [
     "oops_cce_echo_batch_create", "oops_server_create", "oops_server_submit", "oops_server_poll", "oops_server_state",
     "oops_server_result", "oops_server_pending", "oops_server_destroy", "oops_server_times",
     "oops_hahn_batch_kernels", "oops_echo_batch_capture", "oops_slot_form", "oops_tile_plan", "oops_read_xyz", "oops_write_xyz",
+    "oops_hahn_batch_elapsed", "oops_server_failed",
 ]
 
 
@@ -98,6 +99,8 @@ def lib():
         L.oops_server_pending.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]
         L.oops_server_destroy.argtypes = [ctypes.c_void_p]
         L.oops_server_times.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+        L.oops_server_failed.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
+        L.oops_hahn_batch_elapsed.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]
         L.oops_hamvec.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.oops_expv.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
                                 ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_void_p,
@@ -356,6 +359,13 @@ class HahnBatch:
         return [{"n": int(n[i]), "count": int(cnt[i]), "ms": float(ms[i]), "kernel": KERNEL_NAMES.get(int(kind[i]))}
                 for i in range(k)]
 
+    def elapsed(self):
+        """Device time of the last run in ms: one interval around every bucket
+        (buckets overlap on their own streams, so per-bucket times do not add)."""
+        v = ctypes.c_double()
+        _check(lib().oops_hahn_batch_elapsed(self.h, ctypes.byref(v)), self.ctx.h, "oops_hahn_batch_elapsed")
+        return v.value
+
     def close(self):
         if self.h:
             lib().oops_hahn_batch_destroy(self.h)
@@ -500,6 +510,13 @@ class Server:
                self.ctx.h, "oops_server_times")
         return {"L": L, "latency_s": lat.value, "clusters": ncl.value, "footprint": fp.value,
                 "built_s": tb.value, "launched_s": tl.value}
+
+    def failed(self, rid):
+        """Clusters of a finished request whose integration failed."""
+        v = ctypes.c_int64()
+        _check(lib().oops_server_failed(self.h, ctypes.c_int64(rid), ctypes.byref(v)), self.ctx.h,
+               "oops_server_failed")
+        return v.value
 
     def drain(self, sleep=0.002):
         import time

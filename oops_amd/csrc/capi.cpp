@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <cstdlib>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -215,6 +216,27 @@ int dev_upload(oops_ctx* ctx, DevBuf& b, const std::vector<T>& v) {
     return hip_check(ctx, hipMemcpy(b.p, v.data(), b.bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
 }
 
+// L(t) results of a bucket, filled with NaN (all bits set): a cluster whose
+// sweeps fail leaves its outputs unwritten, and NaN cannot pass for a result.
+int alloc_results(oops_ctx* ctx, DevBuf& b, int64_t count) {
+    b.release();
+    const size_t bytes = sizeof(double) * (size_t)count;
+    int rc = hip_check(ctx, hipMalloc(&b.p, bytes), "hipMalloc L");
+    if (rc) return rc;
+    b.bytes = bytes;
+    return hip_check(ctx, hipMemset(b.p, 0xFF, bytes), "hipMemset L");
+}
+
+// Step limit of every Krylov integration (OOPS_MAX_STEPS overrides it, so
+// that a caller can force the failure path from the host).
+int32_t max_steps_limit() {
+    if (const char* e = std::getenv("OOPS_MAX_STEPS")) {
+        const long v = std::strtol(e, nullptr, 10);
+        if (v > 0 && v < 50000000) return (int32_t)v;
+    }
+    return 50000000;
+}
+
 }  // namespace oops
 
 using namespace oops;
@@ -270,6 +292,7 @@ struct oops_hahn_batch {
     hipGraph_t graph = nullptr;        // captured launch sequence (oops_echo_batch_capture)
     hipGraphExec_t graph_exec = nullptr;
     std::vector<hipStream_t> bstreams; // oops_hahn_batch_run: one stream per bucket (buckets overlap)
+    double total_ms = 0.0;             // device interval of the last run, first launch to last completion
     ~oops_hahn_batch() {
         if (graph_exec) hipGraphExecDestroy(graph_exec);
         if (graph) hipGraphDestroy(graph);
@@ -560,10 +583,7 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
             rc = rc ? rc : dev_upload(ctx, B.tau_off, H.tau_off);
             rc = rc ? rc : dev_upload(ctx, B.L_off, H.L_off);
             rc = rc ? rc : dev_upload(ctx, B.tau, H.tau);
-            if (!rc) {
-                rc = hip_check(ctx, hipMalloc(&B.L.p, sizeof(double) * std::max<int64_t>(B.L_len, 1)), "hipMalloc L");
-                if (!rc) B.L.bytes = sizeof(double) * std::max<int64_t>(B.L_len, 1);
-            }
+            if (!rc) rc = alloc_results(ctx, B.L, std::max<int64_t>(B.L_len, 1));
         }
         if (rc) return rc;
         const bool dense = B.n <= m;
@@ -700,7 +720,7 @@ static KrylovArgs bucket_args(const oops_hahn_batch* b, const oops_hahn_batch::B
     a.nitems = (int32_t)B.items.size();
     a.n = B.n;
     a.klim = b->klim;
-    a.max_steps = 50000000;
+    a.max_steps = max_steps_limit();
     a.tol = b->tol;
     if (b->per_item) {
         a.item_nt = (const int32_t*)B.nt.p;
@@ -722,7 +742,7 @@ static int bucket_run_large(oops_hahn_batch* b, oops_hahn_batch::Bucket& B, int3
     B.kernel = 5;
     int64_t total = 0;
     for (size_t i = 0; i < ni; ++i) total += b->per_item ? B.hnt[i] : nt;
-    B.hL.assign((size_t)total, 0.0);
+    B.hL.assign((size_t)total, std::numeric_limits<double>::quiet_NaN());
     B.hstats.assign(ni * kStatsPerItem, 0);
     std::vector<int64_t> off(ni + 1, 0);
     for (size_t i = 0; i < ni; ++i) off[i + 1] = off[i] + (b->per_item ? B.hnt[i] : nt);
@@ -875,12 +895,15 @@ int oops_echo_batch_wait(oops_hahn_batch* b) {
         float ms = 0.f;
         hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
         for (auto& B : b->buckets) B.ms = ms / std::max<size_t>(1, b->buckets.size());
+        b->total_ms = ms;
         return OOPS_OK;
     }
+    b->total_ms = 0.0;
     for (size_t bk = 0; bk < b->buckets.size(); ++bk) {
         float ms = 0.f;
         hipEventElapsedTime(&ms, b->ev[2 * bk], b->ev[2 * bk + 1]);
         b->buckets[bk].ms = ms;
+        b->total_ms += ms;   // one stream: the buckets run one after another
     }
     return OOPS_OK;
 }
@@ -910,8 +933,9 @@ int oops_hahn_batch_run(oops_hahn_batch* b, int32_t nt, const double* tau) {
     // uniform grid from 0: the dense propagator powers apply to n = 64 buckets
     bool uniform = nt >= 2 && tau[0] == 0.0 && tau[1] > 0.0;
     for (int k = 2; k < nt && uniform; ++k) uniform = std::fabs(tau[k] - k * tau[1]) <= 1e-12 * tau[nt - 1];
-    std::vector<hipEvent_t> ev(2 * b->buckets.size());
+    std::vector<hipEvent_t> ev(2 * b->buckets.size() + 2);
     for (auto& e : ev) hipEventCreate(&e);
+    hipEvent_t ev_all0 = ev[ev.size() - 2], ev_all1 = ev[ev.size() - 1];
     // every bucket on its own stream, so that buckets that cannot fill the
     // device (few large clusters) overlap; allocations first, then the
     // launches, then the host-driven buckets of large states
@@ -924,16 +948,19 @@ int oops_hahn_batch_run(oops_hahn_batch* b, int32_t nt, const double* tau) {
     for (size_t bk = 0; bk < b->buckets.size() && !rc; ++bk) {
         auto& B = b->buckets[bk];
         if (B.large) continue;
-        B.L.release();
-        rc = hip_check(ctx, hipMalloc(&B.L.p, sizeof(double) * B.items.size() * nt), "hipMalloc L");
+        rc = alloc_results(ctx, B.L, (int64_t)B.items.size() * nt);
         if (rc) break;
-        B.L.bytes = sizeof(double) * B.items.size() * nt;
         rc = sweep_workspace(ctx, B, nt);
     }
+    // one interval around the whole run: every bucket stream starts after
+    // ev_all0 on the context stream, which waits for every bucket before ev_all1
+    if (!rc) rc = hip_check(ctx, hipEventRecord(ev_all0, ctx->stream), "hipEventRecord");
     for (size_t bk = 0; bk < b->buckets.size() && !rc; ++bk) {
         auto& B = b->buckets[bk];
         if (B.large) continue;
         hipStream_t st = b->bstreams[bk];
+        rc = hip_check(ctx, hipStreamWaitEvent(st, ev_all0, 0), "hipStreamWaitEvent");
+        if (rc) break;
         const KrylovArgs a = bucket_args(b, B, nt, (const double*)b->tau.p);
         hipEventRecord(ev[2 * bk], st);
         rc = bucket_launch(b, B, a, st, uniform);
@@ -946,6 +973,10 @@ int oops_hahn_batch_run(oops_hahn_batch* b, int32_t nt, const double* tau) {
         rc = bucket_run_large(b, B, nt, tau);
         hipEventRecord(ev[2 * bk + 1], ctx->stream);
     }
+    for (size_t bk = 0; bk < b->buckets.size() && !rc; ++bk)
+        if (!b->buckets[bk].large)
+            rc = hip_check(ctx, hipStreamWaitEvent(ctx->stream, ev[2 * bk + 1], 0), "hipStreamWaitEvent");
+    if (!rc) rc = hip_check(ctx, hipEventRecord(ev_all1, ctx->stream), "hipEventRecord");
     for (auto& q : b->bstreams) {
         if (!q) continue;
         const hipError_t e = hipStreamSynchronize(q);
@@ -959,6 +990,9 @@ int oops_hahn_batch_run(oops_hahn_batch* b, int32_t nt, const double* tau) {
             hipEventElapsedTime(&ms, ev[2 * bk], ev[2 * bk + 1]);
             b->buckets[bk].ms = ms;
         }
+        float all = 0.f;
+        hipEventElapsedTime(&all, ev_all0, ev_all1);
+        b->total_ms = all;
     }
     for (auto& e : ev) hipEventDestroy(e);
     return rc;
@@ -1022,6 +1056,12 @@ int oops_hahn_batch_timing(const oops_hahn_batch* b, int32_t max_buckets, int32_
         if (count) count[k] = (int32_t)b->buckets[k].items.size();
         if (ms) ms[k] = b->buckets[k].ms;
     }
+    return OOPS_OK;
+}
+
+int oops_hahn_batch_elapsed(const oops_hahn_batch* b, double* ms) {
+    if (!b || !ms) return OOPS_ERR_ARG;
+    *ms = b->total_ms;
     return OOPS_OK;
 }
 

@@ -98,6 +98,8 @@ int large_hahn_item(oops_ctx* ctx, const CompiledOp& A, const CompiledOp& B, con
 int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err, int max_locals = 64,
                      const std::complex<double>* ccoeff = nullptr);
 int set_err(oops_ctx* ctx, int code, const std::string& msg);
+// Step limit of every Krylov integration (environment OOPS_MAX_STEPS, else 5e7).
+int32_t max_steps_limit();
 int hip_check(oops_ctx* ctx, hipError_t e, const char* what);
 void dlaran_fill(int32_t* iseed, int64_t count, double* out);
 

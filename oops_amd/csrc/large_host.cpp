@@ -440,7 +440,7 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
     const int m = R.m;
     const int64_t n = R.n;
     const double delta = 1.2, gamma = 0.9, break_tol = 1.0e-7;
-    const int64_t max_steps = 50000000;
+    const int64_t max_steps = max_steps_limit();
     int rc = norm_estimate(R, klim, tol, C.anorm);
     if (rc) return rc;
     const double anorm = C.anorm;
@@ -498,14 +498,6 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
     };
     // one Arnoldi cycle: m products with CGS2 and the extra product of the
     // error estimate, all on R.s
-    hipGraphExec_t arn_exec = nullptr;
-    bool arn_eager = false;
-    if (const char* e = std::getenv("OOPS_LARGE_GRAPH"))   // "0": launch the cycle eagerly
-        arn_eager = e[0] == '0';
-    struct ExecGuard {
-        hipGraphExec_t* e;
-        ~ExecGuard() { if (*e) hipGraphExecDestroy(*e); }
-    } arn_guard{&arn_exec};
     auto arnoldi_launches = [&]() {
         double2* bufs[2] = {(double2*)R.p0.p, (double2*)R.p1.p};
         const double2* x = (const double2*)R.w.p;
@@ -546,24 +538,7 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
             (void)hipMemsetAsync(R.flags.p, 0, sizeof(int) * 4, R.s);
             const double inv_beta = 1.0 / beta;
             (void)hipMemcpyAsync(scal, &inv_beta, sizeof(double), hipMemcpyHostToDevice, R.s);
-            if (!arn_exec && !arn_eager) {
-                // the Arnoldi cycle's launches are the same every step (every
-                // per-step value lives in device memory): capture them once
-                // and replay the graph (the memsets and the 1/beta copy above
-                // stay outside it); launch eagerly if capture fails
-                hipGraph_t g = nullptr;
-                if (hipStreamBeginCapture(R.s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-                    arnoldi_launches();
-                    if (hipStreamEndCapture(R.s, &g) == hipSuccess && g &&
-                        hipGraphInstantiate(&arn_exec, g, nullptr, nullptr, 0) != hipSuccess)
-                        arn_exec = nullptr;
-                    if (g) hipGraphDestroy(g);
-                }
-                (void)hipGetLastError();
-                if (!arn_exec) arn_eager = true;
-            }
-            if (arn_eager) arnoldi_launches();
-            else if ((rc = hip_check(R.ctx, hipGraphLaunch(arn_exec, R.s), "hipGraphLaunch arnoldi")) != 0) return rc;
+            arnoldi_launches();
             int fl[4];
             double sc[4];
             rc = hip_check(R.ctx, hipMemcpyAsync(fl, R.flags.p, sizeof(int) * 4, hipMemcpyDeviceToHost, R.s), "D2H");
@@ -708,7 +683,7 @@ int large_hahn_item(oops_ctx* ctx, const CompiledOp& A, const CompiledOp& B, con
     if (rc) return rc;
     std::vector<double> ov(2 * (size_t)nt);
     Counters CA, CB;
-    hipStreamCreate(&RA.s);
+    hipStreamCreateWithFlags(&RA.s, hipStreamNonBlocking);
     RB.s = RA.s;
     rc = run_expv(RA, u0, nt, tau, klim, tol, 2, nullptr, CA, (double2*)ket.p);
     rc = rc ? rc : run_expv(RB, y0, nt, tau, klim, tol, 3, ov.data(), CB, (double2*)ket.p);
@@ -847,7 +822,7 @@ int oops_expv_ex(oops_ctx* ctx, const oops_term_table* H, const double* v, int32
     int rc = setup(ctx, H, m, R);
     if (rc) return rc;
     if (m >= R.n) return set_err(ctx, OOPS_ERR_ARG, "the Krylov dimension must be smaller than the state dimension");
-    hipStreamCreate(&R.s);
+    hipStreamCreateWithFlags(&R.s, hipStreamNonBlocking);
     Counters C;
     rc = run_expv(R, v, nt, ta, klim, tol, out_mode, out, C, nullptr, (precision & 2) != 0);
     hipStreamDestroy(R.s);
@@ -949,7 +924,7 @@ static int csr_expv(int64_t n, const std::vector<int64_t>& rp, const std::vector
     LargeRun R;
     int rc = setup_csr(ctx, n, rp, ci, cv, m, R);
     if (rc) return rc;
-    hipStreamCreate(&R.s);
+    hipStreamCreateWithFlags(&R.s, hipStreamNonBlocking);
     Counters C;
     rc = run_expv(R, v, tn, ta, klim, tol, 0, w_seq, C);
     hipStreamDestroy(R.s);

@@ -24,6 +24,7 @@
 #include "../../include/oops_expv.h"
 #include "../../include/oops_serve.h"
 #include "host_common.h"
+#include "oops_kernels.h"
 
 using namespace oops;
 
@@ -49,6 +50,7 @@ struct Req {
     oops_hahn_batch* batch = nullptr;
     int64_t footprint = 0;
     int64_t nclusters = 0;
+    int64_t failed = 0;   // clusters whose integration failed (flags in their stats)
     std::vector<double> L;
     double latency = 0.0;
     double t_built = 0.0, t_launched = 0.0;  // since submit
@@ -58,32 +60,60 @@ struct Req {
     }
 };
 
-// Device bytes a request will hold, from its operators' term counts (an upper
-// bound of the merged ELL tables: one d x d block per spin, one d^2 x d^2 per pair),
-// plus the Hahn-sweep state store of the 256-element Lanczos bucket (256 complex
-// FP64 per output time for each of at most 512 workgroups).
-int64_t estimate_bytes(const CceItems& it) {
-    int64_t b = 0, n256 = 0, nt256 = 0;
+// Device bytes a request will hold, matching what batch_create allocates:
+// the merged ELL tables (an upper bound: one d x d block per spin, one
+// d^2 x d^2 per pair), vectors and results, plus per state dimension
+//   n = 64   the dense-propagator workspace (one workgroup per CU),
+//   n = 256  the flip-mask coefficient tables and the Hahn-sweep state store
+//            of the slot kernel (two workgroups per CU),
+//   n = 1024 / 4096 (mixed) the FP32 bases and flip-mask tables of the medium
+//            kernel's workgroups and its state store (both sweeps of every Hahn
+//            item at every output time, left out above 32 GB as batch_create does),
+//   n > 256 on the single-state FP64 path: up to eight items in flight, each
+//            with its basis, work vectors and stored ket states.
+int64_t estimate_bytes(const CceItems& it, int num_cu, int m, int precision) {
+    struct Dim { int64_t items = 0, hahn = 0, nt_hahn = 0, nt_max = 0; };
+    std::map<int64_t, Dim> dims;
+    int64_t b = 0;
     for (const EchoIn& e : it.items) {
-        int64_t dim = 1;
-        for (int s = 0; s < e.opA->nspin; ++s) dim *= e.opA->spin_dim[s];
-        if (dim > 64 && dim <= 256 && e.npulse == 1) {
-            ++n256;
-            nt256 = std::max<int64_t>(nt256, e.nt);
-        }
-    }
-    b += std::min<int64_t>(n256, 512) * 256 * 16 * nt256;
-    for (const EchoIn& e : it.items) {
-        const oops_term_table* ops[4] = {e.opA, e.opB, e.opC, e.opD};
         int64_t n = 1;
         for (int s = 0; s < e.opA->nspin; ++s) n *= e.opA->spin_dim[s];
+        Dim& d = dims[n];
+        ++d.items;
+        d.nt_max = std::max<int64_t>(d.nt_max, e.nt);
+        if (e.npulse == 1) {
+            ++d.hahn;
+            d.nt_hahn = std::max<int64_t>(d.nt_hahn, e.nt);
+        }
+        const oops_term_table* ops[4] = {e.opA, e.opB, e.opC, e.opD};
         for (const oops_term_table* o : ops) {
             if (!o) continue;
-            const int64_t k = o->nspin, d = o->spin_dim[0];
-            const int64_t ent = k * d * 4 + k * (k - 1) / 2 * d * d * 16;
-            b += 64 + (k + k * (k - 1) / 2) * 16 + ent * 20;
+            const int64_t k = o->nspin, dl = o->spin_dim[0];
+            const int64_t ent = k * dl * 4 + k * (k - 1) / 2 * dl * dl * 16;
+            b += 64 + (k + k * (k - 1) / 2) * 16 + ent * 20 + kMedMaskStride * 4 + kSlotMaskStride * 4;
         }
-        b += 32 * n + 8 * (int64_t)e.nt + 4 * 24 + 64;
+        b += 32 * n + 8 * (int64_t)e.nt + 4 * kStatsPerItem + 64;
+    }
+    const int64_t c16 = 16;   // bytes of one complex FP64
+    for (const auto& kv : dims) {
+        const int64_t n = kv.first;
+        const Dim& d = kv.second;
+        if (n == 64) {
+            b += 8 * dense64_workspace_doubles((int)std::min<int64_t>(d.items, num_cu));
+        } else if (n > m && n <= 256 && precision == 1) {
+            const int64_t grid = std::min<int64_t>(d.items, 2 * (int64_t)num_cu);
+            b += c16 * kSlotOpD2 * 4 * grid;
+            b += c16 * 256 * d.nt_hahn * std::min<int64_t>(d.hahn, grid);
+        } else if ((n == 1024 || n == 4096) && precision == 1) {
+            const int64_t grid = std::min<int64_t>(2 * d.items, num_cu);
+            b += 8 * medium_workspace_float2((int)grid, m, (int)n);
+            b += c16 * 4 * ((int64_t)kMedSlot64 * n + (int64_t)kMedSlot32 * n / 2) * grid;
+            const int64_t store = c16 * 2 * n * d.nt_hahn * d.hahn;
+            if (store <= (int64_t(32) << 30)) b += store;
+        } else if (n > 256) {
+            const int64_t inflight = std::min<int64_t>(d.items, 8);
+            b += inflight * c16 * n * (2 * (m + 1) + 8 + d.nt_max);
+        }
     }
     return b + 4096;
 }
@@ -151,7 +181,7 @@ struct oops_server {
         rc = cce_build_items(q.coords.data(), q.C, &P, r.npulse, r.nt, t.data(), inner_threads, q.items);
         if (rc) return rc;
         q.nclusters = (int64_t)q.items.items.size();
-        q.est_bytes = estimate_bytes(q.items);
+        q.est_bytes = estimate_bytes(q.items, ctx->num_cu, m, precision);
         return OOPS_OK;
     }
 
@@ -211,7 +241,15 @@ struct oops_server {
             if (!rc) rc = oops_echo_batch_wait(q->batch);
             const int nt = q->r.nt, K = q->r.order;
             std::vector<double> Lall((size_t)q->nclusters * nt), fin_prod((size_t)nt * K);
-            if (!rc) rc = oops_hahn_batch_results(q->batch, Lall.data(), nullptr);
+            std::vector<int32_t> st((size_t)q->nclusters * kStatsPerItem);
+            if (!rc) rc = oops_hahn_batch_results(q->batch, Lall.data(), st.data());
+            int64_t failed = 0;
+            for (int64_t c = 0; c < q->nclusters; ++c)
+                if ((st[(size_t)c * kStatsPerItem + 8] & 0xFF) != 0) ++failed;
+            if (!rc && failed)
+                rc = set_err(ctx, OOPS_ERR_NUMERIC, "request " + std::to_string(id) + ": " + std::to_string(failed) +
+                                                        " of " + std::to_string(q->nclusters) +
+                                                        " clusters failed (step limit or step-size collapse)");
             if (!rc) rc = oops_cce_reduce(q->C, nt, Lall.data(), nullptr, fin_prod.data(), nullptr, nullptr, nullptr);
             if (!rc) {
                 q->L.resize(nt);
@@ -223,6 +261,7 @@ struct oops_server {
             used -= q->footprint;
             running.erase(std::find(running.begin(), running.end(), id));
             q->latency = now_s() - q->t_submit;
+            q->failed = failed;
             q->state = rc ? rc : kDone;
             q->err = rc;
             ++done;
@@ -306,6 +345,15 @@ int oops_server_times(oops_server* s, int64_t id, double* built_s, double* launc
     if (f == s->reqs.end()) return OOPS_ERR_ARG;
     if (built_s) *built_s = f->second->t_built;
     if (launched_s) *launched_s = f->second->t_launched;
+    return OOPS_OK;
+}
+
+int oops_server_failed(oops_server* s, int64_t id, int64_t* failed) {
+    if (!s || !failed) return OOPS_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto f = s->reqs.find(id);
+    if (f == s->reqs.end()) return OOPS_ERR_ARG;
+    *failed = f->second->failed;
     return OOPS_OK;
 }
 

@@ -168,11 +168,9 @@ def test_expv_outputs_inside_steps_match_reference_schedule(ctx, precision):
 
 
 @pytest.mark.gpu
-def test_expv_graph_replay_equals_eager_launches(ctx, monkeypatch):
-    """The large path replays each Arnoldi cycle from a HIP graph captured once
-    per call; OOPS_LARGE_GRAPH=0 launches the same kernels eagerly.  Same
-    kernels, same order: identical results, on FP64 and FP32 bases, over
-    several calls."""
+def test_expv_repeated_calls_are_identical(ctx):
+    """Each oops_expv call builds and frees its own device state and stream;
+    back-to-back calls on FP64 and FP32 bases give bit-identical results."""
     rng = np.random.default_rng(5)
     X = np.array([p for p in rng.uniform(-15, 15, (600, 3)) if np.linalg.norm(p) <= 15.0][:11])
     T, _, _ = O.cluster_terms(X, O.cce_params(B=B111), 3)
@@ -180,12 +178,34 @@ def test_expv_graph_replay_equals_eager_launches(ctx, monkeypatch):
     v /= np.linalg.norm(v)
     tl = np.arange(31) * 1e-6
     for precision in ("fp64", "fp32"):
-        got = {}
-        for flag in ("1", "0", "1"):
-            monkeypatch.setenv("OOPS_LARGE_GRAPH", flag)
-            ov, st = O.expv(ctx, T, v, tl, m=30, tol=1e-12, overlaps=True, precision=precision)
-            got.setdefault(flag, []).append(ov)
-        print(f"graph vs eager n={T.n} {precision}: max diff {np.abs(got['1'][0] - got['0'][0]).max():.1e}",
-              flush=True)
-        assert np.array_equal(got["1"][0], got["0"][0])
-        assert np.array_equal(got["1"][0], got["1"][1])
+        runs = [O.expv(ctx, T, v, tl, m=30, tol=1e-12, overlaps=True, precision=precision)[0] for _ in range(3)]
+        print(f"repeated expv n={T.n} {precision}", flush=True)
+        assert np.array_equal(runs[0], runs[1]) and np.array_equal(runs[0], runs[2])
+
+
+@pytest.mark.gpu
+def test_hahn_batch_large_items_on_threads_equal_single_items(ctx):
+    """An FP64 batch of three 5-spin clusters (n = 1024) runs its items on
+    host threads, each on its own stream; every item's L(t) equals the same
+    item run alone (1e-12)."""
+    X = O.bath_diamond(3.567, 14.0, 0.05, seed=2)
+    C = O.Clusters(X, 8.0, 5)
+    cl = C.get(4)
+    assert len(cl) >= 3
+    par = O.cce_params(B=B111)
+    items = []
+    for r in cl[:3]:
+        A, rho, D = O.cluster_terms(X[list(r)], par, 0)
+        Bd, _, _ = O.cluster_terms(X[list(r)], par, 2)
+        items.append((A, Bd, rho, rho, D))
+    tau = np.linspace(0.0, 2e-5, 5)
+    print("threaded large batch of 3 items", flush=True)
+    b = O.HahnBatch.from_items(ctx, items, m=30, tol=1e-12, precision="fp64")
+    b.run(tau)
+    L, st = b.results()
+    assert (st[:, 8] & 0xFF).max() == 0
+    for i, it in enumerate(items):
+        one = O.HahnBatch.from_items(ctx, [it], m=30, tol=1e-12, precision="fp64")
+        one.run(tau)
+        L1, _ = one.results()
+        assert np.abs(L[i] - L1[0]).max() <= 1e-12, i

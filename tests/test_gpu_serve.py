@@ -61,3 +61,33 @@ def test_served_hahn_equals_batch_and_reduction(ctx):
     Lc, _ = b.results()
     want = C.reduce(Lc.reshape(-1), len(tl))["final"][:, -1]
     assert np.abs(L - want).max() <= 1e-12
+
+
+@pytest.mark.gpu
+def test_failed_clusters_fail_the_request(ctx, monkeypatch):
+    """A step limit forced from the host (OOPS_MAX_STEPS) makes the 4-spin
+    clusters' integrations fail: the batch reports their failure flags with
+    NaN in place of unwritten outputs, and the server fails the request
+    instead of reducing garbage; a request without failures still completes."""
+    X = O.bath_diamond(3.567, 15.0, 0.011, seed=3)
+    C = O.Clusters(X, 8.0, 4)
+    assert C.count(3) > 0
+    monkeypatch.setenv("OOPS_MAX_STEPS", "2")
+    b = C.hahn_batch(ctx, O.cce_params(B=B111), m=30, tol=1e-4, precision="mixed")
+    b.run(np.linspace(0.0, 5e-4, 11))
+    L, st = b.results()
+    four = slice(sum(C.counts()[:3]), None)
+    bad = (st[:, 8] & 0xFF) != 0
+    print(f"forced failures: {int(bad.sum())} of {len(bad)} clusters", flush=True)
+    assert bad[four].all() and not bad[:sum(C.counts()[:3])].any()
+    assert np.isnan(L[bad]).any()
+    s = O.Server(ctx, hbm_budget=1 << 30, max_inflight=2, tol=1e-4, threads=8)
+    rid = s.submit({"seed": 3, "radius": 15.0, "p": 0.011, "B": list(B111), "cutoff": 8.0, "order": 4,
+                    "npulse": 1, "nt": 11, "t_end": 1e-3})
+    ok = s.submit(_req(4, 15.0, 3, 1, 11))
+    s.drain()
+    assert s.state(rid) == -5 and s.failed(rid) == C.count(3)
+    with pytest.raises(O.OopsError):
+        s.result(rid)
+    assert s.state(ok) == 4 and s.failed(ok) == 0 and np.all(np.isfinite(s.result(ok)["L"]))
+    s.close()
