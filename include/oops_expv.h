@@ -189,7 +189,11 @@ typedef struct {
  * flip-mask form when every operator has at most 9 one-spin and 12 two-spin
  * flip masks: all products of 4-spin spin-1/2 Liouvillians), the others
  * Arnoldi + Pade.  2: mixed, always Arnoldi + Pade.  3: as 1 without the
- * flip-mask form (ELL product in the Lanczos kernel). */
+ * flip-mask form (ELL product in the Lanczos kernel).  4: FP64 throughout
+ * with the mixed path's algorithm for the buckets the flip-mask Lanczos
+ * kernel takes (Lanczos, Chebyshev exponential, output times inside steps,
+ * the same tolerance), the FP64 paths of precision 0 for the others: it
+ * isolates what the FP32 parts of precision 1 gain. */
 int oops_hahn_batch_create(oops_ctx* ctx, int32_t nitems, const oops_hahn_item* items, int32_t m, int32_t klim,
                            double tol, int32_t precision, oops_hahn_batch** out);
 /* Run every bucket for the ascending sweep times tau[0..nt) (device-resident
@@ -255,7 +259,7 @@ int oops_hahn_batch_elapsed(const oops_hahn_batch* b, double* ms);
  * 1 Arnoldi FP64, 2 Arnoldi mixed, 3 Lanczos mixed, 4 dense FP64-MFMA
  * propagator (n = 64), 5 single-state path (n > 256), 6 Lanczos mixed in the
  * flip-mask (slot) form, 7 Lanczos mixed for n = 1024 / 4096 (one 1024-thread
- * workgroup per cluster); -1 not run. */
+ * workgroup per cluster), 8 Lanczos FP64 in the flip-mask form; -1 not run. */
 int oops_hahn_batch_kernels(const oops_hahn_batch* b, int32_t max_buckets, int32_t* kind);
 int oops_hahn_batch_destroy(oops_hahn_batch* b);
 

@@ -113,7 +113,7 @@ def lib():
 
 
 KERNEL_NAMES = {0: "dense_pade", 1: "arnoldi_fp64", 2: "arnoldi_mixed", 3: "lanczos_mixed", 4: "dense64_mfma",
-                5: "single_state", 6: "lanczos_slot", 7: "lanczos_medium", -1: None}
+                5: "single_state", 6: "lanczos_slot", 7: "lanczos_medium", 8: "lanczos_slot_fp64", -1: None}
 
 
 def _ptr(a):
@@ -121,7 +121,7 @@ def _ptr(a):
 
 
 def _precision_code(p):
-    codes = {"fp64": 0, "mixed": 1, "mixed-arnoldi": 2, "mixed-ell": 3}
+    codes = {"fp64": 0, "mixed": 1, "mixed-arnoldi": 2, "mixed-ell": 3, "fp64-lanczos": 4}
     if p not in codes:
         raise ValueError(f"precision must be one of {sorted(codes)}")
     return codes[p]
@@ -694,6 +694,22 @@ class Clusters:
                                               ctypes.c_int32(_precision_code(precision)), ctypes.byref(h))
         _check(rc, ctx.h, "oops_cce_hahn_batch_create")
         return HahnBatch(ctx, h, sum(self.counts()) if sel is None else nsel)
+
+    def echo_batch(self, ctx: Context, params: CCEParamsC, npulse, t, m=30, klim=10, tol=1e-3, precision="mixed"):
+        """Device batch of every cluster under a Hahn echo (npulse 1; t holds
+        the sweep times t/2) or CPMG-N (t holds the times t), one time grid for
+        all clusters (oops_cce_echo_batch_create)."""
+        t = np.ascontiguousarray(t, np.float64)
+        h = ctypes.c_void_p()
+        rc = lib().oops_cce_echo_batch_create(ctx.h, _ptr(self.coords), self.h, ctypes.byref(params),
+                                              ctypes.c_int32(npulse), ctypes.c_int32(len(t)), _ptr(t),
+                                              ctypes.c_int32(m), ctypes.c_int32(klim), ctypes.c_double(tol),
+                                              ctypes.c_int32(_precision_code(precision)), ctypes.byref(h))
+        _check(rc, ctx.h, "oops_cce_echo_batch_create")
+        nclu = sum(self.counts())
+        b = EchoBatch(ctx, h, nclu)
+        b.nts = [len(t)] * nclu
+        return b
 
     def close(self):
         if self.h:

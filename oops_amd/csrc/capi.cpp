@@ -244,6 +244,7 @@ using namespace oops;
 struct oops_hahn_batch {
     oops_ctx* ctx = nullptr;
     int32_t nitems = 0, m = 30, klim = 10, precision = 0;
+    bool lanczos64 = false;   // precision 4: FP64 slot Lanczos for the buckets that fit it, else the FP64 paths
     double tol = 1e-12;
     struct Bucket {
         int n = 0;
@@ -359,11 +360,14 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
                        double tol, int32_t precision, oops_hahn_batch** out) {
     if (!ctx || !out || nitems < 0 || (nitems > 0 && !items)) return set_err(ctx, OOPS_ERR_ARG, "bad arguments");
     *out = nullptr;
-    if (precision < 0 || precision > 3)
+    if (precision < 0 || precision > 4)
         return set_err(ctx, OOPS_ERR_ARG,
-                       "precision must be 0 (FP64), 1 (mixed), 2 (mixed, Arnoldi) or 3 (mixed, ELL product)");
+                       "precision must be 0 (FP64), 1 (mixed), 2 (mixed, Arnoldi), 3 (mixed, ELL product) or 4 "
+                       "(FP64 Lanczos in the flip-mask form)");
     const bool ell_only = precision == 3;   // mixed Lanczos without the slot form
     if (ell_only) precision = 1;
+    const bool lanczos64 = precision == 4;  // FP64 with the mixed path's Lanczos algorithm and schedule
+    if (lanczos64) precision = 0;
     if (m != 20 && m != 30) return set_err(ctx, OOPS_ERR_UNSUPPORTED, "Krylov dimension must be 20 or 30");
     if (klim < 1 || klim > 16) return set_err(ctx, OOPS_ERR_ARG, "klim must be in [1, 16]");
     std::unique_ptr<oops_hahn_batch> b(new oops_hahn_batch());
@@ -373,6 +377,7 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
     b->klim = klim;
     b->tol = tol;
     b->precision = precision;
+    b->lanczos64 = lanczos64;
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
     // compile all operators, bucket by n
     std::map<int64_t, int> bucket_of;
@@ -422,7 +427,8 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
                 uniform_item = std::fabs(I.t[k] - k * dt) <= 1e-12 * I.t[I.nt - 1];
         }
         if (cpmg && !(A.n <= m || (A.n == 64 && uniform_item) ||
-                      (precision == 1 && A.hermitian && B.hermitian && Cc.hermitian && Dd.hermitian)))
+                      ((precision == 1 || (lanczos64 && A.n <= 256)) && A.hermitian && B.hermitian &&
+                       Cc.hermitian && Dd.hermitian)))
             return set_err(ctx, OOPS_ERR_UNSUPPORTED, "item " + std::to_string(it) +
                            ": CPMG-N runs on the dense paths (n <= m, or n = 64 on a uniform grid) or the mixed "
                            "Lanczos path (Hermitian generators)");
@@ -548,7 +554,7 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
         }
     }
     for (auto& B : b->buckets)
-        if (B.cpmg && !(B.n <= m || (B.n == 64 && B.uniform) || (precision == 1 && B.hermitian)))
+        if (B.cpmg && !(B.n <= m || (B.n == 64 && B.uniform) || ((precision == 1 || lanczos64) && B.hermitian)))
             return set_err(ctx, OOPS_ERR_UNSUPPORTED,
                            "CPMG-N items of dimension " + std::to_string(B.n) +
                            " need one uniform time grid from 0 for the whole bucket, or mixed precision");
@@ -608,17 +614,22 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
                 B.medtab.bytes = tb;
             }
         }
-        B.slots = B.slots && !B.medium && !ell_only && precision == 1 && B.hermitian && !dense && B.n <= 256 && (m == 20 || m == 30);
+        B.slots = B.slots && !B.medium && !ell_only && (precision == 1 || lanczos64) && B.hermitian && !dense &&
+                  B.n <= 256 && (m == 20 || m == 30);
         if (B.slots) {
-            // two workgroups per CU; 2 (Hahn) or 4 (CPMG-N) coefficient tables each
-            B.slot_grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu * 2));
+            // mixed: two workgroups per CU, FP64 variant: one; 2 (Hahn) or 4
+            // (CPMG-N) coefficient tables each
+            B.slot_grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu * (lanczos64 ? 1 : 2)));
             rc = rc ? rc : dev_upload(ctx, B.eslot, H.eslot);
             rc = rc ? rc : dev_upload(ctx, B.smask, H.smask);
-            const size_t bytes = sizeof(double) * 2 * (size_t)kSlotOpD2 * 4 * (size_t)B.slot_grid;
+            const size_t bytes = sizeof(double) * 2 * (size_t)slot_table_double2(lanczos64) * 4 * (size_t)B.slot_grid;
             rc = rc ? rc : hip_check(ctx, hipMalloc(&B.slotws.p, bytes), "hipMalloc slot tables");
             if (!rc) B.slotws.bytes = bytes;
             if (rc) return rc;
         }
+        if (lanczos64 && B.cpmg && !B.slots && !(B.n <= m || (B.n == 64 && B.uniform)))
+            return set_err(ctx, OOPS_ERR_UNSUPPORTED, "FP64 Lanczos CPMG-N needs the flip-mask form (n = " +
+                                                          std::to_string(B.n) + ")");
         if (per_item && (B.slots || B.medium)) {
             int ntmax = 0;
             for (size_t i = 0; i < H.nt.size(); ++i)
@@ -787,6 +798,10 @@ static int bucket_launch(oops_hahn_batch* b, oops_hahn_batch::Bucket& B, const K
         B.kernel = 7;
         launch_zero_counter(a.queue, st);
         lrc = launch_hahn_medium(a, (float2*)B.medws.p, b->m, B.medium_grid, st);
+    } else if (b->lanczos64 && B.slots && B.slotws.p && B.hermitian && B.n > b->m) {
+        B.kernel = 8;
+        launch_zero_counter(a.queue, st);
+        lrc = launch_hahn_lanczos_slot(a, b->m, B.slot_grid, st, true);
     } else if (lanczos && B.slots && B.slotws.p) {
         B.kernel = 6;
         launch_zero_counter(a.queue, st);

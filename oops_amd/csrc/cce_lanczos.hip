@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
 #include <utility>
 
 #include "cheb.h"
@@ -301,9 +302,12 @@ __global__ __launch_bounds__(kNT) void cce_hahn_lanczos_kernel(KrylovArgs a) {
 // column i), which leaves registers for the coefficients and fits two
 // workgroups per CU.
 
+template <bool F64>
 struct SlotOp {
+    // pair-flip coefficients: complex FP32 (mixed) or FP64 (the FP64 variant)
+    typedef typename std::conditional<F64, cd, float2>::type PairC;
     cd c[kSlot64];
-    float2 f[kSlot32];
+    PairC f[kSlot32];
     int mc[kSlot64];   // masks (uniform)
     int mf[kSlot32];
     int n;
@@ -314,22 +318,32 @@ struct SlotOp {
         // opaque column index: keeps the 2 x 21 per-slot addresses of every
         // call site from being hoisted (and held) across the kernel's loops
         asm volatile("" : "+v"(i));
-        const float2* t32 = reinterpret_cast<const float2*>(tab + kSlot64 * 256);
 #pragma unroll
         for (int s = 0; s < kSlot64; ++s) {
             const double2 v = tab[s * 256 + i];
             c[s] = mkcd(v.x, v.y);
             mc[s] = masks[s];
         }
+        if constexpr (F64) {
+            const double2* t2 = tab + kSlot64 * 256;
 #pragma unroll
-        for (int s = 0; s < kSlot32; ++s) {
-            f[s] = t32[s * 256 + i];
-            mf[s] = masks[kSlot64 + s];
+            for (int s = 0; s < kSlot32; ++s) {
+                const double2 v = t2[s * 256 + i];
+                f[s] = mkcd(v.x, v.y);
+                mf[s] = masks[kSlot64 + s];
+            }
+        } else {
+            const float2* t32 = reinterpret_cast<const float2*>(tab + kSlot64 * 256);
+#pragma unroll
+            for (int s = 0; s < kSlot32; ++s) {
+                f[s] = t32[s * 256 + i];
+                mf[s] = masks[kSlot64 + s];
+            }
         }
     }
 
     // element i of A x with x staged in LDS (FP64 copy xs, FP32 copy xs32).
-    // Three rounds of 3 FP64 + 4 FP32 gathers bound the registers in flight;
+    // Three rounds of 3 FP64 + 4 pair gathers bound the registers in flight;
     // the empty asm keeps the gather addresses from being hoisted out of the
     // caller's loops (21 more live registers otherwise).
     __device__ __forceinline__ cd apply(const cd* __restrict__ xs, const float2* __restrict__ xs32, int i) const {
@@ -342,18 +356,28 @@ struct SlotOp {
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
             cd x64[3];
-            float2 x32[4];
 #pragma unroll
             for (int t = 0; t < 3; ++t) x64[t] = xs[ii ^ mc[3 * r + t]];
+            if constexpr (F64) {
+                cd xp[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) x32[t] = xs32[ii ^ mf[4 * r + t]];
+                for (int t = 0; t < 4; ++t) xp[t] = xs[ii ^ mf[4 * r + t]];
 #pragma unroll
-            for (int t = 0; t < 3; ++t) acc = cfma(c[3 * r + t], x64[t], acc);
+                for (int t = 0; t < 3; ++t) acc = cfma(c[3 * r + t], x64[t], acc);
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const float2 cf = f[4 * r + t];
-                ar = fmaf(cf.x, x32[t].x, fmaf(-cf.y, x32[t].y, ar));
-                ai = fmaf(cf.x, x32[t].y, fmaf(cf.y, x32[t].x, ai));
+                for (int t = 0; t < 4; ++t) acc = cfma(f[4 * r + t], xp[t], acc);
+            } else {
+                float2 x32[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) x32[t] = xs32[ii ^ mf[4 * r + t]];
+#pragma unroll
+                for (int t = 0; t < 3; ++t) acc = cfma(c[3 * r + t], x64[t], acc);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const float2 cf = f[4 * r + t];
+                    ar = fmaf(cf.x, x32[t].x, fmaf(-cf.y, x32[t].y, ar));
+                    ai = fmaf(cf.x, x32[t].y, fmaf(cf.y, x32[t].x, ai));
+                }
             }
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -362,14 +386,17 @@ struct SlotOp {
 };
 
 // Coefficient table of operator `op` for this thread's element: FP64 slots
-// as double2 [kSlot64][256], then FP32 slots as float2 [kSlot32][256].  Every
-// thread writes and reads only its own column.
+// as double2 [kSlot64][256], then the pair slots as float2 [kSlot32][256]
+// (mixed) or double2 [kSlot32][256] (F64).  Every thread writes and reads
+// only its own column.
+template <bool F64>
 __device__ __noinline__ void slot_build(const KrylovArgs& a, int op, double2* __restrict__ tab, int i) {
+    typedef typename std::conditional<F64, double2, float2>::type PairT;
     const DevOpHeader h = a.hdr[op];
     asm volatile("" : "+v"(i));
-    float2* t32 = reinterpret_cast<float2*>(tab + kSlot64 * 256);
+    PairT* t2 = reinterpret_cast<PairT*>(tab + kSlot64 * 256);
     for (int s = 0; s < kSlot64; ++s) tab[s * 256 + i] = make_double2(0.0, 0.0);
-    for (int s = 0; s < kSlot32; ++s) t32[s * 256 + i] = make_float2(0.f, 0.f);
+    for (int s = 0; s < kSlot32; ++s) t2[s * 256 + i] = PairT{0, 0};
     if (i >= h.n) return;
     tab[i] = make_double2(h.c0re, h.c0im);   // slot 0: mask 0
     const int ld = h.ld_bits;
@@ -385,10 +412,15 @@ __device__ __noinline__ void slot_build(const KrylovArgs& a, int op, double2* __
             const int sl = a.eslot[e0 + e];
             // this thread's own column: read-modify-write in program order
             if (sl & 128) {
-                float2 t = t32[(sl & 127) * 256 + i];
-                t.x += (float)v.x;
-                t.y += (float)v.y;
-                t32[(sl & 127) * 256 + i] = t;
+                PairT t = t2[(sl & 127) * 256 + i];
+                if constexpr (F64) {
+                    t.x += v.x;
+                    t.y += v.y;
+                } else {
+                    t.x += (float)v.x;
+                    t.y += (float)v.y;
+                }
+                t2[(sl & 127) * 256 + i] = t;
             } else {
                 double2 t = tab[sl * 256 + i];
                 t.x += v.x;
@@ -399,9 +431,10 @@ __device__ __noinline__ void slot_build(const KrylovArgs& a, int op, double2* __
     }
 }
 
-template <int M>
+template <int M, bool F64>
 struct LzSlotSmem {
-    float2 V[M][kNT];                   // v_1..v_M (FP32), column i owned by thread i
+    // v_1..v_M (FP32, or FP64 in the F64 variant), column i owned by thread i
+    typename std::conditional<F64, double2, float2>::type V[M][kNT];
     cd xs[256];
     float2 xs32[256];
     double red[3 * (kNT / kWave) * 64];
@@ -454,8 +487,9 @@ __device__ __forceinline__ void emit_state(BlockRed<kNT>& br, const SweepOut& o,
 // and sparing the clamped steps halves its Krylov work on output grids finer
 // than the natural step.  Mode 2 reads the states mode 1 stored for the same
 // output times.
-template <int M>
-__device__ __forceinline__ void advance_slot(LzSlotSmem<M>& S, BlockRed<kNT>& br, IntegState& g, const SlotOp& so,
+template <int M, bool F64>
+__device__ __forceinline__ void advance_slot(LzSlotSmem<M, F64>& S, BlockRed<kNT>& br, IntegState& g,
+                                             const SlotOp<F64>& so,
                                              double t_end, const SweepOut& o, int max_steps, double tol) {
     const int tid = threadIdx.x;
     const double delta = 1.2, gamma = 0.9;
@@ -507,7 +541,8 @@ __device__ __forceinline__ void advance_slot(LzSlotSmem<M>& S, BlockRed<kNT>& br
             }
             vprev = vj;
             vcur = cscale(p, 1.0 / beta);
-            S.V[j][tid] = make_float2((float)vcur.re, (float)vcur.im);
+            if constexpr (F64) S.V[j][tid] = make_double2(vcur.re, vcur.im);
+            else S.V[j][tid] = make_float2((float)vcur.re, (float)vcur.im);
             bprev = beta;
         }
         double avnorm = 0.0;
@@ -527,7 +562,7 @@ __device__ __forceinline__ void advance_slot(LzSlotSmem<M>& S, BlockRed<kNT>& br
             cd acc = cmul(v0, S.F[0]);
 #pragma unroll 1
             for (int j = 1; j < mx; ++j) {
-                const float2 f = S.V[j - 1][tid];
+                const auto f = S.V[j - 1][tid];
                 acc = cfma(mkcd((double)f.x, (double)f.y), S.F[j], acc);
             }
             return cscale(acc, g.beta);
@@ -592,15 +627,20 @@ __device__ __forceinline__ void advance_slot(LzSlotSmem<M>& S, BlockRed<kNT>& br
     }
 }
 
-template <int M>
-__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(2, 2)))
+// Coefficient-table stride (double2) of one operator in the slot kernels.
+template <bool F64>
+constexpr int slot_table_d2() { return F64 ? (kSlot64 + kSlot32) * 256 : kSlotOpD2; }
+
+template <int M, bool F64>
+__global__ __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(F64 ? 1 : 2, F64 ? 1 : 2)))
 void cce_hahn_lanczos_slot_kernel(KrylovArgs a) {
-    __shared__ LzSlotSmem<M> S;
+    __shared__ LzSlotSmem<M, F64> S;
+    constexpr int kTab = slot_table_d2<F64>();
     const int tid = threadIdx.x;
     BlockRed<kNT> br(S.red);
     const double tol = a.tol;
     const int max_steps = a.max_steps;
-    double2* const ws = a.slotws + (int64_t)blockIdx.x * 4 * kSlotOpD2;
+    double2* const ws = a.slotws + (int64_t)blockIdx.x * 4 * kTab;
 #ifdef OOPS_PHASE_TIMERS
     if (tid == 0) S.tbes = 0;
 #endif
@@ -619,22 +659,22 @@ void cce_hahn_lanczos_slot_kernel(KrylovArgs a) {
         const int qB = __builtin_amdgcn_readfirstlane(a.item_opB[item]);
         const int qC = cpmg ? __builtin_amdgcn_readfirstlane(a.item_opC[item]) : qA;
         const int qD = cpmg ? __builtin_amdgcn_readfirstlane(a.item_opD[item]) : qB;
-        slot_build(a, qA, ws, tid);
-        slot_build(a, qB, ws + kSlotOpD2, tid);
+        slot_build<F64>(a, qA, ws, tid);
+        slot_build<F64>(a, qB, ws + kTab, tid);
         if (cpmg) {
-            slot_build(a, qC, ws + 2 * kSlotOpD2, tid);
-            slot_build(a, qD, ws + 3 * kSlotOpD2, tid);
+            slot_build<F64>(a, qC, ws + 2 * kTab, tid);
+            slot_build<F64>(a, qD, ws + 3 * kTab, tid);
         }
         IntegState gA, gB;
         const cd u0 = tid < n ? mkcd(a.u0[voff + tid].x, a.u0[voff + tid].y) : mkcd(0.0, 0.0);
         const cd y0 = tid < n ? mkcd(a.y0[voff + tid].x, a.y0[voff + tid].y) : mkcd(0.0, 0.0);
         const NormWork nw{S.xs, S.xs32, S.tdiag, S.toff, S.scal};
         const double scale = a.scale[item];
-        SlotOp so;
+        SlotOp<F64> so;
         so.load(ws, a.smask + (int64_t)qA * kSlotMaskStride, tid, n);
         init_integ_mv<true>(nw, n, [&](int i) { return so.apply(S.xs, S.xs32, i); }, br, gA, u0, a.rnd, a.klim,
                             tol, M);
-        so.load(ws + kSlotOpD2, a.smask + (int64_t)qB * kSlotMaskStride, tid, n);
+        so.load(ws + kTab, a.smask + (int64_t)qB * kSlotMaskStride, tid, n);
         init_integ_mv<true>(nw, n, [&](int i) { return so.apply(S.xs, S.xs32, i); }, br, gB, y0, a.rnd, a.klim,
                             tol, M);
         const double beta0A = gA.beta, beta0B = gB.beta;
@@ -668,8 +708,8 @@ void cce_hahn_lanczos_slot_kernel(KrylovArgs a) {
                         target = g.t_now + cpmg_fraction(j, q.npulse) * q.tau[k];
                     }
                     const int qo = opi == 0 ? qA : (opi == 1 ? qB : (opi == 2 ? qC : qD));
-                    so.load(ws + opi * kSlotOpD2, a.smask + (int64_t)qo * kSlotMaskStride, tid, n);
-                    advance_slot<M>(S, br, g, so, target, o, max_steps, tol);
+                    so.load(ws + opi * kTab, a.smask + (int64_t)qo * kSlotMaskStride, tid, n);
+                    advance_slot<M, F64>(S, br, g, so, target, o, max_steps, tol);
                 }
                 const IntegState tmp = gA;
                 gA = gB;
@@ -705,13 +745,21 @@ void cce_hahn_lanczos_slot_kernel(KrylovArgs a) {
     }
 }
 
-int launch_hahn_lanczos_slot(const KrylovArgs& a, int m, int grid, hipStream_t stream) {
+int launch_hahn_lanczos_slot(const KrylovArgs& a, int m, int grid, hipStream_t stream, bool fp64) {
     if (a.n > 256 || !a.slotws || !a.eslot || !a.smask) return -11;
-    if (m == 20) hipLaunchKernelGGL((cce_hahn_lanczos_slot_kernel<20>), dim3(grid), dim3(kNT), 0, stream, a);
-    else if (m == 30) hipLaunchKernelGGL((cce_hahn_lanczos_slot_kernel<30>), dim3(grid), dim3(kNT), 0, stream, a);
-    else return -12;
+    if (fp64) {
+        if (m == 20) hipLaunchKernelGGL((cce_hahn_lanczos_slot_kernel<20, true>), dim3(grid), dim3(kNT), 0, stream, a);
+        else if (m == 30) hipLaunchKernelGGL((cce_hahn_lanczos_slot_kernel<30, true>), dim3(grid), dim3(kNT), 0, stream, a);
+        else return -12;
+    } else {
+        if (m == 20) hipLaunchKernelGGL((cce_hahn_lanczos_slot_kernel<20, false>), dim3(grid), dim3(kNT), 0, stream, a);
+        else if (m == 30) hipLaunchKernelGGL((cce_hahn_lanczos_slot_kernel<30, false>), dim3(grid), dim3(kNT), 0, stream, a);
+        else return -12;
+    }
     return (int)hipGetLastError();
 }
+
+int64_t slot_table_double2(bool fp64) { return fp64 ? slot_table_d2<true>() : slot_table_d2<false>(); }
 
 int launch_hahn_lanczos(const KrylovArgs& a, int m, int grid, hipStream_t stream) {
     if (a.n > 256) return -11;

@@ -73,10 +73,13 @@ int launch_hahn_bucket(const KrylovArgs& a, int m, int precision, int grid, hipS
 // Mixed-precision Lanczos kernel for buckets whose generators are all Hermitian
 // (n > m); the caller zeroes the work counter first.
 int launch_hahn_lanczos(const KrylovArgs& a, int m, int grid, hipStream_t stream);
-// Slot form of the Lanczos kernel (two workgroups per CU): every operator of
-// the bucket has at most kSlot64 / kSlot32 slots; a.slotws holds
-// grid * 4 * kSlotOpD2 double2.  The caller zeroes the work counter first.
-int launch_hahn_lanczos_slot(const KrylovArgs& a, int m, int grid, hipStream_t stream);
+// Slot form of the Lanczos kernel: every operator of the bucket has at most
+// kSlot64 / kSlot32 slots; a.slotws holds grid * 4 * slot_table_double2(fp64)
+// double2.  fp64 false: pair coefficients and the basis in FP32 (two
+// workgroups per CU); true: everything in FP64, the same algorithm, schedule
+// and tolerance (one workgroup per CU).  The caller zeroes the work counter.
+int launch_hahn_lanczos_slot(const KrylovArgs& a, int m, int grid, hipStream_t stream, bool fp64 = false);
+int64_t slot_table_double2(bool fp64);
 // Mixed Lanczos kernel for n = 1024 and 4096 (1024 threads per cluster, the
 // product input and the active generator's tables in LDS, the FP32 basis in
 // `work`: medium_workspace_float2(grid, m, n) float2).  Hermitian generators,

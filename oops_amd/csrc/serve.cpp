@@ -100,9 +100,9 @@ int64_t estimate_bytes(const CceItems& it, int num_cu, int m, int precision) {
         const Dim& d = kv.second;
         if (n == 64) {
             b += 8 * dense64_workspace_doubles((int)std::min<int64_t>(d.items, num_cu));
-        } else if (n > m && n <= 256 && precision == 1) {
-            const int64_t grid = std::min<int64_t>(d.items, 2 * (int64_t)num_cu);
-            b += c16 * kSlotOpD2 * 4 * grid;
+        } else if (n > m && n <= 256 && (precision == 1 || precision == 4)) {
+            const int64_t grid = std::min<int64_t>(d.items, (precision == 4 ? 1 : 2) * (int64_t)num_cu);
+            b += c16 * slot_table_double2(precision == 4) * 4 * grid;
             b += c16 * 256 * d.nt_hahn * std::min<int64_t>(d.hahn, grid);
         } else if ((n == 1024 || n == 4096) && precision == 1) {
             const int64_t grid = std::min<int64_t>(2 * d.items, num_cu);

@@ -209,3 +209,104 @@ def test_hahn_batch_large_items_on_threads_equal_single_items(ctx):
         one.run(tau)
         L1, _ = one.results()
         assert np.abs(L[i] - L1[0]).max() <= 1e-12, i
+
+
+# --------------------------------------------------------------- single large cluster at full size
+def _sphere(k, radius=15.0, seed=0):
+    """k positions uniform in a sphere of `radius` A (the large-cluster workload)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < k:
+        p = rng.uniform(-radius, radius, 3)
+        if np.linalg.norm(p) <= radius:
+            out.append(p)
+    return np.array(out)
+
+
+def _unit(rng, n):
+    v = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    return v / np.linalg.norm(v)
+
+
+def _oracle_table(T, sign=1.0):
+    return P.TermTable(T.spin_dim.size, sign * np.asarray(T.coeff), T.nbody, T.pos_offset, T.pos, T.dims,
+                       T.mat_offset, T.mats, T.spin_dim, T.n)
+
+
+def _vec_outer(a, b, k):
+    """vec(|a><b|) in the Liouville ordering: spin s contributes the local
+    index ket_s + 2 bra_s, spin 0 the slowest digit."""
+    D = 1 << k
+    I, J = np.meshgrid(np.arange(D), np.arange(D), indexing="ij")
+    idx = np.zeros((D, D), np.int64)
+    for s in range(k):
+        sh = k - 1 - s
+        idx = idx * 4 + ((I >> sh) & 1) + 2 * ((J >> sh) & 1)
+    v = np.zeros(D * D, np.complex128)
+    v[idx.ravel()] = np.outer(a, b.conj()).ravel()
+    return v
+
+
+@pytest.mark.gpu
+def test_config4_liouville10_rank1_state_equals_two_hilbert_evolutions(ctx):
+    """10 spins in a 1.5 nm sphere, lvA (n = 2^20), m = 40.  With
+    lvA vec(rho) = vec(rho H0 - H1 rho), a rank-1 start |a><b| evolves into
+    exp(i H1 t)|a><b|exp(-i H0 t): the whole 2^20-element state must equal the
+    outer product of two 1024-element Hilbert evolutions, which the oracle
+    computes.  Also: unit norm kept to 1e-10 and the FP32 basis within 1e-6."""
+    k = 10
+    X = _sphere(k)
+    par = O.cce_params(B=B111)
+    A, _, _ = O.cluster_terms(X, par, 0)
+    H0, _, _ = O.cluster_terms(X, par, 3)
+    H1, _, _ = O.cluster_terms(X, par, 4)
+    assert A.n == 1 << 20 and O.tile_plan(A) == 6
+    rng = np.random.default_rng(1)
+    a, b = _unit(rng, 1 << k), _unit(rng, 1 << k)
+    v = _vec_outer(a, b, k)
+    tl = np.arange(11) * 1e-6
+    full_t = tl[[0, 5, 10]]
+    print(f"liouville-10 n={A.n} m=40: full states at {len(full_t)} times", flush=True)
+    W, st = O.expv(ctx, A, v, full_t, m=40, tol=1e-12)
+    at, _, rc1 = P.expv(_oracle_table(H1, -1.0), a, full_t, m=40, tol=1e-12)
+    bt, _, rc2 = P.expv(_oracle_table(H0, -1.0), b, full_t, m=40, tol=1e-12)
+    assert rc1 == 0 and rc2 == 0
+    for q in range(len(full_t)):
+        want = _vec_outer(at[:, q], bt[:, q], k)
+        err = np.abs(W[:, q] - want).max()
+        print(f"  t={full_t[q]:.0e}: max |w - outer| {err:.2e}, |norm - 1| {abs(np.linalg.norm(W[:, q]) - 1):.1e}",
+              flush=True)
+        assert err <= 1e-10
+        assert abs(np.linalg.norm(W[:, q]) - 1.0) <= 1e-10
+    ov64, s64 = O.expv(ctx, A, v, tl, m=40, tol=1e-12, overlaps=True)
+    ov32, s32 = O.expv(ctx, A, v, tl, m=40, tol=1e-12, overlaps=True, precision="fp32")
+    print(f"  fp32 basis vs fp64: max |d<v,w>| {np.abs(ov32 - ov64).max():.2e} "
+          f"({s64['nstep']:.0f} / {s32['nstep']:.0f} Krylov steps)", flush=True)
+    assert np.abs(ov32 - ov64).max() <= 1e-6
+    assert np.abs(ov64[[0, 5, 10]] - v.conj() @ W).max() <= 1e-12
+
+
+@pytest.mark.gpu
+def test_config4_hilbert14_matches_oracle(ctx):
+    """14 spins in a 1.5 nm sphere, the m_s = 0 conditional Hamiltonian
+    (n = 16384), m = 40: states against the oracle's Krylov driver to 1e-10,
+    unit norm to 1e-10, and the FP32 basis within 1e-6 over 11 outputs 1 us
+    apart."""
+    X = _sphere(14)
+    T, _, _ = O.cluster_terms(X, O.cce_params(B=B111), 3)
+    assert T.n == 16384
+    v = _unit(np.random.default_rng(1), T.n)
+    tl = np.array([0.0, 1e-6, 2e-6])
+    print(f"hilbert-14 n={T.n} m=40", flush=True)
+    W, st = O.expv(ctx, T, v, tl, m=40, tol=1e-12)
+    Wr, sr, rc = P.expv(_oracle_table(T), v, tl, m=40, tol=1e-12)
+    assert rc == 0
+    err = np.abs(W - Wr).max()
+    print(f"  max |w - oracle| {err:.2e}; steps {st['nstep']:.0f} vs {sr.nstep}", flush=True)
+    assert err <= 1e-10
+    assert np.abs(np.linalg.norm(W, axis=0) - 1.0).max() <= 1e-10
+    t11 = np.arange(11) * 1e-6
+    ov64, _ = O.expv(ctx, T, v, t11, m=40, tol=1e-12, overlaps=True)
+    ov32, _ = O.expv(ctx, T, v, t11, m=40, tol=1e-12, overlaps=True, precision="fp32")
+    print(f"  fp32 basis vs fp64: max |d<v,w>| {np.abs(ov32 - ov64).max():.2e}", flush=True)
+    assert np.abs(ov32 - ov64).max() <= 1e-6

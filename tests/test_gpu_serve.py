@@ -91,3 +91,50 @@ def test_failed_clusters_fail_the_request(ctx, monkeypatch):
         s.result(rid)
     assert s.state(ok) == 4 and s.failed(ok) == 0 and np.all(np.isfinite(s.result(ok)["L"]))
     s.close()
+
+
+@pytest.mark.gpu
+def test_serving_mix_at_size_equals_batches_and_fp64(ctx):
+    """Three requests drawn from the serving mix at its sizes (r 3-5 nm,
+    CCE 2-4, CPMG-N up to N = 16, 101-401 points on 0-2 ms, B 0.01-0.5 T
+    along [111]) served together: every served L(t) equals the batch API plus
+    the host CCE reduction on the same bath (1e-12), and the CCE-4 request's
+    clusters agree with FP64 (m = 30, tol = 1e-12) to 1e-6 at every point."""
+    t_end, m, tol = 2e-3, 30, 3e-3
+    reqs = [{"seed": 11, "radius": 30.0, "order": 4, "npulse": 1, "nt": 401, "field": 0.05},
+            {"seed": 12, "radius": 40.0, "order": 3, "npulse": 16, "nt": 201, "field": 0.3},
+            {"seed": 13, "radius": 50.0, "order": 2, "npulse": 8, "nt": 101, "field": 0.01}]
+    s = O.Server(ctx, hbm_budget=32 << 30, max_inflight=4, m=m, tol=tol, threads=16)
+    ids = []
+    for r in reqs:
+        q = {"seed": r["seed"], "radius": r["radius"], "p": 0.011, "B": list(np.ones(3) * r["field"] / np.sqrt(3.0)),
+             "cutoff": 8.0, "order": r["order"], "npulse": r["npulse"], "nt": r["nt"], "t_end": t_end}
+        ids.append(s.submit(q))
+    s.drain()
+    served = [s.result(i) for i in ids]
+    s.close()
+    for r, got in zip(reqs, served):
+        X = O.bath_diamond(3.567, r["radius"], 0.011, seed=r["seed"])
+        C = O.Clusters(X, 8.0, r["order"])
+        par = O.cce_params(B=np.ones(3) * r["field"] / np.sqrt(3.0))
+        t = t_end * np.arange(r["nt"]) / (r["nt"] - 1)
+        if r["npulse"] == 1:
+            t = t * 0.5
+        b = C.echo_batch(ctx, par, r["npulse"], t, m=m, tol=tol, precision="mixed")
+        b.run()
+        Ls, st = b.results()
+        assert ((st[:, 8] & 0xFF) == 0).all()
+        Lc = np.stack(Ls)
+        want = C.reduce(Lc.reshape(-1), r["nt"])["final"][:, -1]
+        print(f"request r={r['radius']} CCE-{r['order']} N={r['npulse']} nt={r['nt']}: {got['clusters']} clusters, "
+              f"L(t_end) {got['L'][-1]:.6f}, max |served - batch| {np.abs(got['L'] - want).max():.1e}", flush=True)
+        assert got["clusters"] == sum(C.counts())
+        assert np.abs(got["L"] - want).max() <= 1e-12
+        if r["order"] == 4:
+            ref = C.hahn_batch(ctx, par, m=30, tol=1e-12, precision="fp64")
+            ref.run(t)
+            L64, st64 = ref.results()
+            assert ((st64[:, 8] & 0xFF) == 0).all()
+            d = np.abs(Lc - L64).max()
+            print(f"  CCE-4 request clusters vs FP64: max |dL| {d:.2e}", flush=True)
+            assert d <= 1e-6
