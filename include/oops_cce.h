@@ -22,6 +22,8 @@
  *   oops_cce_hahn_batch_create <- CCE::run_each_clusters (source/app/cce.cpp:54-70)
  *                               with EnsembleCCE::cluster_evolution for a Hahn echo
  *                               (source/app/ensemble_cce.cpp:33-55) on the device expv
+ *   oops_bath_states, oops_cce_single_sample_* <- SingleSampleCCE
+ *                               (source/app/single_sample_cce.cpp:35-128)
  *   oops_bath_diamond        <- no reference counterpart: the seeded synthetic
  *                               13C bath of the benchmark workloads.
  */
@@ -88,6 +90,37 @@ int oops_clusters_get(const oops_clusters* C, int32_t order, int32_t* idx);
 /* Parent positions (in order-1) of each cluster: CSR offsets [count+1] + list. */
 int oops_clusters_parents(const oops_clusters* C, int32_t order, int64_t* offsets, int32_t* list);
 int oops_clusters_destroy(oops_clusters* C);
+
+/* --- single-sample CCE (a pure bath state) ---------------------------------
+ * Bath states of SingleSampleCCE::prepare_bath_state
+ * (source/app/single_sample_cce.cpp:35-46): srand(seed), then spin i in
+ * spin-list order is in basis state rand() % 2 (0: m = +1/2, 1: m = -1/2),
+ * with glibc's rand() restated (the process's own rand() state is untouched). */
+int oops_bath_states(uint32_t seed, int32_t nspin, int32_t* states);
+
+/* Hilbert-space conditional Hamiltonian of one cluster with the bath's mean
+ * field (SingleSampleCCE::create_spin_hamiltonian, single_sample_cce.cpp:75-93:
+ * dipolar + Zeeman + NV field + sum over the bath spins outside the cluster of
+ * their dipolar field in their basis states).  coords: all nspin bath spins;
+ * cluster: k ascending spin indices; which 0: H0 (NV state0), 1: H1 (state1);
+ * psi (2^k complex, may be NULL): the cluster's product state
+ * (create_cluster_state, :95-105). */
+int oops_cce_single_sample_terms(const double* coords, int32_t nspin, const int32_t* cluster, int32_t k,
+                                 const oops_cce_params* P, const int32_t* states, int32_t which, oops_terms** out,
+                                 double* psi);
+
+/* Batch of every cluster of C under the single-sample evolution
+ * (SingleSampleCCE::cluster_evolution, single_sample_cce.cpp:48-73):
+ * psi_1(t) = riffle(H0, H1, N) segments and psi_2(t) = riffle(H1, H0, N)
+ * segments applied to psi (Pulse_Interval fractions, npulse = 1: Hahn echo),
+ * L_c(t) = Re <psi_1(t), psi_2(t)> (calc_observables, :120-128) at the times
+ * t[0..nt).  States of dimension <= 16 run on the dense path, larger ones on
+ * the flip-mask Lanczos kernel (precision 1, or 4 for FP64).  Results through
+ * oops_hahn_batch_results; launch with oops_echo_batch_run. */
+int oops_cce_single_sample_batch_create(oops_ctx* ctx, const double* coords, const oops_clusters* C,
+                                        const oops_cce_params* P, const int32_t* states, int32_t npulse, int32_t nt,
+                                        const double* t, int32_t m, int32_t klim, double tol, int32_t precision,
+                                        oops_hahn_batch** out);
 
 /* CCE reduction of L (all clusters' rows, orders concatenated, nt columns):
  * tilde (same shape), final_result / each_order (nt x max_order, row-major),

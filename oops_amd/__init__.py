@@ -61,7 +61,8 @@ EXPORTS = [
     "oops_cce_echo_batch_create", "oops_server_create", "oops_server_submit", "oops_server_poll", "oops_server_state",
     "oops_server_result", "oops_server_pending", "oops_server_destroy", "oops_server_times",
     "oops_hahn_batch_kernels", "oops_echo_batch_capture", "oops_slot_form", "oops_tile_plan", "oops_read_xyz", "oops_write_xyz",
-    "oops_hahn_batch_elapsed", "oops_server_failed",
+    "oops_hahn_batch_elapsed", "oops_server_failed", "oops_bath_states", "oops_cce_single_sample_terms",
+    "oops_cce_single_sample_batch_create",
 ]
 
 
@@ -572,17 +573,8 @@ def nv_states(pos, B):
     return ev, vec.reshape(3, 3, order="F")
 
 
-def cluster_terms(coords, params: CCEParamsC, which=0):
-    """Term table of one cluster: which 0 lvA, 1 lvB, 2 -lvB^dagger, 3 H0, 4 H1.
-    Returns (TermTable copy, rho0, D)."""
-    X = np.ascontiguousarray(coords, float).reshape(-1, 3)
-    k = X.shape[0]
-    n = 4 ** k
-    rho = np.zeros(n, np.complex128)
-    D = ctypes.c_double()
-    h = ctypes.c_void_p()
-    _check(lib().oops_cce_cluster_terms(_ptr(X), ctypes.c_int32(k), ctypes.byref(params), ctypes.c_int32(which),
-                                        ctypes.byref(h), _ptr(rho), ctypes.byref(D)), None, "oops_cce_cluster_terms")
+def _terms_from_handle(h):
+    """Copy an oops_terms handle into a TermTable and destroy the handle."""
     try:
         v = TermTableC()
         _check(lib().oops_terms_view(h, ctypes.byref(v)), None, "oops_terms_view")
@@ -604,9 +596,23 @@ def cluster_terms(coords, params: CCEParamsC, which=0):
         mat_off = arr(v.mat_offset, npos.value + 1, ctypes.c_int64)
         mats = arr(v.mats, 2 * nmat.value, ctypes.c_double).view(np.complex128)
         spin_dim = arr(v.spin_dim, v.nspin, ctypes.c_int64)
-        tt = TermTable(coeff, nbody, pos_off, pos, dims, mat_off, mats, spin_dim)
+        return TermTable(coeff, nbody, pos_off, pos, dims, mat_off, mats, spin_dim)
     finally:
         lib().oops_terms_destroy(h)
+
+
+def cluster_terms(coords, params: CCEParamsC, which=0):
+    """Term table of one cluster: which 0 lvA, 1 lvB, 2 -lvB^dagger, 3 H0, 4 H1.
+    Returns (TermTable copy, rho0, D)."""
+    X = np.ascontiguousarray(coords, float).reshape(-1, 3)
+    k = X.shape[0]
+    n = 4 ** k
+    rho = np.zeros(n, np.complex128)
+    D = ctypes.c_double()
+    h = ctypes.c_void_p()
+    _check(lib().oops_cce_cluster_terms(_ptr(X), ctypes.c_int32(k), ctypes.byref(params), ctypes.c_int32(which),
+                                        ctypes.byref(h), _ptr(rho), ctypes.byref(D)), None, "oops_cce_cluster_terms")
+    tt = _terms_from_handle(h)
     if which >= 3:
         rho = rho[: 2 ** k]
     return tt, rho, D.value
@@ -711,6 +717,26 @@ class Clusters:
         b.nts = [len(t)] * nclu
         return b
 
+    def single_sample_batch(self, ctx: Context, params: CCEParamsC, states, npulse, t, m=30, klim=10, tol=1e-12,
+                            precision="fp64"):
+        """Device batch of every cluster under the single-sample evolution of
+        a pure bath state (oops_cce_single_sample_batch_create): psi_1 under
+        riffle(H0, H1, N), psi_2 under riffle(H1, H0, N), L = Re <psi_1, psi_2>
+        at the times t (uniform grids from 0 as the reference assumes)."""
+        t = np.ascontiguousarray(t, np.float64)
+        st = np.ascontiguousarray(states, np.int32)
+        h = ctypes.c_void_p()
+        rc = lib().oops_cce_single_sample_batch_create(ctx.h, _ptr(self.coords), self.h, ctypes.byref(params),
+                                                       _ptr(st), ctypes.c_int32(npulse), ctypes.c_int32(len(t)),
+                                                       _ptr(t), ctypes.c_int32(m), ctypes.c_int32(klim),
+                                                       ctypes.c_double(tol), ctypes.c_int32(_precision_code(precision)),
+                                                       ctypes.byref(h))
+        _check(rc, ctx.h, "oops_cce_single_sample_batch_create")
+        nclu = sum(self.counts())
+        b = EchoBatch(ctx, h, nclu)
+        b.nts = [len(t)] * nclu
+        return b
+
     def close(self):
         if self.h:
             lib().oops_clusters_destroy(self.h)
@@ -729,3 +755,25 @@ def dlaran_fill(iseed, count):
     _check(lib().oops_dlaran_fill(seed, ctypes.c_int64(count), _ptr(out)), None, "oops_dlaran_fill")
     iseed[:] = list(seed)
     return out
+
+
+def bath_states(seed, nspin):
+    """Basis state (0: m = +1/2, 1: m = -1/2) of every bath spin for a
+    single-sample bath: glibc rand() % 2 after srand(seed)."""
+    out = np.zeros(nspin, np.int32)
+    _check(lib().oops_bath_states(ctypes.c_uint32(seed), ctypes.c_int32(nspin), _ptr(out)), None, "oops_bath_states")
+    return out
+
+
+def single_sample_terms(coords, cluster, params: CCEParamsC, states, which=0):
+    """(TermTable of H0 (which 0) or H1 (which 1) with the bath mean field,
+    the cluster's product state psi) of one cluster of a single-sample bath."""
+    X = np.ascontiguousarray(coords, np.float64)
+    cl = np.ascontiguousarray(cluster, np.int32)
+    st = np.ascontiguousarray(states, np.int32)
+    psi = np.zeros(1 << len(cl), np.complex128)
+    h = ctypes.c_void_p()
+    _check(lib().oops_cce_single_sample_terms(_ptr(X), ctypes.c_int32(len(X)), _ptr(cl), ctypes.c_int32(len(cl)),
+                                              ctypes.byref(params), _ptr(st), ctypes.c_int32(which), ctypes.byref(h),
+                                              _ptr(psi)), None, "oops_cce_single_sample_terms")
+    return _terms_from_handle(h), psi

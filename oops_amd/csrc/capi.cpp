@@ -254,7 +254,8 @@ struct oops_hahn_batch {
         int64_t L_len = 0;                       // doubles of L (echo batches)
         int grid = 0;
         bool hermitian = true;  // every generator Hermitian: mixed precision may use Lanczos
-        bool cpmg = false;      // some item has N >= 2 pulses
+        bool cpmg = false;      // some item has N >= 2 pulses or two chains
+        bool two_chain = false; // some item runs two forward chains (single-sample form)
         bool uniform = true;    // every item's times are k * dt from 0 (dense propagator powers apply)
         DevBuf work;            // dense64 workspace
         int dense_grid = 0;
@@ -402,8 +403,10 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
     int64_t L_total = 0;
     for (int32_t it = 0; it < nitems; ++it) {
         const EchoIn& I = items[it];
-        const bool cpmg = I.npulse > 1;
+        const bool two = I.chains == 2;   // single-sample form: two forward chains
+        const bool cpmg = I.npulse > 1 || two;
         if (I.npulse < 1 || I.npulse > 64) return set_err(ctx, OOPS_ERR_ARG, "item " + std::to_string(it) + ": npulse must be in [1, 64]");
+        if (I.chains != 1 && I.chains != 2) return set_err(ctx, OOPS_ERR_ARG, "item " + std::to_string(it) + ": chains must be 1 or 2");
         if (per_item) {
             if (I.nt < 1 || !I.t) return set_err(ctx, OOPS_ERR_ARG, "item " + std::to_string(it) + ": no output times");
             for (int k = 0; k < I.nt; ++k)
@@ -442,7 +445,7 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
         auto fits_medium = [&](const CompiledOp& o) {
             return o.hermitian && (int)o.val.size() <= kMediumMaxEnt && (int)o.locs.size() <= kMaxLocals;
         };
-        const bool medium = !dense && (n == 1024 || n == 4096) && precision == 1 && fits_medium(A) &&
+        const bool medium = !dense && !two && (n == 1024 || n == 4096) && precision == 1 && fits_medium(A) &&
                             fits_medium(B) && (!cpmg || (fits_medium(Cc) && fits_medium(Dd)));
         const bool large = !dense && n > 256 && !medium;
         if (large && cpmg)
@@ -489,6 +492,7 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
         b->buckets[bk].hermitian = b->buckets[bk].hermitian && A.hermitian && B.hermitian &&
                                    (!cpmg || (Cc.hermitian && Dd.hermitian));
         b->buckets[bk].cpmg = b->buckets[bk].cpmg || cpmg;
+        b->buckets[bk].two_chain = b->buckets[bk].two_chain || two;
         for (int s = 0; s < 4; ++s) {
             if (s >= 2 && !cpmg) {
                 (s == 2 ? H.opC : H.opD).push_back(-1);
@@ -540,7 +544,7 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
             H.y0.push_back(I.y0[i]);
         }
         H.scale.push_back(I.scale);
-        H.npulse.push_back(I.npulse);
+        H.npulse.push_back(I.npulse | (two ? kTwoChainFlag : 0));
         if (per_item) {
             if (I.nt >= 2) b->buckets[bk].uniform = b->buckets[bk].uniform && uniform_item;
             H.nt.push_back(I.nt);
@@ -627,6 +631,10 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
             if (!rc) B.slotws.bytes = bytes;
             if (rc) return rc;
         }
+        if (B.two_chain && B.n > m && !B.slots)
+            return set_err(ctx, OOPS_ERR_UNSUPPORTED, "two-chain (single-sample) items of dimension " +
+                                                          std::to_string(B.n) + " need the dense path (n <= m) or "
+                                                          "the flip-mask Lanczos kernel (precision 1 or 4, Hermitian)");
         if (lanczos64 && B.cpmg && !B.slots && !(B.n <= m || (B.n == 64 && B.uniform)))
             return set_err(ctx, OOPS_ERR_UNSUPPORTED, "FP64 Lanczos CPMG-N needs the flip-mask form (n = " +
                                                           std::to_string(B.n) + ")");
@@ -787,7 +795,7 @@ static int bucket_run_large(oops_hahn_batch* b, oops_hahn_batch::Bucket& B, int3
 
 static int bucket_launch(oops_hahn_batch* b, oops_hahn_batch::Bucket& B, const KrylovArgs& a, hipStream_t st,
                          bool uniform) {
-    const bool dense64 = B.n == 64 && uniform && B.work.p;
+    const bool dense64 = B.n == 64 && uniform && B.work.p && !B.two_chain;
     const bool lanczos = b->precision == 1 && B.hermitian && B.n > b->m;
     int lrc;
     if (dense64) {

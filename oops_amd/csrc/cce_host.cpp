@@ -242,9 +242,11 @@ std::vector<std::vector<Mat>> single_forms(int mult) {
     return {{sx}, {sy}, {sz}, {matmul(sx, sx, mult)}, {matmul(sy, sy, mult)}, {matmul(sz, sz, mult)}};
 }
 
-// create_spin_hamiltonian: dipolar + Zeeman + field of the centre spin
+// create_spin_hamiltonian: dipolar + Zeeman + field of the centre spin, and
+// for the single-sample variant the bath's mean field on each cluster spin
+// (bath_field: 3 values per spin, single_sample_cce.cpp:75-93)
 std::vector<Term> hamiltonian(const std::vector<SpinH>& spins, const double* B, const SpinH& center,
-                              const cplx* cstate) {
+                              const cplx* cstate, const double* bath_field = nullptr) {
     std::vector<Term> out;
     const int k = (int)spins.size();
     for (int i = 0; i < k; ++i)
@@ -271,7 +273,65 @@ std::vector<Term> hamiltonian(const std::vector<SpinH>& spins, const double* B, 
             for (int c = 0; c < 3; ++c) f[r] += dp[r + 3 * c] * sv[c];  // reshape(dip, 3, 3) * s
         add_filtered(out, {i}, f, single_forms(spins[i].iso.mult));
     }
+    if (bath_field)
+        for (int i = 0; i < k; ++i) {
+            const double f[6] = {bath_field[3 * i], bath_field[3 * i + 1], bath_field[3 * i + 2], 0.0, 0.0, 0.0};
+            add_filtered(out, {i}, f, single_forms(spins[i].iso.mult));
+        }
     return out;
+}
+
+// glibc's rand() after srand(seed) (the additive feedback generator of
+// random(3), TYPE_3: r_i = r_{i-3} + r_{i-31} mod 2^32 from a Park-Miller
+// seeding of 31 words, 310 values discarded, outputs r_i >> 1), restated so
+// that the bath states do not depend on or disturb the process's rand() state.
+class GlibcRand {
+public:
+    explicit GlibcRand(uint32_t seed) {
+        int32_t w = (int32_t)seed;
+        if (w == 0) w = 1;
+        r_[0] = w;
+        for (int i = 1; i < 31; ++i) {
+            const int64_t hi = r_[i - 1] / 127773, lo = r_[i - 1] % 127773;
+            int32_t word = (int32_t)(16807 * lo - 2836 * hi);
+            if (word < 0) word += 2147483647;
+            r_[i] = word;
+        }
+        for (int i = 31; i < 34; ++i) r_[i] = r_[i - 31];
+        n_ = 34;
+        for (int i = 34; i < 344; ++i) step();
+    }
+    int32_t next() { return (int32_t)(step() >> 1); }
+
+private:
+    uint32_t step() {
+        const uint32_t v = (uint32_t)r_[(n_ - 31) % 34] + (uint32_t)r_[(n_ - 3) % 34];
+        r_[n_ % 34] = (int32_t)v;
+        ++n_;
+        return v;
+    }
+    int32_t r_[34];
+    int64_t n_ = 0;
+};
+
+// Mean field of the bath on cluster spin `s`: sum over every bath spin j
+// outside the cluster of dipole(s, j) <S_j>, with <S_j> = (0, 0, +1/2) for
+// state 0 and (0, 0, -1/2) for state 1, in bath order
+// (SpinInteraction.cpp:159-173, SpinInteractionComponent.cpp:433-451).
+void bath_mean_field(const SpinH& s, const double* coords, int nspin, const int32_t* states,
+                     const std::vector<int32_t>& cluster, double* f) {
+    f[0] = f[1] = f[2] = 0.0;
+    SpinH src;
+    isotope_of("13C", src.iso);
+    size_t q = 0;   // cluster is sorted ascending
+    for (int j = 0; j < nspin; ++j) {
+        while (q < cluster.size() && cluster[q] < j) ++q;
+        if (q < cluster.size() && cluster[q] == j) continue;
+        for (int c = 0; c < 3; ++c) src.r[c] = coords[3 * j + c];
+        const auto dp = dipole(s, src);
+        const double sz = states[j] == 0 ? 0.5 : -0.5;
+        for (int r = 0; r < 3; ++r) f[r] += dp[r + 3 * 2] * sz;
+    }
 }
 
 Mat kron(const Mat& A, int da, const Mat& B, int db) {
@@ -442,6 +502,36 @@ ClusterProblem build_problem(const std::vector<SpinH>& spins, const oops_cce_par
     }
     cp.rho = density_vector(spins, P.pol, cp.D);
     return cp;
+}
+
+// SingleSampleCCE::cluster_evolution (single_sample_cce.cpp:48-73, 75-105):
+// the Hilbert-space conditional Hamiltonians with the bath mean field and the
+// product state of the cluster's spins.
+struct SingleSampleProblem {
+    Owned H0, H1;
+    std::vector<cplx> psi;
+};
+
+SingleSampleProblem build_single_sample(const double* coords, int nspin, const std::vector<int32_t>& cluster,
+                                        const int32_t* states, const oops_cce_params& P, const NV& nv) {
+    const int k = (int)cluster.size();
+    std::vector<SpinH> spins(k);
+    std::vector<double> bf(3 * (size_t)k);
+    for (int i = 0; i < k; ++i) {
+        for (int c = 0; c < 3; ++c) spins[i].r[c] = coords[3 * cluster[i] + c];
+        isotope_of("13C", spins[i].iso);
+    }
+    for (int i = 0; i < k; ++i) bath_mean_field(spins[i], coords, nspin, states, cluster, &bf[3 * i]);
+    SingleSampleProblem sp;
+    std::vector<int64_t> hd(k, 2);
+    sp.H0 = flatten(hamiltonian(spins, P.B, nv.espin, &nv.evecs[3 * P.state0], bf.data()), hd);
+    sp.H1 = flatten(hamiltonian(spins, P.B, nv.espin, &nv.evecs[3 * P.state1], bf.data()), hd);
+    // kron of the spins' basis vectors, spin 0 the slowest factor
+    int64_t idx = 0;
+    for (int i = 0; i < k; ++i) idx = 2 * idx + (states[cluster[i]] ? 1 : 0);
+    sp.psi.assign((size_t)1 << k, 0.0);
+    sp.psi[idx] = 1.0;
+    return sp;
 }
 
 }  // namespace
@@ -790,6 +880,77 @@ int oops_cce_echo_batch_create(oops_ctx* ctx, const double* coords, const oops_c
     int rc = oops::cce_build_items(coords, C, P, npulse, nt, t, threads, it);
     if (rc) return oops::set_err(ctx, rc, "building cluster operators failed");
     return oops::batch_create(ctx, (int32_t)it.items.size(), it.items.data(), true, m, klim, tol, precision, out);
+}
+
+int oops_bath_states(uint32_t seed, int32_t nspin, int32_t* states) {
+    if (nspin < 0 || (nspin > 0 && !states)) return OOPS_ERR_ARG;
+    GlibcRand g(seed);
+    for (int32_t i = 0; i < nspin; ++i) states[i] = g.next() % 2;
+    return OOPS_OK;
+}
+
+int oops_cce_single_sample_terms(const double* coords, int32_t nspin, const int32_t* cluster, int32_t k,
+                                 const oops_cce_params* P, const int32_t* states, int32_t which, oops_terms** out,
+                                 double* psi) {
+    if (!coords || nspin <= 0 || !cluster || k <= 0 || !P || !states || !out || (which != 0 && which != 1))
+        return OOPS_ERR_ARG;
+    *out = nullptr;
+    std::vector<int32_t> cl(cluster, cluster + k);
+    for (int i = 0; i < k; ++i)
+        if (cl[i] < 0 || cl[i] >= nspin || (i && cl[i] <= cl[i - 1])) return OOPS_ERR_ARG;
+    if (k > 20) return OOPS_ERR_UNSUPPORTED;
+    NV nv = make_nv(P->nv_pos, P->B);
+    SingleSampleProblem sp = build_single_sample(coords, nspin, cl, states, *P, nv);
+    std::unique_ptr<oops_terms> t(new oops_terms());
+    t->o = which == 0 ? sp.H0 : sp.H1;
+    if (psi) std::memcpy(psi, sp.psi.data(), sizeof(cplx) * sp.psi.size());
+    *out = t.release();
+    return OOPS_OK;
+}
+
+int oops_cce_single_sample_batch_create(oops_ctx* ctx, const double* coords, const oops_clusters* C,
+                                        const oops_cce_params* P, const int32_t* states, int32_t npulse, int32_t nt,
+                                        const double* t, int32_t m, int32_t klim, double tol, int32_t precision,
+                                        oops_hahn_batch** out) {
+    if (!ctx || !coords || !C || !P || !states || !out || nt < 1 || !t || npulse < 1) return OOPS_ERR_ARG;
+    const NV nv = make_nv(P->nv_pos, P->B);
+    std::vector<const std::vector<int32_t>*> cls;
+    for (int o = 0; o < C->max_order; ++o)
+        for (const auto& cl : C->idx[o]) cls.push_back(&cl);
+    const size_t nc = cls.size();
+    std::vector<SingleSampleProblem> probs(nc);
+    auto work = [&](size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; ++i) probs[i] = build_single_sample(coords, C->nspin, *cls[i], states, *P, nv);
+    };
+    const int threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const int T = std::max(1, std::min<int>(threads, (int)(nc / 64) + 1));
+    if (T == 1) {
+        work(0, nc);
+    } else {
+        std::vector<std::thread> pool;
+        for (int q = 0; q < T; ++q) pool.emplace_back(work, nc * q / T, nc * (q + 1) / T);
+        for (auto& th : pool) th.join();
+    }
+    // two chains per cluster: the ket side runs riffle(H1, H0) and the bra
+    // side riffle(H0, H1), both forward from psi; L = Re <psi_bra, psi_ket>
+    std::vector<oops_term_table> views(2 * nc);
+    std::vector<oops::EchoIn> items(nc);
+    for (size_t i = 0; i < nc; ++i) {
+        views[2 * i] = probs[i].H0.view();
+        views[2 * i + 1] = probs[i].H1.view();
+        oops::EchoIn& e = items[i];
+        e.opA = &views[2 * i + 1];
+        e.opC = &views[2 * i];
+        e.opB = &views[2 * i];
+        e.opD = &views[2 * i + 1];
+        e.u0 = e.y0 = reinterpret_cast<const double*>(probs[i].psi.data());
+        e.scale = 1.0;
+        e.npulse = npulse;
+        e.nt = nt;
+        e.t = t;
+        e.chains = 2;
+    }
+    return oops::batch_create(ctx, (int32_t)nc, items.data(), true, m, klim, tol, precision, out);
 }
 
 }  // extern "C"

@@ -332,8 +332,9 @@ __global__ __launch_bounds__(kDenseNT) void cce_hahn_dense_kernel(KrylovArgs a) 
         __syncthreads();
         if (item >= a.nitems) break;
         const ItemSeq q = item_seq(a, item);
-        const bool cpmg = q.npulse > 1;
-        // Hahn echo: sweeps under lvA and -lvB^dagger; CPMG-N: ket-side lvA and lvB
+        const bool cpmg = q.npulse > 1 || q.two_chain;
+        // Hahn echo: sweeps under lvA and -lvB^dagger; CPMG-N and the two-chain
+        // form: the ket-side generators opA (even segments) and opC (odd)
         const int ops[2] = {a.item_opA[item], cpmg ? a.item_opC[item] : a.item_opB[item]};
         const int64_t voff = a.vec_off[item];
         int fail = 0;
@@ -381,6 +382,23 @@ __global__ __launch_bounds__(kDenseNT) void cce_hahn_dense_kernel(KrylovArgs a) 
                             __syncthreads();
                             if (tid < n) S.ws[0][tid] = acc;
                             __syncthreads();
+                        }
+                    }
+                    if (q.two_chain) {
+                        // <y0, bra-chain ket> with the bra chain's generators
+                        // opB = opC (even) and opD = opA (odd): apply the
+                        // adjoints of its segments N..0 to the ket-side state
+                        for (int j = q.npulse; j >= 0; --j) {
+                            const int reps = (j == 0 || j == q.npulse) ? 1 : 2;
+                            const int u = (j & 1) ^ 1;
+                            for (int rep = 0; rep < reps; ++rep) {
+                                cd acc = mkcd(0.0, 0.0);
+                                if (tid < n)
+                                    for (int c = 0; c < n; ++c) acc = cjfma(S.U[u][c + n * tid], S.ws[0][c], acc);
+                                __syncthreads();
+                                if (tid < n) S.ws[0][tid] = acc;
+                                __syncthreads();
+                            }
                         }
                     }
                 }

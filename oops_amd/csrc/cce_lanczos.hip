@@ -652,7 +652,7 @@ void cce_hahn_lanczos_slot_kernel(KrylovArgs a) {
         if (item >= a.nitems) break;
         const int64_t voff = a.vec_off[item];
         const ItemSeq q = item_seq(a, item);
-        const bool cpmg = q.npulse > 1;
+        const bool cpmg = q.npulse > 1 || q.two_chain;
         const int n = a.n;
         // operator indices (uniform): A, -B^dagger, and for CPMG-N B, -A^dagger
         const int qA = __builtin_amdgcn_readfirstlane(a.item_opA[item]);
@@ -693,7 +693,7 @@ void cce_hahn_lanczos_slot_kernel(KrylovArgs a) {
                     g.w = s ? y0 : u0;
                     g.beta = s ? beta0B : beta0A;
                     g.t_now = 0.0;
-                    nseg = s == 0 ? h : q.npulse + 1 - h;
+                    nseg = q.two_chain ? q.npulse + 1 : (s == 0 ? h : q.npulse + 1 - h);
                 }
                 if (hahn) {
                     o.mode = 1 + s;
@@ -702,7 +702,12 @@ void cce_hahn_lanczos_slot_kernel(KrylovArgs a) {
                 for (int sg = 0; sg < nseg; ++sg) {
                     int opi = s;
                     double target = hahn ? q.tau[q.nt - 1] : q.tau[k];
-                    if (cpmg) {
+                    if (q.two_chain) {
+                        // each side its own chain forward: ket opA / opC, bra opB / opD
+                        const int j = sg;
+                        opi = s == 0 ? ((j & 1) ? 2 : 0) : ((j & 1) ? 3 : 1);
+                        target = g.t_now + cpmg_fraction(j, q.npulse) * q.tau[k];
+                    } else if (cpmg) {
                         const int j = s == 0 ? sg : q.npulse - sg;
                         opi = s == 0 ? ((j & 1) ? 2 : 0) : ((j & 1) ? 1 : 3);
                         target = g.t_now + cpmg_fraction(j, q.npulse) * q.tau[k];

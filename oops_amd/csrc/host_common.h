@@ -66,12 +66,18 @@ struct DevBuf {
 };
 
 // One item of a batch (Hahn items have npulse = 1, nt = 0, t = NULL).
+// chains 1: one pulse sequence U_N..U_0 split between the ket side (u0, lvA /
+// lvB segments) and the bra side (y0, adjoint segments), the ensemble form.
+// chains 2: two sequences run forward from u0 and y0, the ket side with opA
+// (even segments) / opC (odd), the bra side with opB (even) / opD (odd), and
+// L = Re(scale <bra, ket>): the single-sample form (opB = opC, opD = opA).
 struct EchoIn {
     const oops_term_table *opA, *opB, *opC, *opD;
     const double *u0, *y0;
     double scale;
     int32_t npulse, nt;
     const double* t;
+    int32_t chains = 1;
 };
 
 // Compile, bucket and upload items (capi.cpp); per_item: times carried by the items.

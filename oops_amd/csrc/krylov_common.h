@@ -100,6 +100,7 @@ __device__ __forceinline__ cd readlane_cd(cd x, int c) {
 struct ItemSeq {
     int nt;
     int npulse;
+    bool two_chain;   // single-sample form: both sides run their own chain forward
     const double* tau;
     double* L;
 };
@@ -107,7 +108,9 @@ struct ItemSeq {
 __device__ __forceinline__ ItemSeq item_seq(const KrylovArgs& a, int item) {
     ItemSeq q;
     q.nt = a.item_nt ? a.item_nt[item] : a.nt;
-    q.npulse = a.item_npulse ? a.item_npulse[item] : 1;
+    const int np = a.item_npulse ? a.item_npulse[item] : 1;
+    q.npulse = np & 0xFFFF;
+    q.two_chain = (np & kTwoChainFlag) != 0;
     q.tau = a.tau + (a.item_tau_off ? a.item_tau_off[item] : 0);
     q.L = a.L + (a.item_L_off ? a.item_L_off[item] : (int64_t)item * a.nt);
     return q;

@@ -27,6 +27,12 @@ constexpr int kSlot32 = 12;
 constexpr int kSlotMaskStride = 32;                            // int32 masks per operator
 constexpr int kSlotOpD2 = kSlot64 * 256 + kSlot32 * 256 / 2;   // double2 per operator table (n <= 256)
 
+// Bit of item_npulse marking an item with two forward chains (single-sample
+// form): the ket side runs opA / opC segments and the bra side opB / opD
+// segments forward from u0 / y0, L = Re(scale <bra, ket>).  The pulse count is
+// the low 16 bits.
+constexpr int32_t kTwoChainFlag = 1 << 16;
+
 // One bucket of Hahn-echo clusters sharing the state dimension n.
 struct KrylovArgs {
     const DevOpHeader* hdr;
@@ -37,7 +43,7 @@ struct KrylovArgs {
     const int32_t* item_opB;   // operator index of sweep B (bra side, already -lvB^dagger)
     const int32_t* item_opC;   // CPMG-N only: lvB (ket side), or NULL
     const int32_t* item_opD;   // CPMG-N only: -lvA^dagger (bra side), or NULL
-    const int32_t* item_npulse;  // pulses N per item (NULL: 1, the Hahn echo)
+    const int32_t* item_npulse;  // pulses N per item (NULL: 1, the Hahn echo) | kTwoChainFlag
     const int32_t* item_nt;      // output times per item (NULL: nt for every item)
     const int64_t* item_tau_off; // offset of the item's times in tau (NULL: 0)
     const int64_t* item_L_off;   // offset of the item's L in L (NULL: item * nt)

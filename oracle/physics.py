@@ -25,6 +25,10 @@ Reference anchors (paths relative to the reference's ``src/``):
                             source/spin/SpinCollection/SpinCollection.cpp:31-57
   sub-clusters              source/spin/SpinCluster/SpinCluster.cpp:82-98
   reduction / product       source/app/cce.cpp:151-189
+  single-sample CCE         source/app/single_sample_cce.cpp:35-128 (bath states, mean
+                            field, Hilbert sweeps, observable), bath field with an
+                            exclusion mask SpinInteraction.cpp:159-173,
+                            SpinInteractionComponent.cpp:433-451
 """
 from __future__ import annotations
 
@@ -601,3 +605,98 @@ def hahn_dense(prob, t_list):
     dt = t_list[1] - t_list[0]
     st = piecewise_dense([A, Bm], [0.5, 0.5], dt, prob["rho"], len(t_list))
     return np.real(prob["D"] * (prob["rho"].conj() @ st))
+
+
+# ---------------------------------------------------------------------------
+# single-sample CCE (a pure bath state, Hilbert space)
+def glibc_rand(seed):
+    """glibc rand() after srand(seed): random(3)'s TYPE_3 additive feedback
+    generator (31 words seeded by Park-Miller 16807 mod 2^31-1 with Schrage's
+    decomposition, r_i = r_{i-31} + r_{i-3} mod 2^32, 310 values discarded,
+    outputs r_i >> 1).  Yields the rand() sequence."""
+    def tdiv(a, b):
+        q = abs(a) // abs(b)
+        return q if (a >= 0) == (b >= 0) else -q
+    w = seed & 0xFFFFFFFF
+    if w >= 2 ** 31:
+        w -= 2 ** 32
+    if w == 0:
+        w = 1
+    r = [w]
+    for i in range(1, 31):
+        hi = tdiv(r[i - 1], 127773)
+        lo = r[i - 1] - hi * 127773
+        x = 16807 * lo - 2836 * hi
+        if x < 0:
+            x += 2147483647
+        r.append(x)
+    for i in range(31, 34):
+        r.append(r[i - 31])
+    for i in range(34, 344):
+        r.append((r[i - 31] + r[i - 3]) & 0xFFFFFFFF)
+    while True:
+        v = (r[-31] + r[-3]) & 0xFFFFFFFF
+        r.append(v)
+        if len(r) > 4096:
+            del r[:-64]
+        yield v >> 1
+
+
+def bath_states(seed, nspin):
+    """prepare_bath_state (single_sample_cce.cpp:35-46): srand(seed), then
+    spin i (spin-list order) in basis state rand() % 2."""
+    g = glibc_rand(seed)
+    return np.array([next(g) % 2 for _ in range(nspin)], np.int32)
+
+
+def basis_state(mult, b):
+    v = np.zeros(mult, complex)
+    v[b] = 1.0
+    return v
+
+
+def bath_field(spin, bath, states, exclude):
+    """DipolarField(spin_list, bath, states, exclude_idx): sum over the bath in
+    its order of mask_j * dipole_field(spin, bath_j, state_j), mask 0 on the
+    excluded (cluster) spins (SpinInteraction.cpp:159-173,
+    SpinInteractionComponent.cpp:433-451)."""
+    mask = np.ones(len(bath))
+    mask[list(exclude)] = 0.0
+    f = np.zeros(3)
+    for j, src in enumerate(bath):
+        f = f + mask[j] * dipole_field(spin, src, basis_state(src.mult, states[j]))
+    return f
+
+
+def single_sample_problem(coords, cluster, B, nv: NVCenter, states, state0=0, state1=1):
+    """create_spin_hamiltonian (single_sample_cce.cpp:75-93) for both NV states
+    and create_cluster_state (:95-105)."""
+    bath = [Spin(np.asarray(x, float)) for x in coords]
+    spins = [bath[i] for i in cluster]
+    dims = [s.mult for s in spins]
+
+    def ham(state):
+        h = dipolar_terms(spins) + zeeman_terms(spins, B) + field_terms(spins, nv.espin, state)
+        for i, s in enumerate(spins):
+            f = bath_field(s, bath, states, cluster)
+            h += _filtered(dims, (i,), [f[0], f[1], f[2], 0.0, 0.0, 0.0], single_forms(s))
+        return h
+    psi = np.ones(1, complex)
+    for i in cluster:
+        psi = np.kron(psi, basis_state(2, states[i]))
+    return dict(h0=ham(nv.state(state0)), h1=ham(nv.state(state1)), psi=psi, dims=dims)
+
+
+def single_sample_dense(prob, t_list, npulse):
+    """cluster_evolution + calc_observables (single_sample_cce.cpp:48-73,
+    120-128) with the dense piecewise propagator
+    (QuantumEvolutionAlgorithm.cpp:60-83): psi_1 under riffle(H0, H1, N),
+    psi_2 under riffle(H1, H0, N), L = Re cdot(psi_1, psi_2)."""
+    n = int(np.prod(prob["dims"]))
+    H0 = full(prob["h0"], n)
+    H1 = full(prob["h1"], n)
+    dt = t_list[1] - t_list[0]
+    seg = pulse_interval(npulse)
+    s1 = piecewise_dense(riffle(H0, H1, npulse), seg, dt, prob["psi"], len(t_list))
+    s2 = piecewise_dense(riffle(H1, H0, npulse), seg, dt, prob["psi"], len(t_list))
+    return np.real(np.sum(s1.conj() * s2, axis=0))
