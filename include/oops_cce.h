@@ -84,6 +84,12 @@ int oops_terms_destroy(oops_terms* t);
 /* Connected clusters of 1..max_order spins for the pair graph d <= cutoff. */
 int oops_clusters_enumerate(const double* coords, int32_t nspin, double cutoff, int32_t max_order,
                             oops_clusters** out);
+/* The same from a connection graph (cDepthFirstPathTracing on a given
+ * connection matrix, SpinClusterAlgorithm.h:68-82): vertex i's neighbours are
+ * neighbors[offsets[i] .. offsets[i+1]); the graph is made symmetric and its
+ * diagonal ignored, as getConnectionMatrix builds it. */
+int oops_clusters_enumerate_graph(int32_t nspin, const int64_t* offsets, const int32_t* neighbors,
+                                  int32_t max_order, oops_clusters** out);
 int oops_clusters_count(const oops_clusters* C, int32_t order, int64_t* count);
 /* Spin indices, count x (order+1), clusters in the reference's set order. */
 int oops_clusters_get(const oops_clusters* C, int32_t order, int32_t* idx);

@@ -680,22 +680,15 @@ int oops_terms_destroy(oops_terms* t) {
     return OOPS_OK;
 }
 
-int oops_clusters_enumerate(const double* coords, int32_t nspin, double cutoff, int32_t max_order,
-                            oops_clusters** out) {
-    if (!out || nspin < 0 || max_order < 1 || max_order > 10 || (nspin > 0 && !coords)) return OOPS_ERR_ARG;
-    *out = nullptr;
-    std::unique_ptr<oops_clusters> C(new oops_clusters());
+}  // extern "C"
+
+// Connected vertex sets of 1..max_order vertices of the graph with the sorted
+// neighbour lists nbr, in the reference's set order with parent lists
+// (cDepthFirstPathTracing::generate, SpinClusterAlgorithm.cpp:98-148).
+static void enumerate_graph(const std::vector<std::vector<int32_t>>& nbr, int nspin, int max_order,
+                            oops_clusters* C) {
     C->nspin = nspin;
     C->max_order = max_order;
-    // connection lists: d(i,j) computed for j >= i and mirrored (SpinCollection.cpp:38-47)
-    std::vector<std::vector<int32_t>> nbr(nspin);
-    for (int i = 0; i < nspin; ++i)
-        for (int j = i + 1; j < nspin; ++j)
-            if (dist3(coords + 3 * i, coords + 3 * j) <= cutoff) {
-                nbr[i].push_back(j);
-                nbr[j].push_back(i);
-            }
-    for (auto& v : nbr) std::sort(v.begin(), v.end());
     C->idx.resize(max_order);
     C->parents.resize(max_order);
     for (int i = 0; i < nspin; ++i) {
@@ -728,6 +721,51 @@ int oops_clusters_enumerate(const double* coords, int32_t nspin, double cutoff, 
             C->parents[order].push_back(kv.second);
         }
     }
+}
+
+extern "C" {
+
+int oops_clusters_enumerate(const double* coords, int32_t nspin, double cutoff, int32_t max_order,
+                            oops_clusters** out) {
+    if (!out || nspin < 0 || max_order < 1 || max_order > 10 || (nspin > 0 && !coords)) return OOPS_ERR_ARG;
+    *out = nullptr;
+    std::unique_ptr<oops_clusters> C(new oops_clusters());
+    // connection lists: d(i,j) computed for j >= i and mirrored (SpinCollection.cpp:38-47)
+    std::vector<std::vector<int32_t>> nbr(nspin);
+    for (int i = 0; i < nspin; ++i)
+        for (int j = i + 1; j < nspin; ++j)
+            if (dist3(coords + 3 * i, coords + 3 * j) <= cutoff) {
+                nbr[i].push_back(j);
+                nbr[j].push_back(i);
+            }
+    for (auto& v : nbr) std::sort(v.begin(), v.end());
+    enumerate_graph(nbr, nspin, max_order, C.get());
+    *out = C.release();
+    return OOPS_OK;
+}
+
+int oops_clusters_enumerate_graph(int32_t nspin, const int64_t* offsets, const int32_t* neighbors,
+                                  int32_t max_order, oops_clusters** out) {
+    if (!out || nspin < 0 || max_order < 1 || max_order > 10 || (nspin > 0 && !offsets)) return OOPS_ERR_ARG;
+    *out = nullptr;
+    std::vector<std::vector<int32_t>> nbr(nspin);
+    for (int i = 0; i < nspin; ++i) {
+        if (offsets[i + 1] < offsets[i] || (offsets[i + 1] > offsets[i] && !neighbors)) return OOPS_ERR_ARG;
+        for (int64_t q = offsets[i]; q < offsets[i + 1]; ++q) {
+            const int32_t j = neighbors[q];
+            if (j < 0 || j >= nspin) return OOPS_ERR_ARG;
+            if (j != i) nbr[i].push_back(j);   // the connection matrix's diagonal is zeroed
+        }
+    }
+    // symmetric: an edge listed on either side connects both (getConnectionMatrix mirrors j >= i)
+    for (int i = 0; i < nspin; ++i)
+        for (int32_t j : std::vector<int32_t>(nbr[i])) nbr[j].push_back(i);
+    for (auto& v : nbr) {
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+    }
+    std::unique_ptr<oops_clusters> C(new oops_clusters());
+    enumerate_graph(nbr, nspin, max_order, C.get());
     *out = C.release();
     return OOPS_OK;
 }
