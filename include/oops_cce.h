@@ -128,6 +128,21 @@ int oops_cce_single_sample_batch_create(oops_ctx* ctx, const double* coords, con
                                         const double* t, int32_t m, int32_t klim, double tol, int32_t precision,
                                         oops_hahn_batch** out);
 
+/* Compiled tables of one cluster's generator (which 0: lvA, 1: -lvB^dagger,
+ * 2: lvB, 3: -lvA^dagger) from the host compiler (device 0) or the device term
+ * builder (device 1, clusters of 1-4 spins).  The batch creators above build
+ * on the device for unpolarised baths of clusters of at most four spins
+ * (environment OOPS_TERM_BUILDER=host keeps the host compiler).  counts[6]:
+ * local operators, ELL entries, widest row, one-spin and two-spin flip-mask
+ * slots, Hermitian flag; locs: 6 int32 per local (bodies, spin, second spin,
+ * row width, first entry, rows); val (2 doubles per entry), off, eslot per
+ * entry (up to max_loc / max_ent); masks: 32 int32 (9 one-spin then 12
+ * two-spin slot masks).  No reference counterpart: it exposes the layout
+ * both builders produce so that they can be compared. */
+int oops_cce_op_tables(oops_ctx* ctx, const double* coords, int32_t k, const oops_cce_params* P, int32_t which,
+                       int32_t device, int32_t max_loc, int32_t max_ent, int32_t* counts, int32_t* locs, double* val,
+                       int32_t* off, uint8_t* eslot, int32_t* masks);
+
 /* CCE reduction of L (all clusters' rows, orders concatenated, nt columns):
  * tilde (same shape), final_result / each_order (nt x max_order, row-major),
  * and the sign / log-magnitude of final_result.  Outputs may be NULL. */

@@ -62,7 +62,7 @@ EXPORTS = [
     "oops_server_result", "oops_server_pending", "oops_server_destroy", "oops_server_times",
     "oops_hahn_batch_kernels", "oops_echo_batch_capture", "oops_slot_form", "oops_tile_plan", "oops_read_xyz", "oops_write_xyz",
     "oops_hahn_batch_elapsed", "oops_server_failed", "oops_bath_states", "oops_cce_single_sample_terms",
-    "oops_cce_single_sample_batch_create", "oops_clusters_enumerate_graph",
+    "oops_cce_single_sample_batch_create", "oops_clusters_enumerate_graph", "oops_cce_op_tables",
 ]
 
 
@@ -777,3 +777,25 @@ def single_sample_terms(coords, cluster, params: CCEParamsC, states, which=0):
                                               ctypes.byref(params), _ptr(st), ctypes.c_int32(which), ctypes.byref(h),
                                               _ptr(psi)), None, "oops_cce_single_sample_terms")
     return _terms_from_handle(h), psi
+
+
+def op_tables(ctx: Context, coords, params: CCEParamsC, which=0, device=True):
+    """Compiled tables of one cluster's generator from the device term builder
+    (device=True) or the host compiler: dict of counts, locs, val, off, eslot, masks."""
+    X = np.ascontiguousarray(coords, np.float64).reshape(-1, 3)
+    k = len(X)
+    max_loc, max_ent = 64, 1 << 14
+    counts = np.zeros(6, np.int32)
+    locs = np.zeros(6 * max_loc, np.int32)
+    val = np.zeros(2 * max_ent)
+    off = np.zeros(max_ent, np.int32)
+    eslot = np.zeros(max_ent, np.uint8)
+    masks = np.zeros(32, np.int32)
+    _check(lib().oops_cce_op_tables(ctx.h, _ptr(X), ctypes.c_int32(k), ctypes.byref(params), ctypes.c_int32(which),
+                                    ctypes.c_int32(1 if device else 0), ctypes.c_int32(max_loc), ctypes.c_int32(max_ent),
+                                    _ptr(counts), _ptr(locs), _ptr(val), _ptr(off), _ptr(eslot), _ptr(masks)),
+           ctx.h, "oops_cce_op_tables")
+    nl, ne = int(counts[0]), int(counts[1])
+    return {"counts": counts.copy(), "locs": locs[:6 * nl].reshape(nl, 6).copy(),
+            "val": val[:2 * ne].view(np.complex128).copy(), "off": off[:ne].copy(), "eslot": eslot[:ne].copy(),
+            "masks": masks.copy()}

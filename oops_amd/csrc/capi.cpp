@@ -22,6 +22,7 @@
 #include "../../include/oops_expv.h"
 #include "oops_kernels.h"
 #include "host_common.h"
+#include "term_builder.h"
 
 using oops::DevOpHeader;
 using oops::DevOpLocal;
@@ -272,6 +273,7 @@ struct oops_hahn_batch {
         int sweep_nt = 0;             // output times per workgroup that sweepws holds
         // states above 256 elements: every item on the single-state path, in turn
         bool large = false;
+        bool device_tables = false;   // hdr, loc, val, off, eslot, smask, medmask written by the term builder
         int kernel = -1;  // kind that ran last: see oops_hahn_batch_kernels
         std::vector<CompiledOp> hA, hB;
         std::vector<std::vector<double>> hu0, hy0;
@@ -357,6 +359,29 @@ int oops_dlaran_fill(int32_t* iseed, int64_t count, double* out) {
 
 static int sweep_workspace(oops_ctx* ctx, oops_hahn_batch::Bucket& B, int ntmax);
 
+// Host side of one bucket before upload: packed operator tables (left empty
+// when the device term builder wrote them to the bucket directly) and the
+// per-item sequences.
+struct BucketHost {
+    std::vector<DevOpHeader> hdr;
+    std::vector<DevOpLocal> loc;
+    std::vector<double> val;
+    std::vector<int32_t> off;
+    std::vector<int32_t> opA, opB;
+    std::vector<int64_t> voff;
+    std::vector<double> u0, y0, scale;
+    std::vector<int32_t> opC, opD, npulse, nt;
+    std::vector<int64_t> tau_off, L_off;
+    std::vector<double> tau;
+    std::vector<uint8_t> eslot;
+    std::vector<int32_t> smask;
+    std::vector<int32_t> mmask;   // kMedMaskStride masks per operator (medium flip-mask form)
+};
+
+static int finalize_batch(oops_ctx* ctx, std::unique_ptr<oops_hahn_batch>& b, std::vector<BucketHost>& hosts,
+                          bool per_item, int32_t m, int32_t precision, bool ell_only, bool lanczos64,
+                          oops_hahn_batch** out);
+
 int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool per_item, int32_t m, int32_t klim,
                        double tol, int32_t precision, oops_hahn_batch** out) {
     if (!ctx || !out || nitems < 0 || (nitems > 0 && !items)) return set_err(ctx, OOPS_ERR_ARG, "bad arguments");
@@ -382,21 +407,7 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
     // compile all operators, bucket by n
     std::map<int64_t, int> bucket_of;
-    struct Host {
-        std::vector<DevOpHeader> hdr;
-        std::vector<DevOpLocal> loc;
-        std::vector<double> val;
-        std::vector<int32_t> off;
-        std::vector<int32_t> opA, opB;
-        std::vector<int64_t> voff;
-        std::vector<double> u0, y0, scale;
-        std::vector<int32_t> opC, opD, npulse, nt;
-        std::vector<int64_t> tau_off, L_off;
-        std::vector<double> tau;
-        std::vector<uint8_t> eslot;
-        std::vector<int32_t> smask;
-        std::vector<int32_t> mmask;   // kMedMaskStride masks per operator (medium flip-mask form)
-    };
+    typedef BucketHost Host;
     b->per_item = per_item;
     std::vector<Host> hosts;
     const int emax = krylov_entry_capacity(m);
@@ -557,6 +568,15 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
             L_total += I.nt;
         }
     }
+    return finalize_batch(ctx, b, hosts, per_item, m, precision, ell_only, lanczos64, out);
+}
+
+// Bucket checks, uploads and workspaces of a batch whose buckets and host
+// tables are assembled (the host compiler's path or the device term builder's).
+static int finalize_batch(oops_ctx* ctx, std::unique_ptr<oops_hahn_batch>& b, std::vector<BucketHost>& hosts,
+                          bool per_item, int32_t m, int32_t precision, bool ell_only, bool lanczos64,
+                          oops_hahn_batch** out) {
+    typedef BucketHost Host;
     for (auto& B : b->buckets)
         if (B.cpmg && !(B.n <= m || (B.n == 64 && B.uniform) || ((precision == 1 || lanczos64) && B.hermitian)))
             return set_err(ctx, OOPS_ERR_UNSUPPORTED,
@@ -570,10 +590,12 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
         int32_t seed[4] = {0, 0, 0, 5};
         dlaran_fill(seed, (int64_t)rnd.size(), rnd.data());
         int rc = 0;
-        rc = rc ? rc : dev_upload(ctx, B.hdr, H.hdr);
-        rc = rc ? rc : dev_upload(ctx, B.loc, H.loc);
-        rc = rc ? rc : dev_upload(ctx, B.val, H.val);
-        rc = rc ? rc : dev_upload(ctx, B.off, H.off);
+        if (!B.device_tables) {
+            rc = rc ? rc : dev_upload(ctx, B.hdr, H.hdr);
+            rc = rc ? rc : dev_upload(ctx, B.loc, H.loc);
+            rc = rc ? rc : dev_upload(ctx, B.val, H.val);
+            rc = rc ? rc : dev_upload(ctx, B.off, H.off);
+        }
         rc = rc ? rc : dev_upload(ctx, B.opA, H.opA);
         rc = rc ? rc : dev_upload(ctx, B.opB, H.opB);
         rc = rc ? rc : dev_upload(ctx, B.voff, H.voff);
@@ -611,8 +633,10 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
                 // flip-mask tables: per workgroup four generators of ms64 FP64 + ms32 FP32 slots
                 const int64_t gstride = (int64_t)B.ms64 * B.n + ((int64_t)B.ms32 * B.n + 1) / 2;
                 const size_t tb = sizeof(double) * 2 * 4 * (size_t)gstride * (size_t)B.medium_grid;
-                rc = dev_upload(ctx, B.eslot, H.eslot);
-                rc = rc ? rc : dev_upload(ctx, B.medmask, H.mmask);
+                if (!B.device_tables) {
+                    rc = dev_upload(ctx, B.eslot, H.eslot);
+                    rc = rc ? rc : dev_upload(ctx, B.medmask, H.mmask);
+                }
                 rc = rc ? rc : hip_check(ctx, hipMalloc(&B.medtab.p, tb), "hipMalloc medium tables");
                 if (rc) return rc;
                 B.medtab.bytes = tb;
@@ -624,8 +648,10 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
             // mixed: two workgroups per CU, FP64 variant: one; 2 (Hahn) or 4
             // (CPMG-N) coefficient tables each
             B.slot_grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu * (lanczos64 ? 1 : 2)));
-            rc = rc ? rc : dev_upload(ctx, B.eslot, H.eslot);
-            rc = rc ? rc : dev_upload(ctx, B.smask, H.smask);
+            if (!B.device_tables) {
+                rc = rc ? rc : dev_upload(ctx, B.eslot, H.eslot);
+                rc = rc ? rc : dev_upload(ctx, B.smask, H.smask);
+            }
             const size_t bytes = sizeof(double) * 2 * (size_t)slot_table_double2(lanczos64) * 4 * (size_t)B.slot_grid;
             rc = rc ? rc : hip_check(ctx, hipMalloc(&B.slotws.p, bytes), "hipMalloc slot tables");
             if (!rc) B.slotws.bytes = bytes;
@@ -667,6 +693,157 @@ int oops::batch_create(oops_ctx* ctx, int32_t nitems, const EchoIn* items, bool 
     }
     *out = b.release();
     return OOPS_OK;
+}
+
+int oops::batch_create_built(oops_ctx* ctx, const double* coords, int32_t nspin, const std::vector<BuiltGroup>& groups,
+                             int64_t nitems, const TermBuilderArgs& consts,
+                             const std::vector<std::vector<double>>& rho, int32_t npulse, bool per_item, int32_t nt,
+                             const double* t, int32_t m, int32_t klim, double tol, int32_t precision,
+                             oops_hahn_batch** out) {
+    if (!ctx || !out || !coords || nitems < 0) return set_err(ctx, OOPS_ERR_ARG, "bad arguments");
+    *out = nullptr;
+    if (precision < 0 || precision > 4) return set_err(ctx, OOPS_ERR_ARG, "precision must be 0..4");
+    const bool ell_only = precision == 3;
+    if (ell_only) precision = 1;
+    const bool lanczos64 = precision == 4;
+    if (lanczos64) precision = 0;
+    if (m != 20 && m != 30) return set_err(ctx, OOPS_ERR_UNSUPPORTED, "Krylov dimension must be 20 or 30");
+    if (klim < 1 || klim > 16) return set_err(ctx, OOPS_ERR_ARG, "klim must be in [1, 16]");
+    if (npulse < 1 || npulse > 64) return set_err(ctx, OOPS_ERR_ARG, "npulse must be in [1, 64]");
+    if (per_item) {
+        if (nt < 1 || !t) return set_err(ctx, OOPS_ERR_ARG, "no output times");
+        for (int q = 0; q < nt; ++q)
+            if (!(t[q] >= (q ? t[q - 1] : 0.0))) return set_err(ctx, OOPS_ERR_ARG, "times must be ascending and >= 0");
+    }
+    const bool cpmg = npulse > 1;
+    const int nops = cpmg ? 4 : 2;
+    std::unique_ptr<oops_hahn_batch> b(new oops_hahn_batch());
+    b->ctx = ctx;
+    b->nitems = (int32_t)nitems;
+    b->m = m;
+    b->klim = klim;
+    b->tol = tol;
+    b->precision = precision;
+    b->lanczos64 = lanczos64;
+    b->per_item = per_item;
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
+    bool uniform = false;
+    if (per_item && nt >= 2) {
+        const double dt = t[1] - t[0];
+        uniform = t[0] == 0.0 && dt > 0.0;
+        for (int q = 2; q < nt && uniform; ++q) uniform = std::fabs(t[q] - q * dt) <= 1e-12 * t[nt - 1];
+    }
+    DevBuf dcoords;
+    int rc = dev_upload(ctx, dcoords, std::vector<double>(coords, coords + 3 * (size_t)nspin));
+    if (rc) return rc;
+    std::vector<BucketHost> hosts;
+    if (per_item) {
+        b->item_L_off.assign((size_t)nitems, 0);
+        b->item_nt.assign((size_t)nitems, nt);
+    }
+    std::vector<int64_t> L_len_of_item_order;
+    for (const BuiltGroup& g : groups) {
+        const int k = g.k;
+        const int64_t ni = (int64_t)g.items.size();
+        if (ni == 0) continue;
+        if (k < 1 || k > 4 || (int64_t)g.spins.size() != ni * k)
+            return set_err(ctx, OOPS_ERR_ARG, "device term builder: clusters of 1-4 spins");
+        const int n = 1 << (2 * k);
+        const int P = k * (k - 1) / 2;
+        const int loc_stride = k + P;
+        const int ent_stride = 16 * k + 128 * P;
+        b->buckets.emplace_back();
+        auto& B = b->buckets.back();
+        hosts.emplace_back();
+        BucketHost& H = hosts.back();
+        B.n = n;
+        B.device_tables = true;
+        B.cpmg = cpmg;
+        B.uniform = uniform;
+        B.items.assign(g.items.begin(), g.items.end()) ;
+        const int64_t nop = ni * nops;
+        DevBuf dcl, summary;
+        rc = dev_upload(ctx, dcl, g.spins);
+        rc = rc ? rc : hip_check(ctx, hipMalloc(&B.hdr.p, sizeof(DevOpHeader) * nop), "hipMalloc hdr");
+        if (!rc) B.hdr.bytes = sizeof(DevOpHeader) * nop;
+        rc = rc ? rc : hip_check(ctx, hipMalloc(&B.loc.p, sizeof(DevOpLocal) * nop * loc_stride), "hipMalloc loc");
+        if (!rc) B.loc.bytes = sizeof(DevOpLocal) * nop * loc_stride;
+        rc = rc ? rc : hip_check(ctx, hipMalloc(&B.val.p, sizeof(double2) * nop * ent_stride), "hipMalloc val");
+        if (!rc) B.val.bytes = sizeof(double2) * nop * ent_stride;
+        rc = rc ? rc : hip_check(ctx, hipMalloc(&B.off.p, sizeof(int32_t) * nop * ent_stride), "hipMalloc off");
+        if (!rc) B.off.bytes = sizeof(int32_t) * nop * ent_stride;
+        rc = rc ? rc : hip_check(ctx, hipMalloc(&B.eslot.p, nop * ent_stride), "hipMalloc eslot");
+        if (!rc) B.eslot.bytes = nop * ent_stride;
+        rc = rc ? rc : hip_check(ctx, hipMalloc(&B.smask.p, sizeof(int32_t) * nop * kSlotMaskStride), "hipMalloc smask");
+        if (!rc) B.smask.bytes = sizeof(int32_t) * nop * kSlotMaskStride;
+        rc = rc ? rc : hip_check(ctx, hipMalloc(&B.medmask.p, sizeof(int32_t) * nop * kMedMaskStride), "hipMalloc mmask");
+        if (!rc) B.medmask.bytes = sizeof(int32_t) * nop * kMedMaskStride;
+        rc = rc ? rc : hip_check(ctx, hipMalloc(&summary.p, sizeof(int32_t) * nop * kTBSummary), "hipMalloc summary");
+        if (rc) return rc;
+        TermBuilderArgs a = consts;
+        a.coords = (const double*)dcoords.p;
+        a.clusters = (const int32_t*)dcl.p;
+        a.nitems = (int32_t)ni;
+        a.k = k;
+        a.nops = nops;
+        a.hdr = (DevOpHeader*)B.hdr.p;
+        a.loc = (DevOpLocal*)B.loc.p;
+        a.val = (double2*)B.val.p;
+        a.off = (int32_t*)B.off.p;
+        a.eslot = (uint8_t*)B.eslot.p;
+        a.smask = (int32_t*)B.smask.p;
+        a.mmask = (int32_t*)B.medmask.p;
+        a.summary = (int32_t*)summary.p;
+        a.loc_stride = loc_stride;
+        a.ent_stride = ent_stride;
+        const int lrc = launch_term_builder(a, (int)nop, ctx->stream);
+        if (lrc) return set_err(ctx, OOPS_ERR_HIP, "term builder launch failed: " + std::to_string(lrc));
+        std::vector<int32_t> sm((size_t)nop * kTBSummary);
+        rc = hip_check(ctx, hipMemcpyAsync(sm.data(), summary.p, sizeof(int32_t) * sm.size(), hipMemcpyDeviceToHost,
+                                           ctx->stream), "D2H builder summary");
+        rc = rc ? rc : hip_check(ctx, hipStreamSynchronize(ctx->stream), "term builder");
+        if (rc) return rc;
+        // bucket properties from every operator, as the host compiler decides them
+        const bool dense = n <= m;
+        const int cap = dense ? kDenseMaxEnt : krylov_entry_capacity(m);
+        for (int64_t o = 0; o < nop; ++o) {
+            const int32_t* q = &sm[(size_t)o * kTBSummary];
+            if (q[6]) return set_err(ctx, OOPS_ERR_CAPACITY, "device term builder: table overflow");
+            if (q[1] > cap) return set_err(ctx, OOPS_ERR_CAPACITY, "operator table exceeds LDS capacity");
+            if (q[2] > 16) return set_err(ctx, OOPS_ERR_UNSUPPORTED, "local operator rows wider than 16 entries");
+            B.hermitian = B.hermitian && q[5] != 0;
+            if (q[3] < 0 || q[3] > kSlot64 || q[4] > kSlot32) B.slots = false;
+            if (q[3] < 1 || q[3] > kMedSlot64 || q[4] > kMedSlot32) B.medslot = false;
+        }
+        if (cpmg && !(n <= m || (n == 64 && uniform) || ((precision == 1 || lanczos64) && B.hermitian)))
+            return set_err(ctx, OOPS_ERR_UNSUPPORTED, "CPMG-N items of dimension " + std::to_string(n) +
+                                                          " need the dense paths or Lanczos (Hermitian generators)");
+        for (int64_t i = 0; i < ni; ++i) {
+            H.opA.push_back((int32_t)(i * nops));
+            H.opB.push_back((int32_t)(i * nops + 1));
+            H.opC.push_back(cpmg ? (int32_t)(i * nops + 2) : -1);
+            H.opD.push_back(cpmg ? (int32_t)(i * nops + 3) : -1);
+            H.voff.push_back(0);   // every item starts from the same rho0
+            H.scale.push_back((double)(1 << k));
+            H.npulse.push_back(npulse);
+            if (per_item) {
+                H.nt.push_back(nt);
+                H.tau_off.push_back(0);
+                H.L_off.push_back(B.L_len);
+                B.L_len += nt;
+            }
+        }
+        if (per_item) H.tau.assign(t, t + nt);
+        const std::vector<double>& r0 = rho.at(k);
+        if ((int)r0.size() != 2 * n) return set_err(ctx, OOPS_ERR_ARG, "rho0 of the wrong size");
+        H.u0 = r0;
+        H.y0 = r0;
+    }
+    if (per_item) {
+        // items are numbered by cluster; every item has nt outputs
+        for (int64_t i = 0; i < nitems; ++i) b->item_L_off[i] = i * nt;
+    }
+    return finalize_batch(ctx, b, hosts, per_item, m, precision, ell_only, lanczos64, out);
 }
 
 // Hahn-sweep state stores, kept when already large enough.  Slot kernel: the

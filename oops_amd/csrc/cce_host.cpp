@@ -33,6 +33,7 @@
 #include "../../include/oops_cce.h"
 #include "../../include/oops_expv.h"
 #include "host_common.h"
+#include "term_builder.h"
 
 namespace {
 
@@ -371,9 +372,8 @@ std::vector<Term> expand(const std::vector<Term>& h, bool use_sharp, double sign
     return out;
 }
 
-std::vector<Term> dephasing(int k, double rate_hz, const double* axis) {
-    std::vector<Term> out;
-    const double rate = 2.0 * kPi * rate_hz;
+// i (sigma^T (x) sigma - I) of the dephasing axis (SpinInteractionComponent.cpp:285-304)
+Mat dephasing_matrix(const double* axis) {
     double nrm = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
     if (nrm == 0.0) nrm = 1.0;
     const double nx = axis[0] / nrm, ny = axis[1] / nrm, nz = axis[2] / nrm;
@@ -381,6 +381,13 @@ std::vector<Term> dephasing(int k, double rate_hz, const double* axis) {
     Mat dm = kron(transpose(sig, 2), 2, sig, 2);
     for (int i = 0; i < 4; ++i) dm[i + 4 * i] -= 1.0;
     for (auto& z : dm) z *= cplx(0.0, 1.0);
+    return dm;
+}
+
+std::vector<Term> dephasing(int k, double rate_hz, const double* axis) {
+    std::vector<Term> out;
+    const double rate = 2.0 * kPi * rate_hz;
+    const Mat dm = dephasing_matrix(axis);
     for (int i = 0; i < k; ++i) {
         const double c[1] = {rate};
         add_filtered(out, {i}, c, {{dm}});
@@ -534,6 +541,54 @@ SingleSampleProblem build_single_sample(const double* coords, int nspin, const s
     return sp;
 }
 
+// Constants of the device term builder: Zeeman coefficients, NV position and
+// state spin vectors, dephasing superoperator, SHARP / FLAT of the spin-1/2
+// operators (computed by the same host functions as the host path), and
+// rho0 = vec(I / 2^k) for k = 1..4.
+void builder_consts(const oops_cce_params& P, const NV& nv, oops::TermBuilderArgs& a,
+                    std::vector<std::vector<double>>& rho) {
+    std::memset(&a, 0, sizeof(a));
+    Isotope c13;
+    isotope_of("13C", c13);
+    for (int c = 0; c < 3; ++c) {
+        a.zee[c] = -c13.gamma * P.B[c];
+        a.center[c] = nv.espin.r[c];
+    }
+    a.gamma_c = c13.gamma;
+    a.gamma_e = nv.espin.iso.gamma;
+    const auto s0 = spin_vector(nv.espin.iso.mult, &nv.evecs[3 * P.state0]);
+    const auto s1 = spin_vector(nv.espin.iso.mult, &nv.evecs[3 * P.state1]);
+    for (int c = 0; c < 3; ++c) {
+        a.sv0[c] = s0[c];
+        a.sv1[c] = s1[c];
+    }
+    a.deph_rate = 2.0 * kPi * P.dephasing_rate;
+    const Mat dm = dephasing_matrix(P.dephasing_axis);
+    for (int q = 0; q < 16; ++q) a.dm[q] = make_double2(dm[q].real(), dm[q].imag());
+    Mat sx, sy, sz;
+    spin_ops(2, sx, sy, sz);
+    const Mat* ops[3] = {&sx, &sy, &sz};
+    for (int c = 0; c < 3; ++c) {
+        const Mat sh = sharp(*ops[c]), fl = flat(*ops[c]);
+        for (int q = 0; q < 16; ++q) {
+            a.sharp[16 * c + q] = make_double2(sh[q].real(), sh[q].imag());
+            a.flat[16 * c + q] = make_double2(fl[q].real(), fl[q].imag());
+        }
+    }
+    rho.assign(5, {});
+    const double zero[3] = {0.0, 0.0, 0.0};
+    for (int k = 1; k <= 4; ++k) {
+        std::vector<SpinH> spins(k);
+        for (auto& sp : spins) {
+            isotope_of("13C", sp.iso);
+            sp.r[0] = sp.r[1] = sp.r[2] = 0.0;
+        }
+        double D = 1.0;
+        const auto v = density_vector(spins, zero, D);
+        for (const cplx& z : v) { rho[k].push_back(z.real()); rho[k].push_back(z.imag()); }
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -547,6 +602,41 @@ struct oops_clusters {
     std::vector<std::vector<std::vector<int32_t>>> idx;      // [order][cluster] -> sorted spins
     std::vector<std::vector<std::vector<int32_t>>> parents;  // [order][cluster] -> parent positions
 };
+
+// The device term builder takes the 13C clusters of at most four spins when
+// the bath is unpolarised (rho0 = I / D for every cluster); OOPS_TERM_BUILDER
+// = host keeps the host compiler.
+static bool device_builder_applies(const oops_clusters* C, const oops_cce_params* P) {
+    if (const char* e = std::getenv("OOPS_TERM_BUILDER"))
+        if (std::string(e) == "host") return false;
+    if (P->pol[0] != 0.0 || P->pol[1] != 0.0 || P->pol[2] != 0.0) return false;
+    return C->max_order <= 4 && C->nspin > 0;
+}
+
+
+// Selected clusters grouped by size for the device term builder; items are
+// numbered in selection order (global cluster order).
+static std::vector<oops::BuiltGroup> builder_groups(const oops_clusters* C, int64_t nsel, const int64_t* sel, int64_t& nitems,
+                                             bool& bad_selection) {
+    std::vector<oops::BuiltGroup> groups(C->max_order);
+    int64_t g = 0, next = 0;
+    nitems = 0;
+    for (int o = 0; o < C->max_order; ++o) {
+        groups[o].k = o + 1;
+        for (const auto& cl : C->idx[o]) {
+            const int64_t gi = g++;
+            if (sel) {
+                if (next >= nsel || sel[next] != gi) continue;
+                ++next;
+            }
+            groups[o].spins.insert(groups[o].spins.end(), cl.begin(), cl.end());
+            groups[o].items.push_back(nitems++);
+        }
+    }
+    bad_selection = sel && next != nsel;
+    return groups;
+}
+
 
 extern "C" {
 
@@ -878,6 +968,17 @@ int oops_cce_hahn_batch_create(oops_ctx* ctx, const double* coords, const oops_c
                                int32_t precision, oops_hahn_batch** out) {
     if (!ctx || !coords || !C || !P || !out || nsel < 0 || (nsel > 0 && !sel)) return OOPS_ERR_ARG;
     NV nv = make_nv(P->nv_pos, P->B);
+    if (device_builder_applies(C, P)) {
+        int64_t nitems = 0;
+        bool bad = false;
+        const auto groups = builder_groups(C, nsel, sel, nitems, bad);
+        if (bad) return oops::set_err(ctx, OOPS_ERR_ARG, "selection must be ascending cluster indices");
+        oops::TermBuilderArgs consts;
+        std::vector<std::vector<double>> rho;
+        builder_consts(*P, nv, consts, rho);
+        return oops::batch_create_built(ctx, coords, C->nspin, groups, nitems, consts, rho, 1, false, 0, nullptr, m,
+                                        klim, tol, precision, out);
+    }
     std::vector<ClusterProblem> probs;
     int64_t g = 0, next = 0;  // global cluster index, next selection
     for (int o = 0; o < C->max_order; ++o)
@@ -913,11 +1014,136 @@ int oops_cce_echo_batch_create(oops_ctx* ctx, const double* coords, const oops_c
                                int32_t npulse, int32_t nt, const double* t, int32_t m, int32_t klim, double tol,
                                int32_t precision, oops_hahn_batch** out) {
     if (!ctx || !coords || !C || !P || !out || nt < 1 || !t || npulse < 1) return OOPS_ERR_ARG;
+    if (device_builder_applies(C, P)) {
+        int64_t nitems = 0;
+        bool bad = false;
+        const auto groups = builder_groups(C, 0, nullptr, nitems, bad);
+        NV nv = make_nv(P->nv_pos, P->B);
+        oops::TermBuilderArgs consts;
+        std::vector<std::vector<double>> rho;
+        builder_consts(*P, nv, consts, rho);
+        return oops::batch_create_built(ctx, coords, C->nspin, groups, nitems, consts, rho, npulse, true, nt, t, m,
+                                        klim, tol, precision, out);
+    }
     oops::CceItems it;
     const int threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     int rc = oops::cce_build_items(coords, C, P, npulse, nt, t, threads, it);
     if (rc) return oops::set_err(ctx, rc, "building cluster operators failed");
     return oops::batch_create(ctx, (int32_t)it.items.size(), it.items.data(), true, m, klim, tol, precision, out);
+}
+
+int oops_cce_op_tables(oops_ctx* ctx, const double* coords, int32_t k, const oops_cce_params* P, int32_t which,
+                       int32_t device, int32_t max_loc, int32_t max_ent, int32_t* counts, int32_t* locs, double* val,
+                       int32_t* off, uint8_t* eslot, int32_t* masks) {
+    if (!ctx || !coords || k < 1 || !P || which < 0 || which > 3 || !counts) return OOPS_ERR_ARG;
+    std::vector<oops::DevOpHeader> hdr(1);
+    std::vector<oops::DevOpLocal> lv;
+    std::vector<double> vv;
+    std::vector<int32_t> ov, mv(oops::kSlotMaskStride, 0), sm(oops::kTBSummary, 0);
+    std::vector<uint8_t> ev;
+    NV nv = make_nv(P->nv_pos, P->B);
+    if (!device) {
+        std::vector<SpinH> spins(k);
+        for (int i = 0; i < k; ++i) {
+            for (int c = 0; c < 3; ++c) spins[i].r[c] = coords[3 * i + c];
+            isotope_of("13C", spins[i].iso);
+        }
+        ClusterProblem cp = build_problem(spins, *P, nv);
+        const Owned& o = which == 0 ? cp.A : (which == 1 ? cp.Bd : (which == 2 ? cp.lvB : cp.Ad));
+        const oops_term_table T = o.view();
+        oops::CompiledOp C;
+        std::string err;
+        int rc = oops::compile_operator(&T, C, err, 1 << 20);
+        if (rc) return oops::set_err(ctx, rc, err);
+        lv = C.locs;
+        for (const auto& z : C.val) { vv.push_back(z.real()); vv.push_back(z.imag()); }
+        ov = C.off;
+        ev = C.eslot;
+        for (int q = 0; q < oops::kSlotMaskStride; ++q) {
+            if (q < oops::kSlot64 && q < (int)C.mask64.size()) mv[q] = C.mask64[q];
+            if (q >= oops::kSlot64 && q - oops::kSlot64 < oops::kSlot32 && q - oops::kSlot64 < (int)C.mask32.size())
+                mv[q] = C.mask32[q - oops::kSlot64];
+        }
+        sm = {(int32_t)C.locs.size(), (int32_t)C.val.size(), C.max_row, C.nslot64, C.nslot32, C.hermitian ? 1 : 0, 0,
+              0};
+    } else {
+        if (k > 4) return oops::set_err(ctx, OOPS_ERR_UNSUPPORTED, "the device term builder takes 1-4 spins");
+        if (hipSetDevice(ctx->device) != hipSuccess) return oops::set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
+        oops::TermBuilderArgs a;
+        std::vector<std::vector<double>> rho;
+        builder_consts(*P, nv, a, rho);
+        const int P2 = k * (k - 1) / 2, ls = k + P2, es = 16 * k + 128 * P2, nops = 4;
+        std::vector<int32_t> cl(k);
+        for (int i = 0; i < k; ++i) cl[i] = i;
+        oops::DevBuf dc, dcl, dh, dl, dv, doff, de, dsm, dmm, dsum;
+        auto al = [&](oops::DevBuf& b, size_t bytes) {
+            const int rc = oops::hip_check(ctx, hipMalloc(&b.p, bytes), "hipMalloc");
+            if (!rc) b.bytes = bytes;
+            return rc;
+        };
+        int rc = al(dc, sizeof(double) * 3 * k);
+        rc = rc ? rc : oops::hip_check(ctx, hipMemcpy(dc.p, coords, sizeof(double) * 3 * k, hipMemcpyHostToDevice), "H2D");
+        rc = rc ? rc : al(dcl, sizeof(int32_t) * k);
+        rc = rc ? rc : oops::hip_check(ctx, hipMemcpy(dcl.p, cl.data(), sizeof(int32_t) * k, hipMemcpyHostToDevice), "H2D");
+        rc = rc ? rc : al(dh, sizeof(oops::DevOpHeader) * nops);
+        rc = rc ? rc : al(dl, sizeof(oops::DevOpLocal) * nops * ls);
+        rc = rc ? rc : al(dv, sizeof(double2) * nops * es);
+        rc = rc ? rc : al(doff, sizeof(int32_t) * nops * es);
+        rc = rc ? rc : al(de, (size_t)nops * es);
+        rc = rc ? rc : al(dsm, sizeof(int32_t) * nops * oops::kSlotMaskStride);
+        rc = rc ? rc : al(dmm, sizeof(int32_t) * nops * oops::kMedMaskStride);
+        rc = rc ? rc : al(dsum, sizeof(int32_t) * nops * oops::kTBSummary);
+        if (rc) return rc;
+        a.coords = (const double*)dc.p;
+        a.clusters = (const int32_t*)dcl.p;
+        a.nitems = 1;
+        a.k = k;
+        a.nops = nops;
+        a.hdr = (oops::DevOpHeader*)dh.p;
+        a.loc = (oops::DevOpLocal*)dl.p;
+        a.val = (double2*)dv.p;
+        a.off = (int32_t*)doff.p;
+        a.eslot = (uint8_t*)de.p;
+        a.smask = (int32_t*)dsm.p;
+        a.mmask = (int32_t*)dmm.p;
+        a.summary = (int32_t*)dsum.p;
+        a.loc_stride = ls;
+        a.ent_stride = es;
+        if (oops::launch_term_builder(a, nops, ctx->stream) != 0)
+            return oops::set_err(ctx, OOPS_ERR_HIP, "term builder launch failed");
+        rc = oops::hip_check(ctx, hipStreamSynchronize(ctx->stream), "term builder");
+        if (rc) return rc;
+        rc = oops::hip_check(ctx, hipMemcpy(sm.data(), (int32_t*)dsum.p + which * oops::kTBSummary,
+                                            sizeof(int32_t) * oops::kTBSummary, hipMemcpyDeviceToHost), "D2H");
+        if (rc) return rc;
+        lv.resize(sm[0]);
+        vv.resize(2 * (size_t)sm[1]);
+        ov.resize(sm[1]);
+        ev.resize(sm[1]);
+        rc = oops::hip_check(ctx, hipMemcpy(lv.data(), (oops::DevOpLocal*)dl.p + (size_t)which * ls, sizeof(oops::DevOpLocal) * lv.size(),
+                                            hipMemcpyDeviceToHost), "D2H");
+        rc = rc ? rc : oops::hip_check(ctx, hipMemcpy(vv.data(), (double2*)dv.p + (size_t)which * es,
+                                                      sizeof(double) * vv.size(), hipMemcpyDeviceToHost), "D2H");
+        rc = rc ? rc : oops::hip_check(ctx, hipMemcpy(ov.data(), (int32_t*)doff.p + (size_t)which * es,
+                                                      sizeof(int32_t) * ov.size(), hipMemcpyDeviceToHost), "D2H");
+        rc = rc ? rc : oops::hip_check(ctx, hipMemcpy(ev.data(), (uint8_t*)de.p + (size_t)which * es, ev.size(),
+                                                      hipMemcpyDeviceToHost), "D2H");
+        rc = rc ? rc : oops::hip_check(ctx, hipMemcpy(mv.data(), (int32_t*)dsm.p + which * oops::kSlotMaskStride,
+                                                      sizeof(int32_t) * mv.size(), hipMemcpyDeviceToHost), "D2H");
+        if (rc) return rc;
+        if (sm[6]) return oops::set_err(ctx, OOPS_ERR_CAPACITY, "device term builder: table overflow");
+    }
+    for (int q = 0; q < 6; ++q) counts[q] = sm[q];
+    if ((int)lv.size() > max_loc || (int)ov.size() > max_ent) return OOPS_ERR_CAPACITY;
+    for (size_t l = 0; l < lv.size() && locs; ++l) {
+        const int32_t f[6] = {lv[l].nb, lv[l].s, lv[l].t, lv[l].R, lv[l].ent, lv[l].rows};
+        for (int q = 0; q < 6; ++q) locs[6 * l + q] = f[q];
+    }
+    if (val) std::memcpy(val, vv.data(), sizeof(double) * vv.size());
+    if (off) std::memcpy(off, ov.data(), sizeof(int32_t) * ov.size());
+    if (eslot) std::memcpy(eslot, ev.data(), ev.size());
+    if (masks) std::memcpy(masks, mv.data(), sizeof(int32_t) * mv.size());
+    return OOPS_OK;
 }
 
 int oops_bath_states(uint32_t seed, int32_t nspin, int32_t* states) {

@@ -95,6 +95,21 @@ struct CceItems {
 int cce_build_items(const double* coords, const oops_clusters* C, const oops_cce_params* P, int32_t npulse,
                     int32_t nt, const double* t, int threads, CceItems& out);
 
+// Clusters of one size for the device term builder (term_builder.hip).
+struct BuiltGroup {
+    int k = 0;
+    std::vector<int32_t> spins;    // k spin indices per cluster
+    std::vector<int64_t> items;    // item index of each cluster in the batch
+};
+struct TermBuilderArgs;
+// Batch whose operator tables the device builds from coordinates: groups by
+// cluster size, constants (couplings aside) in `consts`, rho0 = vec(I / 2^k)
+// per group (rho[k]); npulse >= 2: CPMG-N (four generators per item).
+int batch_create_built(oops_ctx* ctx, const double* coords, int32_t nspin, const std::vector<BuiltGroup>& groups,
+                       int64_t nitems, const TermBuilderArgs& consts, const std::vector<std::vector<double>>& rho,
+                       int32_t npulse, bool per_item, int32_t nt, const double* t, int32_t m, int32_t klim, double tol,
+                       int32_t precision, oops_hahn_batch** out);
+
 // ccoeff (optional, nterm complex): complex multipliers replacing T->coeff.
 // One Hahn-echo item on the single-state path (large_host.cpp), FP64;
 // L[0..nt) and 24 stats slots as the batched kernels write them.

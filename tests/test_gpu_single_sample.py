@@ -14,12 +14,11 @@ from oracle import physics as P
 B111 = np.ones(3) * 0.05 / np.sqrt(3.0)
 
 
-def _oracle(X, C, states, order_max, npulse, tl, per_order=None):
+def _oracle(X, C, states, order_max, npulse, tl):
     nv = P.NVCenter((0.0, 0.0, 0.0), B111)
     out = []
     for o in range(order_max):
-        cls = C.get(o) if per_order is None else C.get(o)[:per_order]
-        for cl in cls:
+        for cl in C.get(o):
             prob = P.single_sample_problem(X, list(cl), B111, nv, states)
             out.append(P.single_sample_dense(prob, tl, npulse))
     return np.array(out)
@@ -50,25 +49,31 @@ def test_single_sample_batch_matches_oracle(ctx, npulse):
     assert np.abs(got - ref).max() <= 1e-9
 
 
+def _local_bath():
+    """The 9 spins nearest to the centre of one 5-spin cluster of a dense bath:
+    clusters [9, 17, 35, 62, 78] of orders 1-5 at cutoff 8 A."""
+    X = O.bath_diamond(3.567, 14.0, 0.05, seed=2)
+    C = O.Clusters(X, 8.0, 5)
+    c0 = X[list(C.get(4)[0])].mean(axis=0)
+    keep = np.sort(np.argsort(np.linalg.norm(X - c0, axis=1))[:9])
+    return X[keep]
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("npulse", [1, 2])
 def test_single_sample_five_spin_clusters_on_the_lanczos_kernel(ctx, npulse):
-    X = O.bath_diamond(3.567, 14.0, 0.05, seed=2)
+    X = _local_bath()
     C = O.Clusters(X, 8.0, 5)
     assert C.count(4) > 0
     states = O.bath_states(5, len(X))
     tl = np.linspace(0.0, 1e-4, 21)
-    want = _oracle(X, C, states, 5, npulse, tl, per_order=2)
-    idx, base = [], 0
-    for o in range(5):
-        idx += list(range(base, base + min(2, C.count(o))))
-        base += C.count(o)
-    print(f"single-sample 5 spins N={npulse}: {C.count(4)} five-spin clusters", flush=True)
+    want = _oracle(X, C, states, 5, npulse, tl)
+    print(f"single-sample 5 spins N={npulse}: clusters {C.counts()}", flush=True)
     for precision, tol, bar in (("fp64-lanczos", 1e-10, 1e-8), ("mixed", 1e-4, 1e-6)):
         b = C.single_sample_batch(ctx, O.cce_params(B=B111), states, npulse, tl, m=30, tol=tol, precision=precision)
         b.run()
         Ls, st = b.results()
-        L = np.stack(Ls)[idx]
+        L = np.stack(Ls)
         kinds = {t["n"]: t["kernel"] for t in b.timing()}
         err = np.abs(L - want).max()
         print(f"  {precision}: kernels {kinds}, max |L - oracle| {err:.2e}", flush=True)
