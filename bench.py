@@ -296,6 +296,7 @@ def main():
     tau = tlist / 2.0
     t0 = time.perf_counter()
     batch = C.hahn_batch(ctx, par, m=mm, tol=mtol, precision="mixed")
+    setup_mixed_s = time.perf_counter() - t0
     sel = stratified(counts, args.fp64_frac)
     batch64 = None if args.no_fp64 else C.hahn_batch(ctx, par, m=args.m, tol=args.tol, precision="fp64",
                                                         select=sel)
@@ -384,6 +385,8 @@ def main():
         "config": {"workload": "batched CCE-4 Hahn echo, diamond 13C bath r=5nm p=0.011 seed 0, cutoff 8A, "
                                "B=0.05T||[111], 401 points on 0-5ms",
                    "clusters": nclu, "clusters_per_order": counts, "spins": int(len(X)),
+                   "setup_s": setup_mixed_s,
+                   "setup": "operator tables of every cluster built and uploaded (device term builder), untimed",
                    "mixed": {"m": mm, "tol": mtol, "source": msrc, "buckets": timing,
                              "failed_clusters": int(((st[:, 8] & 0xFF) != 0).sum())},
                    "fp64": fp64,
