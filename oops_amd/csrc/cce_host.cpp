@@ -1221,6 +1221,18 @@ int oops_cce_single_sample_batch_create(oops_ctx* ctx, const double* coords, con
 
 namespace oops {
 
+// Inputs of the device term builder for every cluster of C (false when the
+// builder does not take them: clusters above four spins, a polarised bath).
+bool cce_builder_inputs(const oops_clusters* C, const oops_cce_params* P, std::vector<BuiltGroup>& groups,
+                        int64_t& nitems, TermBuilderArgs& consts, std::vector<std::vector<double>>& rho) {
+    if (!C || !P || !device_builder_applies(C, P)) return false;
+    bool bad = false;
+    groups = builder_groups(C, 0, nullptr, nitems, bad);
+    NV nv = make_nv(P->nv_pos, P->B);
+    builder_consts(*P, nv, consts, rho);
+    return true;
+}
+
 // Every cluster's sweep operators (lvA, -lvB^dagger and, for CPMG-N, lvB and
 // -lvA^dagger) and rho0, built in parallel over clusters.
 int cce_build_items(const double* coords, const oops_clusters* C, const oops_cce_params* P, int32_t npulse,

@@ -110,6 +110,11 @@ int batch_create_built(oops_ctx* ctx, const double* coords, int32_t nspin, const
                        int32_t npulse, bool per_item, int32_t nt, const double* t, int32_t m, int32_t klim, double tol,
                        int32_t precision, oops_hahn_batch** out);
 
+// Inputs of batch_create_built for every cluster of C (cce_host.cpp); false
+// when the device term builder does not take them.
+bool cce_builder_inputs(const oops_clusters* C, const oops_cce_params* P, std::vector<BuiltGroup>& groups,
+                        int64_t& nitems, TermBuilderArgs& consts, std::vector<std::vector<double>>& rho);
+
 // ccoeff (optional, nterm complex): complex multipliers replacing T->coeff.
 // One Hahn-echo item on the single-state path (large_host.cpp), FP64;
 // L[0..nt) and 24 stats slots as the batched kernels write them.

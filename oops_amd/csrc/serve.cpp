@@ -25,6 +25,7 @@
 #include "../../include/oops_serve.h"
 #include "host_common.h"
 #include "oops_kernels.h"
+#include "term_builder.h"
 
 using namespace oops;
 
@@ -46,6 +47,13 @@ struct Req {
     std::vector<double> coords;
     oops_clusters* C = nullptr;
     CceItems items;
+    // device term builder path: clusters by size, builder constants, rho0 and times
+    bool device_tables = false;
+    std::vector<BuiltGroup> groups;
+    int64_t nitems = 0;
+    TermBuilderArgs consts{};
+    std::vector<std::vector<double>> rho;
+    std::vector<double> t;
     int64_t est_bytes = 0;
     oops_hahn_batch* batch = nullptr;
     int64_t footprint = 0;
@@ -71,6 +79,9 @@ struct Req {
 //            item at every output time, left out above 32 GB as batch_create does),
 //   n > 256 on the single-state FP64 path: up to eight items in flight, each
 //            with its basis, work vectors and stored ket states.
+int64_t workspace_bytes(int64_t n, int64_t items, int64_t hahn, int64_t nt_hahn, int64_t nt_max, int num_cu, int m,
+                        int precision);
+
 int64_t estimate_bytes(const CceItems& it, int num_cu, int m, int precision) {
     struct Dim { int64_t items = 0, hahn = 0, nt_hahn = 0, nt_max = 0; };
     std::map<int64_t, Dim> dims;
@@ -94,28 +105,53 @@ int64_t estimate_bytes(const CceItems& it, int num_cu, int m, int precision) {
         }
         b += 32 * n + 8 * (int64_t)e.nt + 4 * kStatsPerItem + 64;
     }
-    const int64_t c16 = 16;   // bytes of one complex FP64
-    for (const auto& kv : dims) {
-        const int64_t n = kv.first;
-        const Dim& d = kv.second;
-        if (n == 64) {
-            b += 8 * dense64_workspace_doubles((int)std::min<int64_t>(d.items, num_cu));
-        } else if (n > m && n <= 256 && (precision == 1 || precision == 4)) {
-            const int64_t grid = std::min<int64_t>(d.items, (precision == 4 ? 1 : 2) * (int64_t)num_cu);
-            b += c16 * slot_table_double2(precision == 4) * 4 * grid;
-            b += c16 * 256 * d.nt_hahn * std::min<int64_t>(d.hahn, grid);
-        } else if ((n == 1024 || n == 4096) && precision == 1) {
-            const int64_t grid = std::min<int64_t>(2 * d.items, num_cu);
-            b += 8 * medium_workspace_float2((int)grid, m, (int)n);
-            b += c16 * 4 * ((int64_t)kMedSlot64 * n + (int64_t)kMedSlot32 * n / 2) * grid;
-            const int64_t store = c16 * 2 * n * d.nt_hahn * d.hahn;
-            if (store <= (int64_t(32) << 30)) b += store;
-        } else if (n > 256) {
-            const int64_t inflight = std::min<int64_t>(d.items, 8);
-            b += inflight * c16 * n * (2 * (m + 1) + 8 + d.nt_max);
-        }
-    }
+    for (const auto& kv : dims)
+        b += workspace_bytes(kv.first, kv.second.items, kv.second.hahn, kv.second.nt_hahn, kv.second.nt_max, num_cu,
+                             m, precision);
     return b + 4096;
+}
+
+// Device bytes of the per-bucket workspaces of `items` items of dimension n
+// (hahn of them Hahn echoes), as batch_create allocates them.
+int64_t workspace_bytes(int64_t n, int64_t items, int64_t hahn, int64_t nt_hahn, int64_t nt_max, int num_cu, int m,
+                        int precision) {
+    int64_t b = 0;
+    const int64_t c16 = 16;   // bytes of one complex FP64
+    if (n == 64) {
+        b += 8 * dense64_workspace_doubles((int)std::min<int64_t>(items, num_cu));
+    } else if (n > m && n <= 256 && (precision == 1 || precision == 4)) {
+        const int64_t grid = std::min<int64_t>(items, (precision == 4 ? 1 : 2) * (int64_t)num_cu);
+        b += c16 * slot_table_double2(precision == 4) * 4 * grid;
+        b += c16 * 256 * nt_hahn * std::min<int64_t>(hahn, grid);
+    } else if ((n == 1024 || n == 4096) && precision == 1) {
+        const int64_t grid = std::min<int64_t>(2 * items, num_cu);
+        b += 8 * medium_workspace_float2((int)grid, m, (int)n);
+        b += c16 * 4 * ((int64_t)kMedSlot64 * n + (int64_t)kMedSlot32 * n / 2) * grid;
+        const int64_t store = c16 * 2 * n * nt_hahn * hahn;
+        if (store <= (int64_t(32) << 30)) b += store;
+    } else if (n > 256) {
+        const int64_t inflight = std::min<int64_t>(items, 8);
+        b += inflight * c16 * n * (2 * (m + 1) + 8 + nt_max);
+    }
+    return b;
+}
+
+// Device bytes of a batch built by the device term builder: the fixed-stride
+// tables of every generator (batch_create_built), the shared rho0 and DLARAN
+// vectors, the per-item sequences and results, and the workspaces.
+int64_t estimate_bytes_built(const std::vector<BuiltGroup>& groups, int npulse, int64_t nt, int num_cu, int m,
+                             int precision) {
+    int64_t b = 4096;
+    const int nops = npulse > 1 ? 4 : 2;
+    for (const BuiltGroup& g : groups) {
+        const int64_t k = g.k, ni = (int64_t)g.items.size(), n = int64_t(1) << (2 * k), P = k * (k - 1) / 2;
+        if (!ni) continue;
+        const int64_t ent = 16 * k + 128 * P;
+        b += ni * nops * (64 + (k + P) * 16 + ent * 21 + (kSlotMaskStride + kMedMaskStride + kTBSummary) * 4);
+        b += ni * (4 * 4 + 8 + 8 + 4 + 4 + 8 + 8 + 4 * kStatsPerItem + 8 * nt) + 48 * n + 8 * nt;
+        b += workspace_bytes(n, ni, npulse == 1 ? ni : 0, npulse == 1 ? nt : 0, nt, num_cu, m, precision);
+    }
+    return b;
 }
 
 }  // namespace
@@ -178,6 +214,14 @@ struct oops_server {
         P.dephasing_axis[2] = 1.0;
         std::vector<double> t(r.nt);
         for (int k = 0; k < r.nt; ++k) t[k] = r.t_end * k / (r.nt - 1) * (r.npulse == 1 ? 0.5 : 1.0);
+        if (cce_builder_inputs(q.C, &P, q.groups, q.nitems, q.consts, q.rho)) {
+            // the operator tables are built on the device when the request is admitted
+            q.device_tables = true;
+            q.t = t;
+            q.nclusters = q.nitems;
+            q.est_bytes = estimate_bytes_built(q.groups, r.npulse, r.nt, ctx->num_cu, m, precision);
+            return OOPS_OK;
+        }
         rc = cce_build_items(q.coords.data(), q.C, &P, r.npulse, r.nt, t.data(), inner_threads, q.items);
         if (rc) return rc;
         q.nclusters = (int64_t)q.items.items.size();
@@ -202,8 +246,12 @@ struct oops_server {
                 if (!running.empty() && used + q->est_bytes > budget) return OOPS_OK;
                 built.pop_front();
             }
-            int rc = batch_create(ctx, (int32_t)q->items.items.size(), q->items.items.data(), true, m, 10, tol,
-                                  precision, &q->batch);
+            int rc = q->device_tables
+                         ? batch_create_built(ctx, q->coords.data(), (int32_t)(q->coords.size() / 3), q->groups,
+                                              q->nitems, q->consts, q->rho, q->r.npulse, true, q->r.nt, q->t.data(), m,
+                                              10, tol, precision, &q->batch)
+                         : batch_create(ctx, (int32_t)q->items.items.size(), q->items.items.data(), true, m, 10, tol,
+                                        precision, &q->batch);
             if (!rc) rc = oops_echo_batch_footprint(q->batch, &q->footprint);
             if (!rc) rc = oops_echo_batch_launch(q->batch);
             q->items.items.clear();

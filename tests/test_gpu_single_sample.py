@@ -2,7 +2,7 @@
 source/app/single_sample_cce.cpp:48-128) against the oracle's dense
 restatement: every cluster of orders 1-4 of a small bath (Hilbert dims 2-16,
 the dense path) in FP64 to 1e-10 under a Hahn echo and CPMG-2/4, and 5-spin
-clusters (n = 32) on the flip-mask Lanczos kernel in FP64 (1e-8 at tol 1e-10)
+clusters (n = 32) on the flip-mask Lanczos kernel in FP64 (1e-7 at tol 1e-12)
 and mixed precision (1e-6).  The CCE product of the batch equals the host
 reduction of the oracle's L_c(t)."""
 import numpy as np
@@ -69,7 +69,9 @@ def test_single_sample_five_spin_clusters_on_the_lanczos_kernel(ctx, npulse):
     tl = np.linspace(0.0, 1e-4, 21)
     want = _oracle(X, C, states, 5, npulse, tl)
     print(f"single-sample 5 spins N={npulse}: clusters {C.counts()}", flush=True)
-    for precision, tol, bar in (("fp64-lanczos", 1e-10, 1e-8), ("mixed", 1e-4, 1e-6)):
+    # FP64 with the mixed path's Lanczos / Chebyshev scheme at tol 1e-12, and
+    # mixed precision at the bar of the mixed path
+    for precision, tol, bar in (("fp64-lanczos", 1e-12, 1e-7), ("mixed", 1e-4, 1e-6)):
         b = C.single_sample_batch(ctx, O.cce_params(B=B111), states, npulse, tl, m=30, tol=tol, precision=precision)
         b.run()
         Ls, st = b.results()
