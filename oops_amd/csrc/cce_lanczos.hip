@@ -390,7 +390,7 @@ struct SlotOp {
 // (mixed) or double2 [kSlot32][256] (F64).  Every thread writes and reads
 // only its own column.
 template <bool F64>
-__device__ __noinline__ void slot_build(const KrylovArgs& a, int op, double2* __restrict__ tab, int i) {
+__device__ __noinline__ void slot_build(SlotTables a, int op, double2* __restrict__ tab, int i) {
     typedef typename std::conditional<F64, double2, float2>::type PairT;
     const DevOpHeader h = a.hdr[op];
     asm volatile("" : "+v"(i));
@@ -659,11 +659,12 @@ void cce_hahn_lanczos_slot_kernel(KrylovArgs a) {
         const int qB = __builtin_amdgcn_readfirstlane(a.item_opB[item]);
         const int qC = cpmg ? __builtin_amdgcn_readfirstlane(a.item_opC[item]) : qA;
         const int qD = cpmg ? __builtin_amdgcn_readfirstlane(a.item_opD[item]) : qB;
-        slot_build<F64>(a, qA, ws, tid);
-        slot_build<F64>(a, qB, ws + kTab, tid);
+        const SlotTables tabs = slot_tables(a);
+        slot_build<F64>(tabs, qA, ws, tid);
+        slot_build<F64>(tabs, qB, ws + kTab, tid);
         if (cpmg) {
-            slot_build<F64>(a, qC, ws + 2 * kTab, tid);
-            slot_build<F64>(a, qD, ws + 3 * kTab, tid);
+            slot_build<F64>(tabs, qC, ws + 2 * kTab, tid);
+            slot_build<F64>(tabs, qD, ws + 3 * kTab, tid);
         }
         IntegState gA, gB;
         const cd u0 = tid < n ? mkcd(a.u0[voff + tid].x, a.u0[voff + tid].y) : mkcd(0.0, 0.0);

@@ -164,7 +164,7 @@ __device__ __forceinline__ void med_product_all(const MedSmem<M, E>& S, cd (&y)[
 // [ms64][n] then FP32 slots [ms32][n]); every thread writes and reads only its
 // own columns.  Slot 0 is mask 0 (the diagonal, with the constant term).
 template <int E>
-__device__ __noinline__ void med_slot_build(const KrylovArgs& a, int op, double2* __restrict__ tab) {
+__device__ __noinline__ void med_slot_build(SlotTables a, int op, double2* __restrict__ tab) {
     const DevOpHeader h = a.hdr[op];
     const int n = kMT * E;
     float2* t32 = reinterpret_cast<float2*>(tab + (int64_t)a.ms64 * n);
@@ -557,14 +557,15 @@ __global__ __launch_bounds__(kMT) void cce_hahn_medium_kernel(KrylovArgs a, floa
         const int qA = a.item_opA[item], qB = a.item_opB[item];
         const int qC = cpmg ? a.item_opC[item] : qA, qD = cpmg ? a.item_opD[item] : qB;
         if (slot) {   // this unit's generator tables (each thread its own columns)
+            const SlotTables tabs = slot_tables(a);
             if (hahn) {
-                med_slot_build<E>(a, side ? qB : qA, wtab + side * gstride);
+                med_slot_build<E>(tabs, side ? qB : qA, wtab + side * gstride);
             } else {
-                med_slot_build<E>(a, qA, wtab);
-                med_slot_build<E>(a, qB, wtab + gstride);
+                med_slot_build<E>(tabs, qA, wtab);
+                med_slot_build<E>(tabs, qB, wtab + gstride);
                 if (cpmg) {
-                    med_slot_build<E>(a, qC, wtab + 2 * gstride);
-                    med_slot_build<E>(a, qD, wtab + 3 * gstride);
+                    med_slot_build<E>(tabs, qC, wtab + 2 * gstride);
+                    med_slot_build<E>(tabs, qD, wtab + 3 * gstride);
                 }
             }
         }

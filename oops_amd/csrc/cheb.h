@@ -178,7 +178,17 @@ __device__ int cheb_coeffs(SM& S, int mb, int k1, double tau, double beta_m, boo
                 step(k + 2, 0);
                 step(k + 3, 1);
             }
-            for (; k < K; ++k) step(k, k & 3);
+            for (; k < K; ++k) {   // the tail, i^k by selects (no indexed accumulator)
+                const double tn = apply2(tk) - tm;
+                tm = tk;
+                tk = tn;
+                const int R = k & 3;
+                const double bk = (R & 2) ? -bes(k) : bes(k);
+                const double eb = (R & 1) ? 0.0 : bk;
+                const double ob = (R & 1) ? bk : 0.0;
+                E = fma(eb, tk, E);
+                O = fma(ob, tk, O);
+            }
             // (E_re + i O_re) + i (E_im + i O_im): re = E_re - O_im, im = E_im + O_re
             const double r = fma(sgn, other(O), E);
             // times e^{i c tau / ns}

@@ -13,6 +13,20 @@ namespace oops {
 
 constexpr int kNT = 256;
 
+// Compiled operator tables handed to the out-of-line slot-table builders by
+// value: a reference to the kernel's argument struct would make the kernel
+// keep a copy of the struct in scratch memory and read every field from there.
+struct SlotTables {
+    const DevOpHeader* hdr;
+    const DevOpLocal* loc;
+    const double2* val;
+    const uint8_t* eslot;
+    int ms64, ms32;   // slot counts (medium kernel)
+};
+__device__ __forceinline__ SlotTables slot_tables(const KrylovArgs& a) {
+    return SlotTables{a.hdr, a.loc, a.val, a.eslot, a.ms64, a.ms32};
+}
+
 struct IntegState {
     cd w;
     double beta, t_now, t_new, anorm, rndoff, xm;
