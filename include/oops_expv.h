@@ -99,6 +99,19 @@ int oops_ctx_info(const oops_ctx* ctx, int32_t* num_cu, int32_t* lds_bytes, char
 /* --- single operator ---------------------------------------------------- */
 /* w = H v for host vectors of length prod(spin_dim) (hamvec_cuda3_). */
 int oops_hamvec(oops_ctx* ctx, const oops_term_table* H, const double* v, double* w);
+/* y = H x like oops_hamvec, but x and y are DEVICE pointers (prod(spin_dim)
+ * complex doubles each, owned by the caller) and the call allocates no device
+ * memory: the compiled operator tables are copied from the host into the
+ * library's fixed 16 MiB device array on every call (one array per device,
+ * set up by oops_ctx_create) and the product runs from there (flat form, or
+ * the tiled form for states of 2^19 elements and more).  Tables beyond
+ * 16 MiB return OOPS_ERR_CAPACITY.  The call returns after the product has
+ * finished.  Calls that overlap on one device (several threads, several
+ * contexts) share the array and race.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). */
+int oops_hamvec_noalloc(oops_ctx* ctx, const oops_term_table* H, const double* x, double* y);
 
 /* Multi-output Krylov expv of one state: w(t_k) = exp(-i H t_k) v for the
  * ascending times ta[0..nt) (MAIN_CACHE / zgexpv_cache: Lanczos(klim) norm

@@ -63,6 +63,7 @@ EXPORTS = [
     "oops_hahn_batch_kernels", "oops_echo_batch_capture", "oops_slot_form", "oops_tile_plan", "oops_read_xyz", "oops_write_xyz",
     "oops_hahn_batch_elapsed", "oops_server_failed", "oops_bath_states", "oops_cce_single_sample_terms",
     "oops_cce_single_sample_batch_create", "oops_clusters_enumerate_graph", "oops_cce_op_tables",
+    "oops_hamvec_noalloc",
 ]
 
 

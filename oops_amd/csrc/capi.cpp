@@ -326,6 +326,7 @@ int oops_ctx_create(int32_t device, oops_ctx** out) {
     c->num_cu = prop.multiProcessorCount;
     c->lds_bytes = (int)prop.sharedMemPerBlock;
     c->name = prop.gcnArchName;
+    if (oops::noalloc_array_address(device, &c->noalloc) != OOPS_OK) return OOPS_ERR_HIP;
     *out = c.release();
     return OOPS_OK;
 }

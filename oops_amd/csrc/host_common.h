@@ -19,6 +19,7 @@ struct oops_ctx {
     int lds_bytes = 0;
     std::string name;
     std::string err;
+    void* noalloc = nullptr;   // device address of the no-allocation array on this device
 };
 
 namespace oops {
@@ -124,6 +125,9 @@ int large_hahn_item(oops_ctx* ctx, const CompiledOp& A, const CompiledOp& B, con
 int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err, int max_locals = 64,
                      const std::complex<double>* ccoeff = nullptr);
 int set_err(oops_ctx* ctx, int code, const std::string& msg);
+// Device address of the no-allocation array (large_host.cpp) on the current
+// device, looked up once per device.
+int noalloc_array_address(int device, void** ptr);
 // Step limit of every Krylov integration (environment OOPS_MAX_STEPS, else 5e7).
 int32_t max_steps_limit();
 int hip_check(oops_ctx* ctx, hipError_t e, const char* what);

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <cstdlib>
 #include <limits>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -159,43 +160,69 @@ static bool plan_tiles(const CompiledOp& op, std::vector<TilePlan>& plan) {
 template <typename T>
 int upload(oops_ctx* ctx, DevBuf& b, const std::vector<T>& v);
 
+// Device form of a compiled operator's tables: header, local operators, ELL
+// values and offsets (at least one of each).
+struct OpImage {
+    DevOpHeader hdr;
+    std::vector<DevOpLocal> locs;
+    std::vector<double> vals;
+    std::vector<int32_t> offs;
+};
+
+static OpImage op_image(const CompiledOp& T) {
+    OpImage im;
+    std::memset(&im.hdr, 0, sizeof(im.hdr));
+    im.hdr.n = (int32_t)T.n;
+    im.hdr.nspin = T.nspin;
+    im.hdr.ld_bits = T.ld_bits;
+    im.hdr.nloc = (int32_t)T.locs.size();
+    im.hdr.nent = (int32_t)T.val.size();
+    im.hdr.max_row = T.max_row;
+    im.hdr.c0re = T.c0.real();
+    im.hdr.c0im = T.c0.imag();
+    im.hdr.ent2_first = T.ent2_first;
+    for (auto& z : T.val) { im.vals.push_back(z.real()); im.vals.push_back(z.imag()); }
+    if (im.vals.empty()) { im.vals.push_back(0.0); im.vals.push_back(0.0); }
+    im.offs = T.off;
+    if (im.offs.empty()) im.offs.push_back(0);
+    im.locs = T.locs;
+    if (im.locs.empty()) im.locs.push_back(DevOpLocal{});
+    return im;
+}
+
+// Bit positions of a pass's tile spins ([0,16)) and block spins ([16,48)).
+static std::vector<int32_t> tile_map(const CompiledOp& op, const std::vector<int>& g) {
+    const int ns = op.nspin, ld = op.ld_bits;
+    std::vector<int32_t> map(kLargeTileMapStride, 0);
+    std::vector<int> in(ns, 0);
+    for (size_t q = 0; q < g.size(); ++q) { map[q] = ld * (ns - 1 - g[q]); in[g[q]] = 1; }
+    int nb = 0;
+    for (int s2 = 0; s2 < ns; ++s2)
+        if (!in[s2]) map[16 + nb++] = ld * (ns - 1 - s2);
+    return map;
+}
+
+// The no-allocation array: a fixed device array that oops_hamvec_noalloc
+// stages an operator's tables in instead of allocating (each device holds
+// its own copy until the process ends).  Calls on one device share it.
+constexpr size_t kNoAllocBytes = size_t(16) << 20;   // 16 MiB
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 static int setup_tiles(oops_ctx* ctx, LargeRun& R) {
     std::vector<TilePlan> plan;
     if (!plan_tiles(R.op, plan)) return OOPS_OK;
     const int ns = R.op.nspin, ld = R.op.ld_bits;
     R.tiles.resize(plan.size());
     for (size_t p = 0; p < plan.size(); ++p) {
-        const CompiledOp& T = plan[p].op;
         LargeRun::Tile& t = R.tiles[p];
-        DevOpHeader h;
-        std::memset(&h, 0, sizeof(h));
-        h.n = (int32_t)T.n;
-        h.nspin = T.nspin;
-        h.ld_bits = T.ld_bits;
-        h.nloc = (int32_t)T.locs.size();
-        h.nent = (int32_t)T.val.size();
-        h.max_row = T.max_row;
-        h.c0re = T.c0.real();
-        h.c0im = T.c0.imag();
-        h.ent2_first = T.ent2_first;
-        std::vector<double> vals;
-        for (auto& z : T.val) { vals.push_back(z.real()); vals.push_back(z.imag()); }
-        if (vals.empty()) { vals.push_back(0.0); vals.push_back(0.0); }
-        std::vector<int32_t> offs(T.off);
-        if (offs.empty()) offs.push_back(0);
-        std::vector<DevOpLocal> locs(T.locs);
-        if (locs.empty()) locs.push_back(DevOpLocal{});
-        std::vector<int32_t> map(kLargeTileMapStride, 0);
-        std::vector<int> in(ns, 0);
+        const OpImage im = op_image(plan[p].op);
         const std::vector<int>& g = plan[p].G;
-        for (size_t q = 0; q < g.size(); ++q) { map[q] = ld * (ns - 1 - g[q]); in[g[q]] = 1; }
-        int nb = 0;
-        for (int s2 = 0; s2 < ns; ++s2)
-            if (!in[s2]) map[16 + nb++] = ld * (ns - 1 - s2);
-        int rc = upload(ctx, t.hdr, std::vector<DevOpHeader>{h});
-        rc = rc ? rc : upload(ctx, t.loc, locs);
-        rc = rc ? rc : upload(ctx, t.val, vals);
-        rc = rc ? rc : upload(ctx, t.off, offs);
+        const std::vector<int32_t> map = tile_map(R.op, g);
+        const int nb = ns - (int)g.size();
+        int rc = upload(ctx, t.hdr, std::vector<DevOpHeader>{im.hdr});
+        rc = rc ? rc : upload(ctx, t.loc, im.locs);
+        rc = rc ? rc : upload(ctx, t.val, im.vals);
+        rc = rc ? rc : upload(ctx, t.off, im.offs);
         rc = rc ? rc : upload(ctx, t.map, map);
         if (rc) return rc;
         t.pass.op.hdr = (const DevOpHeader*)t.hdr.p;
@@ -266,28 +293,11 @@ int setup_compiled(oops_ctx* ctx, int m, LargeRun& R) {
     R.ctx = ctx;
     R.n = R.op.n;
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
-    DevOpHeader h;
-    std::memset(&h, 0, sizeof(h));
-    h.n = (int32_t)R.op.n;
-    h.nspin = R.op.nspin;
-    h.ld_bits = R.op.ld_bits;
-    h.nloc = (int32_t)R.op.locs.size();
-    h.nent = (int32_t)R.op.val.size();
-    h.max_row = R.op.max_row;
-    h.c0re = R.op.c0.real();
-    h.c0im = R.op.c0.imag();
-    h.ent2_first = R.op.ent2_first;
-    std::vector<double> vals;
-    for (auto& z : R.op.val) { vals.push_back(z.real()); vals.push_back(z.imag()); }
-    if (vals.empty()) { vals.push_back(0.0); vals.push_back(0.0); }
-    std::vector<int32_t> offs(R.op.off);
-    if (offs.empty()) offs.push_back(0);
-    std::vector<DevOpLocal> locs(R.op.locs);
-    if (locs.empty()) locs.push_back(DevOpLocal{});
-    rc = upload(ctx, R.hdr, std::vector<DevOpHeader>{h});
-    rc = rc ? rc : upload(ctx, R.loc, locs);
-    rc = rc ? rc : upload(ctx, R.val, vals);
-    rc = rc ? rc : upload(ctx, R.off, offs);
+    const OpImage im = op_image(R.op);
+    rc = upload(ctx, R.hdr, std::vector<DevOpHeader>{im.hdr});
+    rc = rc ? rc : upload(ctx, R.loc, im.locs);
+    rc = rc ? rc : upload(ctx, R.val, im.vals);
+    rc = rc ? rc : upload(ctx, R.off, im.offs);
     if (rc) return rc;
     R.args.hdr = (const DevOpHeader*)R.hdr.p;
     R.args.loc = (const DevOpLocal*)R.loc.p;
@@ -703,6 +713,25 @@ int large_hahn_item(oops_ctx* ctx, const CompiledOp& A, const CompiledOp& B, con
     return OOPS_OK;
 }
 
+// hipGetSymbolAddress of the no-allocation array, once per device (with that
+// device current); the host-side name `scratch` is not a device address.
+int noalloc_array_address(int device, void** ptr) {
+    static std::mutex mu;
+    static std::map<int, void*> addr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto f = addr.find(device);
+    if (f != addr.end()) {
+        *ptr = f->second;
+        return OOPS_OK;
+    }
+    if (hipSetDevice(device) != hipSuccess) return OOPS_ERR_HIP;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return OOPS_ERR_HIP;
+    addr[device] = p;
+    *ptr = p;
+    return OOPS_OK;
+}
+
 }  // namespace oops
 
 // Host evaluation of the compiled product (ELL rows of the merged local
@@ -785,6 +814,83 @@ int oops_tile_plan(const oops_term_table* T, int32_t* npasses, int32_t mode, con
         }
     }
     return OOPS_OK;
+}
+
+int oops_hamvec_noalloc(oops_ctx* ctx, const oops_term_table* H, const double* x, double* y) {
+    if (!ctx || !H || !x || !y) return set_err(ctx, OOPS_ERR_ARG, "bad arguments");
+    if (!ctx->noalloc) return set_err(ctx, OOPS_ERR_HIP, "no-allocation array not available on this device");
+    CompiledOp op;
+    std::string err;
+    int rc = compile_operator(H, op, err, kLargeMaxLocals);
+    if (rc) return set_err(ctx, rc, err);
+    // host image of the array: the tables oops_hamvec uploads, 256-byte aligned
+    std::vector<unsigned char> img;
+    auto put = [&](const void* src, size_t bytes) {
+        const size_t at = (img.size() + 255) & ~size_t(255);
+        img.resize(at + bytes);
+        std::memcpy(img.data() + at, src, bytes);
+        return at;
+    };
+    struct Table { size_t hdr, loc, val, off, map; };
+    auto put_op = [&](const CompiledOp& o) {
+        const OpImage im = op_image(o);
+        Table t{};
+        t.hdr = put(&im.hdr, sizeof(DevOpHeader));
+        t.loc = put(im.locs.data(), sizeof(DevOpLocal) * im.locs.size());
+        t.val = put(im.vals.data(), sizeof(double) * im.vals.size());
+        t.off = put(im.offs.data(), sizeof(int32_t) * im.offs.size());
+        return t;
+    };
+    std::vector<TilePlan> plan;
+    const bool tiled = plan_tiles(op, plan);
+    std::vector<Table> tabs;
+    std::vector<std::vector<int32_t>> maps;
+    if (tiled) {
+        for (const TilePlan& tp : plan) {
+            tabs.push_back(put_op(tp.op));
+            maps.push_back(tile_map(op, tp.G));
+            tabs.back().map = put(maps.back().data(), sizeof(int32_t) * maps.back().size());
+        }
+    } else {
+        tabs.push_back(put_op(op));
+    }
+    if (img.size() > kNoAllocBytes)
+        return set_err(ctx, OOPS_ERR_CAPACITY, "operator tables of " + std::to_string(img.size()) +
+                                                   " bytes exceed the no-allocation array (" +
+                                                   std::to_string(kNoAllocBytes) + " bytes)");
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
+    unsigned char* base = static_cast<unsigned char*>(ctx->noalloc);
+    rc = hip_check(ctx, hipMemcpyAsync(base, img.data(), img.size(), hipMemcpyHostToDevice, ctx->stream),
+                   "H2D no-allocation tables");
+    if (rc) return rc;
+    auto args_of = [&](const Table& t) {
+        LargeOpArgs a;
+        a.hdr = (const DevOpHeader*)(base + t.hdr);
+        a.loc = (const DevOpLocal*)(base + t.loc);
+        a.val = (const double2*)(base + t.val);
+        a.off = (const int32_t*)(base + t.off);
+        return a;
+    };
+    if (tiled) {
+        const int ld = op.ld_bits;
+        for (size_t p = 0; p < plan.size(); ++p) {
+            LargeTilePass tp{};
+            tp.op = args_of(tabs[p]);
+            tp.map = (const int32_t*)(base + tabs[p].map);
+            tp.ng = (int32_t)plan[p].G.size();
+            tp.nb = op.nspin - tp.ng;
+            tp.ld = ld;
+            tp.ntiles = (int32_t)(op.n >> (ld * tp.ng));
+            launch_large_tile_matvec(tp, (const double2*)x, nullptr, (double2*)y, nullptr, 0, p > 0 ? 1 : 0,
+                                     ctx->stream);
+        }
+    } else {
+        launch_large_matvec(args_of(tabs[0]), (const double2*)x, nullptr, (double2*)y, nullptr, op.n, 0,
+                            large_grid(op.n, ctx->num_cu), ctx->stream);
+    }
+    rc = hip_check(ctx, hipGetLastError(), "no-allocation product launch");
+    // the array is reused by the next call: wait for this one
+    return rc ? rc : hip_check(ctx, hipStreamSynchronize(ctx->stream), "no-allocation product");
 }
 
 int oops_hamvec(oops_ctx* ctx, const oops_term_table* H, const double* v, double* w) {
