@@ -192,7 +192,7 @@ def measured_traffic(kernel):
     import re
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_cce4_r5nm_pmc.md"))):
         txt = open(path).read()
-        m = re.search(r"kernel:\s*(\S+)", txt)
+        m = re.search(r"^kernel:\s*(.+?)\s*$", txt, re.M)
         t = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+-]+)", txt)
         if m and t and m.group(1) == kernel:
             return float(t.group(1))

@@ -272,7 +272,9 @@ int oops_hahn_batch_elapsed(const oops_hahn_batch* b, double* ms);
  * 1 Arnoldi FP64, 2 Arnoldi mixed, 3 Lanczos mixed, 4 dense FP64-MFMA
  * propagator (n = 64), 5 single-state path (n > 256), 6 Lanczos mixed in the
  * flip-mask (slot) form, 7 Lanczos mixed for n = 1024 / 4096 (one 1024-thread
- * workgroup per cluster), 8 Lanczos FP64 in the flip-mask form; -1 not run. */
+ * workgroup per cluster), 8 Lanczos FP64 in the flip-mask form, 9 factored
+ * FP64 propagator powers (CPMG-N of unpolarised 4-spin clusters on a uniform
+ * grid, built by oops_cce_echo_batch_create); -1 not run. */
 int oops_hahn_batch_kernels(const oops_hahn_batch* b, int32_t max_buckets, int32_t* kind);
 int oops_hahn_batch_destroy(oops_hahn_batch* b);
 

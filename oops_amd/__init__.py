@@ -115,7 +115,8 @@ def lib():
 
 
 KERNEL_NAMES = {0: "dense_pade", 1: "arnoldi_fp64", 2: "arnoldi_mixed", 3: "lanczos_mixed", 4: "dense64_mfma",
-                5: "single_state", 6: "lanczos_slot", 7: "lanczos_medium", 8: "lanczos_slot_fp64", -1: None}
+                5: "single_state", 6: "lanczos_slot", 7: "lanczos_medium", 8: "lanczos_slot_fp64",
+                9: "factored_powers", -1: None}
 
 
 def _ptr(a):

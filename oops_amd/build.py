@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liboops_amd.so")
 ARCH = os.environ.get("OOPS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["cce_krylov.hip", "cce_lanczos.hip", "cce_dense64.hip", "cce_medium.hip", "expv_large.hip", "term_builder.hip",
+SOURCES = ["cce_krylov.hip", "cce_lanczos.hip", "cce_dense64.hip", "cce_medium.hip", "expv_large.hip", "term_builder.hip", "cce_factored.hip",
            "capi.cpp", "cce_host.cpp", "large_host.cpp", "serve.cpp"]
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -274,6 +274,11 @@ struct oops_hahn_batch {
         // states above 256 elements: every item on the single-state path, in turn
         bool large = false;
         bool device_tables = false;   // hdr, loc, val, off, eslot, smask, medmask written by the term builder
+        // factored propagator powers (unpolarised k-spin clusters, uniform grid):
+        // the kernel builds H0 / H1 from the coordinates, no operator tables
+        bool factored = false;
+        DevBuf fcoords, fspins;
+        FactoredArgs fconst;          // physics constants, k and N (pointers filled at launch)
         int kernel = -1;  // kind that ran last: see oops_hahn_batch_kernels
         std::vector<CompiledOp> hA, hB;
         std::vector<std::vector<double>> hu0, hy0;
@@ -579,7 +584,8 @@ static int finalize_batch(oops_ctx* ctx, std::unique_ptr<oops_hahn_batch>& b, st
                           oops_hahn_batch** out) {
     typedef BucketHost Host;
     for (auto& B : b->buckets)
-        if (B.cpmg && !(B.n <= m || (B.n == 64 && B.uniform) || ((precision == 1 || lanczos64) && B.hermitian)))
+        if (B.cpmg && !B.factored &&
+            !(B.n <= m || (B.n == 64 && B.uniform) || ((precision == 1 || lanczos64) && B.hermitian)))
             return set_err(ctx, OOPS_ERR_UNSUPPORTED,
                            "CPMG-N items of dimension " + std::to_string(B.n) +
                            " need one uniform time grid from 0 for the whole bucket, or mixed precision");
@@ -587,6 +593,18 @@ static int finalize_batch(oops_ctx* ctx, std::unique_ptr<oops_hahn_batch>& b, st
         auto& B = b->buckets[bk];
         Host& H = hosts[bk];
         if (B.large) continue;
+        if (B.factored) {
+            int rc = dev_upload(ctx, B.queue, std::vector<int32_t>(4, 0));
+            rc = rc ? rc : dev_upload(ctx, B.stats, std::vector<int32_t>(B.items.size() * kStatsPerItem, 0));
+            rc = rc ? rc : dev_upload(ctx, B.nt, H.nt);
+            rc = rc ? rc : dev_upload(ctx, B.tau_off, H.tau_off);
+            rc = rc ? rc : dev_upload(ctx, B.L_off, H.L_off);
+            rc = rc ? rc : dev_upload(ctx, B.tau, H.tau);
+            if (!rc) rc = alloc_results(ctx, B.L, std::max<int64_t>(B.L_len, 1));
+            if (rc) return rc;
+            B.grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu * 2));
+            continue;
+        }
         std::vector<double> rnd(2 * B.n + 16);
         int32_t seed[4] = {0, 0, 0, 5};
         dlaran_fill(seed, (int64_t)rnd.size(), rnd.data());
@@ -684,7 +702,8 @@ static int finalize_batch(oops_ctx* ctx, std::unique_ptr<oops_hahn_batch>& b, st
     for (auto& B : b->buckets)
         for (DevBuf* d : {&B.hdr, &B.loc, &B.val, &B.off, &B.opA, &B.opB, &B.opC, &B.opD, &B.voff, &B.u0, &B.y0,
                           &B.scale, &B.rnd, &B.L, &B.stats, &B.queue, &B.npulse, &B.nt, &B.tau_off, &B.L_off, &B.tau,
-                          &B.work, &B.eslot, &B.smask, &B.slotws, &B.sweepws, &B.medws, &B.medmask, &B.medtab})
+                          &B.work, &B.eslot, &B.smask, &B.slotws, &B.sweepws, &B.medws, &B.medmask, &B.medtab,
+                          &B.fcoords, &B.fspins})
             b->footprint += (int64_t)d->bytes;
     if (per_item) {
         if (hipStreamCreate(&b->stream) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipStreamCreate");
@@ -694,6 +713,19 @@ static int finalize_batch(oops_ctx* ctx, std::unique_ptr<oops_hahn_batch>& b, st
     }
     *out = b.release();
     return OOPS_OK;
+}
+
+// The factored propagator powers take the 4-spin clusters of an unpolarised
+// CPMG-N batch (N >= 2) without dephasing on a uniform grid: the dense
+// D x D propagators are exact and far cheaper than restarting the Krylov
+// sweeps at every output time.  OOPS_FACTORED=0 keeps them on the Lanczos
+// path; OOPS_FACTORED=all takes the Hahn echo (N = 1) too.
+bool oops::factored_applies(int k, int npulse, double deph_rate) {
+    const char* e = std::getenv("OOPS_FACTORED");
+    const std::string mode = e ? e : "";
+    if (mode == "0") return false;
+    if (k != 4 || deph_rate != 0.0) return false;
+    return npulse >= 2 || mode == "all";
 }
 
 int oops::batch_create_built(oops_ctx* ctx, const double* coords, int32_t nspin, const std::vector<BuiltGroup>& groups,
@@ -750,6 +782,43 @@ int oops::batch_create_built(oops_ctx* ctx, const double* coords, int32_t nspin,
         if (k < 1 || k > 4 || (int64_t)g.spins.size() != ni * k)
             return set_err(ctx, OOPS_ERR_ARG, "device term builder: clusters of 1-4 spins");
         const int n = 1 << (2 * k);
+        if (per_item && uniform && factored_applies(k, npulse, consts.deph_rate)) {
+            // factored propagator powers: H0 / H1 built in the kernel, no tables
+            b->buckets.emplace_back();
+            auto& B = b->buckets.back();
+            hosts.emplace_back();
+            BucketHost& H = hosts.back();
+            B.n = n;
+            B.factored = true;
+            B.cpmg = cpmg;
+            B.uniform = true;
+            B.slots = false;
+            B.medslot = false;
+            B.items.assign(g.items.begin(), g.items.end());
+            rc = dev_upload(ctx, B.fspins, g.spins);
+            rc = rc ? rc : dev_upload(ctx, B.fcoords, std::vector<double>(coords, coords + 3 * (size_t)nspin));
+            if (rc) return rc;
+            std::memset(&B.fconst, 0, sizeof(B.fconst));
+            for (int c = 0; c < 3; ++c) {
+                B.fconst.zee[c] = consts.zee[c];
+                B.fconst.center[c] = consts.center[c];
+                B.fconst.sv0[c] = consts.sv0[c];
+                B.fconst.sv1[c] = consts.sv1[c];
+            }
+            B.fconst.gamma_c = consts.gamma_c;
+            B.fconst.gamma_e = consts.gamma_e;
+            B.fconst.k = k;
+            B.fconst.npulse = npulse;
+            for (int64_t i = 0; i < ni; ++i) {
+                H.npulse.push_back(npulse);
+                H.nt.push_back(nt);
+                H.tau_off.push_back(0);
+                H.L_off.push_back(B.L_len);
+                B.L_len += nt;
+            }
+            H.tau.assign(t, t + nt);
+            continue;
+        }
         const int P = k * (k - 1) / 2;
         const int loc_stride = k + P;
         const int ent_stride = 16 * k + 128 * P;
@@ -976,7 +1045,22 @@ static int bucket_launch(oops_hahn_batch* b, oops_hahn_batch::Bucket& B, const K
     const bool dense64 = B.n == 64 && uniform && B.work.p && !B.two_chain;
     const bool lanczos = b->precision == 1 && B.hermitian && B.n > b->m;
     int lrc;
-    if (dense64) {
+    if (B.factored) {
+        B.kernel = 9;
+        FactoredArgs f = B.fconst;
+        f.coords = (const double*)B.fcoords.p;
+        f.clusters = (const int32_t*)B.fspins.p;
+        f.nitems = (int32_t)B.items.size();
+        f.item_nt = a.item_nt;
+        f.item_tau_off = a.item_tau_off;
+        f.item_L_off = a.item_L_off;
+        f.tau = a.tau;
+        f.L = a.L;
+        f.stats = a.stats;
+        f.queue = a.queue;
+        launch_zero_counter(a.queue, st);
+        lrc = launch_factored(f, B.grid, st);
+    } else if (dense64) {
         B.kernel = 4;
         launch_zero_counter(a.queue, st);
         lrc = launch_dense64(a, (double*)B.work.p, B.dense_grid, st);

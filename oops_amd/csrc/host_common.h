@@ -127,6 +127,9 @@ int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err
 int set_err(oops_ctx* ctx, int code, const std::string& msg);
 // Device address of the no-allocation array (large_host.cpp) on the current
 // device, looked up once per device.
+// The factored propagator powers take the group of k-spin clusters of an
+// unpolarised batch with N pulses (capi.cpp).
+bool factored_applies(int k, int npulse, double deph_rate);
 int noalloc_array_address(int device, void** ptr);
 // Step limit of every Krylov integration (environment OOPS_MAX_STEPS, else 5e7).
 int32_t max_steps_limit();

@@ -100,4 +100,27 @@ int launch_dense64(const KrylovArgs& a, double* work, int grid, hipStream_t stre
 int64_t dense64_workspace_doubles(int grid);
 int krylov_entry_capacity(int m);
 
+// Factored propagator powers (cce_factored.hip): CPMG-N / Hahn coherence of
+// unpolarised clusters of k <= 4 13C spins without dephasing, on uniform time
+// grids from 0; one 256-thread workgroup per cluster (work queue).
+struct FactoredArgs {
+    const double* coords;       // every bath spin, x y z (Angstrom)
+    const int32_t* clusters;    // k spin indices per item
+    int32_t nitems;
+    int32_t k;
+    int32_t npulse;             // N (1: Hahn echo, segments of tau each)
+    double zee[3];              // -gamma_C B
+    double center[3];           // NV electron position
+    double gamma_c, gamma_e;
+    double sv0[3], sv1[3];      // <S> of the two NV states
+    const int32_t* item_nt;
+    const int64_t* item_tau_off;
+    const int64_t* item_L_off;
+    const double* tau;
+    double* L;
+    int32_t* stats;             // nitems x kStatsPerItem (slot 8: failure flags)
+    int32_t* queue;             // work counter, zeroed by the caller
+};
+int launch_factored(const FactoredArgs& a, int grid, hipStream_t stream);
+
 }  // namespace oops

@@ -146,6 +146,10 @@ int64_t estimate_bytes_built(const std::vector<BuiltGroup>& groups, int npulse, 
     for (const BuiltGroup& g : groups) {
         const int64_t k = g.k, ni = (int64_t)g.items.size(), n = int64_t(1) << (2 * k), P = k * (k - 1) / 2;
         if (!ni) continue;
+        if (factored_applies((int)k, npulse, 0.0)) {   // spins, per-item arrays and L only
+            b += ni * (4 * k + 4 * 4 + 8 + 8 + 4 * kStatsPerItem + 8 * nt) + 8 * nt;
+            continue;
+        }
         const int64_t ent = 16 * k + 128 * P;
         b += ni * nops * (64 + (k + P) * 16 + ent * 21 + (kSlotMaskStride + kMedMaskStride + kTBSummary) * 4);
         b += ni * (4 * 4 + 8 + 8 + 4 + 4 + 8 + 8 + 4 * kStatsPerItem + 8 * nt) + 48 * n + 8 * nt;

@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "couplings.h"
 #include "term_builder.h"
 
 #pragma clang fp contract(off)
@@ -46,31 +47,6 @@ __device__ __forceinline__ cdx mul(cdx a, cdx b) { return mk(a.re * b.re - a.im 
 __device__ __forceinline__ cdx add(cdx a, cdx b) { return mk(a.re + b.re, a.im + b.im); }
 __device__ __forceinline__ cdx conjx(cdx a) { return mk(a.re, -a.im); }
 __device__ __forceinline__ bool nonzero(cdx a) { return a.re != 0.0 || a.im != 0.0; }
-
-constexpr double kHbar = 1.0545718e-34;
-constexpr double kMu0 = 1.25663706143592e-06;
-constexpr double kPi = 3.14159265358979323846;
-constexpr double kDistEps = 1e-10;
-
-// dipole(s1, s2) of two point dipoles (misc.cpp:11-36) with the host's
-// operation order: distance ((dx^2 + dz^2) + dy^2), n = r / d, prefactor
-// ((hbar mu0) / (4 pi)) (g1 g2) / (d0 d0 d0)
-__device__ void dipole9(const double* r1, const double* r2, double g1, double g2, double* res) {
-    const double dx = r1[0] - r2[0], dy = r1[1] - r2[1], dz = r1[2] - r2[2];
-    double acc1 = dx * dx;
-    const double acc2 = dy * dy;
-    acc1 = acc1 + dz * dz;
-    const double d = sqrt(acc1 + acc2);
-    if (d <= kDistEps) {
-        for (int q = 0; q < 9; ++q) res[q] = 0.0;
-        return;
-    }
-    const double n[3] = {(r1[0] - r2[0]) / d, (r1[1] - r2[1]) / d, (r1[2] - r2[2]) / d};
-    const double d0 = d * 1e-10;
-    const double pref = kHbar * kMu0 / (4.0 * kPi) * (g1 * g2) / (d0 * d0 * d0);
-    for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) res[3 * a + b] = pref * ((a == b ? 1.0 : 0.0) - 3.0 * n[a] * n[b]);
-}
 
 constexpr int kMaxK = 4;
 constexpr int kMaxPairs = kMaxK * (kMaxK - 1) / 2;

@@ -57,6 +57,8 @@ def test_device_tables_equal_host_tables(ctx):
 def _batches(ctx, C, par, precision, tol, npulse, t):
     import os
     out = {}
+    # both builds on the Krylov kernels (the factored CPMG path takes no tables)
+    os.environ["OOPS_FACTORED"] = "0"
     for mode in ("host", "device"):
         os.environ["OOPS_TERM_BUILDER"] = mode
         try:
@@ -72,6 +74,7 @@ def _batches(ctx, C, par, precision, tol, npulse, t):
             out[mode] = (L, st)
         finally:
             os.environ.pop("OOPS_TERM_BUILDER", None)
+    os.environ.pop("OOPS_FACTORED", None)
     return out
 
 
