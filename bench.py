@@ -275,7 +275,7 @@ def main():
     ap.add_argument("--serve-requests", dest="serve_requests", type=int, default=20,
                     help="requests of the serving segment (Poisson arrivals at 50/s); 0 skips it")
     ap.add_argument("--cpu-per", dest="cpu_per", type=int, default=None)
-    ap.add_argument("--serve-cpmg-order", dest="serve_cpmg_order", type=int, default=3,
+    ap.add_argument("--serve-cpmg-order", dest="serve_cpmg_order", type=int, default=4,
                     help="highest CCE order of the serving mix's CPMG-N requests")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -85,7 +85,11 @@ int oops_slot_form(const oops_term_table* T, int32_t max_slots, int32_t* nslot64
  * operators of 2^19 elements and more (*npasses = 0: the flat ELL product).
  * mode 1 with v, w: w = H v on the host through that plan (the kernels'
  * tile-to-index mapping, checked to be a bijection in every pass: status
- * OOPS_ERR_NUMERIC otherwise); mode 2: w = H v through the flat ELL tables.
+ * OOPS_ERR_NUMERIC otherwise); mode 2: w = H v through the flat ELL tables;
+ * mode 3: w = H v through the two-axis product the large path uses for
+ * spin-1/2 Liouvillians of 8-12 spins whose terms act on ket digits only or
+ * on bra digits only (*npasses = 2; OOPS_ERR_UNSUPPORTED when the table does
+ * not split), with the kernels' tile mapping checked to be a bijection.
  * No reference counterpart (host check of the device layout). */
 int oops_tile_plan(const oops_term_table* T, int32_t* npasses, int32_t mode, const double* v, double* w);
 

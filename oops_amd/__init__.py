@@ -212,19 +212,21 @@ def slot_form(H: TermTable, max_slots=4096):
     return masks, coef, n64.value
 
 
-def tile_plan(H: TermTable, v=None, flat=False):
+def tile_plan(H: TermTable, v=None, flat=False, axis=False):
     """Number of passes of the large path's tiled product for H (0: flat ELL
     product); with v, also H v computed on the host through that plan (or
-    through the flat ELL tables with flat=True)."""
+    through the flat ELL tables with flat=True, or through the two-axis
+    product of split Liouvillians with axis=True)."""
     n = ctypes.c_int32()
+    mode = 3 if axis else (2 if flat else 1)
     if v is None:
-        _check(lib().oops_tile_plan(ctypes.byref(H.c), ctypes.byref(n), ctypes.c_int32(0), None, None), None,
-               "oops_tile_plan")
+        _check(lib().oops_tile_plan(ctypes.byref(H.c), ctypes.byref(n), ctypes.c_int32(3 if axis else 0), None,
+                                    None), None, "oops_tile_plan")
         return n.value
     v = np.ascontiguousarray(v, np.complex128)
     w = np.zeros_like(v)
-    _check(lib().oops_tile_plan(ctypes.byref(H.c), ctypes.byref(n), ctypes.c_int32(2 if flat else 1), _ptr(v),
-                                _ptr(w)), None, "oops_tile_plan")
+    _check(lib().oops_tile_plan(ctypes.byref(H.c), ctypes.byref(n), ctypes.c_int32(mode), _ptr(v), _ptr(w)), None,
+           "oops_tile_plan")
     return n.value, w
 
 

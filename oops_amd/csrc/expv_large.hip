@@ -175,6 +175,109 @@ __global__ __launch_bounds__(kTT) void large_tile_matvec(LargeTilePass tp, const
     }
 }
 
+// Two-axis product (see LargeAxisPass).  spread() deposits the bits of v at
+// the even bit positions (v < 2^16).
+__device__ __forceinline__ int64_t spread_bits(int64_t v) {
+    v = (v | (v << 8)) & 0x00FF00FFLL;
+    v = (v | (v << 4)) & 0x0F0F0F0FLL;
+    v = (v | (v << 2)) & 0x33333333LL;
+    v = (v | (v << 1)) & 0x55555555LL;
+    return v;
+}
+
+constexpr int kAT = 1024;   // threads of the two-axis product
+
+template <typename VT>
+__global__ __launch_bounds__(kAT) void large_axis_matvec(LargeAxisPass ap, const double2* __restrict__ x,
+                                                         const double* __restrict__ xscale_p,
+                                                         double2* __restrict__ y, VT* __restrict__ xstore,
+                                                         int alpha_mode, int accumulate) {
+    __shared__ cd xs[kLargeAxisMaxTile];
+    const int tid = threadIdx.x;
+    const int k = ap.k, bb = ap.bbits;
+    const int D = 1 << k, B = 1 << bb;
+    const int tsize = D << bb;
+    const int64_t hi = (int64_t)blockIdx.x;      // the other axis's digits above the tile's
+    // tile element u (consecutive u: increasing global index) -> axis value a,
+    // other-axis low value b, global index
+    auto decode = [&](int u, int& a, int& b) {
+        const int low = u & ((1 << (2 * bb)) - 1);
+        int ev = 0, od = 0;
+        for (int q = 0; q < bb; ++q) {
+            ev |= ((low >> (2 * q)) & 1) << q;
+            od |= ((low >> (2 * q + 1)) & 1) << q;
+        }
+        const int up = u >> (2 * bb);
+        // side 0: even global bits are the ket (axis) digits, odd the bra (b);
+        // side 1: odd bits are the bra (axis) digits, even the ket (b)
+        if (ap.side == 0) { a = ev | (up << bb); b = od; }
+        else { a = od | (up << bb); b = ev; }
+    };
+    auto gidx = [&](int a, int b) -> int64_t {
+        const int64_t other = (hi << bb) | b;
+        const int64_t ket = ap.side == 0 ? a : other;
+        const int64_t bra = ap.side == 0 ? other : a;
+        return spread_bits(ket) | (spread_bits(bra) << 1);
+    };
+    const double xsc = xscale_p ? *xscale_p : 1.0;
+    for (int u = tid; u < tsize; u += kAT) {
+        int a, b;
+        decode(u, a, b);
+        const int64_t g = gidx(a, b);
+        const double2 xv = x[g];
+        xs[a * B + b] = mkcd(xv.x, xv.y);
+        if (xstore) vstore(xstore + g, xv.x * xsc, xv.y * xsc);
+    }
+    __syncthreads();
+    // rows of the axis operator: thread r handles rows r, r + kAT, ..
+    constexpr int kMaxRows = kLargeAxisMaxTile / kAT;   // rows x B <= tile / kAT per thread
+    cd acc[kMaxRows];
+    const int rows_per = (D + kAT - 1) / kAT;
+#pragma unroll
+    for (int q = 0; q < kMaxRows; ++q) acc[q] = mkcd(0.0, 0.0);
+    // accumulator q = (row slot rq, column b): rq * B + b < kMaxRows
+    for (int rq = 0; rq < rows_per; ++rq) {
+        const int r = tid + rq * kAT;
+        if (r >= D) break;
+        for (int j = 0; j < ap.W; ++j) {
+            const double2 v = ap.val[(int64_t)j * D + r];
+            const int c = ap.col[(int64_t)j * D + r];
+            const cd vv = mkcd(v.x, v.y);
+#pragma unroll
+            for (int q = 0; q < kMaxRows; ++q) {
+                const int bq = q - rq * B;
+                if (bq >= 0 && bq < B) acc[q] = cfma(vv, xs[c * B + bq], acc[q]);
+            }
+        }
+    }
+    __syncthreads();   // every thread has read xs
+    for (int rq = 0; rq < rows_per; ++rq) {
+        const int r = tid + rq * kAT;
+        if (r >= D) break;
+#pragma unroll
+        for (int q = 0; q < kMaxRows; ++q) {
+            const int bq = q - rq * B;
+            if (bq >= 0 && bq < B) {
+                cd v = cscale(acc[q], xsc);
+                if (alpha_mode == 1) v = cmulmi(v);
+                xs[r * B + bq] = v;
+            }
+        }
+    }
+    __syncthreads();
+    for (int u = tid; u < tsize; u += kAT) {
+        int a, b;
+        decode(u, a, b);
+        const int64_t g = gidx(a, b);
+        cd v = xs[a * B + b];
+        if (accumulate) {
+            const double2 yo = y[g];
+            v = mkcd(v.re + yo.x, v.im + yo.y);
+        }
+        y[g] = make_double2(v.re, v.im);
+    }
+}
+
 // Same product for a general sparse matrix in CSR form (row_ptr[n+1], col,
 // val): y = alpha * A (xscale * x), alpha_mode 0: +1, 1: -i.
 __global__ __launch_bounds__(kLT) void large_csr_matvec(const int64_t* __restrict__ row_ptr,
@@ -477,6 +580,16 @@ void launch_large_matvec(const LargeOpArgs& o, const double2* x, const double* x
     else
         hipLaunchKernelGGL(large_matvec<double2>, dim3(grid), dim3(kLT), 0, s, o, x, xscale, y, (double2*)xstore, n,
                            alpha_mode);
+}
+
+void launch_large_axis_matvec(const LargeAxisPass& ap, const double2* x, const double* xscale, double2* y,
+                              void* xstore, int alpha_mode, int accumulate, hipStream_t s, bool fp32) {
+    if (fp32)
+        hipLaunchKernelGGL(large_axis_matvec<float2>, dim3(ap.ntiles), dim3(kAT), 0, s, ap, x, xscale, y,
+                           (float2*)xstore, alpha_mode, accumulate);
+    else
+        hipLaunchKernelGGL(large_axis_matvec<double2>, dim3(ap.ntiles), dim3(kAT), 0, s, ap, x, xscale, y,
+                           (double2*)xstore, alpha_mode, accumulate);
 }
 
 void launch_large_tile_matvec(const LargeTilePass& tp, const double2* x, const double* xscale, double2* y,

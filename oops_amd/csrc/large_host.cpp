@@ -65,6 +65,12 @@ struct LargeRun {
         LargeTilePass pass{};
     };
     std::vector<Tile> tiles;
+    // two-axis product (Liouville-space operators split into ket / bra sides)
+    struct Axis {
+        DevBuf val, col;
+        LargeAxisPass pass{};
+    };
+    std::vector<Axis> axis;
     int rc = 0;
 };
 
@@ -210,7 +216,7 @@ static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 static int setup_tiles(oops_ctx* ctx, LargeRun& R) {
     std::vector<TilePlan> plan;
-    if (!plan_tiles(R.op, plan)) return OOPS_OK;
+    if (!R.axis.empty() || !plan_tiles(R.op, plan)) return OOPS_OK;
     const int ns = R.op.nspin, ld = R.op.ld_bits;
     R.tiles.resize(plan.size());
     for (size_t p = 0; p < plan.size(); ++p) {
@@ -245,6 +251,10 @@ void matvec(LargeRun& R, const double2* x, const double* xs, double2* y, void* x
     if (R.csr)
         launch_large_csr_matvec((const int64_t*)R.rp.p, (const int32_t*)R.ci.p, (const double2*)R.cv.p, x, xs, y,
                                 (double2*)xstore, R.n, alpha_mode, R.grid, R.s);
+    else if (!R.axis.empty())
+        for (size_t p = 0; p < R.axis.size(); ++p)
+            launch_large_axis_matvec(R.axis[p].pass, x, xs, y, p == 0 ? xstore : nullptr, alpha_mode, p > 0 ? 1 : 0,
+                                     R.s, R.fp32);
     else if (!R.tiles.empty())
         for (size_t p = 0; p < R.tiles.size(); ++p)
             launch_large_tile_matvec(R.tiles[p].pass, x, xs, y, p == 0 ? xstore : nullptr, alpha_mode, p > 0 ? 1 : 0,
@@ -280,10 +290,143 @@ int setup_buffers(oops_ctx* ctx, int m, LargeRun& R);
 
 int setup_compiled(oops_ctx* ctx, int m, LargeRun& R);
 
+// Ket / bra split of a Liouville-space term table (every spin of local
+// dimension 4, local index ket + 2 bra): a 4 x 4 body matrix kron(I2, B) acts
+// on the ket digit with B, kron(A, I2) on the bra digit with A (the FLAT and
+// SHARP expansions, KronProd.cpp:9-28); the identity is both.  Each term must
+// act on one side only.  HK / HB come back as sparse rows (D = 2^k rows,
+// column -> value, exact zeros dropped); false when some term mixes the sides
+// or has another form (dephasing), or the table is not a spin-1/2 Liouvillian.
+static bool axis_split(const oops_term_table* T, const cplx* ccoeff, std::vector<std::map<int32_t, cplx>> rows[2]) {
+    const int k = T->nspin;
+    if (k < 1 || k > 15) return false;
+    for (int s = 0; s < k; ++s)
+        if (T->spin_dim[s] != 4) return false;
+    const int D = 1 << k;
+    const cplx* mats = reinterpret_cast<const cplx*>(T->mats);
+    rows[0].assign(D, {});
+    rows[1].assign(D, {});
+    cplx c0(0.0, 0.0);
+    for (int t = 0; t < T->nterm; ++t) {
+        cplx c = ccoeff ? ccoeff[t] : cplx(T->coeff[t], 0.0);
+        const int64_t nb = T->nbody[t];
+        const int64_t p0 = T->pos_offset[t];
+        if (nb > 2 || nb < 0) return false;
+        int side = -1;
+        int sp[2];
+        cplx m2[2][4];   // 2 x 2 column-major per remaining body
+        int nbody = 0;
+        for (int64_t b = 0; b < nb; ++b) {
+            if (T->dims[p0 + b] != 4) return false;
+            const cplx* M = mats + T->mat_offset[p0 + b];
+            auto at = [&](int r, int cc) { return M[r + 4 * cc]; };
+            bool ket = true, bra = true;
+            for (int r1 = 0; r1 < 2; ++r1)
+                for (int c1 = 0; c1 < 2; ++c1)
+                    for (int r2 = 0; r2 < 2; ++r2)
+                        for (int c2 = 0; c2 < 2; ++c2) {
+                            const cplx v = at(2 * r1 + r2, 2 * c1 + c2);
+                            if (v != (r1 == c1 ? at(r2, c2) : cplx(0.0))) ket = false;
+                            if (v != (r2 == c2 ? at(2 * r1, 2 * c1) : cplx(0.0))) bra = false;
+                        }
+            if (ket && bra) {   // f times the identity (ket- and bra-form at once): fold f into c
+                if (at(0, 0) != at(1, 1) || at(1, 0) != 0.0 || at(0, 1) != 0.0) return false;
+                c *= at(0, 0);
+                continue;
+            }
+            const int sd = ket ? 0 : (bra ? 1 : -2);
+            if (sd < 0) return false;
+            if (side >= 0 && side != sd) return false;
+            side = sd;
+            sp[nbody] = (int)T->pos[p0 + b];
+            for (int r = 0; r < 2; ++r)
+                for (int cc = 0; cc < 2; ++cc)
+                    m2[nbody][r + 2 * cc] = sd == 0 ? at(r, cc) : at(2 * r, 2 * cc);
+            ++nbody;
+        }
+        if (nbody == 0) {
+            c0 += c;
+            continue;
+        }
+        if (sp[0] < 0 || sp[0] >= k || (nbody == 2 && (sp[1] < 0 || sp[1] >= k || sp[1] == sp[0]))) return false;
+        auto& R = rows[side];
+        for (int r = 0; r < D; ++r) {
+            const int sh0 = k - 1 - sp[0];
+            const int r0 = (r >> sh0) & 1;
+            if (nbody == 1) {
+                for (int c0b = 0; c0b < 2; ++c0b) {
+                    const cplx v = c * m2[0][r0 + 2 * c0b];
+                    if (v == 0.0) continue;
+                    R[r][(r & ~(1 << sh0)) | (c0b << sh0)] += v;
+                }
+            } else {
+                const int sh1 = k - 1 - sp[1];
+                const int r1 = (r >> sh1) & 1;
+                for (int a = 0; a < 2; ++a)
+                    for (int b2 = 0; b2 < 2; ++b2) {
+                        const cplx v = c * m2[0][r0 + 2 * a] * m2[1][r1 + 2 * b2];
+                        if (v == 0.0) continue;
+                        R[r][(r & ~((1 << sh0) | (1 << sh1))) | (a << sh0) | (b2 << sh1)] += v;
+                    }
+            }
+        }
+    }
+    if (c0 != 0.0)
+        for (int r = 0; r < D; ++r) rows[0][r][r] += c0;
+    return true;
+}
+
+// The two-axis product for spin-1/2 Liouvillians of 8-12 spins whose terms
+// split into ket and bra sides; OOPS_LARGE_AXIS=0 keeps the tiled / flat
+// products.
+static int setup_axis(oops_ctx* ctx, const oops_term_table* T, const cplx* ccoeff, LargeRun& R) {
+    R.axis.clear();
+    if (const char* e = std::getenv("OOPS_LARGE_AXIS"))
+        if (e[0] == '0') return OOPS_OK;
+    const int k = T->nspin;
+    if (k < 8 || k > 12) return OOPS_OK;
+    std::vector<std::map<int32_t, cplx>> rows[2];
+    if (!axis_split(T, ccoeff, rows)) return OOPS_OK;
+    const int D = 1 << k;
+    const int bbits = std::min(2, 12 - k);   // tile D x 2^bbits <= 4096 elements
+    R.axis.resize(2);
+    for (int side = 0; side < 2; ++side) {
+        int W = 1;
+        for (const auto& row : rows[side]) W = std::max(W, (int)row.size());
+        std::vector<double> val(2 * (size_t)W * D, 0.0);
+        std::vector<int32_t> col((size_t)W * D, 0);
+        for (int r = 0; r < D; ++r) {
+            int j = 0;
+            for (const auto& e : rows[side][r]) {
+                val[2 * ((size_t)j * D + r)] = e.second.real();
+                val[2 * ((size_t)j * D + r) + 1] = e.second.imag();
+                col[(size_t)j * D + r] = e.first;
+                ++j;
+            }
+            for (; j < W; ++j) col[(size_t)j * D + r] = r;
+        }
+        LargeRun::Axis& A = R.axis[side];
+        int rc = upload(ctx, A.val, val);
+        rc = rc ? rc : upload(ctx, A.col, col);
+        if (rc) return rc;
+        A.pass.val = (const double2*)A.val.p;
+        A.pass.col = (const int32_t*)A.col.p;
+        A.pass.W = W;
+        A.pass.k = k;
+        A.pass.bbits = bbits;
+        A.pass.side = side;
+        A.pass.ntiles = 1 << (k - bbits);
+    }
+    return OOPS_OK;
+}
+
 int setup(oops_ctx* ctx, const oops_term_table* T, int m, LargeRun& R, const std::complex<double>* ccoeff = nullptr) {
     std::string err;
     int rc = compile_operator(T, R.op, err, kLargeMaxLocals, ccoeff);
     if (rc) return set_err(ctx, rc, err);
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
+    rc = setup_axis(ctx, T, ccoeff, R);
+    if (rc) return rc;
     return setup_compiled(ctx, m, R);
 }
 
@@ -767,6 +910,63 @@ extern "C" {
 
 int oops_tile_plan(const oops_term_table* T, int32_t* npasses, int32_t mode, const double* v, double* w) {
     if (!T || !npasses) return OOPS_ERR_ARG;
+    if (mode == 3) {
+        // the two-axis product on the host through the kernel's tile mapping
+        std::vector<std::map<int32_t, cplx>> rows[2];
+        const int k = T->nspin;
+        if (k < 1 || k > 12 || !axis_split(T, nullptr, rows)) { *npasses = 0; return OOPS_ERR_UNSUPPORTED; }
+        *npasses = 2;
+        if (!v || !w) return OOPS_OK;
+        const cplx* x = reinterpret_cast<const cplx*>(v);
+        cplx* y = reinterpret_cast<cplx*>(w);
+        const int64_t n = 1LL << (2 * k);
+        const int bb = std::max(0, std::min(2, 12 - k));
+        const int D = 1 << k, B = 1 << bb;
+        auto spread = [](int64_t q) {
+            int64_t r = 0;
+            for (int j = 0; j < 16; ++j) r |= ((q >> j) & 1) << (2 * j);
+            return r;
+        };
+        std::vector<uint8_t> seen(n);
+        std::vector<cplx> xs((size_t)D * B);
+        for (int side = 0; side < 2; ++side) {
+            std::fill(seen.begin(), seen.end(), 0);
+            for (int64_t hi = 0; hi < (int64_t)(1 << (k - bb)); ++hi) {
+                auto decode = [&](int u, int& a, int& b) {
+                    const int low = u & ((1 << (2 * bb)) - 1);
+                    int ev = 0, od = 0;
+                    for (int q = 0; q < bb; ++q) {
+                        ev |= ((low >> (2 * q)) & 1) << q;
+                        od |= ((low >> (2 * q + 1)) & 1) << q;
+                    }
+                    const int up = u >> (2 * bb);
+                    if (side == 0) { a = ev | (up << bb); b = od; }
+                    else { a = od | (up << bb); b = ev; }
+                };
+                auto gidx = [&](int a, int b) {
+                    const int64_t other = (hi << bb) | b;
+                    const int64_t ket = side == 0 ? a : other, bra = side == 0 ? other : a;
+                    return spread(ket) | (spread(bra) << 1);
+                };
+                for (int u = 0; u < D * B; ++u) {
+                    int a, b;
+                    decode(u, a, b);
+                    const int64_t g = gidx(a, b);
+                    if (g < 0 || g >= n || seen[g]) return OOPS_ERR_NUMERIC;   // not a bijection
+                    seen[g] = 1;
+                    xs[(size_t)a * B + b] = x[g];
+                }
+                for (int a = 0; a < D; ++a)
+                    for (int b = 0; b < B; ++b) {
+                        cplx acc(0.0);
+                        for (const auto& e : rows[side][a]) acc += e.second * xs[(size_t)e.first * B + b];
+                        const int64_t g = gidx(a, b);
+                        y[g] = side == 0 ? acc : y[g] + acc;
+                    }
+            }
+        }
+        return OOPS_OK;
+    }
     CompiledOp op;
     std::string err;
     int rc = compile_operator(T, op, err, kLargeMaxLocals);

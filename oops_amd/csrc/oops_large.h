@@ -40,6 +40,28 @@ struct LargeTilePass {
 void launch_large_tile_matvec(const LargeTilePass& tp, const double2* x, const double* xscale, double2* y,
                               void* xstore, int alpha_mode, int accumulate, hipStream_t s, bool fp32 = false);
 
+// Two-axis product of a Liouville-space operator whose every term acts on the
+// ket digits only or on the bra digits only (the SHARP / FLAT expansions of
+// Hamiltonians): the state index holds spin s's ket digit at bit 2(k-1-s) and
+// its bra digit at bit 2(k-1-s)+1, and L x = HK along the ket axis + HB along
+// the bra axis, HK and HB sparse D x D (D = 2^k) in column-major ELL form
+// (entry j of row r at j * D + r; padding entries have value 0 and col = r).
+// Pass `side` (0: ket axis, 1: bra axis) stages tiles of D x 2^bbits elements
+// (the whole axis times the other axis's lowest bbits digits) in LDS; pass 0
+// writes y and stores the scaled basis vector, pass 1 adds to y.
+constexpr int kLargeAxisMaxTile = 4096;   // elements of one staged tile (64 KB FP64)
+struct LargeAxisPass {
+    const double2* val;   // W x D
+    const int32_t* col;   // W x D
+    int32_t W;
+    int32_t k;            // spins (D = 2^k)
+    int32_t bbits;        // other-axis digits per tile
+    int32_t side;         // 0: ket axis, 1: bra axis
+    int32_t ntiles;       // 2^(k - bbits)
+};
+void launch_large_axis_matvec(const LargeAxisPass& ap, const double2* x, const double* xscale, double2* y,
+                              void* xstore, int alpha_mode, int accumulate, hipStream_t s, bool fp32 = false);
+
 int large_grid(int64_t n, int num_cu);
 // xstore (basis column, may be NULL) is float2 when fp32, else double2.
 void launch_large_matvec(const LargeOpArgs& o, const double2* x, const double* xscale, double2* y, void* xstore,

@@ -4,8 +4,10 @@ Request mix (the serving configuration): bath seed, radius uniform in
 [rmin, rmax] A (p = 0.011, cutoff 8 A), B uniform in [0.01, 0.5] T along
 [111], a CPMG-N sequence with N in {1, 2, 4, 8, 16}, CCE order 2-4 and
 101-401 time points on [0, t_end].  CPMG-N with N >= 2 is drawn with CCE
-order <= --cpmg-max-order (4-spin CPMG-N runs on the Krylov path with one
-restart per output time, minutes per request at r = 5 nm).
+order <= --cpmg-max-order (default 4: the 4-spin clusters of CPMG-N requests
+run on the factored propagator powers; with OOPS_FACTORED=0 they would run on
+the Krylov path with one restart per output time, minutes per request at
+r = 5 nm).
 
 Requests are submitted at their arrival times to an oops_amd.Server (mixed
 precision, continuous batching); latency = completion - arrival.
@@ -87,7 +89,7 @@ def main(argv=None):
     ap.add_argument("--rmax", type=float, default=50.0)
     ap.add_argument("--t-end", dest="t_end", type=float, default=2e-3)
     ap.add_argument("--max-order", dest="max_order", type=int, default=4)
-    ap.add_argument("--cpmg-max-order", dest="cpmg_max_order", type=int, default=3)
+    ap.add_argument("--cpmg-max-order", dest="cpmg_max_order", type=int, default=4)
     ap.add_argument("--budget-gb", dest="budget_gb", type=float, default=32.0)
     ap.add_argument("--inflight", type=int, default=8)
     ap.add_argument("--tol", type=float, default=1e-3)

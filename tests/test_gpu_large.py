@@ -37,19 +37,27 @@ def test_hamvec_against_explicit_kron(ctx):
 
 
 @pytest.mark.gpu
-def test_tiled_product_matches_flat_product(ctx):
-    """The tiled product of the large path (10-spin Liouvillian, n = 2^20,
-    6 passes) against the flat ELL product evaluated on the host from the
-    same compiled tables (pinned by the Kronecker fixtures above)."""
+@pytest.mark.parametrize("axis", ["1", "0"])
+def test_tiled_product_matches_flat_product(ctx, axis, monkeypatch):
+    """The large path's products of a 10-spin Liouvillian (n = 2^20): the
+    two-axis product (ket and bra sides, 2 passes) and, with OOPS_LARGE_AXIS=0,
+    the tiled product (6 passes), against the flat ELL product evaluated on
+    the host from the same compiled tables (pinned by the Kronecker fixtures
+    above); -lvB^dagger and an 8-spin lvA through the two-axis product too."""
+    monkeypatch.setenv("OOPS_LARGE_AXIS", axis)
     rng = np.random.default_rng(0)
     X = np.array([p for p in rng.uniform(-15, 15, (400, 3)) if np.linalg.norm(p) <= 15.0][:10])
-    T, _, _ = O.cluster_terms(X, O.cce_params(B=np.ones(3) * 0.05 / np.sqrt(3.0)), 0)
-    assert O.tile_plan(T) == 6
-    v = rng.standard_normal(T.n) + 1j * rng.standard_normal(T.n)
-    print(f"tiled hamvec n={T.n}", flush=True)
-    got = O.hamvec(ctx, T, v)
-    _, want = O.tile_plan(T, v, flat=True)
-    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    par = O.cce_params(B=np.ones(3) * 0.05 / np.sqrt(3.0))
+    cases = [(10, 0)] if axis == "0" else [(10, 0), (10, 2), (8, 0)]
+    for k, which in cases:
+        T, _, _ = O.cluster_terms(X[:k], par, which)
+        if k == 10:
+            assert O.tile_plan(T) == 6
+        v = rng.standard_normal(T.n) + 1j * rng.standard_normal(T.n)
+        print(f"{'two-axis' if axis == '1' else 'tiled'} hamvec k={k} which={which} n={T.n}", flush=True)
+        got = O.hamvec(ctx, T, v)
+        _, want = O.tile_plan(T, v, flat=True)
+        assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
 
 
 @pytest.mark.gpu

@@ -168,6 +168,27 @@ def test_tile_plan_product_equals_flat_product():
     assert O.tile_plan(T9) == 0
 
 
+def test_two_axis_product_equals_flat_product():
+    """The large path's two-axis product of a split Liouvillian (lvA = H0# -
+    H1b: every term acts on ket digits only or on bra digits only): the host
+    evaluation through the kernels' tile mapping (a bijection in both passes)
+    equals the flat ELL product for 8 and 10 spins (n = 2^16, 2^20);
+    -lvB^dagger splits too, a Liouvillian with dephasing terms does not."""
+    rng = np.random.default_rng(3)
+    X = np.array([p for p in rng.uniform(-15, 15, (400, 3)) if np.linalg.norm(p) <= 15.0][:10])
+    par = O.cce_params(B=B111)
+    for k, which in ((8, 0), (10, 0), (8, 2)):
+        T, _, _ = O.cluster_terms(X[:k], par, which)
+        v = rng.standard_normal(T.n) + 1j * rng.standard_normal(T.n)
+        npass, wa = O.tile_plan(T, v, axis=True)
+        assert npass == 2
+        _, wf = O.tile_plan(T, v, flat=True)
+        assert np.abs(wa - wf).max() <= 1e-12 * np.abs(wf).max(), (k, which)
+    Td, _, _ = O.cluster_terms(X[:8], O.cce_params(B=B111, dephasing_rate=1e3), 0)
+    with pytest.raises(O.OopsError):
+        O.tile_plan(Td, np.zeros(Td.n, complex), axis=True)
+
+
 def test_coordinate_files_round_trip(tmp_path):
     """Coordinate files in the reference's layout (SpinSource.cpp:20-79):
     RoyCoord.xyz reads back 548 13C spins with the coordinates the survey's
