@@ -23,6 +23,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "devcommon.h"
 #include "oops_kernels.h"
 #include "oops_large.h"
@@ -74,6 +76,65 @@ __global__ __launch_bounds__(kLT) void large_matvec(LargeOpArgs o, const double2
         if (xstore) {
             const double2 xv = x[i];
             vstore(xstore + i, xv.x * xs, xv.y * xs);
+        }
+    }
+}
+
+// The same product with each element's local operators split over LS
+// consecutive lanes (lane q takes operators q, q + LS, ..; the partial sums
+// meet by butterfly shuffles): states too small to fill the device with one
+// thread per element (a 14-spin Hilbert state has 16 384 elements and ~100
+// local operators) get LS times the parallelism and shorter gather chains.
+template <typename VT, int LS>
+__global__ __launch_bounds__(kLT) void large_matvec_split(LargeOpArgs o, const double2* __restrict__ x,
+                                                          const double* __restrict__ xscale_p,
+                                                          double2* __restrict__ y, VT* __restrict__ xstore,
+                                                          int64_t n, int alpha_mode) {
+    __shared__ LargeOpLds L;
+    large_op_load(o, L);
+    const double xs = xscale_p ? *xscale_p : 1.0;
+    const cd* xc = reinterpret_cast<const cd*>(x);
+    const cd* vals = reinterpret_cast<const cd*>(o.val);
+    const int sub = threadIdx.x % LS;
+    const int ld = L.hdr.ld_bits;
+    const int lmask = (1 << ld) - 1;
+    const int top = L.hdr.nspin - 1;
+    const int64_t stride = (int64_t)gridDim.x * (kLT / LS);
+    // every lane of an element group runs the same iterations (the shuffles
+    // need the whole group): the loop bound is per group
+    for (int64_t i0 = (int64_t)blockIdx.x * (kLT / LS); i0 < n; i0 += stride) {
+        const int64_t i = i0 + threadIdx.x / LS;
+        const bool live = i < n;
+        const int ii = live ? (int)i : 0;
+        cd acc = (sub == 0 && live) ? cmul(mkcd(L.hdr.c0re, L.hdr.c0im), xc[ii]) : mkcd(0.0, 0.0);
+        if (live)
+            for (int Lo = sub; Lo < L.hdr.nloc; Lo += LS) {
+                const DevOpLocal lo = L.locs[Lo];
+                const int shs = ld * (top - lo.s);
+                const int ls = (ii >> shs) & lmask;
+                int base = ii - (ls << shs);
+                int row = ls;
+                if (lo.nb == 2) {
+                    const int sht = ld * (top - lo.t);
+                    const int lt = (ii >> sht) & lmask;
+                    base -= lt << sht;
+                    row = (ls << ld) | lt;
+                }
+                acc = ell_row(xc, base, vals, o.off, lo.ent + row * lo.R, lo.R, acc);
+            }
+#pragma unroll
+        for (int msk = LS / 2; msk >= 1; msk >>= 1) {
+            acc.re += __shfl_xor(acc.re, msk, kWave);
+            acc.im += __shfl_xor(acc.im, msk, kWave);
+        }
+        if (sub == 0 && live) {
+            acc = cscale(acc, xs);
+            if (alpha_mode == 1) acc = cmulmi(acc);
+            y[i] = make_double2(acc.re, acc.im);
+            if (xstore) {
+                const double2 xv = x[i];
+                vstore(xstore + i, xv.x * xs, xv.y * xs);
+            }
         }
     }
 }
@@ -236,17 +297,31 @@ __global__ __launch_bounds__(kAT) void large_axis_matvec(LargeAxisPass ap, const
 #pragma unroll
     for (int q = 0; q < kMaxRows; ++q) acc[q] = mkcd(0.0, 0.0);
     // accumulator q = (row slot rq, column b): rq * B + b < kMaxRows
+    // the row's entries in chunks of 8: the table loads of a chunk are issued
+    // together (one cache-miss latency per chunk, not per entry)
+    constexpr int kChunk = 8;
     for (int rq = 0; rq < rows_per; ++rq) {
         const int r = tid + rq * kAT;
         if (r >= D) break;
-        for (int j = 0; j < ap.W; ++j) {
-            const double2 v = ap.val[(int64_t)j * D + r];
-            const int c = ap.col[(int64_t)j * D + r];
-            const cd vv = mkcd(v.x, v.y);
+        for (int j0 = 0; j0 < ap.W; j0 += kChunk) {
+            double2 v[kChunk];
+            int c[kChunk];
 #pragma unroll
-            for (int q = 0; q < kMaxRows; ++q) {
-                const int bq = q - rq * B;
-                if (bq >= 0 && bq < B) acc[q] = cfma(vv, xs[c * B + bq], acc[q]);
+            for (int u = 0; u < kChunk; ++u) {
+                const bool in = j0 + u < ap.W;
+                const int64_t at = (int64_t)(in ? j0 + u : 0) * D + r;
+                v[u] = ap.val[at];
+                c[u] = ap.col[at];
+                if (!in) v[u] = make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                const cd vv = mkcd(v[u].x, v[u].y);
+#pragma unroll
+                for (int q = 0; q < kMaxRows; ++q) {
+                    const int bq = q - rq * B;
+                    if (bq >= 0 && bq < B) acc[q] = cfma(vv, xs[c[u] * B + bq], acc[q]);
+                }
             }
         }
     }
@@ -351,13 +426,13 @@ __global__ __launch_bounds__(kLT) void large_dots(const double2* __restrict__ V,
 constexpr int kFinishThreads = 1024;
 
 // One wave per row: the lanes sum strided partials (independent loads), then
-// a wave reduction; 16 rows at a time.
-__global__ __launch_bounds__(kFinishThreads) void large_finish(const double* __restrict__ part, int nblk, int nrows,
-                                                               int jcol, int mode, double2* __restrict__ hess, int ldh,
-                                                               double2* __restrict__ coef, double* __restrict__ scal,
-                                                               int* __restrict__ flags, double break_tol) {
+// a wave reduction; the block's waves take the rows in turn.
+__device__ void finish_rows(const double* __restrict__ part, int nblk, int nrows, int jcol, int mode,
+                            double2* __restrict__ hess, int ldh, double2* __restrict__ coef,
+                            double* __restrict__ scal, int* __restrict__ flags, double break_tol) {
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    for (int l = wave; l < nrows; l += kFinishThreads / kWave) {
+    const int nw = blockDim.x / kWave;
+    for (int l = wave; l < nrows; l += nw) {
         double a = 0.0, b = 0.0;
         for (int k = lane; k < nblk; k += kWave) {
             a += part[((int64_t)l * nblk + k) * 2];
@@ -388,6 +463,13 @@ __global__ __launch_bounds__(kFinishThreads) void large_finish(const double* __r
             else hv = make_double2(hv.x + a, hv.y + b);
         }
     }
+}
+
+__global__ __launch_bounds__(kFinishThreads) void large_finish(const double* __restrict__ part, int nblk, int nrows,
+                                                               int jcol, int mode, double2* __restrict__ hess, int ldh,
+                                                               double2* __restrict__ coef, double* __restrict__ scal,
+                                                               int* __restrict__ flags, double break_tol) {
+    finish_rows(part, nblk, nrows, jcol, mode, hess, ldh, coef, scal, flags, break_tol);
 }
 
 // Fused classical Gram-Schmidt passes (one read of each basis vector per pass):
@@ -574,6 +656,17 @@ int large_grid(int64_t n, int num_cu) {
 
 void launch_large_matvec(const LargeOpArgs& o, const double2* x, const double* xscale, double2* y, void* xstore,
                          int64_t n, int alpha_mode, int grid, hipStream_t s, bool fp32) {
+    if (n <= (int64_t(1) << 17)) {
+        // 16 lanes per element: n / 16 elements per 256-thread workgroup
+        const int g = (int)std::min<int64_t>((n * 16 + kLT - 1) / kLT, 4096);
+        if (fp32)
+            hipLaunchKernelGGL((large_matvec_split<float2, 16>), dim3(g), dim3(kLT), 0, s, o, x, xscale, y,
+                               (float2*)xstore, n, alpha_mode);
+        else
+            hipLaunchKernelGGL((large_matvec_split<double2, 16>), dim3(g), dim3(kLT), 0, s, o, x, xscale, y,
+                               (double2*)xstore, n, alpha_mode);
+        return;
+    }
     if (fp32)
         hipLaunchKernelGGL(large_matvec<float2>, dim3(grid), dim3(kLT), 0, s, o, x, xscale, y, (float2*)xstore, n,
                            alpha_mode);
@@ -614,24 +707,46 @@ void launch_large_finish(const double* part, int nblk, int nrows, int jcol, int 
                        coef, scal, flags, break_tol);
 }
 
+// Gram-Schmidt passes; the register footprint follows the number of basis
+// vectors (J + 1 <= 8, 16, 24, 32 or 40).
+template <int MAXJ>
+static void gs_dots_j(const void* V, int64_t ldv, const double2* p, int64_t n, int J, double* part, int nblk,
+                      hipStream_t s, bool fp32) {
+    if (fp32)
+        hipLaunchKernelGGL((large_gs_dots<MAXJ, float2>), dim3(nblk), dim3(kLT), 0, s, (const float2*)V, ldv, p, n, J,
+                           part);
+    else
+        hipLaunchKernelGGL((large_gs_dots<MAXJ, double2>), dim3(nblk), dim3(kLT), 0, s, (const double2*)V, ldv, p, n,
+                           J, part);
+}
+
+template <int MAXJ>
+static void gs_update_j(const void* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef, int norm_only,
+                        double* part, int nblk, hipStream_t s, bool fp32) {
+    if (fp32)
+        hipLaunchKernelGGL((large_gs_update<MAXJ, float2>), dim3(nblk), dim3(kLT), 0, s, (const float2*)V, ldv, p, n,
+                           J, coef, norm_only, part);
+    else
+        hipLaunchKernelGGL((large_gs_update<MAXJ, double2>), dim3(nblk), dim3(kLT), 0, s, (const double2*)V, ldv, p,
+                           n, J, coef, norm_only, part);
+}
+
 void launch_large_gs_dots(const void* V, int64_t ldv, const double2* p, int64_t n, int J, double* part, int nblk,
                           hipStream_t s, bool fp32) {
-    if (fp32)
-        hipLaunchKernelGGL((large_gs_dots<kLargeMaxM, float2>), dim3(nblk), dim3(kLT), 0, s, (const float2*)V, ldv, p,
-                           n, J, part);
-    else
-        hipLaunchKernelGGL((large_gs_dots<kLargeMaxM, double2>), dim3(nblk), dim3(kLT), 0, s, (const double2*)V, ldv,
-                           p, n, J, part);
+    if (J < 8) gs_dots_j<8>(V, ldv, p, n, J, part, nblk, s, fp32);
+    else if (J < 16) gs_dots_j<16>(V, ldv, p, n, J, part, nblk, s, fp32);
+    else if (J < 24) gs_dots_j<24>(V, ldv, p, n, J, part, nblk, s, fp32);
+    else if (J < 32) gs_dots_j<32>(V, ldv, p, n, J, part, nblk, s, fp32);
+    else gs_dots_j<kLargeMaxM>(V, ldv, p, n, J, part, nblk, s, fp32);
 }
 
 void launch_large_gs_update(const void* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef,
                             int norm_only, double* part, int nblk, hipStream_t s, bool fp32) {
-    if (fp32)
-        hipLaunchKernelGGL((large_gs_update<kLargeMaxM, float2>), dim3(nblk), dim3(kLT), 0, s, (const float2*)V, ldv,
-                           p, n, J, coef, norm_only, part);
-    else
-        hipLaunchKernelGGL((large_gs_update<kLargeMaxM, double2>), dim3(nblk), dim3(kLT), 0, s, (const double2*)V, ldv,
-                           p, n, J, coef, norm_only, part);
+    if (J < 8) gs_update_j<8>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32);
+    else if (J < 16) gs_update_j<16>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32);
+    else if (J < 24) gs_update_j<24>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32);
+    else if (J < 32) gs_update_j<32>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32);
+    else gs_update_j<kLargeMaxM>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32);
 }
 
 int launch_large_pade(const double2* hess, int ldh, int mx, double t, double2* F, int* status, hipStream_t s) {
