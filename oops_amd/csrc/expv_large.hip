@@ -529,40 +529,44 @@ __global__ __launch_bounds__(kLT) void large_gs_dots(const VT* __restrict__ V, i
     gs_block_reduce<MAXJ>(sr, si, J + 1, red, part, 0);
 }
 
-template <int MAXJ, typename VT>
+template <int MAXJ, typename VT, bool NORM>
 __global__ __launch_bounds__(kLT) void large_gs_update(const VT* __restrict__ V, int64_t ldv,
                                                        double2* __restrict__ p, int64_t n, int J,
-                                                       const double2* __restrict__ coef, int norm_only,
+                                                       const double2* __restrict__ coef,
                                                        double* __restrict__ part) {
-    __shared__ double red[2 * MAXJ * (kLT / kWave)];
+    constexpr int NR = NORM ? 1 : MAXJ;   // partial sums per thread
+    __shared__ double red[2 * NR * (kLT / kWave)];
     __shared__ double2 c[MAXJ];
     for (int l = threadIdx.x; l <= J; l += kLT) c[l] = coef[l];
     __syncthreads();
-    double sr[MAXJ], si[MAXJ];
+    double sr[NR], si[NR];
 #pragma unroll
-    for (int l = 0; l < MAXJ; ++l) { sr[l] = 0.0; si[l] = 0.0; }
+    for (int l = 0; l < NR; ++l) { sr[l] = 0.0; si[l] = 0.0; }
     for (int64_t i = (int64_t)blockIdx.x * kLT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kLT) {
-        double2 v[MAXJ];
         cd acc = mkcd(p[i].x, p[i].y);
 #pragma unroll
         for (int l = 0; l < MAXJ; ++l)
             if (l <= J) {
-                v[l] = vload(V + (int64_t)l * ldv + i);
-                acc = cfms(mkcd(c[l].x, c[l].y), mkcd(v[l].x, v[l].y), acc);
+                const double2 v = vload(V + (int64_t)l * ldv + i);
+                acc = cfms(mkcd(c[l].x, c[l].y), mkcd(v.x, v.y), acc);
             }
         p[i] = make_double2(acc.re, acc.im);
-        if (norm_only) {
+        if constexpr (NORM) {
             sr[0] = fma(acc.re, acc.re, fma(acc.im, acc.im, sr[0]));
         } else {
+            // the basis values again (from the caches: they were just read),
+            // rather than held in registers across the update
+            asm volatile("" ::: "memory");
 #pragma unroll
             for (int l = 0; l < MAXJ; ++l)
                 if (l <= J) {
-                    sr[l] = fma(v[l].x, acc.re, fma(v[l].y, acc.im, sr[l]));
-                    si[l] = fma(v[l].x, acc.im, fma(-v[l].y, acc.re, si[l]));
+                    const double2 v = vload(V + (int64_t)l * ldv + i);
+                    sr[l] = fma(v.x, acc.re, fma(v.y, acc.im, sr[l]));
+                    si[l] = fma(v.x, acc.im, fma(-v.y, acc.re, si[l]));
                 }
         }
     }
-    gs_block_reduce<MAXJ>(sr, si, norm_only ? 1 : J + 1, red, part, 0);
+    gs_block_reduce<NR>(sr, si, NORM ? 1 : J + 1, red, part, 0);
 }
 
 // F = exp(t H(0:mx,0:mx)) e1 for the augmented Hessenberg matrix (Pade(6),
@@ -720,15 +724,22 @@ static void gs_dots_j(const void* V, int64_t ldv, const double2* p, int64_t n, i
                            J, part);
 }
 
+template <int MAXJ, bool NORM>
+static void gs_update_k(const void* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef, double* part,
+                        int nblk, hipStream_t s, bool fp32) {
+    if (fp32)
+        hipLaunchKernelGGL((large_gs_update<MAXJ, float2, NORM>), dim3(nblk), dim3(kLT), 0, s, (const float2*)V, ldv,
+                           p, n, J, coef, part);
+    else
+        hipLaunchKernelGGL((large_gs_update<MAXJ, double2, NORM>), dim3(nblk), dim3(kLT), 0, s, (const double2*)V,
+                           ldv, p, n, J, coef, part);
+}
+
 template <int MAXJ>
 static void gs_update_j(const void* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef, int norm_only,
                         double* part, int nblk, hipStream_t s, bool fp32) {
-    if (fp32)
-        hipLaunchKernelGGL((large_gs_update<MAXJ, float2>), dim3(nblk), dim3(kLT), 0, s, (const float2*)V, ldv, p, n,
-                           J, coef, norm_only, part);
-    else
-        hipLaunchKernelGGL((large_gs_update<MAXJ, double2>), dim3(nblk), dim3(kLT), 0, s, (const double2*)V, ldv, p,
-                           n, J, coef, norm_only, part);
+    if (norm_only) gs_update_k<MAXJ, true>(V, ldv, p, n, J, coef, part, nblk, s, fp32);
+    else gs_update_k<MAXJ, false>(V, ldv, p, n, J, coef, part, nblk, s, fp32);
 }
 
 void launch_large_gs_dots(const void* V, int64_t ldv, const double2* p, int64_t n, int J, double* part, int nblk,
