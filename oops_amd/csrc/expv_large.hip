@@ -569,6 +569,114 @@ __global__ __launch_bounds__(kLT) void large_gs_update(const VT* __restrict__ V,
     gs_block_reduce<NR>(sr, si, NORM ? 1 : J + 1, red, part, 0);
 }
 
+// Dot-product passes with the basis range split over lane halves: lanes q and
+// q + 32 of a wave own the same element, the low half the vectors
+// l < L0 = (J + 2) / 2, the high half the rest, so each lane holds half the
+// partial sums (long bases keep two waves per SIMD).  Per wave 32 elements;
+// the two halves' partial products of the update meet by one shuffle.
+template <int HJ>
+__device__ __forceinline__ void gs_half_reduce(const double (&sr)[HJ], const double (&si)[HJ], int l0, int nrows,
+                                               double* red, double* __restrict__ part) {
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int hi = lane >> 5;
+    constexpr int MAXJ = 2 * HJ;
+#pragma unroll
+    for (int q = 0; q < HJ; ++q) {
+        double a = sr[q], b = si[q];
+#pragma unroll
+        for (int mask = 16; mask >= 1; mask >>= 1) {
+            a += __shfl_xor(a, mask, kWave);
+            b += __shfl_xor(b, mask, kWave);
+        }
+        const int l = hi ? l0 + q : q;
+        if ((lane & 31) == 0 && l < nrows && (hi || q < l0)) {
+            red[(wave * MAXJ + l) * 2] = a;
+            red[(wave * MAXJ + l) * 2 + 1] = b;
+        }
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < nrows; l += kLT) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < kLT / kWave; ++w) { a += red[(w * MAXJ + l) * 2]; b += red[(w * MAXJ + l) * 2 + 1]; }
+        part[((int64_t)l * gridDim.x + blockIdx.x) * 2] = a;
+        part[((int64_t)l * gridDim.x + blockIdx.x) * 2 + 1] = b;
+    }
+}
+
+template <int HJ, typename VT>
+__global__ __launch_bounds__(kLT) void large_gs_dots_half(const VT* __restrict__ V, int64_t ldv,
+                                                          const double2* __restrict__ p, int64_t n, int J,
+                                                          double* __restrict__ part) {
+    constexpr int MAXJ = 2 * HJ;
+    __shared__ double red[2 * MAXJ * (kLT / kWave)];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int hi = lane >> 5;
+    const int l0 = (J + 2) / 2;
+    const int lb = hi ? l0 : 0, le = hi ? J + 1 : l0;
+    double sr[HJ], si[HJ];
+#pragma unroll
+    for (int q = 0; q < HJ; ++q) { sr[q] = 0.0; si[q] = 0.0; }
+    const int64_t w0 = ((int64_t)blockIdx.x * kLT + threadIdx.x - lane) / 2;   // this wave's first element
+    for (int64_t i = w0 + (lane & 31); i < n; i += (int64_t)gridDim.x * (kLT / 2)) {
+        const double2 b = p[i];
+#pragma unroll
+        for (int q = 0; q < HJ; ++q)
+            if (lb + q < le) {
+                const double2 a = vload(V + (int64_t)(lb + q) * ldv + i);
+                sr[q] = fma(a.x, b.x, fma(a.y, b.y, sr[q]));
+                si[q] = fma(a.x, b.y, fma(-a.y, b.x, si[q]));
+            }
+    }
+    gs_half_reduce<HJ>(sr, si, l0, J + 1, red, part);
+}
+
+template <int HJ, typename VT>
+__global__ __launch_bounds__(kLT) void large_gs_update_half(const VT* __restrict__ V, int64_t ldv,
+                                                            double2* __restrict__ p, int64_t n, int J,
+                                                            const double2* __restrict__ coef,
+                                                            double* __restrict__ part) {
+    constexpr int MAXJ = 2 * HJ;
+    __shared__ double red[2 * MAXJ * (kLT / kWave)];
+    __shared__ double2 c[MAXJ];
+    for (int l = threadIdx.x; l <= J; l += kLT) c[l] = coef[l];
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int hi = lane >> 5;
+    const int l0 = (J + 2) / 2;
+    const int lb = hi ? l0 : 0, le = hi ? J + 1 : l0;
+    double sr[HJ], si[HJ];
+#pragma unroll
+    for (int q = 0; q < HJ; ++q) { sr[q] = 0.0; si[q] = 0.0; }
+    const int64_t w0 = ((int64_t)blockIdx.x * kLT + threadIdx.x - lane) / 2;
+    // every lane of a wave runs the same iterations (the shuffle needs both halves)
+    for (int64_t i0 = w0; i0 < n; i0 += (int64_t)gridDim.x * (kLT / 2)) {
+        const int64_t i = i0 + (lane & 31);
+        const bool live = i < n;
+        cd part_acc = mkcd(0.0, 0.0);
+        if (live)
+#pragma unroll
+            for (int q = 0; q < HJ; ++q)
+                if (lb + q < le) {
+                    const double2 v = vload(V + (int64_t)(lb + q) * ldv + i);
+                    part_acc = cfma(mkcd(c[lb + q].x, c[lb + q].y), mkcd(v.x, v.y), part_acc);
+                }
+        const double ore = __shfl_xor(part_acc.re, 32, kWave), oim = __shfl_xor(part_acc.im, 32, kWave);
+        if (!live) continue;
+        const double2 pv = p[i];
+        const cd acc = mkcd(pv.x - (part_acc.re + ore), pv.y - (part_acc.im + oim));
+        if (!hi) p[i] = make_double2(acc.re, acc.im);
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int q = 0; q < HJ; ++q)
+            if (lb + q < le) {
+                const double2 v = vload(V + (int64_t)(lb + q) * ldv + i);
+                sr[q] = fma(v.x, acc.re, fma(v.y, acc.im, sr[q]));
+                si[q] = fma(v.x, acc.im, fma(-v.y, acc.re, si[q]));
+            }
+    }
+    gs_half_reduce<HJ>(sr, si, l0, J + 1, red, part);
+}
+
 // F = exp(t H(0:mx,0:mx)) e1 for the augmented Hessenberg matrix (Pade(6),
 // one workgroup).  status[0] = number of squarings or a negative error.
 struct LargePadeSmem {
@@ -742,8 +850,31 @@ static void gs_update_j(const void* V, int64_t ldv, double2* p, int64_t n, int J
     else gs_update_k<MAXJ, false>(V, ldv, p, n, J, coef, part, nblk, s, fp32);
 }
 
+template <int HJ>
+static void gs_dots_h(const void* V, int64_t ldv, const double2* p, int64_t n, int J, double* part, int nblk,
+                      hipStream_t s, bool fp32) {
+    if (fp32)
+        hipLaunchKernelGGL((large_gs_dots_half<HJ, float2>), dim3(nblk), dim3(kLT), 0, s, (const float2*)V, ldv, p, n,
+                           J, part);
+    else
+        hipLaunchKernelGGL((large_gs_dots_half<HJ, double2>), dim3(nblk), dim3(kLT), 0, s, (const double2*)V, ldv, p,
+                           n, J, part);
+}
+
+template <int HJ>
+static void gs_update_h(const void* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef, double* part,
+                        int nblk, hipStream_t s, bool fp32) {
+    if (fp32)
+        hipLaunchKernelGGL((large_gs_update_half<HJ, float2>), dim3(nblk), dim3(kLT), 0, s, (const float2*)V, ldv, p,
+                           n, J, coef, part);
+    else
+        hipLaunchKernelGGL((large_gs_update_half<HJ, double2>), dim3(nblk), dim3(kLT), 0, s, (const double2*)V, ldv,
+                           p, n, J, coef, part);
+}
+
 void launch_large_gs_dots(const void* V, int64_t ldv, const double2* p, int64_t n, int J, double* part, int nblk,
                           hipStream_t s, bool fp32) {
+    if (J >= 24) { gs_dots_h<kLargeMaxM / 2>(V, ldv, p, n, J, part, nblk, s, fp32); return; }
     if (J < 8) gs_dots_j<8>(V, ldv, p, n, J, part, nblk, s, fp32);
     else if (J < 16) gs_dots_j<16>(V, ldv, p, n, J, part, nblk, s, fp32);
     else if (J < 24) gs_dots_j<24>(V, ldv, p, n, J, part, nblk, s, fp32);
@@ -753,6 +884,11 @@ void launch_large_gs_dots(const void* V, int64_t ldv, const double2* p, int64_t 
 
 void launch_large_gs_update(const void* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef,
                             int norm_only, double* part, int nblk, hipStream_t s, bool fp32) {
+    if (!norm_only && J >= 16) {
+        if (J < 32) gs_update_h<16>(V, ldv, p, n, J, coef, part, nblk, s, fp32);
+        else gs_update_h<kLargeMaxM / 2>(V, ldv, p, n, J, coef, part, nblk, s, fp32);
+        return;
+    }
     if (J < 8) gs_update_j<8>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32);
     else if (J < 16) gs_update_j<16>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32);
     else if (J < 24) gs_update_j<24>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32);
