@@ -101,14 +101,22 @@ const char* oops_last_error(const oops_ctx* ctx);
 int oops_ctx_info(const oops_ctx* ctx, int32_t* num_cu, int32_t* lds_bytes, char* name, int32_t name_len);
 
 /* --- single operator ---------------------------------------------------- */
-/* w = H v for host vectors of length prod(spin_dim) (hamvec_cuda3_). */
+/* w = H v for host vectors of length prod(spin_dim) (hamvec_cuda3_).
+ * Local dimensions need not be powers of two (spin-1: 3, its Liouville
+ * space: 9, mixed kinds; hamvec_func3.cpp:307-389 takes any): such an
+ * operator is embedded in the next power of two of its largest local
+ * dimension on the device and the vectors are mapped to and from the
+ * caller's index order on the host.  The same holds for oops_expv and
+ * oops_expv_ex; oops_hamvec_noalloc, whose vectors stay on the device,
+ * returns OOPS_ERR_UNSUPPORTED for them. */
 int oops_hamvec(oops_ctx* ctx, const oops_term_table* H, const double* v, double* w);
 /* y = H x like oops_hamvec, but x and y are DEVICE pointers (prod(spin_dim)
  * complex doubles each, owned by the caller) and the call allocates no device
  * memory: the compiled operator tables are copied from the host into the
  * library's fixed 16 MiB device array on every call (one array per device,
  * set up by oops_ctx_create) and the product runs from there (flat form, or
- * the tiled form for states of 2^19 elements and more).  Tables beyond
+ * the tiled form for states of 2^19 elements and more).  Power-of-two local
+ * dimensions only (OOPS_ERR_UNSUPPORTED otherwise).  Tables beyond
  * 16 MiB return OOPS_ERR_CAPACITY.  The call returns after the product has
  * finished.  Calls that overlap on one device (several threads, several
  * contexts) share the array and race.

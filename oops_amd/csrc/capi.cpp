@@ -48,7 +48,7 @@ int hip_check(oops_ctx* ctx, hipError_t e, const char* what) {
 // matrix per spin and per spin pair, then store each as an ELL row table.
 // max_locals: capacity of the kernel's local-operator table (LDS).
 int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err, int max_locals,
-                     const std::complex<double>* ccoeff) {
+                     const std::complex<double>* ccoeff, int max_row) {
     if (!T || T->nspin <= 0 || T->nterm < 0) { err = "empty term table"; return OOPS_ERR_ARG; }
     const int ns = T->nspin;
     const int64_t d = T->spin_dim[0];
@@ -137,7 +137,7 @@ int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err
             R = std::max(R, cnt);
         }
         if (R == 0) return;
-        R = R <= 4 ? 4 : (R <= 8 ? 8 : 16);  // kernel row widths
+        R = R <= 4 ? 4 : (R <= 8 ? 8 : (R <= 16 ? 16 : (R + 3) / 4 * 4));  // kernel row widths
         DevOpLocal lo;
         lo.nb = (int16_t)nb; lo.s = (int16_t)s; lo.t = (int16_t)u; lo.R = (int16_t)R;
         lo.ent = (int32_t)out.val.size();
@@ -187,7 +187,10 @@ int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err
     for (auto& kv : pair)
         if (out.hermitian && !is_herm(kv.second, d * d)) out.hermitian = false;
     if ((int)out.locs.size() > max_locals) { err = "too many local operators"; return OOPS_ERR_CAPACITY; }
-    if (out.max_row > 16) { err = "local operator rows wider than 16 entries"; return OOPS_ERR_UNSUPPORTED; }
+    if (out.max_row > max_row) {
+        err = "local operator rows wider than " + std::to_string(max_row) + " entries";
+        return OOPS_ERR_UNSUPPORTED;
+    }
     return OOPS_OK;
 }
 

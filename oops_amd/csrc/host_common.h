@@ -122,8 +122,10 @@ bool cce_builder_inputs(const oops_clusters* C, const oops_cce_params* P, std::v
 int large_hahn_item(oops_ctx* ctx, const CompiledOp& A, const CompiledOp& B, const double* u0, const double* y0,
                     double scale, int nt, const double* tau, int m, int klim, double tol, double* L, int32_t* stats);
 
+// max_row: widest merged local-operator row (ELL rows are padded to 4, 8 or
+// 16 entries, wider ones to a multiple of 4; the batched kernels take 16).
 int compile_operator(const oops_term_table* T, CompiledOp& out, std::string& err, int max_locals = 64,
-                     const std::complex<double>* ccoeff = nullptr);
+                     const std::complex<double>* ccoeff = nullptr, int max_row = 16);
 int set_err(oops_ctx* ctx, int code, const std::string& msg);
 // Device address of the no-allocation array (large_host.cpp) on the current
 // device, looked up once per device.

@@ -71,6 +71,12 @@ struct LargeRun {
         LargeAxisPass pass{};
     };
     std::vector<Axis> axis;
+    // local dimensions that are not one power of two (spin-1: 3, its
+    // Liouville space: 9, mixed spins): every spin is embedded in the next
+    // power of two; pos[i] is the device index of caller index i
+    bool padded = false;
+    int64_t n_logical = 0;
+    std::vector<int64_t> pos;
     int rc = 0;
 };
 
@@ -420,9 +426,107 @@ static int setup_axis(oops_ctx* ctx, const oops_term_table* T, const cplx* ccoef
     return OOPS_OK;
 }
 
+// A term table whose spins do not share one power-of-two local dimension
+// (spin-1 spins: 3, or 9 in Liouville space; mixed spin kinds) embedded in the
+// local dimension P = the next power of two of the largest: every local matrix
+// becomes the top-left block of a P x P zero matrix, and the caller's index
+// (mixed radix dims[s], spin 0 slowest) maps to sum_s i_s P^(k-1-s).  The
+// padding states are never reached: the operator has zero rows and columns
+// there and the start vector is zero there.
+struct PaddedTable {
+    oops_term_table T{};
+    std::vector<double> coeff, mats;
+    std::vector<int64_t> nbody, pos_offset, pos, dims, mat_offset, spin_dim;
+    std::vector<int64_t> map;   // caller index -> padded index
+    int64_t n_logical = 0;
+};
+
+static bool needs_padding(const oops_term_table* T) {
+    if (!T || T->nspin <= 0) return false;
+    const int64_t d0 = T->spin_dim[0];
+    for (int s = 0; s < T->nspin; ++s)
+        if (T->spin_dim[s] != d0) return true;
+    return d0 < 2 || (d0 & (d0 - 1)) != 0;
+}
+
+static int pad_table(const oops_term_table* T, PaddedTable& P, std::string& err) {
+    const int k = T->nspin;
+    int64_t dmax = 1;
+    for (int s = 0; s < k; ++s) {
+        if (T->spin_dim[s] < 1) { err = "bad local dimension"; return OOPS_ERR_ARG; }
+        dmax = std::max(dmax, (int64_t)T->spin_dim[s]);
+    }
+    int64_t Pd = 2;
+    while (Pd < dmax) Pd *= 2;
+    int64_t nlog = 1, npad = 1;
+    for (int s = 0; s < k; ++s) {
+        nlog *= T->spin_dim[s];
+        npad *= Pd;
+        if (npad > (int64_t(1) << 30)) { err = "state too large after padding"; return OOPS_ERR_UNSUPPORTED; }
+    }
+    P.n_logical = nlog;
+    const int64_t nb_total = T->pos_offset[T->nterm];
+    P.coeff.assign(T->coeff, T->coeff + T->nterm);
+    P.nbody.assign(T->nbody, T->nbody + T->nterm);
+    P.pos_offset.assign(T->pos_offset, T->pos_offset + T->nterm + 1);
+    P.pos.assign(T->pos, T->pos + nb_total);
+    P.dims.assign((size_t)nb_total, Pd);
+    P.mat_offset.assign((size_t)nb_total + 1, 0);
+    P.mats.assign((size_t)2 * nb_total * Pd * Pd, 0.0);
+    for (int64_t b = 0; b < nb_total; ++b) {
+        const int64_t d = T->dims[b];
+        const int s = (int)T->pos[b];
+        if (s < 0 || s >= k || d != T->spin_dim[s]) { err = "body dimension differs from its spin's"; return OOPS_ERR_ARG; }
+        P.mat_offset[b] = b * Pd * Pd;
+        P.mat_offset[b + 1] = (b + 1) * Pd * Pd;
+        const double* src = T->mats + 2 * T->mat_offset[b];
+        double* dst = P.mats.data() + 2 * P.mat_offset[b];
+        for (int64_t c = 0; c < d; ++c)
+            for (int64_t r = 0; r < d; ++r) {
+                dst[2 * (r + Pd * c)] = src[2 * (r + d * c)];
+                dst[2 * (r + Pd * c) + 1] = src[2 * (r + d * c) + 1];
+            }
+    }
+    P.spin_dim.assign((size_t)k, Pd);
+    P.T.nspin = k;
+    P.T.nterm = T->nterm;
+    P.T.coeff = P.coeff.data();
+    P.T.nbody = P.nbody.data();
+    P.T.pos_offset = P.pos_offset.data();
+    P.T.pos = P.pos.data();
+    P.T.dims = P.dims.data();
+    P.T.mat_offset = P.mat_offset.data();
+    P.T.mats = P.mats.data();
+    P.T.spin_dim = P.spin_dim.data();
+    P.map.resize((size_t)nlog);
+    for (int64_t i = 0; i < nlog; ++i) {
+        int64_t q = i, g = 0, stride = 1;
+        for (int s = k - 1; s >= 0; --s) {
+            const int64_t d = T->spin_dim[s];
+            g += (q % d) * stride;
+            q /= d;
+            stride *= Pd;
+        }
+        P.map[(size_t)i] = g;
+    }
+    return OOPS_OK;
+}
+
 int setup(oops_ctx* ctx, const oops_term_table* T, int m, LargeRun& R, const std::complex<double>* ccoeff = nullptr) {
     std::string err;
-    int rc = compile_operator(T, R.op, err, kLargeMaxLocals, ccoeff);
+    if (needs_padding(T)) {
+        PaddedTable P;
+        int rc = pad_table(T, P, err);
+        if (rc) return set_err(ctx, rc, err);
+        R.padded = true;
+        R.n_logical = P.n_logical;
+        R.pos = std::move(P.map);
+        rc = compile_operator(&P.T, R.op, err, kLargeMaxLocals, ccoeff, kLargeMaxRow);
+        if (rc) return set_err(ctx, rc, err);
+        if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
+        return setup_compiled(ctx, m, R);
+    }
+    int rc = compile_operator(T, R.op, err, kLargeMaxLocals, ccoeff, kLargeMaxRow);
     if (rc) return set_err(ctx, rc, err);
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
     rc = setup_axis(ctx, T, ccoeff, R);
@@ -518,11 +622,21 @@ int dev_norm(LargeRun& R, const double2* a, double& out) {
 // Lanczos + power iteration norm estimate on +H (reference :210-265)
 int norm_estimate(LargeRun& R, int klim, double tol, double& anorm) {
     const int64_t n = R.n;
+    const int64_t nl = R.padded ? R.n_logical : n;   // the draws follow the caller's indices
     int k = klim;
-    if (k > n) k = (int)n;
-    std::vector<double> rnd(2 * n + k);
+    if (k > nl) k = (int)nl;
+    std::vector<double> rnd(2 * nl + k);
     int32_t seed[4] = {0, 0, 0, 5};
     dlaran_fill(seed, (int64_t)rnd.size(), rnd.data());
+    if (R.padded) {
+        std::vector<double> full(2 * n + k, 0.0);
+        for (int64_t i = 0; i < nl; ++i) {
+            full[2 * R.pos[(size_t)i]] = rnd[2 * i];
+            full[2 * R.pos[(size_t)i] + 1] = rnd[2 * i + 1];
+        }
+        for (int j = 0; j < k; ++j) full[2 * n + j] = rnd[2 * nl + j];
+        rnd.swap(full);
+    }
     double2* q = (double2*)R.p0.p;
     double2* qm = (double2*)R.p1.p;
     double2* wv = (double2*)R.w.p;
@@ -969,50 +1083,64 @@ int oops_tile_plan(const oops_term_table* T, int32_t* npasses, int32_t mode, con
     }
     CompiledOp op;
     std::string err;
-    int rc = compile_operator(T, op, err, kLargeMaxLocals);
+    // local dimensions that are not one power of two are padded to one: the
+    // product runs in the padded layout and is read back in the caller's
+    PaddedTable P;
+    const bool pad = needs_padding(T);
+    int rc = pad ? pad_table(T, P, err) : OOPS_OK;
+    rc = rc ? rc : compile_operator(pad ? &P.T : T, op, err, kLargeMaxLocals, nullptr, kLargeMaxRow);
     if (rc) return rc;
     std::vector<TilePlan> plan;
     const bool tiled = plan_tiles(op, plan);
     *npasses = tiled ? (int32_t)plan.size() : 0;
     if (mode == 0 || !v || !w) return OOPS_OK;
-    typedef std::complex<double> cplx;
+    const int64_t n = op.n;
+    std::vector<cplx> xp, yp;
     const cplx* x = reinterpret_cast<const cplx*>(v);
     cplx* y = reinterpret_cast<cplx*>(w);
-    const int64_t n = op.n;
+    if (pad) {
+        xp.assign((size_t)n, cplx(0.0));
+        yp.assign((size_t)n, cplx(0.0));
+        for (int64_t i = 0; i < P.n_logical; ++i) xp[(size_t)P.map[(size_t)i]] = x[i];
+        x = xp.data();
+        y = yp.data();
+    }
     if (mode == 2 || !tiled) {
         for (int64_t i = 0; i < n; ++i) y[i] = host_elem(op, i, x, [](int64_t g) { return g; });
-        return OOPS_OK;
-    }
-    // the kernel's mapping: tile b, element e -> base(b) | scatter(e)
-    const int ns = op.nspin, ld = op.ld_bits;
-    const int64_t lmask = (1LL << ld) - 1;
-    std::vector<uint8_t> seen(n);
-    for (size_t p = 0; p < plan.size(); ++p) {
-        const std::vector<int>& g = plan[p].G;
-        std::vector<int> in(ns, 0), blk;
-        for (int s2 : g) in[s2] = 1;
-        for (int s2 = 0; s2 < ns; ++s2)
-            if (!in[s2]) blk.push_back(s2);
-        const int ng = (int)g.size(), nb = (int)blk.size();
-        const int64_t tsize = 1LL << (ld * ng), ntiles = n >> (ld * ng);
-        std::fill(seen.begin(), seen.end(), 0);
-        for (int64_t b = 0; b < ntiles; ++b) {
-            int64_t base = 0, bb = b;
-            for (int q = nb - 1; q >= 0; --q) { base |= (bb & lmask) << (ld * (ns - 1 - blk[q])); bb >>= ld; }
-            auto scatter = [&](int64_t e) {
-                int64_t gi = base;
-                for (int q = ng - 1; q >= 0; --q) { gi |= (e & lmask) << (ld * (ns - 1 - g[q])); e >>= ld; }
-                return gi;
-            };
-            for (int64_t e = 0; e < tsize; ++e) {
-                const int64_t gi = scatter(e);
-                if (gi < 0 || gi >= n || seen[gi]) return OOPS_ERR_NUMERIC;   // not a bijection
-                seen[gi] = 1;
-                const cplx a = host_elem(plan[p].op, e, x, scatter, true);
-                y[gi] = p == 0 ? a : y[gi] + a;
+    } else {
+        // the kernel's mapping: tile b, element e -> base(b) | scatter(e)
+        const int ns = op.nspin, ld = op.ld_bits;
+        const int64_t lmask = (1LL << ld) - 1;
+        std::vector<uint8_t> seen(n);
+        for (size_t p = 0; p < plan.size(); ++p) {
+            const std::vector<int>& g = plan[p].G;
+            std::vector<int> in(ns, 0), blk;
+            for (int s2 : g) in[s2] = 1;
+            for (int s2 = 0; s2 < ns; ++s2)
+                if (!in[s2]) blk.push_back(s2);
+            const int ng = (int)g.size(), nb = (int)blk.size();
+            const int64_t tsize = 1LL << (ld * ng), ntiles = n >> (ld * ng);
+            std::fill(seen.begin(), seen.end(), 0);
+            for (int64_t b = 0; b < ntiles; ++b) {
+                int64_t base = 0, bb = b;
+                for (int q = nb - 1; q >= 0; --q) { base |= (bb & lmask) << (ld * (ns - 1 - blk[q])); bb >>= ld; }
+                auto scatter = [&](int64_t e) {
+                    int64_t gi = base;
+                    for (int q = ng - 1; q >= 0; --q) { gi |= (e & lmask) << (ld * (ns - 1 - g[q])); e >>= ld; }
+                    return gi;
+                };
+                for (int64_t e = 0; e < tsize; ++e) {
+                    const int64_t gi = scatter(e);
+                    if (gi < 0 || gi >= n || seen[gi]) return OOPS_ERR_NUMERIC;   // not a bijection
+                    seen[gi] = 1;
+                    const cplx a = host_elem(plan[p].op, e, x, scatter, true);
+                    y[gi] = p == 0 ? a : y[gi] + a;
+                }
             }
         }
     }
+    if (pad)
+        for (int64_t i = 0; i < P.n_logical; ++i) reinterpret_cast<cplx*>(w)[i] = yp[(size_t)P.map[(size_t)i]];
     return OOPS_OK;
 }
 
@@ -1021,7 +1149,7 @@ int oops_hamvec_noalloc(oops_ctx* ctx, const oops_term_table* H, const double* x
     if (!ctx->noalloc) return set_err(ctx, OOPS_ERR_HIP, "no-allocation array not available on this device");
     CompiledOp op;
     std::string err;
-    int rc = compile_operator(H, op, err, kLargeMaxLocals);
+    int rc = compile_operator(H, op, err, kLargeMaxLocals, nullptr, kLargeMaxRow);
     if (rc) return set_err(ctx, rc, err);
     // host image of the array: the tables oops_hamvec uploads, 256-byte aligned
     std::vector<unsigned char> img;
@@ -1099,11 +1227,27 @@ int oops_hamvec(oops_ctx* ctx, const oops_term_table* H, const double* v, double
     int rc = setup(ctx, H, 1, R);
     if (rc) return rc;
     R.s = nullptr;
-    rc = hip_check(ctx, hipMemcpy(R.p0.p, v, sizeof(double2) * R.n, hipMemcpyHostToDevice), "H2D v");
+    std::vector<double> vp, wp;
+    if (R.padded) {   // caller's layout <-> the padded device layout
+        vp.assign(2 * (size_t)R.n, 0.0);
+        wp.assign(2 * (size_t)R.n, 0.0);
+        for (int64_t i = 0; i < R.n_logical; ++i) {
+            vp[2 * R.pos[(size_t)i]] = v[2 * i];
+            vp[2 * R.pos[(size_t)i] + 1] = v[2 * i + 1];
+        }
+    }
+    rc = hip_check(ctx, hipMemcpy(R.p0.p, R.padded ? vp.data() : v, sizeof(double2) * R.n, hipMemcpyHostToDevice),
+                   "H2D v");
     if (rc) return rc;
     matvec(R, (const double2*)R.p0.p, nullptr, (double2*)R.p1.p, nullptr, 0);
     rc = hip_check(ctx, hipGetLastError(), "matvec launch");
-    rc = rc ? rc : hip_check(ctx, hipMemcpy(w, R.p1.p, sizeof(double2) * R.n, hipMemcpyDeviceToHost), "D2H w");
+    rc = rc ? rc : hip_check(ctx, hipMemcpy(R.padded ? wp.data() : w, R.p1.p, sizeof(double2) * R.n,
+                                            hipMemcpyDeviceToHost), "D2H w");
+    if (!rc && R.padded)
+        for (int64_t i = 0; i < R.n_logical; ++i) {
+            w[2 * i] = wp[2 * R.pos[(size_t)i]];
+            w[2 * i + 1] = wp[2 * R.pos[(size_t)i] + 1];
+        }
     return rc;
 }
 
@@ -1127,11 +1271,28 @@ int oops_expv_ex(oops_ctx* ctx, const oops_term_table* H, const double* v, int32
     R.fp32 = (precision & 1) != 0;
     int rc = setup(ctx, H, m, R);
     if (rc) return rc;
-    if (m >= R.n) return set_err(ctx, OOPS_ERR_ARG, "the Krylov dimension must be smaller than the state dimension");
+    if (m >= (R.padded ? R.n_logical : R.n))
+        return set_err(ctx, OOPS_ERR_ARG, "the Krylov dimension must be smaller than the state dimension");
+    std::vector<double> vp, outp;
+    if (R.padded) {   // caller's layout <-> the padded device layout
+        vp.assign(2 * (size_t)R.n, 0.0);
+        for (int64_t i = 0; i < R.n_logical; ++i) {
+            vp[2 * R.pos[(size_t)i]] = v[2 * i];
+            vp[2 * R.pos[(size_t)i] + 1] = v[2 * i + 1];
+        }
+        if (out_mode == 0) outp.assign(2 * (size_t)R.n * nt, 0.0);
+    }
     hipStreamCreateWithFlags(&R.s, hipStreamNonBlocking);
     Counters C;
-    rc = run_expv(R, v, nt, ta, klim, tol, out_mode, out, C, nullptr, (precision & 2) != 0);
+    rc = run_expv(R, R.padded ? vp.data() : v, nt, ta, klim, tol, out_mode,
+                  (R.padded && out_mode == 0) ? outp.data() : out, C, nullptr, (precision & 2) != 0);
     hipStreamDestroy(R.s);
+    if (!rc && R.padded && out_mode == 0 && out)
+        for (int q = 0; q < nt; ++q)
+            for (int64_t i = 0; i < R.n_logical; ++i) {
+                out[2 * ((size_t)q * R.n_logical + i)] = outp[2 * ((size_t)q * R.n + R.pos[(size_t)i])];
+                out[2 * ((size_t)q * R.n_logical + i) + 1] = outp[2 * ((size_t)q * R.n + R.pos[(size_t)i]) + 1];
+            }
     if (stats) {
         stats[0] = (double)C.nmult;
         stats[1] = (double)C.nexph;
@@ -1300,6 +1461,28 @@ static int hamvec_complex(oops_ctx* ctx, const oops_term_table& T, const std::co
     int rc = setup(ctx, &T, 1, R, cc);
     if (rc) return rc;
     R.s = st;
+    if (R.padded) {
+        // the caller's device vectors are in its own layout: through the host
+        // into the padded layout and back
+        std::vector<double> hv(2 * (size_t)R.n_logical), vp(2 * (size_t)R.n, 0.0), wp(2 * (size_t)R.n);
+        rc = hip_check(ctx, hipMemcpy(hv.data(), dv, sizeof(double2) * R.n_logical, hipMemcpyDeviceToHost), "D2H v");
+        if (rc) return rc;
+        for (int64_t i = 0; i < R.n_logical; ++i) {
+            vp[2 * R.pos[(size_t)i]] = hv[2 * i];
+            vp[2 * R.pos[(size_t)i] + 1] = hv[2 * i + 1];
+        }
+        rc = hip_check(ctx, hipMemcpy(R.p0.p, vp.data(), sizeof(double2) * R.n, hipMemcpyHostToDevice), "H2D v");
+        if (rc) return rc;
+        matvec(R, (const double2*)R.p0.p, nullptr, (double2*)R.p1.p, nullptr, 0);
+        rc = hip_check(ctx, hipGetLastError(), "matvec launch");
+        rc = rc ? rc : hip_check(ctx, hipMemcpy(wp.data(), R.p1.p, sizeof(double2) * R.n, hipMemcpyDeviceToHost), "D2H w");
+        if (rc) return rc;
+        for (int64_t i = 0; i < R.n_logical; ++i) {
+            hv[2 * i] = wp[2 * R.pos[(size_t)i]];
+            hv[2 * i + 1] = wp[2 * R.pos[(size_t)i] + 1];
+        }
+        return hip_check(ctx, hipMemcpy(dw, hv.data(), sizeof(double2) * R.n_logical, hipMemcpyHostToDevice), "H2D w");
+    }
     matvec(R, dv, nullptr, dw, nullptr, 0);
     rc = hip_check(ctx, hipGetLastError(), "matvec launch");
     return rc ? rc : hip_check(ctx, hipStreamSynchronize(st), "sync matvec");

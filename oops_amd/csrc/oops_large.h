@@ -9,6 +9,7 @@ namespace oops {
 
 constexpr int kLargeMaxM = 40;      // Krylov dimension limit of the large path (LDS of the Pade kernel)
 constexpr int kLargeMaxLocals = 1024;  // merged local operators of one operator (LDS of the product kernel)
+constexpr int kLargeMaxRow = 256;      // widest merged local-operator row (padded spin-1 pairs: 81 entries)
 
 struct LargeOpArgs {
     const DevOpHeader* hdr;  // device copy of the header

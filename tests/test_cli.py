@@ -54,8 +54,9 @@ def _project(tmp_path, monkeypatch):
     monkeypatch.setenv("CCE_PROJ_PATH", str(tmp_path))
     E.main(["-I"])
     X = O.bath_diamond(3.567, 16.0, 0.011, seed=2)
-    O.write_xyz(str(tmp_path / "dat" / "input" / "bath.xyz"), X, ["13C"] * len(X))
-    return X
+    path = str(tmp_path / "dat" / "input" / "bath.xyz")
+    O.write_xyz(path, X, ["13C"] * len(X))
+    return O.read_xyz(path)[0]    # the file holds the coordinates to its text precision
 
 
 @pytest.mark.gpu

@@ -109,6 +109,38 @@ def test_expv_matches_oracle_on_cluster_liouvillian(ctx, k, m):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("dims", [[3, 2, 3], [2, 3, 4], [9, 9, 9], [3] * 7])
+def test_padded_local_dimensions_on_device(ctx, dims):
+    """Spin-1, its Liouville space and mixed kinds (local dimensions 3, 9 and
+    3/2/4 embedded in the next power of two on the device; the reference's
+    hamvec_func3.cpp:307-389 takes any local dimension): H v against the
+    explicit Kronecker matrix, exp(-i H t) v against scipy expm and against
+    the oracle's Krylov driver on the unpadded table, with the same step
+    schedule (the norm estimate's random draws follow the caller's indices)."""
+    import scipy.linalg as sla
+    from mixed_dims import random_table
+    T, H = random_table(dims, seed=len(dims))
+    oT = P.TermTable(len(dims), T.coeff, T.nbody, T.pos_offset, T.pos, T.dims, T.mat_offset, T.mats,
+                     T.spin_dim, T.n)
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(T.n) + 1j * rng.standard_normal(T.n)
+    print(f"padded dims {dims} n={T.n}", flush=True)
+    got = O.hamvec(ctx, T, v)
+    assert np.abs(got - H @ v).max() <= 1e-12 * np.abs(H @ v).max()
+    v /= np.linalg.norm(v)
+    ta = np.array([0.0, 2.0, 7.0, 20.0]) / np.abs(H).sum(0).max()
+    m = min(30, T.n - 1)
+    W, st = O.expv(ctx, T, v, ta, m=m, tol=1e-12)
+    want = np.stack([sla.expm(-1j * H * t) @ v for t in ta], axis=1)
+    assert np.abs(W - want).max() < 1e-10
+    Wr, sr, rc = P.expv(oT, v, ta, m=m, tol=1e-12)
+    assert rc == 0
+    assert np.abs(W - Wr).max() < 1e-10
+    assert st["nstep"] == sr.nstep and st["nreject"] == sr.nreject
+    assert abs(st["anorm"] - sr.anorm) <= 1e-4 * sr.anorm
+
+
+@pytest.mark.gpu
 def test_expv_rejects_bad_arguments(ctx):
     g = np.load(os.path.join(GOLD, "expv_cases.npz"), allow_pickle=False)
     T, _ = _case(g, 0)  # n = 4
