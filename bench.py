@@ -200,12 +200,14 @@ def measured_traffic(kernel):
 
 
 def tuned_mixed():
+    """The autotuner's result: (m, tol, product form, launch shapes, source)."""
     path = os.path.join(ROOT, "oops_amd", "tuning.json")
     if os.path.exists(path):
         with open(path) as f:
             t = json.load(f)
-        return int(t["mixed"]["m"]), float(t["mixed"]["tol"]), "oops_amd/tuning.json"
-    return 30, 1e-4, "default"
+        return (int(t["mixed"]["m"]), float(t["mixed"]["tol"]), t["mixed"].get("precision", "mixed"),
+                t.get("launch", {}), "oops_amd/tuning.json")
+    return 30, 1e-4, "mixed", {}, "default"
 
 
 def roofline(counts, st, timing, m, kernel):
@@ -286,16 +288,18 @@ def main():
         sys.exit(2)
 
     import oops_amd as O
-    mm, mtol, msrc = tuned_mixed()
+    mm, mtol, mprec, launch, msrc = tuned_mixed()
     mm = args.mixed_m or mm
     mtol = args.mixed_tol or mtol
     ctx = O.Context(0)
+    for kname, wg in launch.items():
+        ctx.set_launch(kname, int(wg))
     X, C, par, tlist = build_workload(args)
     counts = C.counts()
     nclu = sum(counts)
     tau = tlist / 2.0
     t0 = time.perf_counter()
-    batch = C.hahn_batch(ctx, par, m=mm, tol=mtol, precision="mixed")
+    batch = C.hahn_batch(ctx, par, m=mm, tol=mtol, precision=mprec)
     setup_mixed_s = time.perf_counter() - t0
     sel = stratified(counts, args.fp64_frac)
     batch64 = None if args.no_fp64 else C.hahn_batch(ctx, par, m=args.m, tol=args.tol, precision="fp64",
@@ -305,7 +309,7 @@ def main():
           f"setup {setup_s:.2f}s", file=sys.stderr, flush=True)
     # prime code objects on a small batch (not a step)
     Cs = O.Clusters(X[:40], args.cutoff, 2)
-    Cs.hahn_batch(ctx, par, m=mm, tol=mtol, precision="mixed").run(tau[:3])
+    Cs.hahn_batch(ctx, par, m=mm, tol=mtol, precision=mprec).run(tau[:3])
     if batch64 is not None:
         Cs.hahn_batch(ctx, par, m=args.m, tol=args.tol).run(tau[:3])
     for w in range(args.warmup):
@@ -355,7 +359,7 @@ def main():
         bl.run(tau)
         Ll, stl = bl.results()
         ell = time.perf_counter() - t2
-        mixed_sample = C.hahn_batch(ctx, par, m=mm, tol=mtol, precision="mixed", select=sel)
+        mixed_sample = C.hahn_batch(ctx, par, m=mm, tol=mtol, precision=mprec, select=sel)
         t3 = time.perf_counter()
         mixed_sample.run(tau)
         Lms, _ = mixed_sample.results()
@@ -387,7 +391,7 @@ def main():
                    "clusters": nclu, "clusters_per_order": counts, "spins": int(len(X)),
                    "setup_s": setup_mixed_s,
                    "setup": "operator tables of every cluster built and uploaded (device term builder), untimed",
-                   "mixed": {"m": mm, "tol": mtol, "source": msrc, "buckets": timing,
+                   "mixed": {"m": mm, "tol": mtol, "precision": mprec, "launch": launch, "source": msrc, "buckets": timing,
                              "failed_clusters": int(((st[:, 8] & 0xFF) != 0).sum())},
                    "fp64": fp64,
                    "mixed_over_fp64": fp64["mixed_over_fp64"] if fp64 else None,

@@ -99,6 +99,13 @@ int oops_ctx_destroy(oops_ctx* ctx);
 const char* oops_last_error(const oops_ctx* ctx);
 /* Device properties: number of compute units, LDS bytes per workgroup. */
 int oops_ctx_info(const oops_ctx* ctx, int32_t* num_cu, int32_t* lds_bytes, char* name, int32_t name_len);
+/* Launch shape of the persistent batch kernels: wg_per_cu workgroups per
+ * compute unit for kernel kind `kind` (the kinds oops_hahn_batch_timing
+ * reports: 0-4, 6-9) in batches created on ctx after the call; 0 restores
+ * the default (the number one CU holds: 4 for kind 0, 2 for 1-3, 6 and 9,
+ * 1 for 4, 7 and 8).  Work units pull from a device queue, so results do
+ * not depend on it.  No reference counterpart (the autotuner's knob). */
+int oops_ctx_set_launch(oops_ctx* ctx, int32_t kind, int32_t wg_per_cu);
 
 /* --- single operator ---------------------------------------------------- */
 /* w = H v for host vectors of length prod(spin_dim) (hamvec_cuda3_).

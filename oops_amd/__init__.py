@@ -50,7 +50,7 @@ class CCEParamsC(ctypes.Structure):
 _lib = None
 
 EXPORTS = [
-    "oops_ctx_create", "oops_ctx_destroy", "oops_last_error", "oops_ctx_info",
+    "oops_ctx_create", "oops_ctx_destroy", "oops_last_error", "oops_ctx_info", "oops_ctx_set_launch",
     "oops_hahn_batch_create", "oops_hahn_batch_run", "oops_hahn_batch_results",
     "oops_hahn_batch_timing", "oops_hahn_batch_destroy", "oops_dlaran_fill",
     "oops_bath_diamond", "oops_nv_states", "oops_cce_cluster_terms", "oops_terms_view",
@@ -183,6 +183,14 @@ class Context:
         name = ctypes.create_string_buffer(64)
         _check(lib().oops_ctx_info(self.h, ctypes.byref(cu), ctypes.byref(lds), name, 64), self.h, "ctx_info")
         return {"num_cu": cu.value, "lds_bytes": lds.value, "arch": name.value.decode()}
+
+    def set_launch(self, kernel, wg_per_cu):
+        """Workgroups per CU of the persistent batch kernel `kernel` (a name
+        of KERNEL_NAMES or its kind) in batches created afterwards; 0 restores
+        the default."""
+        kind = kernel if isinstance(kernel, int) else {v: k for k, v in KERNEL_NAMES.items()}[kernel]
+        _check(lib().oops_ctx_set_launch(self.h, ctypes.c_int32(kind), ctypes.c_int32(wg_per_cu)), self.h,
+               "oops_ctx_set_launch")
 
     def close(self):
         if self.h:

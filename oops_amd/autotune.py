@@ -1,12 +1,18 @@
 """Autotuner of the mixed-precision batched Hahn-echo path.
 
-Picks the Krylov dimension m and the Expokit tolerance of the mixed kernel
-(complex FP32 couplings and basis, FP64 orthogonalisation and Hessenberg
-exponential) that give the highest cluster throughput while every L_c(t) of
-a tuning sample stays within budget/safety of the FP64 path (m = 30,
-tol = 1e-12, the reference's settings).  The tuning sample is a CCE-4
-enumeration of a bath drawn from a different seed than the benchmark's, so
-the benchmark checks the choice on data it was not tuned on.
+Stage 1 picks the Krylov dimension m, the Expokit tolerance and the product
+form of the n >= 64 Lanczos buckets (flip-mask slots: "mixed", merged ELL
+rows: "mixed-ell") of the mixed kernels (complex FP32 couplings and basis,
+FP64 orthogonalisation and Hessenberg exponential) that give the highest
+cluster throughput while every L_c(t) of a tuning sample stays within
+budget/safety of the FP64 path (m = 30, tol = 1e-12, the reference's
+settings).  Stage 2 sets the launch shape of every persistent kernel the
+chosen batch runs (workgroups per CU, 1-4), one kernel kind at a time with
+the others at their best so far, on the batch's device time; work units
+pull from device queues, so every shape must give bit-identical L(t) (the
+tuner checks it).  The tuning sample is a CCE-4 enumeration of a bath drawn
+from a different seed than the benchmark's, so the benchmark checks the
+choice on data it was not tuned on.
 
     python -m oops_amd.autotune [--per-order 512] [--out oops_amd/tuning.json]
 """
@@ -36,6 +42,58 @@ def kernel_seconds(batch):
     return batch.elapsed() * 1e-3
 
 
+def apply(ctx, tuning):
+    """Set the launch shapes of a tuning result on a context (batches created
+    afterwards use them); returns the mixed path's (m, tol, precision)."""
+    for name, wg in tuning.get("launch", {}).items():
+        ctx.set_launch(name, int(wg))
+    t = tuning["mixed"]
+    return int(t["m"]), float(t["tol"]), t.get("precision", "mixed")
+
+
+def tune_launch(ctx, C, par, sel, tau, m, tol, prec, L_ref, reps=2, shapes=(1, 2, 3, 4)):
+    """Coordinate search over workgroups per CU of each kernel kind the
+    batch runs, on the best of `reps` device times; returns ({kind: wg},
+    rows)."""
+    import oops_amd as O
+
+    def timed():
+        b = C.hahn_batch(ctx, par, m=m, tol=tol, precision=prec, select=sel)
+        best = None
+        for _ in range(reps):
+            b.run(tau)
+            t = kernel_seconds(b)
+            best = t if best is None else min(best, t)
+        L, _ = b.results()
+        kinds = sorted({x["kernel"] for x in b.timing() if x["kernel"] not in (None, "single_state")})
+        b.close()
+        return best, L, kinds
+
+    t0, L0, kinds = timed()
+    if not np.array_equal(L0, L_ref):
+        raise RuntimeError("default launch shape changed L(t)")
+    chosen, rows = {}, [{"launch": {}, "kernel_s": t0}]
+    print(f"# launch: default {t0:.3f} s, kernels {kinds}", flush=True)
+    for kind in kinds:
+        best_wg, best_t = 0, t0
+        for wg in shapes:
+            ctx.set_launch(kind, wg)
+            t, L, _ = timed()
+            if not np.array_equal(L, L_ref):
+                raise RuntimeError(f"{kind} at {wg} workgroups per CU changed L(t)")
+            rows.append({"launch": {**chosen, kind: wg}, "kernel_s": t})
+            print(f"# launch {kind} x{wg}/CU: {t:.3f} s", flush=True)
+            if t < best_t * 0.98:     # keep the default unless a shape is clearly faster
+                best_wg, best_t = wg, t
+        ctx.set_launch(kind, best_wg)
+        if best_wg:
+            chosen[kind] = best_wg
+            t0 = best_t
+    for kind in kinds:
+        ctx.set_launch(kind, 0)
+    return chosen, rows
+
+
 def main(argv=None):
     import oops_amd as O
     ap = argparse.ArgumentParser()
@@ -51,6 +109,8 @@ def main(argv=None):
     ap.add_argument("--safety", type=float, default=2.5)
     ap.add_argument("--ms", default="20,30")
     ap.add_argument("--tols", default="1e-2,3e-3,1e-3,3e-4,1e-4,1e-5")
+    ap.add_argument("--forms", default="mixed,mixed-ell", help="product forms of the Lanczos buckets")
+    ap.add_argument("--no-launch", dest="launch", action="store_false", help="skip the launch-shape stage")
     ap.add_argument("--out", default=os.path.join(HERE, "tuning.json"))
     a = ap.parse_args(argv)
 
@@ -71,31 +131,39 @@ def main(argv=None):
     print(f"# fp64: {t64:.3f} s kernel time", flush=True)
     rows = []
     limit = a.budget / a.safety
-    for m in [int(x) for x in a.ms.split(",")]:
-        for tol in [float(x) for x in a.tols.split(",")]:
-            b = C.hahn_batch(ctx, par, m=m, tol=tol, precision="mixed", select=sel)
-            b.run(tau)
-            L, st = b.results()
-            t = kernel_seconds(b)
-            err = float(np.abs(L - L64).max())
-            ok = bool(err <= limit and not ((st[:, 8] & 0xFF) != 0).any())
-            rows.append({"m": m, "tol": tol, "kernel_s": t, "max_abs_dL": err, "speedup_vs_fp64": t64 / t,
-                         "accepted": ok})
-            print(f"# m={m} tol={tol:g}: {t:.3f} s, max|dL| {err:.2e}, x{t64 / t:.2f} {'ok' if ok else 'rejected'}",
-                  flush=True)
-            b.close()
+    Lbest = {}
+    for form in a.forms.split(","):
+        for m in [int(x) for x in a.ms.split(",")]:
+            for tol in [float(x) for x in a.tols.split(",")]:
+                b = C.hahn_batch(ctx, par, m=m, tol=tol, precision=form, select=sel)
+                b.run(tau)
+                L, st = b.results()
+                t = kernel_seconds(b)
+                err = float(np.abs(L - L64).max())
+                ok = bool(err <= limit and not ((st[:, 8] & 0xFF) != 0).any())
+                rows.append({"precision": form, "m": m, "tol": tol, "kernel_s": t, "max_abs_dL": err,
+                             "speedup_vs_fp64": t64 / t, "accepted": ok})
+                Lbest[(form, m, tol)] = L
+                print(f"# {form} m={m} tol={tol:g}: {t:.3f} s, max|dL| {err:.2e}, x{t64 / t:.2f} "
+                      f"{'ok' if ok else 'rejected'}", flush=True)
+                b.close()
     good = [r for r in rows if r["accepted"]]
     if not good:
         raise RuntimeError("no candidate met the error budget")
     best = min(good, key=lambda r: r["kernel_s"])
-    out = {"mixed": {"m": best["m"], "tol": best["tol"]},
+    launch, lrows = {}, []
+    if a.launch:
+        launch, lrows = tune_launch(ctx, C, par, sel, tau, best["m"], best["tol"], best["precision"],
+                                    Lbest[(best["precision"], best["m"], best["tol"])])
+    out = {"mixed": {"m": best["m"], "tol": best["tol"], "precision": best["precision"]},
+           "launch": launch, "launch_candidates": lrows,
            "criterion": f"fastest with max|L_mixed - L_fp64| <= {a.budget:g}/{a.safety:g} on every cluster and time",
            "sample": {"radius": a.radius, "seed": a.seed, "cutoff": a.cutoff, "order": a.order,
                       "clusters": int(len(sel)), "nt": a.nt, "t_end": a.t_end},
            "fp64_kernel_s": t64, "device": ctx.info(), "candidates": rows}
     with open(a.out, "w") as f:
         json.dump(out, f, indent=1)
-    print(json.dumps(out["mixed"]), flush=True)
+    print(json.dumps({**out["mixed"], "launch": launch}), flush=True)
     return out
 
 

@@ -348,6 +348,14 @@ int oops_ctx_destroy(oops_ctx* ctx) {
 
 const char* oops_last_error(const oops_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
 
+int oops_ctx_set_launch(oops_ctx* ctx, int32_t kind, int32_t wg_per_cu) {
+    if (!ctx) return OOPS_ERR_ARG;
+    if (kind < 0 || kind > 9 || kind == 5 || wg_per_cu < 0 || wg_per_cu > 16)
+        return set_err(ctx, OOPS_ERR_ARG, "kind must be a batch kernel kind (0-4, 6-9), wg_per_cu in [0, 16]");
+    ctx->wg_per_cu[kind] = wg_per_cu;
+    return OOPS_OK;
+}
+
 int oops_ctx_info(const oops_ctx* ctx, int32_t* num_cu, int32_t* lds_bytes, char* name, int32_t name_len) {
     if (!ctx) return OOPS_ERR_ARG;
     if (num_cu) *num_cu = ctx->num_cu;
@@ -367,6 +375,15 @@ int oops_dlaran_fill(int32_t* iseed, int64_t count, double* out) {
 }  // extern "C"
 
 static int sweep_workspace(oops_ctx* ctx, oops_hahn_batch::Bucket& B, int ntmax);
+
+// Workgroups of a persistent batch kernel: units of work (items or sweeps)
+// pull from the bucket's device queue, so any grid gives the same results;
+// the default per-CU count is what the kernel's LDS and registers let one CU
+// hold, oops_ctx_set_launch overrides it per kernel kind.
+static int persistent_grid(const oops_ctx* ctx, int kind, int def_per_cu, int64_t units) {
+    const int per_cu = ctx->wg_per_cu[kind] > 0 ? ctx->wg_per_cu[kind] : def_per_cu;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(units, (int64_t)ctx->num_cu * per_cu));
+}
 
 // Host side of one bucket before upload: packed operator tables (left empty
 // when the device term builder wrote them to the bucket directly) and the
@@ -605,7 +622,7 @@ static int finalize_batch(oops_ctx* ctx, std::unique_ptr<oops_hahn_batch>& b, st
             rc = rc ? rc : dev_upload(ctx, B.tau, H.tau);
             if (!rc) rc = alloc_results(ctx, B.L, std::max<int64_t>(B.L_len, 1));
             if (rc) return rc;
-            B.grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu * 2));
+            B.grid = persistent_grid(ctx, 9, 2, (int64_t)B.items.size());
             continue;
         }
         std::vector<double> rnd(2 * B.n + 16);
@@ -641,11 +658,13 @@ static int finalize_batch(oops_ctx* ctx, std::unique_ptr<oops_hahn_batch>& b, st
         }
         if (rc) return rc;
         const bool dense = B.n <= m;
-        const int per_cu = dense ? 4 : 2;
-        B.grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu * per_cu));
+        // the kind bucket_launch reports for B.grid: dense Pade, FP64 or
+        // mixed Arnoldi, mixed ELL Lanczos
+        const int gkind = dense ? 0 : (precision == 1 && B.hermitian ? 3 : (precision == 0 ? 1 : 2));
+        B.grid = persistent_grid(ctx, gkind, dense ? 4 : 2, (int64_t)B.items.size());
         if (B.medium) {
             // one workgroup per CU; Hahn items run as two units (one per sweep)
-            B.medium_grid = std::max(1, std::min(2 * (int)B.items.size(), ctx->num_cu));
+            B.medium_grid = persistent_grid(ctx, 7, 1, 2 * (int64_t)B.items.size());
             const size_t bytes = sizeof(float) * 2 * (size_t)medium_workspace_float2(B.medium_grid, m, B.n);
             rc = hip_check(ctx, hipMalloc(&B.medws.p, bytes), "hipMalloc medium workspace");
             if (rc) return rc;
@@ -669,7 +688,7 @@ static int finalize_batch(oops_ctx* ctx, std::unique_ptr<oops_hahn_batch>& b, st
         if (B.slots) {
             // mixed: two workgroups per CU, FP64 variant: one; 2 (Hahn) or 4
             // (CPMG-N) coefficient tables each
-            B.slot_grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu * (lanczos64 ? 1 : 2)));
+            B.slot_grid = persistent_grid(ctx, lanczos64 ? 8 : 6, lanczos64 ? 1 : 2, (int64_t)B.items.size());
             if (!B.device_tables) {
                 rc = rc ? rc : dev_upload(ctx, B.eslot, H.eslot);
                 rc = rc ? rc : dev_upload(ctx, B.smask, H.smask);
@@ -695,7 +714,7 @@ static int finalize_batch(oops_ctx* ctx, std::unique_ptr<oops_hahn_batch>& b, st
         }
         if (B.n == 64) {
             // one workgroup per CU (the kernel holds 152 KB of LDS)
-            B.dense_grid = std::max(1, std::min((int)B.items.size(), ctx->num_cu));
+            B.dense_grid = persistent_grid(ctx, 4, 1, (int64_t)B.items.size());
             const size_t bytes = sizeof(double) * (size_t)dense64_workspace_doubles(B.dense_grid);
             rc = hip_check(ctx, hipMalloc(&B.work.p, bytes), "hipMalloc dense workspace");
             if (rc) return rc;

@@ -20,6 +20,7 @@ struct oops_ctx {
     std::string name;
     std::string err;
     void* noalloc = nullptr;   // device address of the no-allocation array on this device
+    int wg_per_cu[16] = {};    // persistent batch grids: workgroups per CU by kernel kind (0: the default)
 };
 
 namespace oops {

@@ -5,8 +5,11 @@ B = 0.05 T along [111], Hahn echo at 401 points on 0-5 ms.
 * FP64 (the reference's algorithm, m = 30, tol = 1e-12) against the oracle's
   L_c(t) on 8 clusters of every order (committed fixture, written by
   tests/golden/make_cce4_window.py), relative error <= 1e-10 at every point;
-* mixed precision at the tuned (m, tol) of oops_amd/tuning.json against FP64
-  on 256 clusters of every order: max |dL| <= 1e-6 over every L(t);
+* mixed precision at the tuned (m, tol, product form, launch shapes) of
+  oops_amd/tuning.json against FP64 on 256 clusters of every order:
+  max |dL| <= 1e-6 over every L(t);
+* every launch shape the autotuner may pick (1-4 workgroups per CU for each
+  kernel kind the batch runs) gives bit-identical L(t);
 * FP64 with the mixed path's algorithm and schedule at the same (m, tol):
   also within 1e-6 of the reference schedule, so the FP32 parts and the
   Lanczos/Chebyshev scheme each stay inside the bar.
@@ -35,8 +38,8 @@ def _workload():
 
 def _tuned():
     with open(os.path.join(ROOT, "oops_amd", "tuning.json")) as f:
-        t = json.load(f)["mixed"]
-    return int(t["m"]), float(t["tol"])
+        t = json.load(f)
+    return int(t["mixed"]["m"]), float(t["mixed"]["tol"]), t["mixed"].get("precision", "mixed"), t.get("launch", {})
 
 
 def _sample(counts, per):
@@ -75,15 +78,21 @@ def test_fp64_batch_matches_oracle_on_the_full_window(ctx):
 
 @pytest.mark.gpu
 def test_tuned_mixed_and_fp64_lanczos_within_the_bar_on_256_clusters_per_order(ctx):
-    m, tol = _tuned()
+    m, tol, prec, launch = _tuned()
     X, C, par, tau = _workload()
     sel = _sample(C.counts(), 256)
-    print(f"tuned m={m} tol={tol:g} on {len(sel)} clusters", flush=True)
+    print(f"tuned m={m} tol={tol:g} {prec} launch {launch} on {len(sel)} clusters", flush=True)
     ref = C.hahn_batch(ctx, par, m=30, tol=1e-12, precision="fp64", select=sel)
     ref.run(tau)
     L64, st64 = ref.results()
     assert ((st64[:, 8] & 0xFF) == 0).all()
-    mixed = C.hahn_batch(ctx, par, m=m, tol=tol, precision="mixed", select=sel)
+    try:
+        for name, wg in launch.items():
+            ctx.set_launch(name, int(wg))
+        mixed = C.hahn_batch(ctx, par, m=m, tol=tol, precision=prec, select=sel)
+    finally:
+        for name in launch:
+            ctx.set_launch(name, 0)
     mixed.run(tau)
     Lm, stm = mixed.results()
     kinds = {b["n"]: b["kernel"] for b in mixed.timing()}
@@ -93,7 +102,41 @@ def test_tuned_mixed_and_fp64_lanczos_within_the_bar_on_256_clusters_per_order(c
     kinds64 = {b["n"]: b["kernel"] for b in lz.timing()}
     dm, dl, dml = np.abs(Lm - L64).max(), np.abs(Ll - L64).max(), np.abs(Lm - Ll).max()
     print(f"max |dL| mixed {dm:.2e}, fp64-lanczos {dl:.2e}, mixed vs fp64-lanczos {dml:.2e}", flush=True)
-    assert kinds[256] == "lanczos_slot" and kinds64[256] == "lanczos_slot_fp64"
+    assert kinds[256] == ("lanczos_slot" if prec == "mixed" else "lanczos_mixed")
+    assert kinds64[256] == "lanczos_slot_fp64"
     assert ((stm[:, 8] & 0xFF) == 0).all() and ((stl[:, 8] & 0xFF) == 0).all()
     assert dm <= 1e-6 and dl <= 1e-6
     assert dml <= 1e-6
+
+
+@pytest.mark.gpu
+def test_launch_shapes_give_identical_results(ctx):
+    """oops_ctx_set_launch: every kernel kind of a mixed CCE-4 batch (and of
+    the FP64 Lanczos and Arnoldi batches) at 1, 3 and 4 workgroups per CU
+    gives the same L(t) bit for bit as the default shape (work units pull from
+    device queues); bad kinds and counts are rejected."""
+    X, C, par, tau = _workload()
+    sel = _sample(C.counts(), 64)
+    t = tau[::8]
+    for prec in ("mixed", "mixed-ell", "fp64-lanczos", "fp64"):
+        b = C.hahn_batch(ctx, par, m=30, tol=1e-3 if prec != "fp64" else 1e-12, precision=prec, select=sel)
+        b.run(t)
+        L0, _ = b.results()
+        kinds = sorted({x["kernel"] for x in b.timing()})
+        for wg in (1, 3, 4):
+            try:
+                for k in kinds:
+                    ctx.set_launch(k, wg)
+                bw = C.hahn_batch(ctx, par, m=30, tol=1e-3 if prec != "fp64" else 1e-12, precision=prec,
+                                  select=sel)
+            finally:
+                for k in kinds:
+                    ctx.set_launch(k, 0)
+            bw.run(t)
+            L, st = bw.results()
+            print(f"{prec} kernels {kinds} at {wg}/CU: max |dL| {np.abs(L - L0).max():.1e}", flush=True)
+            assert np.array_equal(L, L0)
+    with pytest.raises(O.OopsError):
+        ctx.set_launch(5, 2)
+    with pytest.raises(O.OopsError):
+        ctx.set_launch("lanczos_slot", 17)
