@@ -50,7 +50,7 @@ struct LanczosSmem {
     double tdiag[16];
     double toff[16];
     double scal[16];
-    double bes[2 * kWave];             // Bessel values J_k of the Chebyshev series
+    ChebTables cheb;                   // Bessel tables of the Chebyshev series
     int ints[8];
 #ifdef OOPS_PHASE_TIMERS
     unsigned long long tbes;   // cycles of the Chebyshev set-up (interval, Bessel values)
@@ -444,7 +444,7 @@ struct LzSlotSmem {
     double tdiag[16];
     double toff[16];
     double scal[16];
-    double bes[2 * kWave];             // Bessel values J_k of the Chebyshev series
+    ChebTables cheb;                   // Bessel tables of the Chebyshev series
     int ints[8];
 #ifdef OOPS_PHASE_TIMERS
     unsigned long long tbes;   // cycles of the Chebyshev set-up (interval, Bessel values)
@@ -576,20 +576,11 @@ __device__ __forceinline__ void advance_slot(LzSlotSmem<M, F64>& S, BlockRed<kNT
         int kend = k;
         for (;;) {
             t1 = t_step >= t_end - g.t_now ? t_end : g.t_now + t_step;
-            double tp = g.t_now;
-            int q = 0;
-            if (o.mode) {
-                for (kend = k; kend < o.nt && o.tau[kend] < t1; ++kend, ++q) {
-                    if (q > 0) __syncthreads();   // every thread has read F
-                    ++g.nexph;
-                    g.nscale += cheb_coeffs<M>(S, mb, k1, o.tau[kend] - tp, bprev, q > 0);
-                    tp = o.tau[kend];
-                    emit_state<M>(br, o, kend, combine());
-                }
-                if (q > 0) __syncthreads();
-            }
-            ++g.nexph;
-            g.nscale += cheb_coeffs<M>(S, mb, k1, t1 - tp, bprev, q > 0);
+            kend = k;
+            if (o.mode)
+                while (kend < o.nt && o.tau[kend] < t1) ++kend;
+            g.nscale += cheb_walk<M>(S, mb, k1, bprev, g.t_now, t1, o.tau, k, kend - k, g.nexph,
+                                     [&](int i) { emit_state<M>(br, o, k + i, combine()); });
             if (k1 == 0) {
                 err_loc = tol;
             } else {

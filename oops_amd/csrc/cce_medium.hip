@@ -40,7 +40,7 @@ struct MedSmem {
     double e[M + 2];
     cd F[M + 2];
     double scal[16];
-    double bes[2 * kWave];
+    ChebTables cheb;
     int ints[8];
     // flip-mask form: the active generator's coefficient tables and masks
     const double2* tab64;
@@ -446,21 +446,13 @@ __device__ void med_advance(MedSmem<M, E>& S, BlockRed<kMT>& br, MedState& g, cd
         int kend = k;
         for (;;) {
             t1 = t_step >= t_out - g.t_now ? t_out : g.t_now + t_step;
-            double tp = g.t_now;
-            int q = 0;
-            if (ob) {
-                for (kend = k; kend < ont && otau[kend] < t1; ++kend, ++q) {
-                    if (q > 0) __syncthreads();   // every thread has read F
-                    ++g.nexph;
-                    g.nscale += cheb_coeffs<M>(S, mb, k1, otau[kend] - tp, bprev, q > 0);
-                    tp = otau[kend];
-                    combine(vcur);
-                    med_emit<E>(ob, n, kend, vcur);
-                }
-                if (q > 0) __syncthreads();
-            }
-            ++g.nexph;
-            g.nscale += cheb_coeffs<M>(S, mb, k1, t1 - tp, bprev, q > 0);
+            kend = k;
+            if (ob)
+                while (kend < ont && otau[kend] < t1) ++kend;
+            g.nscale += cheb_walk<M>(S, mb, k1, bprev, g.t_now, t1, otau, k, kend - k, g.nexph, [&](int i) {
+                combine(vcur);
+                med_emit<E>(ob, n, k + i, vcur);
+            });
             if (k1 == 0) {
                 err_loc = tol;
             } else {
