@@ -193,7 +193,10 @@ void hamvec_cuda3_(size_t* handle, int* nspin, int* nTerm, double* coeff_lst_zpl
                    size_t* dev_mat_i_lst, size_t* ham_dim, size_t* nspin_dim, size_t* nspin_m_lst,
                    size_t* nspin_n_lst, size_t* dev_v, size_t* dev_w, size_t* dev_w_med, size_t* dev_coeff,
                    size_t* maxThreadsPerBlock, size_t* maxGridSize);
-/* exp(t H) (m x m, m <= 42, ideg = 6) into wsp(1..m*m), *iexph = 1. */
+/* exp(t H) (m x m, ideg = 6) into wsp(1..m*m), *iexph = 1, *ns squarings as
+ * ZGPADM counts them.  m <= 42 runs in one workgroup; larger m (the dense
+ * propagators of MatExp::pade_exp_mat) on rocBLAS products and a rocSOLVER
+ * LU solve, loaded at first use (*iflag = -2 when they are missing). */
 void zgpadm_(int* ideg, int* m, double* t, double* H, int* ldh, double* wsp, int* lwsp, int* ipiv, int* iexph,
              int* ns, int* iflag);
 

@@ -929,6 +929,18 @@ int launch_large_pade_full(const double2* hess, int ldh, int mx, double t, doubl
     return 0;
 }
 
+__global__ __launch_bounds__(kLT) void large_diag_add(double2* A, int m, double2 v) {
+    const int i = blockIdx.x * kLT + threadIdx.x;
+    if (i < m) {
+        double2* p = A + (int64_t)i * (m + 1);
+        *p = make_double2(p->x + v.x, p->y + v.y);
+    }
+}
+
+void launch_large_diag_add(double2* A, int m, double2 v, hipStream_t s) {
+    hipLaunchKernelGGL(large_diag_add, dim3((m + kLT - 1) / kLT), dim3(kLT), 0, s, A, m, v);
+}
+
 void launch_large_combine(const void* V, int64_t ldv, int mx, const double2* F, double beta, double2* w, int64_t n,
                           double* part, int grid, hipStream_t s, bool fp32) {
     if (fp32)

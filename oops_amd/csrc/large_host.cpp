@@ -9,7 +9,9 @@
 // delta 1.2), update, 2-significant-digit step rounding, output at every
 // requested time.  Only scalars (breakdown flag, norms, two entries of the
 // small exponential) come back per Krylov step.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cmath>
@@ -1548,17 +1550,171 @@ void hamvec_cuda3_(size_t* handle, int* nspin, int* nTerm, double* coeff_lst_zpl
                    (double2*)*dev_w, ctx->stream);
 }
 
+}  // extern "C"
+
+// rocBLAS products and the rocSOLVER LU solve of the dense Pade path,
+// resolved when it is first used (no link-time dependency: a process that
+// already holds a copy of these libraries, PyTorch's for one, shares it).
+namespace {
+struct DenseLib {
+    bool ok = false;
+    rocblas_handle h = nullptr;
+    rocblas_status (*create)(rocblas_handle*) = nullptr;
+    rocblas_status (*set_stream)(rocblas_handle, hipStream_t) = nullptr;
+    rocblas_status (*zgemm)(rocblas_handle, rocblas_operation, rocblas_operation, rocblas_int, rocblas_int,
+                            rocblas_int, const rocblas_double_complex*, const rocblas_double_complex*, rocblas_int,
+                            const rocblas_double_complex*, rocblas_int, const rocblas_double_complex*,
+                            rocblas_double_complex*, rocblas_int) = nullptr;
+    rocblas_status (*zaxpy)(rocblas_handle, rocblas_int, const rocblas_double_complex*,
+                            const rocblas_double_complex*, rocblas_int, rocblas_double_complex*,
+                            rocblas_int) = nullptr;
+    rocblas_status (*zdscal)(rocblas_handle, rocblas_int, const double*, rocblas_double_complex*,
+                             rocblas_int) = nullptr;
+    rocblas_status (*zgesv)(rocblas_handle, rocblas_int, rocblas_int, rocblas_double_complex*, rocblas_int,
+                            rocblas_int*, rocblas_double_complex*, rocblas_int, rocblas_int*) = nullptr;
+};
+
+template <typename F>
+bool resolve(void* lib, const char* name, F& f) {
+    f = lib ? reinterpret_cast<F>(dlsym(lib, name)) : nullptr;
+    return f != nullptr;
+}
+
+// One handle on the given context's stream (the process-wide context of the
+// reference ABI symbols).
+DenseLib* dense_lib(oops_ctx* ctx) {
+    static DenseLib L;
+    static std::once_flag once;
+    std::call_once(once, [ctx] {
+        void* b = dlopen("librocblas.so.5", RTLD_NOW | RTLD_GLOBAL);
+        if (!b) b = dlopen("librocblas.so", RTLD_NOW | RTLD_GLOBAL);
+        void* s = dlopen("librocsolver.so.0", RTLD_NOW | RTLD_GLOBAL);
+        if (!s) s = dlopen("librocsolver.so", RTLD_NOW | RTLD_GLOBAL);
+        L.ok = resolve(b, "rocblas_create_handle", L.create) && resolve(b, "rocblas_set_stream", L.set_stream) &&
+               resolve(b, "rocblas_zgemm", L.zgemm) && resolve(b, "rocblas_zaxpy", L.zaxpy) &&
+               resolve(b, "rocblas_zdscal", L.zdscal) && resolve(s, "rocsolver_zgesv", L.zgesv);
+        L.ok = L.ok && L.create(&L.h) == rocblas_status_success &&
+               L.set_stream(L.h, ctx->stream) == rocblas_status_success;
+    });
+    return L.ok ? &L : nullptr;
+}
+}  // namespace
+
+// exp(t H) of a dense m x m complex matrix (column-major, leading dimension
+// ldh) by ZGPADM's Pade(6) with scaling and squaring (expokit.f:762-934),
+// step for step: the infinity norm and squaring count, the Horner recurrence
+// in H^2 for the numerator and denominator, the final product with H, the
+// solve (Q - P) X = P, E = +-(I + 2X) and ns squarings; products on rocBLAS,
+// the LU solve on rocSOLVER, on the context's stream.  out: m*m complex.
+static int pade_dense(oops_ctx* ctx, int m, double t, const double* H, int ldh, double* out, int* ns_out) {
+    DenseLib* L = dense_lib(ctx);
+    if (!L) return set_err(ctx, OOPS_ERR_UNSUPPORTED, "rocBLAS / rocSOLVER not available for the dense Pade path");
+    typedef rocblas_double_complex zc;
+    const size_t mm = (size_t)m * m;
+    double hnorm = 0.0;
+    for (int i = 0; i < m; ++i) {
+        double row = 0.0;
+        for (int j = 0; j < m; ++j) row += std::hypot(H[2 * (i + (size_t)ldh * j)], H[2 * (i + (size_t)ldh * j) + 1]);
+        hnorm = std::max(hnorm, row);
+    }
+    hnorm = std::fabs(t * hnorm);
+    if (hnorm == 0.0) return set_err(ctx, OOPS_ERR_ARG, "null H");
+    const int ideg = 6;
+    const int ns = std::max(0, (int)(std::log(hnorm) / std::log(2.0)) + 2);
+    const double scale = t / std::ldexp(1.0, ns);
+    double coef[ideg + 1];
+    coef[0] = 1.0;
+    for (int k = 1; k <= ideg; ++k) coef[k] = coef[k - 1] * (double)(ideg + 1 - k) / (double)(k * (2 * ideg + 1 - k));
+    std::vector<double> hp(2 * mm);
+    for (int j = 0; j < m; ++j)
+        std::memcpy(&hp[2 * (size_t)m * j], &H[2 * (size_t)ldh * j], sizeof(double) * 2 * m);
+    DevBuf dH, dH2, b0, b1, b2, piv, info;
+    int rc = upload(ctx, dH, hp);
+    for (DevBuf* d : {&dH2, &b0, &b1, &b2}) rc = rc ? rc : alloc(ctx, *d, sizeof(zc) * mm);
+    rc = rc ? rc : alloc(ctx, piv, sizeof(rocblas_int) * m);
+    rc = rc ? rc : alloc(ctx, info, sizeof(rocblas_int));
+    if (rc) return rc;
+    auto Z = [](const DevBuf& d) { return (zc*)d.p; };
+    const zc one{1.0, 0.0}, zero{0.0, 0.0}, mone{-1.0, 0.0}, s2{scale * scale, 0.0}, sc{scale, 0.0};
+    bool ok = true;
+    auto blas = [&](rocblas_status st) { ok = ok && st == rocblas_status_success; };
+    const rocblas_operation N = rocblas_operation_none;
+    blas(L->zgemm(L->h, N, N, m, m, m, &s2, Z(dH), m, Z(dH), m, &zero, Z(dH2), m));   // (scale H)^2
+    zc* p = Z(b0);
+    zc* q = Z(b1);
+    zc* f = Z(b2);
+    ok = ok && hipMemsetAsync(p, 0, sizeof(zc) * mm, ctx->stream) == hipSuccess &&
+         hipMemsetAsync(q, 0, sizeof(zc) * mm, ctx->stream) == hipSuccess;
+    launch_large_diag_add((double2*)p, m, make_double2(coef[ideg - 1], 0.0), ctx->stream);
+    launch_large_diag_add((double2*)q, m, make_double2(coef[ideg], 0.0), ctx->stream);
+    bool odd = true;
+    for (int k = ideg - 1; k > 0; --k) {   // Horner in H^2, numerator and denominator alternately
+        zc* used = odd ? q : p;
+        blas(L->zgemm(L->h, N, N, m, m, m, &one, used, m, Z(dH2), m, &zero, f, m));
+        launch_large_diag_add((double2*)f, m, make_double2(coef[k - 1], 0.0), ctx->stream);
+        if (odd) q = f;
+        else p = f;
+        f = used;
+        odd = !odd;
+    }
+    if (odd) {
+        blas(L->zgemm(L->h, N, N, m, m, m, &sc, q, m, Z(dH), m, &zero, f, m));
+        std::swap(q, f);
+    } else {
+        blas(L->zgemm(L->h, N, N, m, m, m, &sc, p, m, Z(dH), m, &zero, f, m));
+        std::swap(p, f);
+    }
+    blas(L->zaxpy(L->h, (rocblas_int)mm, &mone, p, 1, q, 1));                          // Q - P
+    blas(L->zgesv(L->h, m, m, q, m, (rocblas_int*)piv.p, p, m, (rocblas_int*)info.p));   // X = (Q - P)^-1 P
+    const double two = 2.0, neg = -1.0;
+    blas(L->zdscal(L->h, (rocblas_int)mm, &two, p, 1));
+    launch_large_diag_add((double2*)p, m, make_double2(1.0, 0.0), ctx->stream);      // I + 2X
+    zc* res = p;
+    if (ns == 0 && odd) {
+        blas(L->zdscal(L->h, (rocblas_int)mm, &neg, p, 1));
+    } else {
+        bool parity = true;
+        for (int k = 0; k < ns; ++k) {   // squarings, alternating between p and q
+            zc* get = parity ? p : q;
+            zc* put = parity ? q : p;
+            blas(L->zgemm(L->h, N, N, m, m, m, &one, get, m, get, m, &zero, put, m));
+            res = put;
+            parity = !parity;
+        }
+    }
+    if (!ok) return set_err(ctx, OOPS_ERR_HIP, "rocBLAS / rocSOLVER call failed");
+    rocblas_int st = 0;
+    rc = hip_check(ctx, hipMemcpyAsync(out, res, sizeof(zc) * mm, hipMemcpyDeviceToHost, ctx->stream), "copy");
+    rc = rc ? rc : hip_check(ctx, hipMemcpyAsync(&st, info.p, sizeof(st), hipMemcpyDeviceToHost, ctx->stream), "copy");
+    rc = rc ? rc : hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    if (rc) return rc;
+    if (st != 0) return set_err(ctx, OOPS_ERR_NUMERIC, "singular Pade denominator");
+    *ns_out = ns;
+    return OOPS_OK;
+}
+
+extern "C" {
+
 // zgpadm_ (expokit.h:6): exp(t H) of an m x m matrix by Pade(ideg = 6) with
 // scaling and squaring on the device; the result is written to wsp(1..m*m)
-// (*iexph = 1), *ns squarings; *iflag < 0 on error.  m <= 42.
+// (*iexph = 1), *ns squarings; *iflag < 0 on error.  m <= 42 runs in one
+// workgroup (the Krylov driver's small exponential); larger m (the dense
+// propagators of MatExp::pade_exp_mat, MatExp.cpp:29-77) on rocBLAS /
+// rocSOLVER.
 void zgpadm_(int* ideg, int* m, double* t, double* H, int* ldh, double* wsp, int* lwsp, int* ipiv, int* iexph,
              int* ns, int* iflag) {
     (void)ipiv;
     *iflag = 0;
     const int mm = *m;
-    if (*ideg != 6 || mm < 1 || mm > kLargeMaxM + 2 || *ldh < mm || *lwsp < mm * mm) { *iflag = -1; return; }
+    if (*ideg != 6 || mm < 1 || *ldh < mm || *lwsp < mm * mm) { *iflag = -1; return; }
     oops_ctx* ctx = abi_ctx();
     if (!ctx) { *iflag = -2; return; }
+    if (mm > kLargeMaxM + 2) {
+        const int rc = pade_dense(ctx, mm, *t, H, *ldh, wsp, ns);
+        if (rc) { *iflag = rc == OOPS_ERR_NUMERIC ? -8 : -2; return; }
+        *iexph = 1;
+        return;
+    }
     std::vector<double> h((size_t)2 * mm * (mm + 1), 0.0);  // one spare column (tiled reads)
     for (int j = 0; j < mm; ++j)
         for (int i = 0; i < mm; ++i) {

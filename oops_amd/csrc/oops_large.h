@@ -86,6 +86,8 @@ void launch_large_csr_matvec(const int64_t* row_ptr, const int32_t* col, const d
                              hipStream_t s);
 // exp(t H) for H m x m (m <= kLargeMaxM + 2), full matrix, leading dimension m in out
 int launch_large_pade_full(const double2* hess, int ldh, int mx, double t, double2* out, int* status, hipStream_t s);
+// A(i, i) += v for an m x m column-major matrix (leading dimension m).
+void launch_large_diag_add(double2* A, int m, double2 v, hipStream_t s);
 void launch_large_axpby(double2 a, const double2* x, double2 b, double2* y, int64_t n, int grid, hipStream_t s);
 
 }  // namespace oops
