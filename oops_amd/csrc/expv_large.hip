@@ -444,7 +444,16 @@ __device__ void finish_rows(const double* __restrict__ part, int nblk, int nrows
             b += __shfl_xor(b, mask, kWave);
         }
         if (lane != 0) continue;
-        if (mode == 2) {
+        if (mode == 2 || (mode == 1 && l == jcol + 1)) {
+            if (mode == 1) {
+                // |p|^2 after the first pass: keep p as it is when the pass
+                // removed at most 3/4 of |p|^2, |p1| >= |p0| / 2 (the
+                // Daniel-Gragg-Kaufman-Stewart test): the second pass would
+                // change p by rounding only and is skipped
+                const int skip = a >= 0.25 * scal[3];
+                flags[2] = skip;
+                if (!skip) continue;
+            }
             const double h = sqrt(a);
             const bool broke = flags[0] != 0 || h <= break_tol;
             if (!broke) hess[(jcol + 1) + (int64_t)ldh * jcol] = make_double2(h, 0.0);
@@ -456,19 +465,35 @@ __device__ void finish_rows(const double* __restrict__ part, int nblk, int nrows
             scal[1] = h;
         } else if (mode == 3) {
             scal[2] = sqrt(a);
+        } else if (mode == 0 && l == jcol + 1) {
+            scal[3] = a;   // |p|^2 before the first pass
         } else {
             coef[l] = make_double2(a, b);
-            double2& hv = hess[l + (int64_t)ldh * jcol];
-            if (mode == 0) hv = make_double2(a, b);
-            else hv = make_double2(hv.x + a, hv.y + b);
+            if (mode == 0) hess[l + (int64_t)ldh * jcol] = make_double2(a, b);
         }
     }
 }
 
+// mode 0 (after the dots pass, nrows = J + 2): coef = hess column = V^H p,
+//   row J + 1 is |p|^2;
+// mode 1 (after the update-and-dots pass, nrows = J + 2): coef = the second
+//   projection, row J + 1 is |p|^2 after the first pass, which decides
+//   whether the second pass runs (flags[2] = 0) or is skipped (flags[2] = 1:
+//   the column's norm entry and scal are set here);
+// mode 2 (after the second pass; nothing when skipped): the second
+//   projection joins the Hessenberg column, row 0 is |p|^2;
+// mode 3: the extra product's norm.
 __global__ __launch_bounds__(kFinishThreads) void large_finish(const double* __restrict__ part, int nblk, int nrows,
                                                                int jcol, int mode, double2* __restrict__ hess, int ldh,
                                                                double2* __restrict__ coef, double* __restrict__ scal,
                                                                int* __restrict__ flags, double break_tol) {
+    if (mode == 2) {
+        if (flags[2]) return;
+        for (int l = threadIdx.x; l <= jcol; l += blockDim.x) {
+            double2& hv = hess[l + (int64_t)ldh * jcol];
+            hv = make_double2(hv.x + coef[l].x, hv.y + coef[l].y);
+        }
+    }
     finish_rows(part, nblk, nrows, jcol, mode, hess, ldh, coef, scal, flags, break_tol);
 }
 
@@ -479,9 +504,12 @@ __global__ __launch_bounds__(kFinishThreads) void large_finish(const double* __r
 // Each thread keeps the J+1 basis values of its element in registers between
 // the update and the dot products; partial sums are reduced per wave with
 // shuffles, then across the block's waves in LDS.  MAXJ bounds J + 1.
-template <int MAXJ>
+// Rows 0..nrows-1 from (sr, si) and, with EXTRA, row nrows = (x, 0); red
+// holds (MAXJ + 1) rows per wave.
+template <int MAXJ, bool EXTRA = false>
 __device__ __forceinline__ void gs_block_reduce(const double (&sr)[MAXJ], const double (&si)[MAXJ], int nrows,
-                                                double* red, double* __restrict__ part, int64_t row0) {
+                                                double* red, double* __restrict__ part, double x = 0.0) {
+    constexpr int RS = MAXJ + 1;   // row stride of red
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
 #pragma unroll
     for (int l = 0; l < MAXJ; ++l) {
@@ -493,31 +521,41 @@ __device__ __forceinline__ void gs_block_reduce(const double (&sr)[MAXJ], const 
                 b += __shfl_xor(b, mask, kWave);
             }
             if (lane == 0) {
-                red[(wave * MAXJ + l) * 2] = a;
-                red[(wave * MAXJ + l) * 2 + 1] = b;
+                red[(wave * RS + l) * 2] = a;
+                red[(wave * RS + l) * 2 + 1] = b;
             }
         }
     }
+    if constexpr (EXTRA) {
+#pragma unroll
+        for (int mask = 32; mask >= 1; mask >>= 1) x += __shfl_xor(x, mask, kWave);
+        if (lane == 0) {
+            red[(wave * RS + nrows) * 2] = x;
+            red[(wave * RS + nrows) * 2 + 1] = 0.0;
+        }
+    }
     __syncthreads();
-    for (int l = threadIdx.x; l < nrows; l += kLT) {
+    const int rows = nrows + (EXTRA ? 1 : 0);
+    for (int l = threadIdx.x; l < rows; l += kLT) {
         double a = 0.0, b = 0.0;
-        for (int w = 0; w < kLT / kWave; ++w) { a += red[(w * MAXJ + l) * 2]; b += red[(w * MAXJ + l) * 2 + 1]; }
+        for (int w = 0; w < kLT / kWave; ++w) { a += red[(w * RS + l) * 2]; b += red[(w * RS + l) * 2 + 1]; }
         part[((int64_t)l * gridDim.x + blockIdx.x) * 2] = a;
         part[((int64_t)l * gridDim.x + blockIdx.x) * 2 + 1] = b;
     }
-    (void)row0;
 }
 
 template <int MAXJ, typename VT>
 __global__ __launch_bounds__(kLT) void large_gs_dots(const VT* __restrict__ V, int64_t ldv,
                                                      const double2* __restrict__ p, int64_t n, int J,
                                                      double* __restrict__ part) {
-    __shared__ double red[2 * MAXJ * (kLT / kWave)];
+    __shared__ double red[2 * (MAXJ + 1) * (kLT / kWave)];
     double sr[MAXJ], si[MAXJ];
+    double nr = 0.0;   // |p|^2
 #pragma unroll
     for (int l = 0; l < MAXJ; ++l) { sr[l] = 0.0; si[l] = 0.0; }
     for (int64_t i = (int64_t)blockIdx.x * kLT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kLT) {
         const double2 b = p[i];
+        nr = fma(b.x, b.x, fma(b.y, b.y, nr));
 #pragma unroll
         for (int l = 0; l < MAXJ; ++l)
             if (l <= J) {
@@ -526,20 +564,22 @@ __global__ __launch_bounds__(kLT) void large_gs_dots(const VT* __restrict__ V, i
                 si[l] = fma(a.x, b.y, fma(-a.y, b.x, si[l]));
             }
     }
-    gs_block_reduce<MAXJ>(sr, si, J + 1, red, part, 0);
+    gs_block_reduce<MAXJ, true>(sr, si, J + 1, red, part, nr);
 }
 
 template <int MAXJ, typename VT, bool NORM>
 __global__ __launch_bounds__(kLT) void large_gs_update(const VT* __restrict__ V, int64_t ldv,
                                                        double2* __restrict__ p, int64_t n, int J,
                                                        const double2* __restrict__ coef,
-                                                       double* __restrict__ part) {
+                                                       double* __restrict__ part, const int* __restrict__ skip) {
     constexpr int NR = NORM ? 1 : MAXJ;   // partial sums per thread
-    __shared__ double red[2 * NR * (kLT / kWave)];
+    if (NORM && skip && *skip) return;    // the second pass is not needed (decided by the finish kernel)
+    __shared__ double red[2 * (NR + 1) * (kLT / kWave)];
     __shared__ double2 c[MAXJ];
     for (int l = threadIdx.x; l <= J; l += kLT) c[l] = coef[l];
     __syncthreads();
     double sr[NR], si[NR];
+    double nr = 0.0;   // |p|^2 after the update (the dot-product variant)
 #pragma unroll
     for (int l = 0; l < NR; ++l) { sr[l] = 0.0; si[l] = 0.0; }
     for (int64_t i = (int64_t)blockIdx.x * kLT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kLT) {
@@ -554,6 +594,7 @@ __global__ __launch_bounds__(kLT) void large_gs_update(const VT* __restrict__ V,
         if constexpr (NORM) {
             sr[0] = fma(acc.re, acc.re, fma(acc.im, acc.im, sr[0]));
         } else {
+            nr = fma(acc.re, acc.re, fma(acc.im, acc.im, nr));
             // the basis values again (from the caches: they were just read),
             // rather than held in registers across the update
             asm volatile("" ::: "memory");
@@ -566,7 +607,8 @@ __global__ __launch_bounds__(kLT) void large_gs_update(const VT* __restrict__ V,
                 }
         }
     }
-    gs_block_reduce<NR>(sr, si, NORM ? 1 : J + 1, red, part, 0);
+    if constexpr (NORM) gs_block_reduce<NR>(sr, si, 1, red, part);
+    else gs_block_reduce<NR, true>(sr, si, J + 1, red, part, nr);
 }
 
 // Dot-product passes with the basis range split over lane halves: lanes q and
@@ -576,10 +618,16 @@ __global__ __launch_bounds__(kLT) void large_gs_update(const VT* __restrict__ V,
 // the two halves' partial products of the update meet by one shuffle.
 template <int HJ>
 __device__ __forceinline__ void gs_half_reduce(const double (&sr)[HJ], const double (&si)[HJ], int l0, int nrows,
-                                               double* red, double* __restrict__ part) {
+                                               double* red, double* __restrict__ part, double x) {
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const int hi = lane >> 5;
-    constexpr int MAXJ = 2 * HJ;
+    constexpr int MAXJ = 2 * HJ + 1;   // row stride of red (the last row: x, row nrows of part)
+#pragma unroll
+    for (int mask = 32; mask >= 1; mask >>= 1) x += __shfl_xor(x, mask, kWave);
+    if (lane == 0) {
+        red[(wave * MAXJ + nrows) * 2] = x;
+        red[(wave * MAXJ + nrows) * 2 + 1] = 0.0;
+    }
 #pragma unroll
     for (int q = 0; q < HJ; ++q) {
         double a = sr[q], b = si[q];
@@ -595,7 +643,7 @@ __device__ __forceinline__ void gs_half_reduce(const double (&sr)[HJ], const dou
         }
     }
     __syncthreads();
-    for (int l = threadIdx.x; l < nrows; l += kLT) {
+    for (int l = threadIdx.x; l <= nrows; l += kLT) {
         double a = 0.0, b = 0.0;
         for (int w = 0; w < kLT / kWave; ++w) { a += red[(w * MAXJ + l) * 2]; b += red[(w * MAXJ + l) * 2 + 1]; }
         part[((int64_t)l * gridDim.x + blockIdx.x) * 2] = a;
@@ -608,7 +656,7 @@ __global__ __launch_bounds__(kLT) void large_gs_dots_half(const VT* __restrict__
                                                           const double2* __restrict__ p, int64_t n, int J,
                                                           double* __restrict__ part) {
     constexpr int MAXJ = 2 * HJ;
-    __shared__ double red[2 * MAXJ * (kLT / kWave)];
+    __shared__ double red[2 * (MAXJ + 1) * (kLT / kWave)];
     const int lane = threadIdx.x & (kWave - 1);
     const int hi = lane >> 5;
     const int l0 = (J + 2) / 2;
@@ -617,8 +665,10 @@ __global__ __launch_bounds__(kLT) void large_gs_dots_half(const VT* __restrict__
 #pragma unroll
     for (int q = 0; q < HJ; ++q) { sr[q] = 0.0; si[q] = 0.0; }
     const int64_t w0 = ((int64_t)blockIdx.x * kLT + threadIdx.x - lane) / 2;   // this wave's first element
+    double nr = 0.0;   // |p|^2, counted by the low half (each element once)
     for (int64_t i = w0 + (lane & 31); i < n; i += (int64_t)gridDim.x * (kLT / 2)) {
         const double2 b = p[i];
+        if (!hi) nr = fma(b.x, b.x, fma(b.y, b.y, nr));
 #pragma unroll
         for (int q = 0; q < HJ; ++q)
             if (lb + q < le) {
@@ -627,7 +677,7 @@ __global__ __launch_bounds__(kLT) void large_gs_dots_half(const VT* __restrict__
                 si[q] = fma(a.x, b.y, fma(-a.y, b.x, si[q]));
             }
     }
-    gs_half_reduce<HJ>(sr, si, l0, J + 1, red, part);
+    gs_half_reduce<HJ>(sr, si, l0, J + 1, red, part, nr);
 }
 
 template <int HJ, typename VT>
@@ -636,7 +686,7 @@ __global__ __launch_bounds__(kLT) void large_gs_update_half(const VT* __restrict
                                                             const double2* __restrict__ coef,
                                                             double* __restrict__ part) {
     constexpr int MAXJ = 2 * HJ;
-    __shared__ double red[2 * MAXJ * (kLT / kWave)];
+    __shared__ double red[2 * (MAXJ + 1) * (kLT / kWave)];
     __shared__ double2 c[MAXJ];
     for (int l = threadIdx.x; l <= J; l += kLT) c[l] = coef[l];
     __syncthreads();
@@ -648,6 +698,7 @@ __global__ __launch_bounds__(kLT) void large_gs_update_half(const VT* __restrict
 #pragma unroll
     for (int q = 0; q < HJ; ++q) { sr[q] = 0.0; si[q] = 0.0; }
     const int64_t w0 = ((int64_t)blockIdx.x * kLT + threadIdx.x - lane) / 2;
+    double nr = 0.0;   // |p|^2 after the update, counted by the low half
     // every lane of a wave runs the same iterations (the shuffle needs both halves)
     for (int64_t i0 = w0; i0 < n; i0 += (int64_t)gridDim.x * (kLT / 2)) {
         const int64_t i = i0 + (lane & 31);
@@ -664,7 +715,10 @@ __global__ __launch_bounds__(kLT) void large_gs_update_half(const VT* __restrict
         if (!live) continue;
         const double2 pv = p[i];
         const cd acc = mkcd(pv.x - (part_acc.re + ore), pv.y - (part_acc.im + oim));
-        if (!hi) p[i] = make_double2(acc.re, acc.im);
+        if (!hi) {
+            p[i] = make_double2(acc.re, acc.im);
+            nr = fma(acc.re, acc.re, fma(acc.im, acc.im, nr));
+        }
         asm volatile("" ::: "memory");
 #pragma unroll
         for (int q = 0; q < HJ; ++q)
@@ -674,7 +728,7 @@ __global__ __launch_bounds__(kLT) void large_gs_update_half(const VT* __restrict
                 si[q] = fma(v.x, acc.im, fma(-v.y, acc.re, si[q]));
             }
     }
-    gs_half_reduce<HJ>(sr, si, l0, J + 1, red, part);
+    gs_half_reduce<HJ>(sr, si, l0, J + 1, red, part, nr);
 }
 
 // F = exp(t H(0:mx,0:mx)) e1 for the augmented Hessenberg matrix (Pade(6),
@@ -834,20 +888,20 @@ static void gs_dots_j(const void* V, int64_t ldv, const double2* p, int64_t n, i
 
 template <int MAXJ, bool NORM>
 static void gs_update_k(const void* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef, double* part,
-                        int nblk, hipStream_t s, bool fp32) {
+                        int nblk, hipStream_t s, bool fp32, const int* skip) {
     if (fp32)
         hipLaunchKernelGGL((large_gs_update<MAXJ, float2, NORM>), dim3(nblk), dim3(kLT), 0, s, (const float2*)V, ldv,
-                           p, n, J, coef, part);
+                           p, n, J, coef, part, skip);
     else
         hipLaunchKernelGGL((large_gs_update<MAXJ, double2, NORM>), dim3(nblk), dim3(kLT), 0, s, (const double2*)V,
-                           ldv, p, n, J, coef, part);
+                           ldv, p, n, J, coef, part, skip);
 }
 
 template <int MAXJ>
 static void gs_update_j(const void* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef, int norm_only,
-                        double* part, int nblk, hipStream_t s, bool fp32) {
-    if (norm_only) gs_update_k<MAXJ, true>(V, ldv, p, n, J, coef, part, nblk, s, fp32);
-    else gs_update_k<MAXJ, false>(V, ldv, p, n, J, coef, part, nblk, s, fp32);
+                        double* part, int nblk, hipStream_t s, bool fp32, const int* skip) {
+    if (norm_only) gs_update_k<MAXJ, true>(V, ldv, p, n, J, coef, part, nblk, s, fp32, skip);
+    else gs_update_k<MAXJ, false>(V, ldv, p, n, J, coef, part, nblk, s, fp32, skip);
 }
 
 template <int HJ>
@@ -883,17 +937,17 @@ void launch_large_gs_dots(const void* V, int64_t ldv, const double2* p, int64_t 
 }
 
 void launch_large_gs_update(const void* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef,
-                            int norm_only, double* part, int nblk, hipStream_t s, bool fp32) {
+                            int norm_only, double* part, int nblk, hipStream_t s, bool fp32, const int* skip) {
     if (!norm_only && J >= 16) {
         if (J < 32) gs_update_h<16>(V, ldv, p, n, J, coef, part, nblk, s, fp32);
         else gs_update_h<kLargeMaxM / 2>(V, ldv, p, n, J, coef, part, nblk, s, fp32);
         return;
     }
-    if (J < 8) gs_update_j<8>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32);
-    else if (J < 16) gs_update_j<16>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32);
-    else if (J < 24) gs_update_j<24>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32);
-    else if (J < 32) gs_update_j<32>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32);
-    else gs_update_j<kLargeMaxM>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32);
+    if (J < 8) gs_update_j<8>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32, skip);
+    else if (J < 16) gs_update_j<16>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32, skip);
+    else if (J < 24) gs_update_j<24>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32, skip);
+    else if (J < 32) gs_update_j<32>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32, skip);
+    else gs_update_j<kLargeMaxM>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32, skip);
 }
 
 int launch_large_pade(const double2* hess, int ldh, int mx, double t, double2* F, int* status, hipStream_t s) {
