@@ -414,15 +414,22 @@ __global__ __launch_bounds__(kLT) void large_dots(const double2* __restrict__ V,
     }
 }
 
-// Sum the partials of large_dots.
-//   mode 0: first Gram-Schmidt pass of Arnoldi column jcol: h(l, jcol) = sums,
-//           coef = sums (rows 0..nrows-1 of part)
-//   mode 1: re-orthogonalisation pass: h(l, jcol) += sums, coef = sums
-//   mode 2: norm (row 0 of part): h(jcol+1, jcol) = sqrt(|p|^2) unless it is
-//           <= break_tol (then flags = {1, jcol+1} once); scal[0] = 1/h
-//           (0 after a breakdown), scal[1] = h
-//   mode 3: norm (row 0 of part) into scal[2] (the extra product's norm)
-// One block.
+// Sum the partials of the Gram-Schmidt passes and of large_dots for
+// Arnoldi column jcol (one block):
+//   mode 0: after the first dots pass (nrows = jcol + 2): h(l, jcol) = coef_l
+//           = sums for l <= jcol; row jcol + 1 is |p|^2, kept in scal[3];
+//   mode 1: after the first update-and-norm pass (row 0 = |p'|^2): when
+//           |p'| >= |p| / 2 (the first pass removed at most 3/4 of |p|^2:
+//           the Daniel-Gragg-Kaufman-Stewart test) the second pass would
+//           change p' by rounding only, so flags[2] = 1 (skip) and the norm
+//           entry is set as in mode 2; otherwise flags[2] = 0;
+//   mode 4: after the second dots pass (skipped with flags[2]):
+//           h(l, jcol) += sums, coef_l = sums for l <= jcol;
+//   mode 2: after the second update-and-norm pass (skipped with flags[2]):
+//           h(jcol+1, jcol) = sqrt(|p|^2) unless it is <= break_tol (then
+//           flags = {1, jcol+1} once); scal[0] = 1/h (0 after a
+//           breakdown), scal[1] = h;
+//   mode 3: norm (row 0 of part) into scal[2] (the extra product's norm).
 constexpr int kFinishThreads = 1024;
 
 // One wave per row: the lanes sum strided partials (independent loads), then
@@ -444,12 +451,8 @@ __device__ void finish_rows(const double* __restrict__ part, int nblk, int nrows
             b += __shfl_xor(b, mask, kWave);
         }
         if (lane != 0) continue;
-        if (mode == 2 || (mode == 1 && l == jcol + 1)) {
+        if (mode == 1 || mode == 2) {
             if (mode == 1) {
-                // |p|^2 after the first pass: keep p as it is when the pass
-                // removed at most 3/4 of |p|^2, |p1| >= |p0| / 2 (the
-                // Daniel-Gragg-Kaufman-Stewart test): the second pass would
-                // change p by rounding only and is skipped
                 const int skip = a >= 0.25 * scal[3];
                 flags[2] = skip;
                 if (!skip) continue;
@@ -465,50 +468,38 @@ __device__ void finish_rows(const double* __restrict__ part, int nblk, int nrows
             scal[1] = h;
         } else if (mode == 3) {
             scal[2] = sqrt(a);
-        } else if (mode == 0 && l == jcol + 1) {
-            scal[3] = a;   // |p|^2 before the first pass
+        } else if (l == jcol + 1) {
+            if (mode == 0) scal[3] = a;   // |p|^2 before the first pass
         } else {
             coef[l] = make_double2(a, b);
-            if (mode == 0) hess[l + (int64_t)ldh * jcol] = make_double2(a, b);
+            double2& hv = hess[l + (int64_t)ldh * jcol];
+            if (mode == 0) hv = make_double2(a, b);
+            else hv = make_double2(hv.x + a, hv.y + b);
         }
     }
 }
 
-// mode 0 (after the dots pass, nrows = J + 2): coef = hess column = V^H p,
-//   row J + 1 is |p|^2;
-// mode 1 (after the update-and-dots pass, nrows = J + 2): coef = the second
-//   projection, row J + 1 is |p|^2 after the first pass, which decides
-//   whether the second pass runs (flags[2] = 0) or is skipped (flags[2] = 1:
-//   the column's norm entry and scal are set here);
-// mode 2 (after the second pass; nothing when skipped): the second
-//   projection joins the Hessenberg column, row 0 is |p|^2;
-// mode 3: the extra product's norm.
 __global__ __launch_bounds__(kFinishThreads) void large_finish(const double* __restrict__ part, int nblk, int nrows,
                                                                int jcol, int mode, double2* __restrict__ hess, int ldh,
                                                                double2* __restrict__ coef, double* __restrict__ scal,
                                                                int* __restrict__ flags, double break_tol) {
-    if (mode == 2) {
-        if (flags[2]) return;
-        for (int l = threadIdx.x; l <= jcol; l += blockDim.x) {
-            double2& hv = hess[l + (int64_t)ldh * jcol];
-            hv = make_double2(hv.x + coef[l].x, hv.y + coef[l].y);
-        }
-    }
+    if ((mode == 2 || mode == 4) && flags[2]) return;   // no second pass
     finish_rows(part, nblk, nrows, jcol, mode, hess, ldh, coef, scal, flags, break_tol);
 }
 
-// Fused classical Gram-Schmidt passes (one read of each basis vector per pass):
-//   large_gs_dots:    part_l = V_l^H p                      (l <= J)
-//   large_gs_update:  p -= sum_l coef_l V_l, then part_l = V_l^H p (next pass)
-//                     or, with norm_only, part_0 = |p|^2
-// Each thread keeps the J+1 basis values of its element in registers between
-// the update and the dot products; partial sums are reduced per wave with
-// shuffles, then across the block's waves in LDS.  MAXJ bounds J + 1.
-// Rows 0..nrows-1 from (sr, si) and, with EXTRA, row nrows = (x, 0); red
-// holds (MAXJ + 1) rows per wave.
-template <int MAXJ, bool EXTRA = false>
+// Classical Gram-Schmidt passes (one read of each basis vector per pass):
+//   large_gs_dots:    part_l = V_l^H p (l <= J), part_{J+1} = |p|^2
+//   large_gs_update:  p -= sum_l coef_l V_l, part_0 = |p|^2
+// A pass launched with a skip flag returns at once while *skip is set (the
+// second pass of a column that did not need it).  Partial sums are reduced
+// per wave with shuffles, then across the block's waves in LDS.  MAXJ bounds
+// J + 1.
+
+// Rows 0..nrows-1 from (sr, si) and row nrows = (x, 0); red holds MAXJ + 1
+// rows per wave.
+template <int MAXJ>
 __device__ __forceinline__ void gs_block_reduce(const double (&sr)[MAXJ], const double (&si)[MAXJ], int nrows,
-                                                double* red, double* __restrict__ part, double x = 0.0) {
+                                                double* red, double* __restrict__ part, double x) {
     constexpr int RS = MAXJ + 1;   // row stride of red
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
 #pragma unroll
@@ -526,17 +517,14 @@ __device__ __forceinline__ void gs_block_reduce(const double (&sr)[MAXJ], const 
             }
         }
     }
-    if constexpr (EXTRA) {
 #pragma unroll
-        for (int mask = 32; mask >= 1; mask >>= 1) x += __shfl_xor(x, mask, kWave);
-        if (lane == 0) {
-            red[(wave * RS + nrows) * 2] = x;
-            red[(wave * RS + nrows) * 2 + 1] = 0.0;
-        }
+    for (int mask = 32; mask >= 1; mask >>= 1) x += __shfl_xor(x, mask, kWave);
+    if (lane == 0) {
+        red[(wave * RS + nrows) * 2] = x;
+        red[(wave * RS + nrows) * 2 + 1] = 0.0;
     }
     __syncthreads();
-    const int rows = nrows + (EXTRA ? 1 : 0);
-    for (int l = threadIdx.x; l < rows; l += kLT) {
+    for (int l = threadIdx.x; l <= nrows; l += kLT) {
         double a = 0.0, b = 0.0;
         for (int w = 0; w < kLT / kWave; ++w) { a += red[(w * RS + l) * 2]; b += red[(w * RS + l) * 2 + 1]; }
         part[((int64_t)l * gridDim.x + blockIdx.x) * 2] = a;
@@ -547,7 +535,8 @@ __device__ __forceinline__ void gs_block_reduce(const double (&sr)[MAXJ], const 
 template <int MAXJ, typename VT>
 __global__ __launch_bounds__(kLT) void large_gs_dots(const VT* __restrict__ V, int64_t ldv,
                                                      const double2* __restrict__ p, int64_t n, int J,
-                                                     double* __restrict__ part) {
+                                                     double* __restrict__ part, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     __shared__ double red[2 * (MAXJ + 1) * (kLT / kWave)];
     double sr[MAXJ], si[MAXJ];
     double nr = 0.0;   // |p|^2
@@ -564,24 +553,20 @@ __global__ __launch_bounds__(kLT) void large_gs_dots(const VT* __restrict__ V, i
                 si[l] = fma(a.x, b.y, fma(-a.y, b.x, si[l]));
             }
     }
-    gs_block_reduce<MAXJ, true>(sr, si, J + 1, red, part, nr);
+    gs_block_reduce<MAXJ>(sr, si, J + 1, red, part, nr);
 }
 
-template <int MAXJ, typename VT, bool NORM>
+template <int MAXJ, typename VT>
 __global__ __launch_bounds__(kLT) void large_gs_update(const VT* __restrict__ V, int64_t ldv,
                                                        double2* __restrict__ p, int64_t n, int J,
                                                        const double2* __restrict__ coef,
                                                        double* __restrict__ part, const int* __restrict__ skip) {
-    constexpr int NR = NORM ? 1 : MAXJ;   // partial sums per thread
-    if (NORM && skip && *skip) return;    // the second pass is not needed (decided by the finish kernel)
-    __shared__ double red[2 * (NR + 1) * (kLT / kWave)];
+    if (skip && *skip) return;
+    __shared__ double red[2 * (kLT / kWave)];
     __shared__ double2 c[MAXJ];
     for (int l = threadIdx.x; l <= J; l += kLT) c[l] = coef[l];
     __syncthreads();
-    double sr[NR], si[NR];
-    double nr = 0.0;   // |p|^2 after the update (the dot-product variant)
-#pragma unroll
-    for (int l = 0; l < NR; ++l) { sr[l] = 0.0; si[l] = 0.0; }
+    double nr = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kLT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kLT) {
         cd acc = mkcd(p[i].x, p[i].y);
 #pragma unroll
@@ -591,31 +576,26 @@ __global__ __launch_bounds__(kLT) void large_gs_update(const VT* __restrict__ V,
                 acc = cfms(mkcd(c[l].x, c[l].y), mkcd(v.x, v.y), acc);
             }
         p[i] = make_double2(acc.re, acc.im);
-        if constexpr (NORM) {
-            sr[0] = fma(acc.re, acc.re, fma(acc.im, acc.im, sr[0]));
-        } else {
-            nr = fma(acc.re, acc.re, fma(acc.im, acc.im, nr));
-            // the basis values again (from the caches: they were just read),
-            // rather than held in registers across the update
-            asm volatile("" ::: "memory");
-#pragma unroll
-            for (int l = 0; l < MAXJ; ++l)
-                if (l <= J) {
-                    const double2 v = vload(V + (int64_t)l * ldv + i);
-                    sr[l] = fma(v.x, acc.re, fma(v.y, acc.im, sr[l]));
-                    si[l] = fma(v.x, acc.im, fma(-v.y, acc.re, si[l]));
-                }
-        }
+        nr = fma(acc.re, acc.re, fma(acc.im, acc.im, nr));
     }
-    if constexpr (NORM) gs_block_reduce<NR>(sr, si, 1, red, part);
-    else gs_block_reduce<NR, true>(sr, si, J + 1, red, part, nr);
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int mask = 32; mask >= 1; mask >>= 1) nr += __shfl_xor(nr, mask, kWave);
+    if (lane == 0) red[wave] = nr;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0;
+        for (int w = 0; w < kLT / kWave; ++w) a += red[w];
+        part[(int64_t)blockIdx.x * 2] = a;
+        part[(int64_t)blockIdx.x * 2 + 1] = 0.0;
+    }
 }
 
-// Dot-product passes with the basis range split over lane halves: lanes q and
+// The dots pass with the basis range split over lane halves: lanes q and
 // q + 32 of a wave own the same element, the low half the vectors
 // l < L0 = (J + 2) / 2, the high half the rest, so each lane holds half the
 // partial sums (long bases keep two waves per SIMD).  Per wave 32 elements;
-// the two halves' partial products of the update meet by one shuffle.
+// |p|^2 is summed by the low half.
 template <int HJ>
 __device__ __forceinline__ void gs_half_reduce(const double (&sr)[HJ], const double (&si)[HJ], int l0, int nrows,
                                                double* red, double* __restrict__ part, double x) {
@@ -654,7 +634,8 @@ __device__ __forceinline__ void gs_half_reduce(const double (&sr)[HJ], const dou
 template <int HJ, typename VT>
 __global__ __launch_bounds__(kLT) void large_gs_dots_half(const VT* __restrict__ V, int64_t ldv,
                                                           const double2* __restrict__ p, int64_t n, int J,
-                                                          double* __restrict__ part) {
+                                                          double* __restrict__ part, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     constexpr int MAXJ = 2 * HJ;
     __shared__ double red[2 * (MAXJ + 1) * (kLT / kWave)];
     const int lane = threadIdx.x & (kWave - 1);
@@ -675,57 +656,6 @@ __global__ __launch_bounds__(kLT) void large_gs_dots_half(const VT* __restrict__
                 const double2 a = vload(V + (int64_t)(lb + q) * ldv + i);
                 sr[q] = fma(a.x, b.x, fma(a.y, b.y, sr[q]));
                 si[q] = fma(a.x, b.y, fma(-a.y, b.x, si[q]));
-            }
-    }
-    gs_half_reduce<HJ>(sr, si, l0, J + 1, red, part, nr);
-}
-
-template <int HJ, typename VT>
-__global__ __launch_bounds__(kLT) void large_gs_update_half(const VT* __restrict__ V, int64_t ldv,
-                                                            double2* __restrict__ p, int64_t n, int J,
-                                                            const double2* __restrict__ coef,
-                                                            double* __restrict__ part) {
-    constexpr int MAXJ = 2 * HJ;
-    __shared__ double red[2 * (MAXJ + 1) * (kLT / kWave)];
-    __shared__ double2 c[MAXJ];
-    for (int l = threadIdx.x; l <= J; l += kLT) c[l] = coef[l];
-    __syncthreads();
-    const int lane = threadIdx.x & (kWave - 1);
-    const int hi = lane >> 5;
-    const int l0 = (J + 2) / 2;
-    const int lb = hi ? l0 : 0, le = hi ? J + 1 : l0;
-    double sr[HJ], si[HJ];
-#pragma unroll
-    for (int q = 0; q < HJ; ++q) { sr[q] = 0.0; si[q] = 0.0; }
-    const int64_t w0 = ((int64_t)blockIdx.x * kLT + threadIdx.x - lane) / 2;
-    double nr = 0.0;   // |p|^2 after the update, counted by the low half
-    // every lane of a wave runs the same iterations (the shuffle needs both halves)
-    for (int64_t i0 = w0; i0 < n; i0 += (int64_t)gridDim.x * (kLT / 2)) {
-        const int64_t i = i0 + (lane & 31);
-        const bool live = i < n;
-        cd part_acc = mkcd(0.0, 0.0);
-        if (live)
-#pragma unroll
-            for (int q = 0; q < HJ; ++q)
-                if (lb + q < le) {
-                    const double2 v = vload(V + (int64_t)(lb + q) * ldv + i);
-                    part_acc = cfma(mkcd(c[lb + q].x, c[lb + q].y), mkcd(v.x, v.y), part_acc);
-                }
-        const double ore = __shfl_xor(part_acc.re, 32, kWave), oim = __shfl_xor(part_acc.im, 32, kWave);
-        if (!live) continue;
-        const double2 pv = p[i];
-        const cd acc = mkcd(pv.x - (part_acc.re + ore), pv.y - (part_acc.im + oim));
-        if (!hi) {
-            p[i] = make_double2(acc.re, acc.im);
-            nr = fma(acc.re, acc.re, fma(acc.im, acc.im, nr));
-        }
-        asm volatile("" ::: "memory");
-#pragma unroll
-        for (int q = 0; q < HJ; ++q)
-            if (lb + q < le) {
-                const double2 v = vload(V + (int64_t)(lb + q) * ldv + i);
-                sr[q] = fma(v.x, acc.re, fma(v.y, acc.im, sr[q]));
-                si[q] = fma(v.x, acc.im, fma(-v.y, acc.re, si[q]));
             }
     }
     gs_half_reduce<HJ>(sr, si, l0, J + 1, red, part, nr);
@@ -877,77 +807,52 @@ void launch_large_finish(const double* part, int nblk, int nrows, int jcol, int 
 // vectors (J + 1 <= 8, 16, 24, 32 or 40).
 template <int MAXJ>
 static void gs_dots_j(const void* V, int64_t ldv, const double2* p, int64_t n, int J, double* part, int nblk,
-                      hipStream_t s, bool fp32) {
+                      hipStream_t s, bool fp32, const int* skip) {
     if (fp32)
         hipLaunchKernelGGL((large_gs_dots<MAXJ, float2>), dim3(nblk), dim3(kLT), 0, s, (const float2*)V, ldv, p, n, J,
-                           part);
+                           part, skip);
     else
         hipLaunchKernelGGL((large_gs_dots<MAXJ, double2>), dim3(nblk), dim3(kLT), 0, s, (const double2*)V, ldv, p, n,
-                           J, part);
-}
-
-template <int MAXJ, bool NORM>
-static void gs_update_k(const void* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef, double* part,
-                        int nblk, hipStream_t s, bool fp32, const int* skip) {
-    if (fp32)
-        hipLaunchKernelGGL((large_gs_update<MAXJ, float2, NORM>), dim3(nblk), dim3(kLT), 0, s, (const float2*)V, ldv,
-                           p, n, J, coef, part, skip);
-    else
-        hipLaunchKernelGGL((large_gs_update<MAXJ, double2, NORM>), dim3(nblk), dim3(kLT), 0, s, (const double2*)V,
-                           ldv, p, n, J, coef, part, skip);
+                           J, part, skip);
 }
 
 template <int MAXJ>
-static void gs_update_j(const void* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef, int norm_only,
-                        double* part, int nblk, hipStream_t s, bool fp32, const int* skip) {
-    if (norm_only) gs_update_k<MAXJ, true>(V, ldv, p, n, J, coef, part, nblk, s, fp32, skip);
-    else gs_update_k<MAXJ, false>(V, ldv, p, n, J, coef, part, nblk, s, fp32, skip);
+static void gs_update_j(const void* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef, double* part,
+                        int nblk, hipStream_t s, bool fp32, const int* skip) {
+    if (fp32)
+        hipLaunchKernelGGL((large_gs_update<MAXJ, float2>), dim3(nblk), dim3(kLT), 0, s, (const float2*)V, ldv, p, n,
+                           J, coef, part, skip);
+    else
+        hipLaunchKernelGGL((large_gs_update<MAXJ, double2>), dim3(nblk), dim3(kLT), 0, s, (const double2*)V, ldv, p,
+                           n, J, coef, part, skip);
 }
 
 template <int HJ>
 static void gs_dots_h(const void* V, int64_t ldv, const double2* p, int64_t n, int J, double* part, int nblk,
-                      hipStream_t s, bool fp32) {
+                      hipStream_t s, bool fp32, const int* skip) {
     if (fp32)
         hipLaunchKernelGGL((large_gs_dots_half<HJ, float2>), dim3(nblk), dim3(kLT), 0, s, (const float2*)V, ldv, p, n,
-                           J, part);
+                           J, part, skip);
     else
         hipLaunchKernelGGL((large_gs_dots_half<HJ, double2>), dim3(nblk), dim3(kLT), 0, s, (const double2*)V, ldv, p,
-                           n, J, part);
-}
-
-template <int HJ>
-static void gs_update_h(const void* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef, double* part,
-                        int nblk, hipStream_t s, bool fp32) {
-    if (fp32)
-        hipLaunchKernelGGL((large_gs_update_half<HJ, float2>), dim3(nblk), dim3(kLT), 0, s, (const float2*)V, ldv, p,
-                           n, J, coef, part);
-    else
-        hipLaunchKernelGGL((large_gs_update_half<HJ, double2>), dim3(nblk), dim3(kLT), 0, s, (const double2*)V, ldv,
-                           p, n, J, coef, part);
+                           n, J, part, skip);
 }
 
 void launch_large_gs_dots(const void* V, int64_t ldv, const double2* p, int64_t n, int J, double* part, int nblk,
-                          hipStream_t s, bool fp32) {
-    if (J >= 24) { gs_dots_h<kLargeMaxM / 2>(V, ldv, p, n, J, part, nblk, s, fp32); return; }
-    if (J < 8) gs_dots_j<8>(V, ldv, p, n, J, part, nblk, s, fp32);
-    else if (J < 16) gs_dots_j<16>(V, ldv, p, n, J, part, nblk, s, fp32);
-    else if (J < 24) gs_dots_j<24>(V, ldv, p, n, J, part, nblk, s, fp32);
-    else if (J < 32) gs_dots_j<32>(V, ldv, p, n, J, part, nblk, s, fp32);
-    else gs_dots_j<kLargeMaxM>(V, ldv, p, n, J, part, nblk, s, fp32);
+                          hipStream_t s, bool fp32, const int* skip) {
+    if (J >= 24) { gs_dots_h<kLargeMaxM / 2>(V, ldv, p, n, J, part, nblk, s, fp32, skip); return; }
+    if (J < 8) gs_dots_j<8>(V, ldv, p, n, J, part, nblk, s, fp32, skip);
+    else if (J < 16) gs_dots_j<16>(V, ldv, p, n, J, part, nblk, s, fp32, skip);
+    else gs_dots_j<24>(V, ldv, p, n, J, part, nblk, s, fp32, skip);
 }
 
 void launch_large_gs_update(const void* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef,
-                            int norm_only, double* part, int nblk, hipStream_t s, bool fp32, const int* skip) {
-    if (!norm_only && J >= 16) {
-        if (J < 32) gs_update_h<16>(V, ldv, p, n, J, coef, part, nblk, s, fp32);
-        else gs_update_h<kLargeMaxM / 2>(V, ldv, p, n, J, coef, part, nblk, s, fp32);
-        return;
-    }
-    if (J < 8) gs_update_j<8>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32, skip);
-    else if (J < 16) gs_update_j<16>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32, skip);
-    else if (J < 24) gs_update_j<24>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32, skip);
-    else if (J < 32) gs_update_j<32>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32, skip);
-    else gs_update_j<kLargeMaxM>(V, ldv, p, n, J, coef, norm_only, part, nblk, s, fp32, skip);
+                            double* part, int nblk, hipStream_t s, bool fp32, const int* skip) {
+    if (J < 8) gs_update_j<8>(V, ldv, p, n, J, coef, part, nblk, s, fp32, skip);
+    else if (J < 16) gs_update_j<16>(V, ldv, p, n, J, coef, part, nblk, s, fp32, skip);
+    else if (J < 24) gs_update_j<24>(V, ldv, p, n, J, coef, part, nblk, s, fp32, skip);
+    else if (J < 32) gs_update_j<32>(V, ldv, p, n, J, coef, part, nblk, s, fp32, skip);
+    else gs_update_j<kLargeMaxM>(V, ldv, p, n, J, coef, part, nblk, s, fp32, skip);
 }
 
 int launch_large_pade(const double2* hess, int ldh, int mx, double t, double2* F, int* status, hipStream_t s) {

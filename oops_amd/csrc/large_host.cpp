@@ -773,21 +773,22 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
         for (int j = 0; j < m; ++j) {
             double2* p = bufs[j & 1];
             matvec(R, x, scal, p, vcol(R, j), arnoldi_mode(R));
-            // classical Gram-Schmidt with a second pass when the first one
-            // cancelled much of p (one read of V per pass): dots and |p|;
-            // update + dots and |p|; the finish kernel decides whether the
-            // update + norm pass runs (its kernels return at once otherwise)
-            launch_large_gs_dots(R.V.p, n, p, n, j, (double*)R.part.p, R.nblk, R.s, R.fp32);
-            launch_large_finish((double*)R.part.p, R.nblk, j + 2, j, 0, hess, R.ldh, (double2*)R.coef.p, scal,
-                                (int*)R.flags.p, break_tol, R.s);
-            launch_large_gs_update(R.V.p, n, p, n, j, (double2*)R.coef.p, 0, (double*)R.part.p, R.nblk, R.s,
-                                   R.fp32);
-            launch_large_finish((double*)R.part.p, R.nblk, j + 2, j, 1, hess, R.ldh, (double2*)R.coef.p, scal,
-                                (int*)R.flags.p, break_tol, R.s);
-            launch_large_gs_update(R.V.p, n, p, n, j, (double2*)R.coef.p, 1, (double*)R.part.p, R.nblk, R.s,
-                                   R.fp32, (const int*)R.flags.p + 2);
-            launch_large_finish((double*)R.part.p, R.nblk, 1, j, 2, hess, R.ldh, (double2*)R.coef.p, scal,
-                                (int*)R.flags.p, break_tol, R.s);
+            // classical Gram-Schmidt, one read of V per pass: dots and |p|,
+            // update and |p'|; then, only when the first update cancelled
+            // more than 3/4 of |p|^2 (decided on the device by the finish
+            // kernel; the skipped kernels return at once), dots and update
+            // again
+            const int* skip = (const int*)R.flags.p + 2;
+            double* part = (double*)R.part.p;
+            double2* coef = (double2*)R.coef.p;
+            launch_large_gs_dots(R.V.p, n, p, n, j, part, R.nblk, R.s, R.fp32);
+            launch_large_finish(part, R.nblk, j + 2, j, 0, hess, R.ldh, coef, scal, (int*)R.flags.p, break_tol, R.s);
+            launch_large_gs_update(R.V.p, n, p, n, j, coef, part, R.nblk, R.s, R.fp32);
+            launch_large_finish(part, R.nblk, 1, j, 1, hess, R.ldh, coef, scal, (int*)R.flags.p, break_tol, R.s);
+            launch_large_gs_dots(R.V.p, n, p, n, j, part, R.nblk, R.s, R.fp32, skip);
+            launch_large_finish(part, R.nblk, j + 2, j, 4, hess, R.ldh, coef, scal, (int*)R.flags.p, break_tol, R.s);
+            launch_large_gs_update(R.V.p, n, p, n, j, coef, part, R.nblk, R.s, R.fp32, skip);
+            launch_large_finish(part, R.nblk, 1, j, 2, hess, R.ldh, coef, scal, (int*)R.flags.p, break_tol, R.s);
             x = p;
         }
         // extra product for the error estimate: V[m] = p / h(m, m-1), p' = -iH V[m]

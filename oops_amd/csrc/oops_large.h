@@ -71,15 +71,14 @@ void launch_large_dots(const double2* V, int64_t ldv, const double2* p, int64_t 
                        int nblk, hipStream_t s);
 void launch_large_finish(const double* part, int nblk, int nrows, int jcol, int mode, double2* hess, int ldh,
                          double2* coef, double* scal, int* flags, double break_tol, hipStream_t s);
-// Fused Gram-Schmidt passes: part_l = V_l^H p (l <= J < kLargeMaxM) and
-// part_{J+1} = |p|^2; p -= sum_l coef_l V_l followed by part_l = V_l^H p and
-// part_{J+1} = |p|^2 (norm_only = 0), or by part_0 = |p|^2 (norm_only = 1;
-// nothing at all while *skip is set).  part rows as for large_dots.
+// Classical Gram-Schmidt passes: part_l = V_l^H p (l <= J < kLargeMaxM) and
+// part_{J+1} = |p|^2; p -= sum_l coef_l V_l and part_0 = |p|^2.  Launched
+// with a skip flag, a pass does nothing while *skip is set.  part rows as for
+// large_dots.
 void launch_large_gs_dots(const void* V, int64_t ldv, const double2* p, int64_t n, int J, double* part, int nblk,
-                          hipStream_t s, bool fp32 = false);
+                          hipStream_t s, bool fp32 = false, const int* skip = nullptr);
 void launch_large_gs_update(const void* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef,
-                            int norm_only, double* part, int nblk, hipStream_t s, bool fp32 = false,
-                            const int* skip = nullptr);
+                            double* part, int nblk, hipStream_t s, bool fp32 = false, const int* skip = nullptr);
 int launch_large_pade(const double2* hess, int ldh, int mx, double t, double2* F, int* status, hipStream_t s);
 void launch_large_combine(const void* V, int64_t ldv, int mx, const double2* F, double beta, double2* w, int64_t n,
                           double* part, int grid, hipStream_t s, bool fp32 = false);
