@@ -441,9 +441,22 @@ __device__ void finish_rows(const double* __restrict__ part, int nblk, int nrows
     const int nw = blockDim.x / kWave;
     for (int l = wave; l < nrows; l += nw) {
         double a = 0.0, b = 0.0;
-        for (int k = lane; k < nblk; k += kWave) {
-            a += part[((int64_t)l * nblk + k) * 2];
-            b += part[((int64_t)l * nblk + k) * 2 + 1];
+        // eight loads in flight per lane (the sums keep their order)
+        const double2* pr = reinterpret_cast<const double2*>(part) + (int64_t)l * nblk;
+        int k = lane;
+        for (; k + 7 * kWave < nblk; k += 8 * kWave) {
+            double2 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = pr[k + u * kWave];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                a += v[u].x;
+                b += v[u].y;
+            }
+        }
+        for (; k < nblk; k += kWave) {
+            a += pr[k].x;
+            b += pr[k].y;
         }
 #pragma unroll
         for (int mask = 32; mask >= 1; mask >>= 1) {
@@ -493,7 +506,10 @@ __global__ __launch_bounds__(kFinishThreads) void large_finish(const double* __r
 // A pass launched with a skip flag returns at once while *skip is set (the
 // second pass of a column that did not need it).  Partial sums are reduced
 // per wave with shuffles, then across the block's waves in LDS.  MAXJ bounds
-// J + 1.
+// J + 1.  Every thread issues all MAXJ basis loads of an element without a
+// branch (indices past J repeat column J, a cache hit; their partial sums
+// are not reduced, their update coefficients are zero), so the loads are in
+// flight together instead of one at a time behind a per-column branch.
 
 // Rows 0..nrows-1 from (sr, si) and row nrows = (x, 0); red holds MAXJ + 1
 // rows per wave.
@@ -546,12 +562,11 @@ __global__ __launch_bounds__(kLT) void large_gs_dots(const VT* __restrict__ V, i
         const double2 b = p[i];
         nr = fma(b.x, b.x, fma(b.y, b.y, nr));
 #pragma unroll
-        for (int l = 0; l < MAXJ; ++l)
-            if (l <= J) {
-                const double2 a = vload(V + (int64_t)l * ldv + i);
-                sr[l] = fma(a.x, b.x, fma(a.y, b.y, sr[l]));
-                si[l] = fma(a.x, b.y, fma(-a.y, b.x, si[l]));
-            }
+        for (int l = 0; l < MAXJ; ++l) {
+            const double2 a = vload(V + (int64_t)min(l, J) * ldv + i);
+            sr[l] = fma(a.x, b.x, fma(a.y, b.y, sr[l]));
+            si[l] = fma(a.x, b.y, fma(-a.y, b.x, si[l]));
+        }
     }
     gs_block_reduce<MAXJ>(sr, si, J + 1, red, part, nr);
 }
@@ -564,17 +579,16 @@ __global__ __launch_bounds__(kLT) void large_gs_update(const VT* __restrict__ V,
     if (skip && *skip) return;
     __shared__ double red[2 * (kLT / kWave)];
     __shared__ double2 c[MAXJ];
-    for (int l = threadIdx.x; l <= J; l += kLT) c[l] = coef[l];
+    for (int l = threadIdx.x; l < MAXJ; l += kLT) c[l] = l <= J ? coef[l] : make_double2(0.0, 0.0);
     __syncthreads();
     double nr = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * kLT + threadIdx.x; i < n; i += (int64_t)gridDim.x * kLT) {
         cd acc = mkcd(p[i].x, p[i].y);
 #pragma unroll
-        for (int l = 0; l < MAXJ; ++l)
-            if (l <= J) {
-                const double2 v = vload(V + (int64_t)l * ldv + i);
-                acc = cfms(mkcd(c[l].x, c[l].y), mkcd(v.x, v.y), acc);
-            }
+        for (int l = 0; l < MAXJ; ++l) {
+            const double2 v = vload(V + (int64_t)min(l, J) * ldv + i);
+            acc = cfms(mkcd(c[l].x, c[l].y), mkcd(v.x, v.y), acc);
+        }
         p[i] = make_double2(acc.re, acc.im);
         nr = fma(acc.re, acc.re, fma(acc.im, acc.im, nr));
     }
@@ -591,74 +605,75 @@ __global__ __launch_bounds__(kLT) void large_gs_update(const VT* __restrict__ V,
     }
 }
 
-// The dots pass with the basis range split over lane halves: lanes q and
-// q + 32 of a wave own the same element, the low half the vectors
-// l < L0 = (J + 2) / 2, the high half the rest, so each lane holds half the
-// partial sums (long bases keep two waves per SIMD).  Per wave 32 elements;
-// |p|^2 is summed by the low half.
-template <int HJ>
-__device__ __forceinline__ void gs_half_reduce(const double (&sr)[HJ], const double (&si)[HJ], int l0, int nrows,
-                                               double* red, double* __restrict__ part, double x) {
+// The dots pass with the basis range split over S lane groups: the S lanes
+// q, q + 64/S, .. of a wave own the same element, group g the vectors
+// l in [g L, (g + 1) L), L = ceil((J + 1) / S), so each lane holds L partial
+// sums and loads (long bases keep four or five waves per SIMD with all loads
+// of a lane in flight).  Per wave 64/S elements; |p|^2 is summed by group 0.
+template <int S, int HJ>
+__device__ __forceinline__ void gs_split_reduce(const double (&sr)[HJ], const double (&si)[HJ], int L, int nrows,
+                                                double* red, double* __restrict__ part, double x) {
+    constexpr int GL = kWave / S;      // lanes per group
+    constexpr int RS = S * HJ + 1;     // row stride of red (the last row: x, row nrows of part)
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const int hi = lane >> 5;
-    constexpr int MAXJ = 2 * HJ + 1;   // row stride of red (the last row: x, row nrows of part)
+    const int g = lane / GL;
 #pragma unroll
     for (int mask = 32; mask >= 1; mask >>= 1) x += __shfl_xor(x, mask, kWave);
     if (lane == 0) {
-        red[(wave * MAXJ + nrows) * 2] = x;
-        red[(wave * MAXJ + nrows) * 2 + 1] = 0.0;
+        red[(wave * RS + nrows) * 2] = x;
+        red[(wave * RS + nrows) * 2 + 1] = 0.0;
     }
 #pragma unroll
     for (int q = 0; q < HJ; ++q) {
         double a = sr[q], b = si[q];
 #pragma unroll
-        for (int mask = 16; mask >= 1; mask >>= 1) {
+        for (int mask = GL / 2; mask >= 1; mask >>= 1) {
             a += __shfl_xor(a, mask, kWave);
             b += __shfl_xor(b, mask, kWave);
         }
-        const int l = hi ? l0 + q : q;
-        if ((lane & 31) == 0 && l < nrows && (hi || q < l0)) {
-            red[(wave * MAXJ + l) * 2] = a;
-            red[(wave * MAXJ + l) * 2 + 1] = b;
+        const int l = g * L + q;
+        if ((lane & (GL - 1)) == 0 && q < L && l < nrows) {
+            red[(wave * RS + l) * 2] = a;
+            red[(wave * RS + l) * 2 + 1] = b;
         }
     }
     __syncthreads();
     for (int l = threadIdx.x; l <= nrows; l += kLT) {
         double a = 0.0, b = 0.0;
-        for (int w = 0; w < kLT / kWave; ++w) { a += red[(w * MAXJ + l) * 2]; b += red[(w * MAXJ + l) * 2 + 1]; }
+        for (int w = 0; w < kLT / kWave; ++w) { a += red[(w * RS + l) * 2]; b += red[(w * RS + l) * 2 + 1]; }
         part[((int64_t)l * gridDim.x + blockIdx.x) * 2] = a;
         part[((int64_t)l * gridDim.x + blockIdx.x) * 2 + 1] = b;
     }
 }
 
-template <int HJ, typename VT>
-__global__ __launch_bounds__(kLT) void large_gs_dots_half(const VT* __restrict__ V, int64_t ldv,
-                                                          const double2* __restrict__ p, int64_t n, int J,
-                                                          double* __restrict__ part, const int* __restrict__ skip) {
+template <int S, int HJ, typename VT>
+__global__ __launch_bounds__(kLT) void large_gs_dots_split(const VT* __restrict__ V, int64_t ldv,
+                                                           const double2* __restrict__ p, int64_t n, int J,
+                                                           double* __restrict__ part, const int* __restrict__ skip) {
     if (skip && *skip) return;
-    constexpr int MAXJ = 2 * HJ;
-    __shared__ double red[2 * (MAXJ + 1) * (kLT / kWave)];
+    constexpr int GL = kWave / S;
+    __shared__ double red[2 * (S * HJ + 1) * (kLT / kWave)];
     const int lane = threadIdx.x & (kWave - 1);
-    const int hi = lane >> 5;
-    const int l0 = (J + 2) / 2;
-    const int lb = hi ? l0 : 0, le = hi ? J + 1 : l0;
+    const int g = lane / GL;
+    const int L = (J + S) / S;                       // ceil((J + 1) / S) <= HJ
+    const int lb = g * L, le = min(lb + L, J + 1);   // this group's vectors (le <= lb: none)
+    const int lmax = max(le - 1, 0);
     double sr[HJ], si[HJ];
 #pragma unroll
     for (int q = 0; q < HJ; ++q) { sr[q] = 0.0; si[q] = 0.0; }
-    const int64_t w0 = ((int64_t)blockIdx.x * kLT + threadIdx.x - lane) / 2;   // this wave's first element
-    double nr = 0.0;   // |p|^2, counted by the low half (each element once)
-    for (int64_t i = w0 + (lane & 31); i < n; i += (int64_t)gridDim.x * (kLT / 2)) {
+    const int64_t w0 = ((int64_t)blockIdx.x * kLT + threadIdx.x - lane) / S;   // this wave's first element
+    double nr = 0.0;   // |p|^2, counted by group 0 (each element once)
+    for (int64_t i = w0 + (lane & (GL - 1)); i < n; i += (int64_t)gridDim.x * (kLT / S)) {
         const double2 b = p[i];
-        if (!hi) nr = fma(b.x, b.x, fma(b.y, b.y, nr));
+        if (g == 0) nr = fma(b.x, b.x, fma(b.y, b.y, nr));
 #pragma unroll
-        for (int q = 0; q < HJ; ++q)
-            if (lb + q < le) {
-                const double2 a = vload(V + (int64_t)(lb + q) * ldv + i);
-                sr[q] = fma(a.x, b.x, fma(a.y, b.y, sr[q]));
-                si[q] = fma(a.x, b.y, fma(-a.y, b.x, si[q]));
-            }
+        for (int q = 0; q < HJ; ++q) {
+            const double2 a = vload(V + (int64_t)min(lb + q, lmax) * ldv + i);
+            sr[q] = fma(a.x, b.x, fma(a.y, b.y, sr[q]));
+            si[q] = fma(a.x, b.y, fma(-a.y, b.x, si[q]));
+        }
     }
-    gs_half_reduce<HJ>(sr, si, l0, J + 1, red, part, nr);
+    gs_split_reduce<S, HJ>(sr, si, L, J + 1, red, part, nr);
 }
 
 // F = exp(t H(0:mx,0:mx)) e1 for the augmented Hessenberg matrix (Pade(6),
@@ -827,23 +842,24 @@ static void gs_update_j(const void* V, int64_t ldv, double2* p, int64_t n, int J
                            n, J, coef, part, skip);
 }
 
-template <int HJ>
-static void gs_dots_h(const void* V, int64_t ldv, const double2* p, int64_t n, int J, double* part, int nblk,
+template <int S, int HJ>
+static void gs_dots_s(const void* V, int64_t ldv, const double2* p, int64_t n, int J, double* part, int nblk,
                       hipStream_t s, bool fp32, const int* skip) {
     if (fp32)
-        hipLaunchKernelGGL((large_gs_dots_half<HJ, float2>), dim3(nblk), dim3(kLT), 0, s, (const float2*)V, ldv, p, n,
-                           J, part, skip);
+        hipLaunchKernelGGL((large_gs_dots_split<S, HJ, float2>), dim3(nblk), dim3(kLT), 0, s, (const float2*)V, ldv,
+                           p, n, J, part, skip);
     else
-        hipLaunchKernelGGL((large_gs_dots_half<HJ, double2>), dim3(nblk), dim3(kLT), 0, s, (const double2*)V, ldv, p,
-                           n, J, part, skip);
+        hipLaunchKernelGGL((large_gs_dots_split<S, HJ, double2>), dim3(nblk), dim3(kLT), 0, s, (const double2*)V, ldv,
+                           p, n, J, part, skip);
 }
 
 void launch_large_gs_dots(const void* V, int64_t ldv, const double2* p, int64_t n, int J, double* part, int nblk,
                           hipStream_t s, bool fp32, const int* skip) {
-    if (J >= 24) { gs_dots_h<kLargeMaxM / 2>(V, ldv, p, n, J, part, nblk, s, fp32, skip); return; }
+    static_assert(kLargeMaxM <= 40, "the split dots pass holds at most 4 x 10 vectors");
     if (J < 8) gs_dots_j<8>(V, ldv, p, n, J, part, nblk, s, fp32, skip);
     else if (J < 16) gs_dots_j<16>(V, ldv, p, n, J, part, nblk, s, fp32, skip);
-    else gs_dots_j<24>(V, ldv, p, n, J, part, nblk, s, fp32, skip);
+    else if (J < 24) gs_dots_s<2, 12>(V, ldv, p, n, J, part, nblk, s, fp32, skip);
+    else gs_dots_s<4, 10>(V, ldv, p, n, J, part, nblk, s, fp32, skip);
 }
 
 void launch_large_gs_update(const void* V, int64_t ldv, double2* p, int64_t n, int J, const double2* coef,

@@ -58,6 +58,7 @@ struct LargeRun {
     // basis stored in complex FP32 (the FP32 variant; Kronecker operators only)
     bool fp32 = false;
     int64_t n = 0;
+    int64_t ldv = 0;   // basis column stride (elements)
     int m = 0, ldh = 0, grid = 0, nblk = 0;
     DevBuf V, p0, p1, w, v0, part, hess, coef, scal, flags, F, status, wpart;
     DevBuf wk, wkpart;   // a state at an output time inside a step (interior-output mode)
@@ -273,7 +274,7 @@ void matvec(LargeRun& R, const double2* x, const double* xs, double2* y, void* x
 
 // Basis column j.
 void* vcol(LargeRun& R, int j) {
-    return R.fp32 ? (void*)((float2*)R.V.p + (size_t)j * R.n) : (void*)((double2*)R.V.p + (size_t)j * R.n);
+    return R.fp32 ? (void*)((float2*)R.V.p + (size_t)j * R.ldv) : (void*)((double2*)R.V.p + (size_t)j * R.ldv);
 }
 int arnoldi_mode(const LargeRun& R) { return R.csr ? 0 : 1; }
 
@@ -576,9 +577,17 @@ int setup_buffers(oops_ctx* ctx, int m, LargeRun& R) {
     R.m = m;
     R.ldh = m + 2;
     R.grid = large_grid(R.n, ctx->num_cu);
-    R.nblk = std::min(R.grid, 512);
+    // blocks of the Gram-Schmidt and dot-product passes: the full grid (up to
+    // eight per CU), so the long dots passes keep enough loads in flight
+    R.nblk = R.grid;
     const size_t vb = sizeof(double2) * (size_t)R.n;
-    rc = rc ? rc : alloc(ctx, R.V, (R.fp32 ? vb / 2 : vb) * (m + 1));
+    // basis columns 4352 B (17 x 256 B) apart beyond a power of two: columns
+    // of 2^k elements would start at addresses equal modulo every power of two
+    // up to their size, so the loads of one element across the basis would
+    // meet in the same memory channels
+    const size_t esz = R.fp32 ? sizeof(float2) : sizeof(double2);
+    R.ldv = R.n + (R.n >= (1 << 16) ? (int64_t)(4352 / esz) : 0);
+    rc = rc ? rc : alloc(ctx, R.V, esz * (size_t)R.ldv * (m + 1));
     rc = rc ? rc : alloc(ctx, R.p0, vb);
     rc = rc ? rc : alloc(ctx, R.p1, vb);
     rc = rc ? rc : alloc(ctx, R.w, vb);
@@ -781,13 +790,13 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
             const int* skip = (const int*)R.flags.p + 2;
             double* part = (double*)R.part.p;
             double2* coef = (double2*)R.coef.p;
-            launch_large_gs_dots(R.V.p, n, p, n, j, part, R.nblk, R.s, R.fp32);
+            launch_large_gs_dots(R.V.p, R.ldv, p, n, j, part, R.nblk, R.s, R.fp32);
             launch_large_finish(part, R.nblk, j + 2, j, 0, hess, R.ldh, coef, scal, (int*)R.flags.p, break_tol, R.s);
-            launch_large_gs_update(R.V.p, n, p, n, j, coef, part, R.nblk, R.s, R.fp32);
+            launch_large_gs_update(R.V.p, R.ldv, p, n, j, coef, part, R.nblk, R.s, R.fp32);
             launch_large_finish(part, R.nblk, 1, j, 1, hess, R.ldh, coef, scal, (int*)R.flags.p, break_tol, R.s);
-            launch_large_gs_dots(R.V.p, n, p, n, j, part, R.nblk, R.s, R.fp32, skip);
+            launch_large_gs_dots(R.V.p, R.ldv, p, n, j, part, R.nblk, R.s, R.fp32, skip);
             launch_large_finish(part, R.nblk, j + 2, j, 4, hess, R.ldh, coef, scal, (int*)R.flags.p, break_tol, R.s);
-            launch_large_gs_update(R.V.p, n, p, n, j, coef, part, R.nblk, R.s, R.fp32, skip);
+            launch_large_gs_update(R.V.p, R.ldv, p, n, j, coef, part, R.nblk, R.s, R.fp32, skip);
             launch_large_finish(part, R.nblk, 1, j, 2, hess, R.ldh, coef, scal, (int*)R.flags.p, break_tol, R.s);
             x = p;
         }
@@ -865,7 +874,7 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
                 break;
             }
             mx = mb + std::max(0, k1 - 1);
-            launch_large_combine(R.V.p, n, mx, (double2*)R.F.p, beta, (double2*)R.w.p, n, (double*)R.wpart.p, R.grid,
+            launch_large_combine(R.V.p, R.ldv, mx, (double2*)R.F.p, beta, (double2*)R.w.p, n, (double*)R.wpart.p, R.grid,
                                  R.s, R.fp32);
             hipEventRecord(e1, R.s);
             std::vector<double> wp(R.grid);
@@ -886,7 +895,7 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
                     if (launch_large_pade(hess, R.ldh, mb + k1, ta[tt] - t_now, (double2*)R.F.p, (int*)R.status.p,
                                           R.s) != 0)
                         return set_err(R.ctx, OOPS_ERR_HIP, "Pade kernel launch");
-                    launch_large_combine(R.V.p, n, mx, (double2*)R.F.p, beta, (double2*)R.wk.p, n,
+                    launch_large_combine(R.V.p, R.ldv, mx, (double2*)R.F.p, beta, (double2*)R.wk.p, n,
                                          (double*)R.wkpart.p, R.grid, R.s, R.fp32);
                     int st = 0;
                     rc = hip_check(R.ctx, hipMemcpyAsync(&st, R.status.p, sizeof(int), hipMemcpyDeviceToHost, R.s),
