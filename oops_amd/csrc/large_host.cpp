@@ -774,7 +774,7 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
         }
         return e;
     };
-    // one Arnoldi cycle: m products with CGS2 and the extra product of the
+    // one Arnoldi cycle: m products with their Gram-Schmidt passes and the extra product of the
     // error estimate, all on R.s
     auto arnoldi_launches = [&]() {
         double2* bufs[2] = {(double2*)R.p0.p, (double2*)R.p1.p};
