@@ -642,6 +642,40 @@ class Clusters:
                None, "oops_clusters_enumerate")
         self.h = h
 
+    @classmethod
+    def union(cls, baths, cutoff, max_order):
+        """The clusters of several baths in one enumeration: their spins
+        concatenated (bath b's spin i is spin first[b] + i), no connection
+        between baths (cDepthFirstPathTracing on the block-diagonal connection
+        graph, SpinClusterAlgorithm.h:68-82), so one batch covers every bath.
+        Within each order the clusters of bath 0 come first, in their own
+        set order.  Returns (Clusters, first)."""
+        baths = [np.ascontiguousarray(X, float).reshape(-1, 3) for X in baths]
+        first = np.cumsum([0] + [len(X) for X in baths])[:-1]
+        offsets, neighbors = [0], []
+        for f, X in zip(first, baths):
+            d = X[:, None, :] - X[None, :, :]
+            # the distance as oops_clusters_enumerate forms it
+            dist = np.sqrt((d[..., 0] * d[..., 0] + d[..., 2] * d[..., 2]) + d[..., 1] * d[..., 1])
+            close = dist <= cutoff
+            np.fill_diagonal(close, False)
+            for i in range(len(X)):
+                nb = np.nonzero(close[i])[0] + f
+                neighbors.extend(nb.tolist())
+                offsets.append(len(neighbors))
+        coords = np.concatenate(baths) if baths else np.zeros((0, 3))
+        off = np.asarray(offsets, np.int64)
+        nbr = np.asarray(neighbors if neighbors else [0], np.int32)
+        h = ctypes.c_void_p()
+        _check(lib().oops_clusters_enumerate_graph(ctypes.c_int32(len(coords)), _ptr(off), _ptr(nbr),
+                                                   ctypes.c_int32(max_order), ctypes.byref(h)),
+               None, "oops_clusters_enumerate_graph")
+        obj = cls.__new__(cls)
+        obj.coords = np.ascontiguousarray(coords)
+        obj.max_order = max_order
+        obj.h = h
+        return obj, first
+
     def count(self, order):
         c = ctypes.c_int64()
         _check(lib().oops_clusters_count(self.h, ctypes.c_int32(order), ctypes.byref(c)), None, "count")

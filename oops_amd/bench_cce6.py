@@ -4,9 +4,11 @@ Workload: diamond 13C bath r = 4 nm, p = 0.011, cutoff 5 A, clusters of 1-6
 spins, B = 0.05 T along [111], 401 points on 0-5 ms, m = 30, tol 1e-3.
 Mixed precision: n <= 256 on the batched kernels (flip-mask Lanczos for
 n = 256), n = 1024 and 4096 on the batched medium kernel (1024 threads per
-work unit, flip-mask product).  One batch per bath seed (seeds 0 .. S-1),
-every batch on its own stream, launched together, so the n = 4096 work units
-of several baths share the device.
+work unit, flip-mask product).  The baths of seeds 0 .. S-1 are enumerated
+together (Clusters.union: their spins concatenated, no connection between
+baths), so one batch holds every cluster and each bucket is one launch whose
+work units fill the device (with one batch per bath, the launches share
+the process's four hardware queues and run four at a time).
 
 Reported: cluster expv/s over all seeds (device time of the concurrent
 launches), per-bucket kernels and times, the CCE-6 coherence of seed 0, and
@@ -39,36 +41,37 @@ def main(argv=None):
     ctx = O.Context(0)
     par = O.cce_params(B=np.ones(3) * 0.05 / np.sqrt(3.0))
     tl = np.linspace(0.0, 5e-3, 401)
-    baths, batches = [], []
     t0 = time.perf_counter()
-    for seed in range(a.seeds):
-        X = O.bath_diamond(3.567, 40.0, 0.011, seed=seed)
-        C = O.Clusters(X, 5.0, 6)
-        baths.append((X, C))
-        batches.append(C.echo_batch(ctx, par, 1, tl / 2.0, m=30, tol=1e-3, precision="mixed"))
+    Xs = [O.bath_diamond(3.567, 40.0, 0.011, seed=seed) for seed in range(a.seeds)]
+    X0 = Xs[0]
+    C0 = O.Clusters(X0, 5.0, 6)
+    CU, first = O.Clusters.union(Xs, 5.0, 6)
+    counts0 = C0.counts()
+    for o in range(6):   # bath 0's clusters lead every order of the union, in their own order
+        assert np.array_equal(CU.get(o)[:counts0[o]], C0.get(o))
+    batch = CU.echo_batch(ctx, par, 1, tl / 2.0, m=30, tol=1e-3, precision="mixed")
     setup = time.perf_counter() - t0
-    counts = [C.counts() for _, C in baths]
-    nclu = int(sum(sum(c) for c in counts))
+    cu = CU.counts()
+    nclu = int(sum(cu))
     print(f"# {a.seeds} baths, {nclu} clusters, setup {setup:.1f}s", file=sys.stderr, flush=True)
     done = threading.Event()
     t1 = time.perf_counter()
-    threading.Thread(target=_alive, args=("mixed batches", t1, done), daemon=True).start()
-    for b in batches:
-        b.launch()
-    for b in batches:
-        b.wait()
+    threading.Thread(target=_alive, args=("mixed batch", t1, done), daemon=True).start()
+    batch.run()
     done.set()
     run_s = time.perf_counter() - t1
     print(f"# mixed: {nclu} clusters in {run_s:.2f}s", file=sys.stderr, flush=True)
-    Ls, st = batches[0].results()
-    L = np.stack(Ls)
-    X0, C0 = baths[0]
+    Ls, st = batch.results()
+    LU = np.stack(Ls)
+    base = np.concatenate([[0], np.cumsum(cu)])
+    L = np.concatenate([LU[base[o]:base[o] + counts0[o]] for o in range(6)])   # bath 0's clusters
     final = C0.reduce(L.reshape(-1), len(tl))["final"]
     out = {"workload": "CCE-6 Hahn echo, r=4nm p=0.011, cutoff 5A, 401 points on 0-5ms, mixed m=30 tol=1e-3",
-           "seeds": a.seeds, "clusters": nclu, "clusters_per_order_seed0": [int(c) for c in counts[0]],
+           "seeds": a.seeds, "clusters": nclu, "clusters_per_order": [int(c) for c in cu],
+           "clusters_per_order_seed0": [int(c) for c in counts0],
            "setup_s": setup, "run_s": run_s, "cluster_expv_per_s": nclu / run_s,
-           "buckets_seed0": batches[0].timing(),
-           "failed_clusters": int(sum(int(((b.results()[1][:, 8] & 0xFF) != 0).sum()) for b in batches)),
+           "buckets": batch.timing(),
+           "failed_clusters": int(((st[:, 8] & 0xFF) != 0).sum()),
            "L_cce6_seed0_at_2_5ms": float(final[200, -1])}
     if not a.no_fp64:
         # FP64 on every cluster of seed 0, on the full window

@@ -669,7 +669,15 @@ static int finalize_batch(oops_ctx* ctx, std::unique_ptr<oops_hahn_batch>& b, st
             rc = hip_check(ctx, hipMalloc(&B.medws.p, bytes), "hipMalloc medium workspace");
             if (rc) return rc;
             B.medws.bytes = bytes;
-            B.medslot = B.medslot && !ell_only;
+            // The flip-mask form streams a work unit's per-element coefficient
+            // tables (1.8 MB for a 6-spin Liouvillian) from HBM every product;
+            // about 110 units running at once saturate it (CCE-6, n = 4096:
+            // 70 units 2.1 s, 232 units 4.5 s, 1736 units 37 s).  The merged
+            // ELL form reads its tables from LDS (1.5x slower per unit when
+            // alone, 3.3 s, but 26.6 s for 1736 units), so buckets with more
+            // units than 2/3 of the CUs take it.
+            const int64_t units = 2 * (int64_t)B.items.size();
+            B.medslot = B.medslot && !ell_only && 3 * units <= 2 * (int64_t)ctx->num_cu;
             if (B.medslot) {
                 // flip-mask tables: per workgroup four generators of ms64 FP64 + ms32 FP32 slots
                 const int64_t gstride = (int64_t)B.ms64 * B.n + ((int64_t)B.ms32 * B.n + 1) / 2;

@@ -242,3 +242,19 @@ def test_cce_result_export(tmp_path):
     r = C.reduce(L.reshape(-1), nt)
     assert np.array_equal(z["final_result"], r["final"])
     assert np.array_equal(z["CCE1"], L[C.counts()[0]:C.counts()[0] + C.counts()[1]].T)
+
+
+def test_union_of_baths_enumerates_each_bath_alone():
+    """Clusters.union (one enumeration over several baths, no connection
+    between them; cDepthFirstPathTracing on the block-diagonal graph): every
+    order holds exactly the clusters of each bath enumerated alone, offset by
+    the bath's first spin, bath 0's first and in its own order."""
+    Xs = [O.bath_diamond(3.567, 25.0, 0.02, seed=s) for s in range(3)]
+    CU, first = O.Clusters.union(Xs, 5.0, 4)
+    Cs = [O.Clusters(X, 5.0, 4) for X in Xs]
+    assert list(first) == [0, len(Xs[0]), len(Xs[0]) + len(Xs[1])]
+    for o in range(4):
+        got = {tuple(r) for r in CU.get(o).tolist()}
+        want = {tuple(int(i) + int(f) for i in r) for f, C in zip(first, Cs) for r in C.get(o).tolist()}
+        assert got == want and CU.count(o) == sum(C.count(o) for C in Cs)
+        assert np.array_equal(CU.get(o)[:Cs[0].count(o)], Cs[0].get(o))
