@@ -208,3 +208,26 @@ def test_graph_replay_of_medium_bucket_matches_eager_launch(ctx):
         assert (st[:, 14:16] == 0).all()
         for i in range(len(items)):
             assert np.array_equal(Lg[i], Le[i]), (rep, i)
+
+
+@pytest.mark.gpu
+def test_union_of_baths_batch_equals_single_bath_batch(ctx):
+    """One batch over several baths (Clusters.union) gives bath 0's clusters
+    the same L(t), bit for bit, as a batch over bath 0 alone: every cluster
+    is evolved on its own whatever else the batch holds."""
+    par = O.cce_params(B=np.ones(3) * 0.05 / np.sqrt(3.0))
+    Xs = [O.bath_diamond(3.567, 22.0, 0.02, seed=s) for s in range(3)]
+    CU, first = O.Clusters.union(Xs, 7.0, 4)
+    C0 = O.Clusters(Xs[0], 7.0, 4)
+    t = np.linspace(0.0, 1e-3, 21)
+    bu = CU.echo_batch(ctx, par, 1, t / 2.0, m=30, tol=1e-3, precision="mixed")
+    bu.run()
+    LU, su = bu.results()
+    b0 = C0.echo_batch(ctx, par, 1, t / 2.0, m=30, tol=1e-3, precision="mixed")
+    b0.run()
+    L0, s0 = b0.results()
+    cu, c0 = CU.counts(), C0.counts()
+    base = np.concatenate([[0], np.cumsum(cu)])
+    rows = np.concatenate([np.arange(base[o], base[o] + c0[o]) for o in range(4)])
+    print(f"union {cu} vs bath 0 {c0}", flush=True)
+    assert np.array_equal(np.stack(LU)[rows], np.stack(L0))

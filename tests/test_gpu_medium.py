@@ -144,3 +144,27 @@ def test_medium_flip_mask_form_matches_ell_form(ctx, k):
     e = np.abs(out["mixed"] - out["mixed-ell"]).max()
     print(f"medium k={k} flip-mask vs ELL: max |dL| {e:.2e}", flush=True)
     assert e <= 1e-6
+
+
+@pytest.mark.gpu
+def test_many_units_take_the_ell_form_and_agree_with_the_flip_mask_form(ctx):
+    """A medium bucket with more work units than 2/3 of the CUs runs the
+    merged ELL product from LDS (the flip-mask coefficient stream would
+    saturate HBM); its L(t) agrees with the same clusters run as a small
+    bucket on the flip-mask form to the mixed path's bar."""
+    ncu = ctx.info()["num_cu"]
+    many = ncu // 3 + 8                     # 2 units per Hahn item: > 2/3 of the CUs
+    coords = _clusters(5, many)
+    items = [_hahn_items(c) for c in coords]
+    tau = np.linspace(0.0, 6e-4, 25) / 2
+    big = O.HahnBatch.from_items(ctx, items, m=30, tol=1e-3, precision="mixed")
+    big.run(tau)
+    Lb, sb = big.results()
+    small = O.HahnBatch.from_items(ctx, items[:8], m=30, tol=1e-3, precision="mixed")
+    small.run(tau)
+    Ls, ss = small.results()
+    assert big.timing()[0]["kernel"] == small.timing()[0]["kernel"] == "lanczos_medium"
+    assert (sb[:, 8] & 0xFF).max() == 0 and (ss[:, 8] & 0xFF).max() == 0
+    err = np.abs(Lb[:8] - Ls).max()
+    print(f"{many} items (ELL form) vs 8 items (flip-mask form): max |dL| {err:.2e}", flush=True)
+    assert err <= 1e-6
