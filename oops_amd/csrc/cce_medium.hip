@@ -116,13 +116,112 @@ __device__ __forceinline__ cd med_product(const MedSmem<M, E>& S, int q) {
     return op_apply_elem_mixed(i, S.xs, S.xs32, S.op.hdr, S.op.locs, S.op.vals, S.op.offs);
 }
 
+// The merged ELL product for the E elements i0 + kMT r of this thread, the
+// local operators walked once.  kMT r sets the digits of spin 0 only (kMT =
+// 2^10, two bits per spin): a local operator without spin 0 has the same row
+// for the E elements, so each chunk of its entries is read from LDS once and
+// gathered for the E elements (E independent chains); operators on spin 0
+// take a row per element.
+template <int E>
+__device__ __forceinline__ void ell_product_all(const cd* __restrict__ xs, const float2* __restrict__ xs32,
+                                                const DevOpHeader& h, const DevOpLocal* __restrict__ locs,
+                                                const cd* __restrict__ vals, const int32_t* __restrict__ offs,
+                                                cd (&y)[E]) {
+    const int i0 = threadIdx.x;
+    const cd c0 = mkcd(h.c0re, h.c0im);
+    cd acc[E];
+    float ar[E], ai[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        acc[r] = cmul(c0, xs[i0 + kMT * r]);
+        ar[r] = 0.f;
+        ai[r] = 0.f;
+    }
+    const float2* v32 = reinterpret_cast<const float2*>(vals + h.ent2_first);
+    const int ld = h.ld_bits;
+    const int lmask = (1 << ld) - 1;
+    const int top = h.nspin - 1;
+    for (int L = 0; L < h.nloc; ++L) {
+        const DevOpLocal lo = locs[L];
+        const bool spin0 = lo.s == 0 || (lo.nb == 2 && lo.t == 0);
+        const int shs = ld * (top - lo.s);
+        const int sht = ld * (top - lo.t);
+        int base[E], e0[E];
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const int i = i0 + kMT * r;
+            const int ls = (i >> shs) & lmask;
+            int b = i - (ls << shs);
+            int row = ls;
+            if (lo.nb == 2) {
+                const int lt = (i >> sht) & lmask;
+                b -= lt << sht;
+                row = (ls << ld) | lt;
+            }
+            base[r] = b;
+            e0[r] = lo.ent + row * lo.R;
+        }
+        if (!spin0) {
+            // one row for the E elements: the chunk's entries once, E gathers each
+            const int e00 = e0[0];
+            if (lo.nb == 2) {
+#pragma unroll 1
+                for (int e = 0; e < lo.R; e += 4) {
+                    const int4 q = *reinterpret_cast<const int4*>(offs + e00 + e);
+                    const int o[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const float2 vv = v32[e00 - h.ent2_first + e + u];
+#pragma unroll
+                        for (int r = 0; r < E; ++r) {
+                            const float2 xv = xs32[base[r] + o[u]];
+                            ar[r] = fmaf(vv.x, xv.x, fmaf(-vv.y, xv.y, ar[r]));
+                            ai[r] = fmaf(vv.x, xv.y, fmaf(vv.y, xv.x, ai[r]));
+                        }
+                    }
+                }
+            } else {
+#pragma unroll 1
+                for (int e = 0; e < lo.R; e += 4) {
+                    const int4 q = *reinterpret_cast<const int4*>(offs + e00 + e);
+                    const int o[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const cd vv = vals[e00 + e + u];
+#pragma unroll
+                        for (int r = 0; r < E; ++r) acc[r] = cfma(vv, xs[base[r] + o[u]], acc[r]);
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < E; ++r) {
+                if (lo.nb == 2) {
+                    const float2 a32 = ell_row32(xs32, base[r], v32, offs, e0[r], e0[r] - h.ent2_first, lo.R,
+                                                 make_float2(ar[r], ai[r]));
+                    ar[r] = a32.x;
+                    ai[r] = a32.y;
+                } else {
+                    acc[r] = ell_row(xs, base[r], vals, offs, e0[r], lo.R, acc[r]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < E; ++r) y[r] = mkcd(acc[r].re + (double)ar[r], acc[r].im + (double)ai[r]);
+}
+
 // All E elements of this thread.  The flip-mask form walks the slots once
 // for the E elements (E independent coefficient loads per slot in flight).
 template <int M, int E>
 __device__ __forceinline__ void med_product_all(const MedSmem<M, E>& S, cd (&y)[E]) {
     if (!S.slot) {
+        if constexpr (E > 1) {
+            ell_product_all<E>(S.xs, S.xs32, S.op.hdr, S.op.locs, S.op.vals, S.op.offs, y);
+        } else {
 #pragma unroll
-        for (int r = 0; r < E; ++r) y[r] = med_product<M, E>(S, r);
+            for (int r = 0; r < E; ++r) y[r] = med_product<M, E>(S, r);
+        }
         return;
     }
     const int n = kMT * E;
