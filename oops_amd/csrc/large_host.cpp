@@ -59,7 +59,7 @@ struct LargeRun {
     bool fp32 = false;
     int64_t n = 0;
     int64_t ldv = 0;   // basis column stride (elements)
-    int m = 0, ldh = 0, grid = 0, nblk = 0;
+    int m = 0, ldh = 0, grid = 0, nblk = 0, nblk_dots = 0;
     DevBuf V, p0, p1, w, v0, part, hess, coef, scal, flags, F, status, wpart;
     DevBuf wk, wkpart;   // a state at an output time inside a step (interior-output mode)
     // tiled product (large Kronecker operators): one entry per pass
@@ -580,6 +580,10 @@ int setup_buffers(oops_ctx* ctx, int m, LargeRun& R) {
     // blocks of the Gram-Schmidt and dot-product passes: the full grid (up to
     // eight per CU), so the long dots passes keep enough loads in flight
     R.nblk = R.grid;
+    // the dots passes' partial sums are reduced row by row in one block: on
+    // full grids half the blocks there halve that reduction (and the passes
+    // themselves run as fast or faster: 2048 -> 1024 blocks for 2^20 elements)
+    R.nblk_dots = R.grid >= 1024 ? R.grid / 2 : R.grid;
     const size_t vb = sizeof(double2) * (size_t)R.n;
     // basis columns 4352 B (17 x 256 B) apart beyond a power of two: columns
     // of 2^k elements would start at addresses equal modulo every power of two
@@ -790,12 +794,14 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
             const int* skip = (const int*)R.flags.p + 2;
             double* part = (double*)R.part.p;
             double2* coef = (double2*)R.coef.p;
-            launch_large_gs_dots(R.V.p, R.ldv, p, n, j, part, R.nblk, R.s, R.fp32);
-            launch_large_finish(part, R.nblk, j + 2, j, 0, hess, R.ldh, coef, scal, (int*)R.flags.p, break_tol, R.s);
+            launch_large_gs_dots(R.V.p, R.ldv, p, n, j, part, R.nblk_dots, R.s, R.fp32);
+            launch_large_finish(part, R.nblk_dots, j + 2, j, 0, hess, R.ldh, coef, scal, (int*)R.flags.p, break_tol,
+                                R.s);
             launch_large_gs_update(R.V.p, R.ldv, p, n, j, coef, part, R.nblk, R.s, R.fp32);
             launch_large_finish(part, R.nblk, 1, j, 1, hess, R.ldh, coef, scal, (int*)R.flags.p, break_tol, R.s);
-            launch_large_gs_dots(R.V.p, R.ldv, p, n, j, part, R.nblk, R.s, R.fp32, skip);
-            launch_large_finish(part, R.nblk, j + 2, j, 4, hess, R.ldh, coef, scal, (int*)R.flags.p, break_tol, R.s);
+            launch_large_gs_dots(R.V.p, R.ldv, p, n, j, part, R.nblk_dots, R.s, R.fp32, skip);
+            launch_large_finish(part, R.nblk_dots, j + 2, j, 4, hess, R.ldh, coef, scal, (int*)R.flags.p, break_tol,
+                                R.s);
             launch_large_gs_update(R.V.p, R.ldv, p, n, j, coef, part, R.nblk, R.s, R.fp32, skip);
             launch_large_finish(part, R.nblk, 1, j, 2, hess, R.ldh, coef, scal, (int*)R.flags.p, break_tol, R.s);
             x = p;
