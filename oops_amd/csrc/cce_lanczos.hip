@@ -54,6 +54,8 @@ struct LanczosSmem {
     int ints[8];
 #ifdef OOPS_PHASE_TIMERS
     unsigned long long tbes;   // cycles of the Chebyshev set-up (interval, Bessel values)
+    unsigned long long tser;   // cycles of the Chebyshev series (wave 0)
+    unsigned long long temit;  // cycles of the output-time states inside a step (thread 0)
 #endif
 };
 
@@ -448,6 +450,8 @@ struct LzSlotSmem {
     int ints[8];
 #ifdef OOPS_PHASE_TIMERS
     unsigned long long tbes;   // cycles of the Chebyshev set-up (interval, Bessel values)
+    unsigned long long tser;   // cycles of the Chebyshev series (wave 0)
+    unsigned long long temit;  // cycles of the output-time states inside a step (thread 0)
 #endif
 };
 
@@ -558,14 +562,29 @@ __device__ __forceinline__ void advance_slot(LzSlotSmem<M, F64>& S, BlockRed<kNT
         OOPS_TIC(te0);
         // w = beta * (sum_{j<mb} F_j v_j + [k1] F_M v_M)
         const int mx = mb + (k1 - 1 > 0 ? k1 - 1 : 0);
+        // four basis vectors per iteration into two accumulators: the LDS
+        // reads of an iteration go out together and the two FMA chains
+        // interleave
         auto combine = [&]() -> cd {
-            cd acc = cmul(v0, S.F[0]);
+            cd acc0 = cmul(v0, S.F[0]), acc1 = mkcd(0.0, 0.0);
+            int j = 1;
 #pragma unroll 1
-            for (int j = 1; j < mx; ++j) {
-                const auto f = S.V[j - 1][tid];
-                acc = cfma(mkcd((double)f.x, (double)f.y), S.F[j], acc);
+            for (; j + 3 < mx; j += 4) {
+                const auto f0 = S.V[j - 1][tid];
+                const auto f1 = S.V[j][tid];
+                const auto f2 = S.V[j + 1][tid];
+                const auto f3 = S.V[j + 2][tid];
+                acc0 = cfma(mkcd((double)f0.x, (double)f0.y), S.F[j], acc0);
+                acc1 = cfma(mkcd((double)f1.x, (double)f1.y), S.F[j + 1], acc1);
+                acc0 = cfma(mkcd((double)f2.x, (double)f2.y), S.F[j + 2], acc0);
+                acc1 = cfma(mkcd((double)f3.x, (double)f3.y), S.F[j + 3], acc1);
             }
-            return cscale(acc, g.beta);
+#pragma unroll 1
+            for (; j < mx; ++j) {
+                const auto f = S.V[j - 1][tid];
+                acc0 = cfma(mkcd((double)f.x, (double)f.y), S.F[j], acc0);
+            }
+            return cscale(mkcd(acc0.re + acc1.re, acc0.im + acc1.im), g.beta);
         };
         // Trial step: F walks from e_1 through the output times inside the
         // step (each evaluated and emitted from this basis) to the step end,
@@ -633,7 +652,7 @@ void cce_hahn_lanczos_slot_kernel(KrylovArgs a) {
     const int max_steps = a.max_steps;
     double2* const ws = a.slotws + (int64_t)blockIdx.x * 4 * kTab;
 #ifdef OOPS_PHASE_TIMERS
-    if (tid == 0) S.tbes = 0;
+    if (tid == 0) S.tbes = S.tser = S.temit = 0;
 #endif
     for (;;) {
         if (tid == 0) S.ints[7] = atomicAdd(a.queue, 1);
@@ -735,7 +754,9 @@ void cce_hahn_lanczos_slot_kernel(KrylovArgs a) {
             st[18] = (int)((gA.texp + gB.texp) >> 10);
             st[19] = (int)((gA.tupd + gB.tupd) >> 10);
             st[20] = (int)(S.tbes >> 10);
-            S.tbes = 0;
+            st[21] = (int)(S.tser >> 10);
+            st[22] = (int)(S.temit >> 10);
+            S.tbes = S.tser = S.temit = 0;
 #endif
         }
         __syncthreads();

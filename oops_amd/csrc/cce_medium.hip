@@ -48,7 +48,7 @@ struct MedSmem {
     int ns64, ns32, slot;
     int mask[kMedMaskStride];
 #ifdef OOPS_PHASE_TIMERS
-    unsigned long long tbes;
+    unsigned long long tbes, tser, temit;
 #endif
 };
 

@@ -151,6 +151,9 @@ __device__ int cheb_series(SM& S, int mb, int k1, double beta_m, int q, double d
     const int tid = threadIdx.x;
     int terms = 0;
     if (tid < kWave) {
+#ifdef OOPS_PHASE_TIMERS
+        const unsigned long long ts0 = clock64();
+#endif
         const ChebTables& T = S.cheb;
         const int lane = tid;
         const int j = tid & 31;          // row of the augmented matrix
@@ -275,6 +278,9 @@ __device__ int cheb_series(SM& S, int mb, int k1, double beta_m, int q, double d
             else F[imh ? 1 : 0] = v;
         }
         terms = K * ns;
+#ifdef OOPS_PHASE_TIMERS
+        if (tid == 0) S.tser += clock64() - ts0;
+#endif
     }
     __syncthreads();
     return terms;
@@ -307,7 +313,15 @@ __device__ int cheb_walk(SM& S, int mb, int k1, double beta_m, double t0, double
         if (i > 0) __syncthreads();   // every thread has read F
         ++nexph;
         terms += cheb_series<M>(S, mb, k1, beta_m, i % kChebTables, dt_of(i), i > 0);
-        if (i < nint) emit(i);
+        if (i < nint) {
+#ifdef OOPS_PHASE_TIMERS
+            const unsigned long long te0 = clock64();
+#endif
+            emit(i);
+#ifdef OOPS_PHASE_TIMERS
+            if (threadIdx.x == 0) S.temit += clock64() - te0;
+#endif
+        }
     }
     return terms;
 }
