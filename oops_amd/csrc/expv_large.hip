@@ -24,6 +24,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "devcommon.h"
 #include "oops_kernels.h"
@@ -136,6 +137,71 @@ __global__ __launch_bounds__(kLT) void large_matvec_split(LargeOpArgs o, const d
                 vstore(xstore + i, xv.x * xs, xv.y * xs);
             }
         }
+    }
+}
+
+// The same product with the local operators split over the NW waves of a
+// workgroup instead of over lanes: lane l of every wave takes element
+// 64 b + l, wave w the operators w, w + NW, ..; the partial sums meet in LDS
+// (summed in wave order).  All lanes of a wave read the tables of one local
+// operator (at most d^2 rows) and gather from 64 neighbouring elements, so
+// each load instruction touches a few cache lines; in the lane-split form
+// the 16 lanes of an element read 16 operators' tables and gather from 16
+// unrelated places.
+template <typename VT, int NW>
+__global__ __launch_bounds__(NW* kWave) void large_matvec_waves(LargeOpArgs o, const double2* __restrict__ x,
+                                                                const double* __restrict__ xscale_p,
+                                                                double2* __restrict__ y, VT* __restrict__ xstore,
+                                                                int64_t n, int alpha_mode) {
+    __shared__ LargeOpLds L;
+    __shared__ cd part[NW][kWave];
+    if (threadIdx.x == 0) L.hdr = *o.hdr;
+    const int nloc = o.hdr->nloc;
+    for (int k = threadIdx.x; k < nloc; k += NW * kWave) L.locs[k] = o.loc[k];
+    __syncthreads();
+    const double xs = xscale_p ? *xscale_p : 1.0;
+    const cd* xc = reinterpret_cast<const cd*>(x);
+    const cd* vals = reinterpret_cast<const cd*>(o.val);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const int ld = L.hdr.ld_bits;
+    const int lmask = (1 << ld) - 1;
+    const int top = L.hdr.nspin - 1;
+    // the bound is uniform over the workgroup (the barriers need every wave)
+    for (int64_t i0 = (int64_t)blockIdx.x * kWave; i0 < n; i0 += (int64_t)gridDim.x * kWave) {
+        const int64_t i = i0 + lane;
+        const bool live = i < n;
+        const int ii = (int)i;
+        cd acc = (w == 0 && live) ? cmul(mkcd(L.hdr.c0re, L.hdr.c0im), xc[ii]) : mkcd(0.0, 0.0);
+        if (live)
+            for (int Lo = w; Lo < L.hdr.nloc; Lo += NW) {
+                const DevOpLocal lo = L.locs[Lo];
+                const int shs = ld * (top - lo.s);
+                const int ls = (ii >> shs) & lmask;
+                int base = ii - (ls << shs);
+                int row = ls;
+                if (lo.nb == 2) {
+                    const int sht = ld * (top - lo.t);
+                    const int lt = (ii >> sht) & lmask;
+                    base -= lt << sht;
+                    row = (ls << ld) | lt;
+                }
+                acc = ell_row(xc, base, vals, o.off, lo.ent + row * lo.R, lo.R, acc);
+            }
+        part[w][lane] = acc;
+        __syncthreads();
+        if (w == 0 && live) {
+            cd s = part[0][lane];
+#pragma unroll
+            for (int q = 1; q < NW; ++q) s = cadd(s, part[q][lane]);
+            s = cscale(s, xs);
+            if (alpha_mode == 1) s = cmulmi(s);
+            y[i] = make_double2(s.re, s.im);
+            if (xstore) {
+                const double2 xv = x[i];
+                vstore(xstore + i, xv.x * xs, xv.y * xs);
+            }
+        }
+        __syncthreads();   // part is rewritten for the next elements
     }
 }
 
@@ -768,13 +834,29 @@ int large_grid(int64_t n, int num_cu) {
 void launch_large_matvec(const LargeOpArgs& o, const double2* x, const double* xscale, double2* y, void* xstore,
                          int64_t n, int alpha_mode, int grid, hipStream_t s, bool fp32) {
     if (n <= (int64_t(1) << 17)) {
-        // 16 lanes per element: n / 16 elements per 256-thread workgroup
-        const int g = (int)std::min<int64_t>((n * 16 + kLT - 1) / kLT, 4096);
+        // 64 elements per 1024-thread workgroup, the operators over its 16
+        // waves (OOPS_LARGE_SPLIT=1: 16 lanes per element instead)
+        static const bool lane_split = [] {
+            const char* e = std::getenv("OOPS_LARGE_SPLIT");
+            return e && e[0] == '1';
+        }();
+        if (lane_split) {
+            const int g = (int)std::min<int64_t>((n * 16 + kLT - 1) / kLT, 4096);
+            if (fp32)
+                hipLaunchKernelGGL((large_matvec_split<float2, 16>), dim3(g), dim3(kLT), 0, s, o, x, xscale, y,
+                                   (float2*)xstore, n, alpha_mode);
+            else
+                hipLaunchKernelGGL((large_matvec_split<double2, 16>), dim3(g), dim3(kLT), 0, s, o, x, xscale, y,
+                                   (double2*)xstore, n, alpha_mode);
+            return;
+        }
+        constexpr int NW = 16;
+        const int g = (int)std::min<int64_t>((n + kWave - 1) / kWave, 4096);
         if (fp32)
-            hipLaunchKernelGGL((large_matvec_split<float2, 16>), dim3(g), dim3(kLT), 0, s, o, x, xscale, y,
+            hipLaunchKernelGGL((large_matvec_waves<float2, NW>), dim3(g), dim3(NW * kWave), 0, s, o, x, xscale, y,
                                (float2*)xstore, n, alpha_mode);
         else
-            hipLaunchKernelGGL((large_matvec_split<double2, 16>), dim3(g), dim3(kLT), 0, s, o, x, xscale, y,
+            hipLaunchKernelGGL((large_matvec_waves<double2, NW>), dim3(g), dim3(NW * kWave), 0, s, o, x, xscale, y,
                                (double2*)xstore, n, alpha_mode);
         return;
     }
