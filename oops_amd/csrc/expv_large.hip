@@ -746,19 +746,31 @@ __global__ __launch_bounds__(kLT) void large_gs_dots_split(const VT* __restrict_
 // one workgroup).  status[0] = number of squarings or a negative error.
 struct LargePadeSmem {
     cd W[4][(kLargeMaxM + 2) * (kLargeMaxM + 3)];  // one spare column per matrix (tiled reads)
+    cd H[(kLargeMaxM + 2) * (kLargeMaxM + 3)];     // the input matrix, copied in once
     cd F[kLargeMaxM + 2];
     int piv[128];
     double scal[16];
 };
 
+// H (mx x mx, leading dimension ldh <= kLargeMaxM + 2) from device memory
+// into LDS: the norm, the two products with H and the solve then read LDS
+// instead of waiting on device-memory latency in their inner loops.
+__device__ __forceinline__ const cd* pade_input(const double2* __restrict__ hess, int ldh, int mx,
+                                                LargePadeSmem& S) {
+    for (int idx = threadIdx.x; idx < mx * mx; idx += kLT) {
+        const int i = idx % mx, j = idx / mx;
+        const double2 v = hess[i + (int64_t)ldh * j];
+        S.H[i + ldh * j] = mkcd(v.x, v.y);
+    }
+    __syncthreads();
+    return S.H;
+}
+
 __global__ __launch_bounds__(kLT) void large_pade(const double2* __restrict__ hess, int ldh, int mx, double t,
                                                   double2* __restrict__ F, int* __restrict__ status) {
     extern __shared__ unsigned char dyn[];
     LargePadeSmem& S = *reinterpret_cast<LargePadeSmem*>(dyn);
-    const int ld = kLargeMaxM + 2;
-    // H is read in place from device memory (leading dimension ldh)
-    const cd* H = reinterpret_cast<const cd*>(hess);
-    (void)ld;
+    const cd* H = pade_input(hess, ldh, mx, S);
     const int ns = pade6_first_column<kLT>(H, ldh, mx, t, S.W[0], S.W[1], S.W[2], S.W[3], S.F, S.piv, S.scal);
     for (int i = threadIdx.x; i < mx; i += kLT) F[i] = make_double2(S.F[i].re, S.F[i].im);
     if (threadIdx.x == 0) status[0] = ns;
@@ -771,7 +783,7 @@ __global__ __launch_bounds__(kLT) void large_pade_full(const double2* __restrict
                                                        double2* __restrict__ out, int* __restrict__ status) {
     extern __shared__ unsigned char dyn[];
     LargePadeSmem& S = *reinterpret_cast<LargePadeSmem*>(dyn);
-    const cd* H = reinterpret_cast<const cd*>(hess);
+    const cd* H = pade_input(hess, ldh, mx, S);
     cd* R = nullptr;
     const int ns = pade6_full<kLT>(H, ldh, mx, t, S.W[0], S.W[1], S.W[2], S.W[3], S.piv, S.scal, &R);
     if (ns >= 0)
