@@ -13,6 +13,8 @@
 //                    scaled x as the next basis vector; one thread per
 //                    element, the compiled operator (merged 1-/2-body ELL
 //                    tables) in LDS, gathers from L2/HBM.
+//   large_matvec_waves  the same for small states, the operators over the
+//                    waves of a workgroup.
 //   large_dots       partial sums of conj(V_l) . p for l <= J and |p|^2
 //                    (grid.y = vector index), 256-thread block reductions.
 //   large_finish     sums the partials, stores h into the Hessenberg column
@@ -24,7 +26,6 @@
 #include <stdint.h>
 
 #include <algorithm>
-#include <cstdlib>
 
 #include "devcommon.h"
 #include "oops_kernels.h"
@@ -81,73 +82,16 @@ __global__ __launch_bounds__(kLT) void large_matvec(LargeOpArgs o, const double2
     }
 }
 
-// The same product with each element's local operators split over LS
-// consecutive lanes (lane q takes operators q, q + LS, ..; the partial sums
-// meet by butterfly shuffles): states too small to fill the device with one
-// thread per element (a 14-spin Hilbert state has 16 384 elements and ~100
-// local operators) get LS times the parallelism and shorter gather chains.
-template <typename VT, int LS>
-__global__ __launch_bounds__(kLT) void large_matvec_split(LargeOpArgs o, const double2* __restrict__ x,
-                                                          const double* __restrict__ xscale_p,
-                                                          double2* __restrict__ y, VT* __restrict__ xstore,
-                                                          int64_t n, int alpha_mode) {
-    __shared__ LargeOpLds L;
-    large_op_load(o, L);
-    const double xs = xscale_p ? *xscale_p : 1.0;
-    const cd* xc = reinterpret_cast<const cd*>(x);
-    const cd* vals = reinterpret_cast<const cd*>(o.val);
-    const int sub = threadIdx.x % LS;
-    const int ld = L.hdr.ld_bits;
-    const int lmask = (1 << ld) - 1;
-    const int top = L.hdr.nspin - 1;
-    const int64_t stride = (int64_t)gridDim.x * (kLT / LS);
-    // every lane of an element group runs the same iterations (the shuffles
-    // need the whole group): the loop bound is per group
-    for (int64_t i0 = (int64_t)blockIdx.x * (kLT / LS); i0 < n; i0 += stride) {
-        const int64_t i = i0 + threadIdx.x / LS;
-        const bool live = i < n;
-        const int ii = live ? (int)i : 0;
-        cd acc = (sub == 0 && live) ? cmul(mkcd(L.hdr.c0re, L.hdr.c0im), xc[ii]) : mkcd(0.0, 0.0);
-        if (live)
-            for (int Lo = sub; Lo < L.hdr.nloc; Lo += LS) {
-                const DevOpLocal lo = L.locs[Lo];
-                const int shs = ld * (top - lo.s);
-                const int ls = (ii >> shs) & lmask;
-                int base = ii - (ls << shs);
-                int row = ls;
-                if (lo.nb == 2) {
-                    const int sht = ld * (top - lo.t);
-                    const int lt = (ii >> sht) & lmask;
-                    base -= lt << sht;
-                    row = (ls << ld) | lt;
-                }
-                acc = ell_row(xc, base, vals, o.off, lo.ent + row * lo.R, lo.R, acc);
-            }
-#pragma unroll
-        for (int msk = LS / 2; msk >= 1; msk >>= 1) {
-            acc.re += __shfl_xor(acc.re, msk, kWave);
-            acc.im += __shfl_xor(acc.im, msk, kWave);
-        }
-        if (sub == 0 && live) {
-            acc = cscale(acc, xs);
-            if (alpha_mode == 1) acc = cmulmi(acc);
-            y[i] = make_double2(acc.re, acc.im);
-            if (xstore) {
-                const double2 xv = x[i];
-                vstore(xstore + i, xv.x * xs, xv.y * xs);
-            }
-        }
-    }
-}
-
-// The same product with the local operators split over the NW waves of a
-// workgroup instead of over lanes: lane l of every wave takes element
-// 64 b + l, wave w the operators w, w + NW, ..; the partial sums meet in LDS
+// The same product for states of up to 2^17 elements (too few to fill the
+// device with one thread per element: a 14-spin Hilbert state has 16 384
+// elements and ~100 local operators), with the local operators split over
+// the NW waves of a workgroup: lane l of every wave takes element 64 b + l,
+// wave w the operators w, w + NW, ..; the partial sums meet in LDS
 // (summed in wave order).  All lanes of a wave read the tables of one local
 // operator (at most d^2 rows) and gather from 64 neighbouring elements, so
-// each load instruction touches a few cache lines; in the lane-split form
-// the 16 lanes of an element read 16 operators' tables and gather from 16
-// unrelated places.
+// each load instruction touches a few cache lines (split over the lanes of
+// one element instead, the 16 lanes would read 16 operators' tables and
+// gather from 16 unrelated places: twice the time for a 14-spin state).
 template <typename VT, int NW>
 __global__ __launch_bounds__(NW* kWave) void large_matvec_waves(LargeOpArgs o, const double2* __restrict__ x,
                                                                 const double* __restrict__ xscale_p,
@@ -846,22 +790,7 @@ int large_grid(int64_t n, int num_cu) {
 void launch_large_matvec(const LargeOpArgs& o, const double2* x, const double* xscale, double2* y, void* xstore,
                          int64_t n, int alpha_mode, int grid, hipStream_t s, bool fp32) {
     if (n <= (int64_t(1) << 17)) {
-        // 64 elements per 1024-thread workgroup, the operators over its 16
-        // waves (OOPS_LARGE_SPLIT=1: 16 lanes per element instead)
-        static const bool lane_split = [] {
-            const char* e = std::getenv("OOPS_LARGE_SPLIT");
-            return e && e[0] == '1';
-        }();
-        if (lane_split) {
-            const int g = (int)std::min<int64_t>((n * 16 + kLT - 1) / kLT, 4096);
-            if (fp32)
-                hipLaunchKernelGGL((large_matvec_split<float2, 16>), dim3(g), dim3(kLT), 0, s, o, x, xscale, y,
-                                   (float2*)xstore, n, alpha_mode);
-            else
-                hipLaunchKernelGGL((large_matvec_split<double2, 16>), dim3(g), dim3(kLT), 0, s, o, x, xscale, y,
-                                   (double2*)xstore, n, alpha_mode);
-            return;
-        }
+        // 64 elements per 1024-thread workgroup, the operators over its 16 waves
         constexpr int NW = 16;
         const int g = (int)std::min<int64_t>((n + kWave - 1) / kWave, 4096);
         if (fp32)
