@@ -523,18 +523,39 @@ __device__ void med_advance(MedSmem<M, E>& S, BlockRed<kMT>& br, MedState& g, cd
         OOPS_TIC(te0);
         // w = beta * (sum_{j<mb} F_j v_j + [k1] F_M v_M)
         const int mx = mb + (k1 - 1 > 0 ? k1 - 1 : 0);
+        // the basis loads of all E elements and of two vectors go out
+        // together (each element's sum keeps the order j = 0, 1, ..)
         auto combine = [&](cd (&x)[E]) {
             const cd F0 = S.F[0];
+            cd acc[E];
 #pragma unroll
-            for (int r = 0; r < E; ++r) {
-                cd acc = cmul(v0[r], F0);
+            for (int r = 0; r < E; ++r) acc[r] = cmul(v0[r], F0);
+            const float2* vb = Vg + tid;
+            int j = 1;
 #pragma unroll 1
-                for (int j = 1; j < mx; ++j) {
-                    const float2 f = Vg[(int64_t)(j - 1) * n + tid + kMT * r];
-                    acc = cfma(mkcd((double)f.x, (double)f.y), S.F[j], acc);
+            for (; j + 1 < mx; j += 2) {
+                float2 fa[E], fb[E];
+#pragma unroll
+                for (int r = 0; r < E; ++r) {
+                    fa[r] = vb[(int64_t)(j - 1) * n + kMT * r];
+                    fb[r] = vb[(int64_t)j * n + kMT * r];
                 }
-                x[r] = cscale(acc, g.beta);
+                const cd Fa = S.F[j], Fb = S.F[j + 1];
+#pragma unroll
+                for (int r = 0; r < E; ++r) {
+                    acc[r] = cfma(mkcd((double)fa[r].x, (double)fa[r].y), Fa, acc[r]);
+                    acc[r] = cfma(mkcd((double)fb[r].x, (double)fb[r].y), Fb, acc[r]);
+                }
             }
+            if (j < mx) {
+#pragma unroll
+                for (int r = 0; r < E; ++r) {
+                    const float2 f = vb[(int64_t)(j - 1) * n + kMT * r];
+                    acc[r] = cfma(mkcd((double)f.x, (double)f.y), S.F[j], acc[r]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < E; ++r) x[r] = cscale(acc[r], g.beta);
         };
         // Trial step: F walks from e_1 through the output times inside the
         // step (each evaluated from this basis and stored; vcur is free here)
