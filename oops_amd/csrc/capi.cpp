@@ -306,11 +306,11 @@ struct oops_hahn_batch {
     std::vector<hipStream_t> bstreams; // oops_hahn_batch_run: one stream per bucket (buckets overlap)
     double total_ms = 0.0;             // device interval of the last run, first launch to last completion
     ~oops_hahn_batch() {
-        if (graph_exec) hipGraphExecDestroy(graph_exec);
-        if (graph) hipGraphDestroy(graph);
-        for (auto& e : ev) hipEventDestroy(e);
-        if (stream) hipStreamDestroy(stream);
-        for (auto& q : bstreams) if (q) hipStreamDestroy(q);
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+        for (auto& q : bstreams) if (q) (void)hipStreamDestroy(q);
     }
 };
 
@@ -341,7 +341,7 @@ int oops_ctx_create(int32_t device, oops_ctx** out) {
 
 int oops_ctx_destroy(oops_ctx* ctx) {
     if (!ctx) return OOPS_OK;
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return OOPS_OK;
 }
@@ -1157,7 +1157,7 @@ int oops_echo_batch_capture(oops_hahn_batch* b) {
     }
     hipGraph_t g = nullptr;
     rc = hip_check(ctx, hipStreamEndCapture(b->stream, &g), "hipStreamEndCapture");
-    if (lrc) { if (g) hipGraphDestroy(g); return lrc; }
+    if (lrc) { if (g) (void)hipGraphDestroy(g); return lrc; }
     if (rc) return rc;
     b->graph = g;
     return hip_check(ctx, hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0), "hipGraphInstantiate");
@@ -1168,25 +1168,25 @@ int oops_echo_batch_launch(oops_hahn_batch* b) {
     oops_ctx* ctx = b->ctx;
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, OOPS_ERR_HIP, "hipSetDevice");
     if (b->graph_exec) {
-        hipEventRecord(b->ev[0], b->stream);
+        (void)hipEventRecord(b->ev[0], b->stream);
         int rc = hip_check(ctx, hipGraphLaunch(b->graph_exec, b->stream), "hipGraphLaunch");
-        hipEventRecord(b->ev[1], b->stream);
+        (void)hipEventRecord(b->ev[1], b->stream);
         b->launched = rc == OOPS_OK;
         return rc;
     }
     for (size_t bk = 0; bk < b->buckets.size(); ++bk) {
         auto& B = b->buckets[bk];
         if (B.large) {
-            hipEventRecord(b->ev[2 * bk], b->stream);
+            (void)hipEventRecord(b->ev[2 * bk], b->stream);
             int rc = bucket_run_large(b, B, 0, nullptr);
-            hipEventRecord(b->ev[2 * bk + 1], b->stream);
+            (void)hipEventRecord(b->ev[2 * bk + 1], b->stream);
             if (rc) return rc;
             continue;
         }
         const KrylovArgs a = bucket_args(b, B, 0, nullptr);
-        hipEventRecord(b->ev[2 * bk], b->stream);
+        (void)hipEventRecord(b->ev[2 * bk], b->stream);
         int rc = bucket_launch(b, B, a, b->stream, B.uniform);
-        hipEventRecord(b->ev[2 * bk + 1], b->stream);
+        (void)hipEventRecord(b->ev[2 * bk + 1], b->stream);
         if (rc) return rc;
     }
     b->launched = true;
@@ -1208,7 +1208,7 @@ int oops_echo_batch_wait(oops_hahn_batch* b) {
     if (rc) return rc;
     if (b->graph_exec) {  // one interval for the whole replayed graph
         float ms = 0.f;
-        hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
+        (void)hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
         for (auto& B : b->buckets) B.ms = ms / std::max<size_t>(1, b->buckets.size());
         b->total_ms = ms;
         return OOPS_OK;
@@ -1216,7 +1216,7 @@ int oops_echo_batch_wait(oops_hahn_batch* b) {
     b->total_ms = 0.0;
     for (size_t bk = 0; bk < b->buckets.size(); ++bk) {
         float ms = 0.f;
-        hipEventElapsedTime(&ms, b->ev[2 * bk], b->ev[2 * bk + 1]);
+        (void)hipEventElapsedTime(&ms, b->ev[2 * bk], b->ev[2 * bk + 1]);
         b->buckets[bk].ms = ms;
         b->total_ms += ms;   // one stream: the buckets run one after another
     }
@@ -1249,13 +1249,13 @@ int oops_hahn_batch_run(oops_hahn_batch* b, int32_t nt, const double* tau) {
     bool uniform = nt >= 2 && tau[0] == 0.0 && tau[1] > 0.0;
     for (int k = 2; k < nt && uniform; ++k) uniform = std::fabs(tau[k] - k * tau[1]) <= 1e-12 * tau[nt - 1];
     std::vector<hipEvent_t> ev(2 * b->buckets.size() + 2);
-    for (auto& e : ev) hipEventCreate(&e);
+    for (auto& e : ev) (void)hipEventCreate(&e);
     hipEvent_t ev_all0 = ev[ev.size() - 2], ev_all1 = ev[ev.size() - 1];
     // every bucket on its own stream, so that buckets that cannot fill the
     // device (few large clusters) overlap; allocations first, then the
     // launches, then the host-driven buckets of large states
     if (b->bstreams.size() != b->buckets.size()) {
-        for (auto& q : b->bstreams) if (q) hipStreamDestroy(q);
+        for (auto& q : b->bstreams) if (q) (void)hipStreamDestroy(q);
         b->bstreams.assign(b->buckets.size(), nullptr);
         for (auto& q : b->bstreams)
             if (!rc) rc = hip_check(ctx, hipStreamCreate(&q), "hipStreamCreate");
@@ -1277,16 +1277,16 @@ int oops_hahn_batch_run(oops_hahn_batch* b, int32_t nt, const double* tau) {
         rc = hip_check(ctx, hipStreamWaitEvent(st, ev_all0, 0), "hipStreamWaitEvent");
         if (rc) break;
         const KrylovArgs a = bucket_args(b, B, nt, (const double*)b->tau.p);
-        hipEventRecord(ev[2 * bk], st);
+        (void)hipEventRecord(ev[2 * bk], st);
         rc = bucket_launch(b, B, a, st, uniform);
-        hipEventRecord(ev[2 * bk + 1], st);
+        (void)hipEventRecord(ev[2 * bk + 1], st);
     }
     for (size_t bk = 0; bk < b->buckets.size() && !rc; ++bk) {
         auto& B = b->buckets[bk];
         if (!B.large) continue;
-        hipEventRecord(ev[2 * bk], ctx->stream);
+        (void)hipEventRecord(ev[2 * bk], ctx->stream);
         rc = bucket_run_large(b, B, nt, tau);
-        hipEventRecord(ev[2 * bk + 1], ctx->stream);
+        (void)hipEventRecord(ev[2 * bk + 1], ctx->stream);
     }
     for (size_t bk = 0; bk < b->buckets.size() && !rc; ++bk)
         if (!b->buckets[bk].large)
@@ -1302,14 +1302,14 @@ int oops_hahn_batch_run(oops_hahn_batch* b, int32_t nt, const double* tau) {
     if (!rc) {
         for (size_t bk = 0; bk < b->buckets.size(); ++bk) {
             float ms = 0.f;
-            hipEventElapsedTime(&ms, ev[2 * bk], ev[2 * bk + 1]);
+            (void)hipEventElapsedTime(&ms, ev[2 * bk], ev[2 * bk + 1]);
             b->buckets[bk].ms = ms;
         }
         float all = 0.f;
-        hipEventElapsedTime(&all, ev_all0, ev_all1);
+        (void)hipEventElapsedTime(&all, ev_all0, ev_all1);
         b->total_ms = all;
     }
-    for (auto& e : ev) hipEventDestroy(e);
+    for (auto& e : ev) (void)hipEventDestroy(e);
     return rc;
 }
 
@@ -1426,7 +1426,7 @@ int oops_slot_form(const oops_term_table* T, int32_t max_slots, int32_t* nslot64
 
 int oops_hahn_batch_destroy(oops_hahn_batch* b) {
     if (!b) return OOPS_OK;
-    if (b->ctx) hipSetDevice(b->ctx->device);
+    if (b->ctx) (void)hipSetDevice(b->ctx->device);
     delete b;
     return OOPS_OK;
 }

@@ -744,9 +744,17 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
     double t_now = 0.0;
     double2* hess = (double2*)R.hess.p;
     double* scal = (double*)R.scal.p;
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
+    // timing events, destroyed on every return
+    struct EventGuard {
+        hipEvent_t e = nullptr;
+        ~EventGuard() {
+            if (e) (void)hipEventDestroy(e);
+        }
+    } g0, g1;
+    rc = hip_check(R.ctx, hipEventCreate(&g0.e), "hipEventCreate");
+    rc = rc ? rc : hip_check(R.ctx, hipEventCreate(&g1.e), "hipEventCreate");
+    if (rc) return rc;
+    const hipEvent_t e0 = g0.e, e1 = g1.e;
     float ms_acc = 0.0f;
     if (interior) {
         rc = alloc(R.ctx, R.wk, sizeof(double2) * (size_t)n);
@@ -821,7 +829,7 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
             ++C.nstep;
             double t_step = std::min(t_out - t_now, t_new);
             int k1 = 2, mb = m;
-            hipEventRecord(e0, R.s);
+            (void)hipEventRecord(e0, R.s);
             (void)hipMemsetAsync(hess, 0, sizeof(double2) * R.ldh * (R.ldh + 1), R.s);
             (void)hipMemsetAsync(R.flags.p, 0, sizeof(int) * 4, R.s);
             const double inv_beta = 1.0 / beta;
@@ -882,20 +890,20 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
             mx = mb + std::max(0, k1 - 1);
             launch_large_combine(R.V.p, R.ldv, mx, (double2*)R.F.p, beta, (double2*)R.w.p, n, (double*)R.wpart.p, R.grid,
                                  R.s, R.fp32);
-            hipEventRecord(e1, R.s);
+            (void)hipEventRecord(e1, R.s);
             std::vector<double> wp(R.grid);
             rc = hip_check(R.ctx, hipMemcpyAsync(wp.data(), R.wpart.p, sizeof(double) * R.grid, hipMemcpyDeviceToHost, R.s),
                            "D2H beta");
             rc = rc ? rc : hip_check(R.ctx, hipStreamSynchronize(R.s), "sync combine");
             if (rc) return rc;
             float ms = 0.f;
-            hipEventElapsedTime(&ms, e0, e1);
+            (void)hipEventElapsedTime(&ms, e0, e1);
             ms_acc += ms;
             if (interior && tt < nt && ta[tt] < (t_step >= t_out - t_now ? t_out : t_now + t_step)) {
                 // output times inside the step, from the same basis (their
                 // exponentials, combinations and overlaps count as device time)
                 const double t1 = t_step >= t_out - t_now ? t_out : t_now + t_step;
-                hipEventRecord(e0, R.s);
+                (void)hipEventRecord(e0, R.s);
                 for (; tt < nt && ta[tt] < t1; ++tt) {
                     ++C.nexph;
                     if (launch_large_pade(hess, R.ldh, mb + k1, ta[tt] - t_now, (double2*)R.F.p, (int*)R.status.p,
@@ -912,11 +920,11 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
                     if (st < 0) return set_err(R.ctx, OOPS_ERR_NUMERIC, "small exponential failed");
                     C.nscale += st;
                 }
-                hipEventRecord(e1, R.s);
+                (void)hipEventRecord(e1, R.s);
                 rc = hip_check(R.ctx, hipEventSynchronize(e1), "sync interior");
                 if (rc) return rc;
                 float mi = 0.f;
-                hipEventElapsedTime(&mi, e0, e1);
+                (void)hipEventElapsedTime(&mi, e0, e1);
                 ms_acc += mi;
             }
             double s2 = 0.0;
@@ -944,8 +952,6 @@ int run_expv(LargeRun& R, const double* v_host, int nt, const double* ta, int kl
             if ((rc = emit(tt, (const double2*)R.w.p)) != 0) return rc;
         }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     C.kernel_ms = ms_acc;
     return OOPS_OK;
 }
@@ -971,11 +977,12 @@ int large_hahn_item(oops_ctx* ctx, const CompiledOp& A, const CompiledOp& B, con
     if (rc) return rc;
     std::vector<double> ov(2 * (size_t)nt);
     Counters CA, CB;
-    hipStreamCreateWithFlags(&RA.s, hipStreamNonBlocking);
+    rc = hip_check(ctx, hipStreamCreateWithFlags(&RA.s, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+    if (rc) return rc;
     RB.s = RA.s;
     rc = run_expv(RA, u0, nt, tau, klim, tol, 2, nullptr, CA, (double2*)ket.p);
     rc = rc ? rc : run_expv(RB, y0, nt, tau, klim, tol, 3, ov.data(), CB, (double2*)ket.p);
-    hipStreamDestroy(RA.s);
+    (void)hipStreamDestroy(RA.s);
     if (rc) return rc;
     for (int k = 0; k < nt; ++k) L[k] = scale * ov[2 * k];
     if (stats) {
@@ -1303,11 +1310,12 @@ int oops_expv_ex(oops_ctx* ctx, const oops_term_table* H, const double* v, int32
         }
         if (out_mode == 0) outp.assign(2 * (size_t)R.n * nt, 0.0);
     }
-    hipStreamCreateWithFlags(&R.s, hipStreamNonBlocking);
+    rc = hip_check(ctx, hipStreamCreateWithFlags(&R.s, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+    if (rc) return rc;
     Counters C;
     rc = run_expv(R, R.padded ? vp.data() : v, nt, ta, klim, tol, out_mode,
                   (R.padded && out_mode == 0) ? outp.data() : out, C, nullptr, (precision & 2) != 0);
-    hipStreamDestroy(R.s);
+    (void)hipStreamDestroy(R.s);
     if (!rc && R.padded && out_mode == 0 && out)
         for (int q = 0; q < nt; ++q)
             for (int64_t i = 0; i < R.n_logical; ++i) {
@@ -1412,10 +1420,11 @@ static int csr_expv(int64_t n, const std::vector<int64_t>& rp, const std::vector
     LargeRun R;
     int rc = setup_csr(ctx, n, rp, ci, cv, m, R);
     if (rc) return rc;
-    hipStreamCreateWithFlags(&R.s, hipStreamNonBlocking);
+    rc = hip_check(ctx, hipStreamCreateWithFlags(&R.s, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+    if (rc) return rc;
     Counters C;
     rc = run_expv(R, v, tn, ta, klim, tol, 0, w_seq, C);
-    hipStreamDestroy(R.s);
+    (void)hipStreamDestroy(R.s);
     return rc;
 }
 

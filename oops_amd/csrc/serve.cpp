@@ -427,7 +427,7 @@ int oops_server_destroy(oops_server* s) {
     }
     s->cv.notify_all();
     for (auto& w : s->workers) w.join();
-    if (s->ctx) hipSetDevice(s->ctx->device);
+    if (s->ctx) (void)hipSetDevice(s->ctx->device);
     for (int64_t id : s->running) {
         Req* q = s->reqs[id].get();
         if (q->batch) oops_echo_batch_wait(q->batch);
