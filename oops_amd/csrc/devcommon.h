@@ -28,7 +28,6 @@ __device__ __forceinline__ cd mkcd(double a, double b) { cd r; r.re = a; r.im = 
 static __device__ __noinline__ double ool_pow(double x, double y) { return pow(x, y); }
 static __device__ __noinline__ double ool_log(double x) { return log(x); }
 static __device__ __noinline__ double ool_log10(double x) { return log10(x); }
-static __device__ __noinline__ double ool_cbrt(double x) { return cbrt(x); }
 static __device__ __noinline__ cd ool_expi(double x) {   // (cos x, sin x)
     double sn, cs;
     sincos(x, &sn, &cs);
