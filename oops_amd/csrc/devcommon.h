@@ -592,14 +592,19 @@ __device__ int lds_pade6(const cd* H, int ldh, int m, double t, cd* w0, cd* w1, 
 // H: m x m block of an LDS matrix with leading dim ld; W0..W3: ld x ld LDS
 // workspaces; F receives exp(t H) e1 (m entries).  NT threads cooperate.
 // Returns ns (>= 0) or a negative code (-1 null H, -2 singular, -3 non-finite).
+// Thread (ii, jj) of the h x h grid (h = ceil(m / 2)) forms rows ii and
+// ii + h of columns 2 jj and 2 jj + 1: the lanes of a wave read consecutive
+// rows of A (16-byte steps; rows two apart put four lanes on the same LDS
+// banks) and a few columns of B (broadcast reads).
 template <int NT>
 __device__ __forceinline__ void tile_matmul(cd* C, const cd* A, const cd* B, int m, int ld, double alpha) {
     const int tiles = (m + 1) >> 1;
     for (int t = threadIdx.x; t < tiles * tiles; t += NT) {
-        const int i0 = 2 * (t % tiles), j0 = 2 * (t / tiles);
+        const int i0 = t % tiles, j0 = 2 * (t / tiles);
+        const int i1 = i0 + tiles < m ? i0 + tiles : i0;   // row of c10 / c11 (a repeat of i0 when odd m leaves none)
         cd c00 = mkcd(0.0, 0.0), c10 = c00, c01 = c00, c11 = c00;
         for (int l = 0; l < m; ++l) {
-            const cd a0 = A[i0 + ld * l], a1 = A[i0 + 1 + ld * l];
+            const cd a0 = A[i0 + ld * l], a1 = A[i1 + ld * l];
             const cd b0 = B[l + ld * j0], b1 = B[l + ld * (j0 + 1)];
             c00 = cfma(a0, b0, c00);
             c10 = cfma(a1, b0, c10);
@@ -607,10 +612,10 @@ __device__ __forceinline__ void tile_matmul(cd* C, const cd* A, const cd* B, int
             c11 = cfma(a1, b1, c11);
         }
         C[i0 + ld * j0] = cscale(c00, alpha);
-        if (i0 + 1 < m) C[i0 + 1 + ld * j0] = cscale(c10, alpha);
+        if (i0 + tiles < m) C[i1 + ld * j0] = cscale(c10, alpha);
         if (j0 + 1 < m) {
             C[i0 + ld * (j0 + 1)] = cscale(c01, alpha);
-            if (i0 + 1 < m) C[i0 + 1 + ld * (j0 + 1)] = cscale(c11, alpha);
+            if (i0 + tiles < m) C[i1 + ld * (j0 + 1)] = cscale(c11, alpha);
         }
     }
     __syncthreads();
@@ -766,11 +771,12 @@ __device__ __forceinline__ void tile_matmul_ip(cd* C, const cd* A, const cd* B, 
     const int tiles = (m + 1) >> 1;
     const int t = threadIdx.x;
     const bool active = t < tiles * tiles;
-    const int i0 = 2 * (t % tiles), j0 = 2 * (t / tiles);
+    const int i0 = t % tiles, j0 = 2 * (t / tiles);   // rows i0 and i0 + tiles, as in tile_matmul
+    const int i1 = i0 + tiles < m ? i0 + tiles : i0;
     cd c00 = mkcd(0.0, 0.0), c10 = c00, c01 = c00, c11 = c00;
     if (active) {
         for (int l = 0; l < m; ++l) {
-            const cd a0 = A[i0 + ld * l], a1 = A[i0 + 1 + ld * l];
+            const cd a0 = A[i0 + ld * l], a1 = A[i1 + ld * l];
             const cd b0 = B[l + ld * j0], b1 = B[l + ld * (j0 + 1)];
             c00 = cfma(a0, b0, c00);
             c10 = cfma(a1, b0, c10);
@@ -781,10 +787,10 @@ __device__ __forceinline__ void tile_matmul_ip(cd* C, const cd* A, const cd* B, 
     __syncthreads();
     if (active) {
         C[i0 + ld * j0] = cscale(c00, alpha);
-        if (i0 + 1 < m) C[i0 + 1 + ld * j0] = cscale(c10, alpha);
+        if (i0 + tiles < m) C[i1 + ld * j0] = cscale(c10, alpha);
         if (j0 + 1 < m) {
             C[i0 + ld * (j0 + 1)] = cscale(c01, alpha);
-            if (i0 + 1 < m) C[i0 + 1 + ld * (j0 + 1)] = cscale(c11, alpha);
+            if (i0 + tiles < m) C[i1 + ld * (j0 + 1)] = cscale(c11, alpha);
         }
     }
     __syncthreads();
