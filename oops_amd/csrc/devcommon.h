@@ -621,6 +621,33 @@ __device__ __forceinline__ void tile_matmul(cd* C, const cd* A, const cd* B, int
     __syncthreads();
 }
 
+// One Gauss-Jordan elimination step on [D | N] (D in W1, N in W2, m <= 64):
+// every row i != p loses f_i = D(i,k) / D(p,k) times row p in the columns
+// k+1.. of D and all columns of N.  Lane i of each wave takes row i (its f_i
+// once), the NT / 64 waves the columns in turn: no index division, and the
+// lanes of a wave touch consecutive rows.  Ends with a block barrier.
+template <int NT>
+__device__ __forceinline__ void gj_eliminate(cd* W1, cd* W2, int ld, int m, int k, int p, cd inv) {
+    static_assert(NT % kWave == 0, "whole waves");
+    constexpr int NG = NT / kWave;
+    const int i = threadIdx.x & (kWave - 1);
+    const int g = threadIdx.x / kWave;
+    const int nd = m - k - 1;   // columns k+1 .. m-1 of D
+    if (i < m && i != p) {
+        const cd f = cmul(W1[i + ld * k], inv);
+        for (int jj = g; jj < nd + m; jj += NG) {
+            if (jj < nd) {
+                const int j = k + 1 + jj;
+                W1[i + ld * j] = cfms(f, W1[p + ld * j], W1[i + ld * j]);
+            } else {
+                const int j = jj - nd;
+                W2[i + ld * j] = cfms(f, W2[p + ld * j], W2[i + ld * j]);
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // Pade(6) rational part: R = r(H t / 2^ns) into W0 (W1..W3 are workspace);
 // returns ns (the squarings still to do) or a negative code.
 template <int NT>
@@ -693,21 +720,7 @@ __device__ int pade6_rational(const cd* H, int ld, int m, double t, cd* W0, cd* 
         if (p < 0) return -2;
         if (tid == 0) piv_lds[p] = 1;
         const cd inv = cdiv(mkcd(1.0, 0.0), W1[p + ld * k]);
-        const int ncol = (m - k - 1) + m;
-        for (int idx = tid; idx < m * ncol; idx += NT) {
-            const int i = idx % m;
-            const int jj = idx / m;
-            if (i == p) continue;
-            const cd f = cmul(W1[i + ld * k], inv);
-            if (jj < m - k - 1) {
-                const int j = k + 1 + jj;
-                W1[i + ld * j] = cfms(f, W1[p + ld * j], W1[i + ld * j]);
-            } else {
-                const int j = jj - (m - k - 1);
-                W2[i + ld * j] = cfms(f, W2[p + ld * j], W2[i + ld * j]);
-            }
-        }
-        __syncthreads();
+        gj_eliminate<NT>(W1, W2, ld, m, k, p, inv);
     }
     // R = I + 2 Y, Y(k,:) = N(perm k,:) / D(perm k, k)   -> W0
     for (int idx = tid; idx < m * m; idx += NT) {
